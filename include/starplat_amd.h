@@ -1,0 +1,144 @@
+/*
+ * starplat_amd.h -- C ABI of the MI355X (gfx950) StarPlat kernel library.
+ *
+ * This is the boundary under StarPlat's generated entry points. The OMP
+ * backend emits (src/backends/backend_omp/dsl_cpp_generator.cpp:3056-3151,
+ * type mapping :2875-3054):
+ *
+ *   void Compute_PR  (graph& g, float beta, float delta, int maxIter, float* pageRank);
+ *   void Compute_SSSP(graph& g, int* dist, int* weight, int src);
+ *   auto Compute_TC  (graph& g);                  // returns long
+ *   void Compute_BC  (graph& g, float* BC, std::set<int>& sourceSet);
+ *
+ * over graphcode/graph.hpp's CSR (graph.hpp:44-57). Code generated for this
+ * backend keeps those C++ signatures and calls the functions below; nothing
+ * here takes C++ or torch types. All pointers are plain host pointers unless
+ * the name says d_ (device memory). Every function returns SPM_OK (0) or a
+ * negative status; spm_last_error() gives the message of the last failure on
+ * the calling thread.
+ *
+ * Threading: a handle owns one HIP stream; calls on the same handle must not
+ * overlap. Different handles are independent.
+ */
+#ifndef STARPLAT_AMD_H
+#define STARPLAT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPM_OK 0
+#define SPM_EINVAL (-1)      /* bad argument                                   */
+#define SPM_EHIP (-2)        /* HIP runtime error                              */
+#define SPM_ENOMEM (-3)      /* device allocation failed                       */
+#define SPM_ENOGRAPH (-4)    /* algorithm needs an array the handle lacks       */
+#define SPM_EDIVERGE (-5)    /* fixed point did not converge within V rounds    */
+
+typedef struct spm_graph spm_graph;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* spm_last_error(void);
+/* Library ABI version (major*100 + minor). */
+int spm_version(void);
+
+/* ---- device CSR mirror ------------------------------------------------
+ * Replaces: graph::parseGraph()'s arrays as consumed by generated code,
+ * graph.hpp:44-57 (indexofNodes, edgeList, rev_indexofNodes, srcList) and
+ * getEdgeLen() :134-137; the CUDA backend's per-call cudaMalloc/cudaMemcpy
+ * of the same arrays (backend_cuda/dsl_cpp_generator.cpp:4440-4500).
+ * Arrays are copied to HBM once; rev_* may be NULL (PR and BC then fail with
+ * SPM_ENOGRAPH); edgeLen may be NULL (SSSP then needs spm_graph_set_weights
+ * or a weight argument). */
+int spm_graph_create(int32_t num_nodes, int64_t num_edges,
+                     const int32_t* indexofNodes, const int32_t* edgeList,
+                     const int32_t* edgeLen,
+                     const int32_t* rev_indexofNodes, const int32_t* srcList,
+                     spm_graph** out);
+int spm_graph_destroy(spm_graph* g);
+/* Replaces: the propEdge<int> weight argument of Compute_SSSP
+ * (graphcode/main_functions/sssp_dslv2main.cpp:33-41 builds it per CSR
+ * position). Uploads E weights. */
+int spm_graph_set_weights(spm_graph* g, const int32_t* weight);
+/* The hipStream_t (as void*) all work on g is ordered on. */
+void* spm_graph_stream(spm_graph* g);
+int spm_graph_sync(spm_graph* g);
+/* Device bytes currently held by the handle (CSR plus workspaces). */
+int64_t spm_graph_device_bytes(spm_graph* g);
+
+/* ---- PageRank ----------------------------------------------------------
+ * Replaces: Compute_PR(graph& g, float beta, float delta, int maxIter,
+ * float* pageRank) generated from graphcode/staticDSLCodes/PageRankDSLV2:1-19
+ * with PageRankDSLV3:15-19 diff semantics. pageRank: host float[V].
+ * iterCount (may be NULL) receives the number of iterations run. */
+int spm_compute_pr(spm_graph* g, float beta, float delta, int32_t maxIter,
+                   float* pageRank, int32_t* iterCount);
+/* Same, result left in device memory d_pageRank (float[V]). */
+int spm_pr_device(spm_graph* g, float beta, float delta, int32_t maxIter,
+                  float* d_pageRank, int32_t* iterCount);
+
+/* ---- SSSP --------------------------------------------------------------
+ * Replaces: Compute_SSSP(graph& g, int* dist, int* weight, int src) generated
+ * from graphcode/staticDSLCodes/sssp_dslV2:1-15 (fixedPoint Bellman-Ford).
+ * dist: host int32[V] (INT_MAX = unreachable); weight: host int32[E] by CSR
+ * position, or NULL to use the handle's weights. */
+int spm_compute_sssp(spm_graph* g, int32_t* dist, const int32_t* weight, int32_t src);
+/* Same, handle weights, result in device memory d_dist (int32[V]). rounds
+ * (may be NULL) receives the number of non-empty relaxation rounds. */
+int spm_sssp_device(spm_graph* g, int32_t src, int32_t* d_dist, int32_t* rounds);
+
+/* ---- Triangle counting --------------------------------------------------
+ * Replaces: auto Compute_TC(graph& g) (returns long) generated from
+ * graphcode/staticDSLCodes/triangle_counting_dsl:1-13; is_an_edge ->
+ * graph::check_if_nbr (graph.hpp:233-287). Requires a symmetric graph with
+ * ascending rows (graph.hpp:722-734 sorts them). */
+int spm_compute_tc(spm_graph* g, uint64_t* triangle_count);
+
+/* ---- Betweenness centrality ---------------------------------------------
+ * Replaces: Compute_BC(graph& g, float* BC, std::set<int>& sourceSet)
+ * generated from graphcode/staticDSLCodes/bc_dslV2:1-21. sources: the set's
+ * elements (any order, duplicates rejected); BC: host float[V]. Path counts
+ * are FP64, dependencies and BC FP32. */
+int spm_compute_bc(spm_graph* g, float* BC, const int32_t* sources, int32_t nsrc);
+/* Same, result in device memory d_BC (float[V]). */
+int spm_bc_device(spm_graph* g, const int32_t* sources, int32_t nsrc, float* d_BC);
+/* Forward results of one source, as bc_dslV2's iterateInBFS leaves them:
+ * bfsDist (int32[V], -1 unreached) and sigma (FP64[V] shortest-path counts).
+ * Replaces reading the generated code's bfsDist/sigma locals
+ * (backend_omp/dsl_cpp_generator.cpp:66-130); host buffers. */
+int spm_bc_paths(spm_graph* g, int32_t src, int32_t* bfsDist, double* sigma);
+
+/* ---- schedules -----------------------------------------------------------
+ * The schedule changes how work maps to lanes, never the arithmetic. No
+ * counterpart in the reference (its CUDA backend hard-codes 512 threads per
+ * vertex grid, backend_cuda/dsl_cpp_generator.cpp:56-77). */
+typedef struct spm_schedule {
+  int32_t pr_t1;          /* in-degree <= t1: one lane per vertex            */
+  int32_t pr_t2;          /* <= t2: pr_group lanes per vertex                */
+  int32_t pr_t3;          /* <= t3: one wavefront per vertex; above: block   */
+  int32_t pr_group;       /* lanes per vertex in the second bin (2..32)      */
+  int32_t sssp_mode;      /* 0 edge-balanced frontier queue, 1 dense sweep   */
+  int32_t sssp_rounds;    /* rounds per captured graph (multiple of 6)       */
+  int32_t sssp_grid;      /* workgroups per relaxation launch (0 = auto)     */
+  int32_t tc_mode;        /* 0 edge-centric search, 1 vertex-staged LDS      */
+  int32_t reserved[8];
+} spm_schedule;
+int spm_get_schedule(spm_graph* g, spm_schedule* s);
+int spm_set_schedule(spm_graph* g, const spm_schedule* s);
+/* The fixed heuristic schedule for g (fallback and judge of the learned one). */
+int spm_default_schedule(spm_graph* g, spm_schedule* s);
+
+/* ---- profiling -----------------------------------------------------------
+ * With profiling on, each kernel launch is bracketed by HIP events on the
+ * handle's stream and graphs are not used, so per-kernel device time can be
+ * read back. Off by default. */
+int spm_profile_enable(spm_graph* g, int enable);
+int spm_profile_reset(spm_graph* g);
+/* total_ms and launches of the named kernel since the last reset. */
+int spm_profile_query(spm_graph* g, const char* kernel, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,93 @@
+/*
+ * starplat_host.h -- host-side graph data layer of the MI355X StarPlat backend.
+ *
+ * Builds the CSR that StarPlat's `class graph` holds after parseGraph()
+ * (graphcode/graph.hpp:44-57, filled by parseEdges :464-515 and
+ * parseEdgesContent :720-859) without graph.hpp's std::map edge table:
+ *
+ *   indexofNodes      int32[V+1]  out-edge offsets            (graph.hpp:44)
+ *   edgeList          int32[E]    destinations, ascending per row (:46,:722-734)
+ *   edgeLen           int32[E]    weights by CSR position      (:37, getEdgeLen :134-137)
+ *   rev_indexofNodes  int32[V+1]  in-edge offsets              (:45,:786-807)
+ *   srcList           int32[E]    sources, ascending per row   (:48,:810-832)
+ *
+ * V follows graph::num_nodes() = max vertex id + 1 (graph.hpp:139-142), and
+ * duplicate (src,dst) pairs are merged by summing their weights (graph.hpp:497).
+ *
+ * Seeded generators (RMAT, Erdos-Renyi, 2-D grid) are counter-based, so a
+ * graph depends only on its parameters and seed, never on the thread count.
+ * All functions are plain C; no GPU is touched.
+ */
+#ifndef STARPLAT_HOST_H
+#define STARPLAT_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spmh_graph {
+  int32_t V;               /* num_nodes(): max id + 1                      */
+  int64_t E;               /* num_edges(): directed CSR entries            */
+  int32_t* indexofNodes;   /* [V+1]                                        */
+  int32_t* edgeList;       /* [E]                                          */
+  int32_t* edgeLen;        /* [E] weights (0 when the input had none)      */
+  int32_t* rev_indexofNodes; /* [V+1]                                      */
+  int32_t* srcList;        /* [E]                                          */
+} spmh_graph;
+
+/* RMAT: edge_factor * 2^scale samples, quadrant probabilities (a,b,c,1-a-b-c),
+ * no noise, no relabelling (vertex 0 is the hub). Self-loops dropped,
+ * duplicates removed; undirected != 0 symmetrises before deduplication.
+ * wmax > 0 assigns int32 weights uniform in [wmin,wmax] per surviving edge
+ * (undirected graphs get the same weight in both directions). */
+spmh_graph* spmh_rmat(int scale, int edge_factor, double a, double b, double c,
+                      uint64_t seed, int undirected, int wmin, int wmax);
+
+/* Erdos-Renyi G(n, m): m uniform endpoint pairs over [0,n), self-loops
+ * dropped, deduplicated, optionally symmetrised. */
+spmh_graph* spmh_er(int64_t n, int64_t m, uint64_t seed, int undirected,
+                    int wmin, int wmax);
+
+/* rows x cols 4-neighbour grid, both directions stored. */
+spmh_graph* spmh_grid(int rows, int cols, uint64_t seed, int wmin, int wmax);
+
+/* CSR from an explicit edge list with graph.hpp semantics: self-loops kept,
+ * duplicates merged by summing weights, V = max id + 1. w may be NULL (0). */
+spmh_graph* spmh_from_edges(int64_t n, const int32_t* src, const int32_t* dst,
+                            const int32_t* w);
+
+/* graph::parseGraph() on a text file (graph.hpp:464-515): lines whose first
+ * character is not a digit are skipped, "src dst [w]" per line, a missing
+ * weight reads as 0. Returns NULL when the file cannot be opened. */
+spmh_graph* spmh_parse_text(const char* path);
+
+/* Writes "src dst w" lines (or "src dst" when weights == 0). 0 on success. */
+int spmh_write_text(const spmh_graph* g, const char* path, int weights);
+
+void spmh_free(spmh_graph* g);
+
+/* n distinct vertices drawn uniformly from [0,V) with the given seed, written
+ * ascending (std::set<int> iteration order, as Compute_BC's sourceSet). */
+int spmh_sources(int32_t V, int32_t n, uint64_t seed, int32_t* out);
+
+/* Algorithmic-byte models used for roofline accounting. */
+/* PR: bytes of one pull iteration = 8E + 20V + 4. */
+double spmh_model_pr_bytes(const spmh_graph* g);
+/* SSSP: replays the synchronous-frontier Bellman-Ford trace from src and
+ * returns sum_k (12|F_k| + 12 M_k + 4|F_k+1|); also the rounds, the
+ * reachable out-edge count m_reach and the relaxation count. */
+double spmh_model_sssp(const spmh_graph* g, const int32_t* weight, int32_t src,
+                       int64_t* rounds, int64_t* m_reach, int64_t* relaxations);
+/* TC: 8(V+1) + 4 m_und + 4 sum_{oriented (u,v)} (d+(u) + d+(v)) where the
+ * orientation is by (degree, id), the one the GPU kernel counts with. */
+double spmh_model_tc_bytes(const spmh_graph* g, int64_t* m_und, int64_t* wedge_work);
+/* BC: per source 32 V_r + 16 E_r + 20 D summed over the sources. */
+double spmh_model_bc_bytes(const spmh_graph* g, const int32_t* sources, int32_t n,
+                           int64_t* edges_traversed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

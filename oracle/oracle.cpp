@@ -1,0 +1,422 @@
+// oracle.cpp -- TEST INFRASTRUCTURE ONLY. CPU restatement of the OpenMP code
+// that StarPlat's OMP backend (src/backends/backend_omp/dsl_cpp_generator.cpp)
+// emits for the four hot-path DSL programs, plus independent textbook
+// routines that pin it. Only the test suite, the smoke check and the CPU
+// baseline leg of the benchmark load this library; the product path never
+// does.
+//
+// Emission rules followed (file:line in the reference tree):
+//   attachNodeProperty -> parallel init loop         dsl_cpp_generator.cpp:744-792
+//   forall(v in g.nodes()) -> omp parallel for        :988-1015, :1056-1066
+//   nested neighbors(v) loop over indexofNodes/edgeList :1075-1093
+//   nodes_to(v) over rev_indexofNodes + srcList       :1094-1113 (srcList: fix 1)
+//   count_outNbrs -> indexofNodes[v+1]-indexofNodes[v] :2475-2484
+//   is_an_edge -> graph::check_if_nbr                  :2485-2495, graph.hpp:233-287
+//   fixedPoint -> while(!finished){finished=true;...}  :2637-2708
+//   Min reduction-list -> atomicMin + flag             :297-527
+//   iterateInBFS / iterateInReverse                    :66-193
+// Reference defects and the fix applied here (each behind `plant`-free code):
+//   fix 1  PR nodes_to reads srcList (the emitted code reads edgeList).
+//   fix 2  atomicMin is a correct CAS-min (graphcode/atomicUtil.h:3-16 can
+//          drop an update after a failed CAS).
+//   fix 3  BC sigma accumulation is atomic, in FP64.
+//   fix 4  PR updates diff every iteration (PageRankDSLV3:15-19 semantics).
+//
+// Every restated function takes `plant`: 0 is the restatement; a non-zero
+// value plants one deliberate mistake so the tests can show that the pinning
+// checks catch it.
+#include <omp.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <cmath>
+#include <queue>
+#include <vector>
+
+namespace {
+
+// Correct CAS-min (fix 2).
+inline void cas_min(int32_t* p, int32_t v) {
+  int32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+  while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {}
+}
+
+// atomicUtil.h:3-16 as written: the loop exits when the CAS returns a
+// previous value that converts to true, i.e. any non-zero value, even when
+// the CAS failed.
+inline void literal_atomic_min(int32_t* p, int32_t update) {
+  int32_t oldv, newv;
+  do {
+    oldv = *(volatile int32_t*)p;
+    newv = std::min(oldv, update);
+    if (oldv == newv) break;
+  } while (__sync_val_compare_and_swap(p, oldv, newv) == 0);
+}
+
+inline void atomic_add_f64(double* p, double v) {
+  double cur, nxt;
+  __atomic_load(p, &cur, __ATOMIC_RELAXED);
+  do {
+    nxt = cur + v;
+  } while (!__atomic_compare_exchange(p, &cur, &nxt, false, __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+}
+
+// graph::check_if_nbr (graph.hpp:279-284): linear scan of s's out-list.
+inline bool check_if_nbr_linear(const int32_t* off, const int32_t* adj, int32_t s, int32_t d) {
+  for (int32_t e = off[s]; e < off[s + 1]; ++e)
+    if (adj[e] == d) return true;
+  return false;
+}
+
+// The commented-out binary search of graph.hpp:235-256 (rows are sorted, so
+// it returns the same answer as the linear scan).
+inline bool check_if_nbr_bsearch(const int32_t* off, const int32_t* adj, int32_t s, int32_t d) {
+  return std::binary_search(adj + off[s], adj + off[s + 1], d);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Compute_PR (PageRankDSLV2 signature, PageRankDSLV3 diff semantics).
+// plant 1: nodes_to reads edgeList (the emitted defect).
+// plant 2: the damping term drops its delta factor.
+void oracle_compute_pr(int32_t V, const int32_t* indexofNodes, const int32_t* edgeList,
+                       const int32_t* rev_indexofNodes, const int32_t* srcList,
+                       float beta, float delta, int32_t maxIter, float* pageRank,
+                       int32_t* iterOut, int plant) {
+  const int32_t* in_nbrs = plant == 1 ? edgeList : srcList;
+  float num_nodes = (float)V;                         // float x = g.num_nodes()
+  float* pageRank_nxt = new float[V];
+#pragma omp parallel for
+  for (int t = 0; t < V; t++) {                       // attachNodeProperty
+    pageRank[t] = 1 / num_nodes;
+    pageRank_nxt[t] = 0;
+  }
+  int iterCount = 0;
+  float diff = 0.0;
+  do {
+    diff = 0.000000;
+#pragma omp parallel for reduction(+ : diff)
+    for (int v = 0; v < V; v++) {
+      float sum = 0.000000;
+      for (int edge = rev_indexofNodes[v]; edge < rev_indexofNodes[v + 1]; edge++) {
+        int nbr = in_nbrs[edge];
+        sum = sum + pageRank[nbr] / (indexofNodes[nbr + 1] - indexofNodes[nbr]);
+      }
+      float newPageRank = plant == 2 ? (1 - delta) / num_nodes + sum
+                                     : (1 - delta) / num_nodes + delta * sum;
+      if (newPageRank - pageRank[v] >= 0) {
+        diff = diff + newPageRank - pageRank[v];
+      } else {
+        diff = diff + pageRank[v] - newPageRank;
+      }
+      pageRank_nxt[v] = newPageRank;
+    }
+#pragma omp parallel for
+    for (int node = 0; node < V; node++) pageRank[node] = pageRank_nxt[node];
+    iterCount++;
+  } while ((diff > beta) && (iterCount < maxIter));
+  delete[] pageRank_nxt;
+  if (iterOut) *iterOut = iterCount;
+}
+
+// ---------------------------------------------------------------------------
+// Compute_SSSP, canonical fixedPoint form with `modified` associated.
+// plant 1: the reference's literal atomicMin (atomicUtil.h:3-16).
+// plant 2: the relaxation reads the weight of the next CSR slot.
+void oracle_compute_sssp(int32_t V, const int32_t* indexofNodes, const int32_t* edgeList,
+                         int32_t* dist, const int32_t* weight, int32_t src, int plant) {
+  const int64_t E = indexofNodes[V];
+  bool* modified = new bool[V];
+  bool* modified_nxt = new bool[V];
+#pragma omp parallel for
+  for (int t = 0; t < V; t++) {
+    dist[t] = INT_MAX;
+    modified[t] = false;
+    modified_nxt[t] = false;
+  }
+  modified[src] = true;
+  dist[src] = 0;
+  bool finished = false;
+  while (!finished) {
+    finished = true;
+#pragma omp parallel
+    {
+#pragma omp for
+      for (int v = 0; v < V; v++) {
+        if (modified[v] == true) {
+          for (int edge = indexofNodes[v]; edge < indexofNodes[v + 1]; edge++) {
+            int nbr = edgeList[edge];
+            int e = edge;
+            int we = plant == 2 ? weight[(e + 1) % E] : weight[e];
+            int dist_new = __atomic_load_n(&dist[v], __ATOMIC_RELAXED) + we;
+            bool modified_new = true;
+            if (__atomic_load_n(&dist[nbr], __ATOMIC_RELAXED) > dist_new) {
+              int oldValue = __atomic_load_n(&dist[nbr], __ATOMIC_RELAXED);
+              if (plant == 1) literal_atomic_min(&dist[nbr], dist_new);
+              else cas_min(&dist[nbr], dist_new);
+              if (oldValue > __atomic_load_n(&dist[nbr], __ATOMIC_RELAXED)) {
+                modified_nxt[nbr] = modified_new;
+                __atomic_store_n(&finished, false, __ATOMIC_RELAXED);
+              }
+            }
+          }
+        }
+      }
+#pragma omp for
+      for (int v = 0; v < V; v++) {
+        modified[v] = modified_nxt[v];
+        modified_nxt[v] = false;
+      }
+    }
+  }
+  delete[] modified;
+  delete[] modified_nxt;
+}
+
+// ---------------------------------------------------------------------------
+// Compute_TC. literal != 0 uses check_if_nbr's linear scan (graph.hpp:279-284);
+// otherwise the binary search over the sorted row (same integer).
+// plant 1: the u filter is reversed (u > v). plant 2: the w filter is dropped.
+int64_t oracle_compute_tc(int32_t V, const int32_t* indexofNodes, const int32_t* edgeList,
+                          int literal, int plant) {
+  long triangle_count = 0;
+#pragma omp parallel for reduction(+ : triangle_count) schedule(dynamic, 64)
+  for (int v = 0; v < V; v++) {
+    for (int edge = indexofNodes[v]; edge < indexofNodes[v + 1]; edge++) {
+      int u = edgeList[edge];
+      if (plant == 1 ? u > v : u < v) {
+        for (int edge_1 = indexofNodes[v]; edge_1 < indexofNodes[v + 1]; edge_1++) {
+          int w = edgeList[edge_1];
+          if (plant == 2 ? true : w > v) {
+            bool nb = literal ? check_if_nbr_linear(indexofNodes, edgeList, u, w)
+                              : check_if_nbr_bsearch(indexofNodes, edgeList, u, w);
+            if (nb) triangle_count = triangle_count + 1;
+          }
+        }
+      }
+    }
+  }
+  return triangle_count;
+}
+
+// ---------------------------------------------------------------------------
+// Compute_BC (bc_dslV2) with FP64 sigma, delta and BC (the FP64 oracle).
+// Sources are processed in the given (ascending) order. When levels/sigma are
+// non-NULL, the forward results of the LAST source are written there.
+// plant 1: the dependency drops its "1 +" term. plant 2: sigma counts every
+// edge into an unvisited-or-next-level vertex twice over (sigma doubled on
+// non-source vertices).
+void oracle_compute_bc(int32_t V, const int32_t* indexofNodes, const int32_t* edgeList,
+                       const int32_t* sources, int32_t nsrc, double* BC,
+                       int32_t* levels_out, double* sigma_out, int plant) {
+#pragma omp parallel for
+  for (int t = 0; t < V; t++) BC[t] = 0;
+  double* sigma = new double[V];
+  double* delta = new double[V];
+  int* bfsDist = new int[V];
+  const int nthreads = omp_get_max_threads();
+  for (int si = 0; si < nsrc; si++) {
+    const int src = sources[si];
+#pragma omp parallel for
+    for (int t = 0; t < V; t++) {
+      delta[t] = 0;
+      bfsDist[t] = -1;
+    }
+#pragma omp parallel for
+    for (int t = 0; t < V; t++) sigma[t] = 0;
+    sigma[src] = 1;
+    bfsDist[src] = 0;
+    std::vector<std::vector<int>> levelNodes(1);
+    std::vector<std::vector<int>> levelNodes_later(nthreads);
+    std::vector<int> levelCount;
+    int phase = 0;
+    levelNodes[phase].push_back(src);
+    int bfsCount = 1;
+    levelCount.push_back(bfsCount);
+    while (bfsCount > 0) {
+      int prev_count = bfsCount;
+      bfsCount = 0;
+#pragma omp parallel for
+      for (int l = 0; l < prev_count; l++) {
+        int v = levelNodes[phase][l];
+        for (int edge = indexofNodes[v]; edge < indexofNodes[v + 1]; edge++) {
+          int nbr = edgeList[edge];
+          int dnbr;
+          if (__atomic_load_n(&bfsDist[nbr], __ATOMIC_RELAXED) < 0) {
+            dnbr = __sync_val_compare_and_swap(&bfsDist[nbr], -1, bfsDist[v] + 1);
+            if (dnbr < 0) {
+              int num_thread = omp_get_thread_num();
+              levelNodes_later[num_thread].push_back(nbr);
+            }
+          }
+        }
+        for (int edge = indexofNodes[v]; edge < indexofNodes[v + 1]; edge++) {
+          int w = edgeList[edge];
+          if (__atomic_load_n(&bfsDist[w], __ATOMIC_RELAXED) == bfsDist[v] + 1) {
+            atomic_add_f64(&sigma[w], plant == 2 ? 2 * sigma[v] : sigma[v]);  // fix 3
+          }
+        }
+      }
+      phase = phase + 1;
+      levelNodes.emplace_back();
+      for (int i = 0; i < nthreads; i++) {
+        levelNodes[phase].insert(levelNodes[phase].end(), levelNodes_later[i].begin(), levelNodes_later[i].end());
+        bfsCount = bfsCount + (int)levelNodes_later[i].size();
+        levelNodes_later[i].clear();
+      }
+      levelCount.push_back(bfsCount);
+    }
+    phase = phase - 1;
+    while (phase > 0) {
+#pragma omp parallel for
+      for (int l = 0; l < levelCount[phase]; l++) {
+        int v = levelNodes[phase][l];
+        for (int edge = indexofNodes[v]; edge < indexofNodes[v + 1]; edge++) {
+          int w = edgeList[edge];
+          if (bfsDist[w] == bfsDist[v] + 1) {
+            delta[v] = plant == 1 ? delta[v] + (sigma[v] / sigma[w]) * delta[w]
+                                  : delta[v] + (sigma[v] / sigma[w]) * (1 + delta[w]);
+          }
+        }
+        BC[v] = BC[v] + delta[v];
+      }
+      phase = phase - 1;
+    }
+    if (si == nsrc - 1) {
+      if (levels_out) std::memcpy(levels_out, bfsDist, sizeof(int) * V);
+      if (sigma_out) std::memcpy(sigma_out, sigma, sizeof(double) * V);
+    }
+  }
+  delete[] sigma;
+  delete[] delta;
+  delete[] bfsDist;
+}
+
+// ===========================================================================
+// Textbook routines (independent of the emission rules) used to pin the
+// restatement.
+
+// Dijkstra with a binary heap; INT_MAX for unreachable vertices.
+void tb_dijkstra(int32_t V, const int32_t* off, const int32_t* adj, const int32_t* w,
+                 int32_t src, int32_t* dist) {
+  std::vector<int64_t> d(V, INT64_MAX);
+  using P = std::pair<int64_t, int32_t>;
+  std::priority_queue<P, std::vector<P>, std::greater<P>> pq;
+  d[src] = 0;
+  pq.push({0, src});
+  while (!pq.empty()) {
+    auto [dv, v] = pq.top();
+    pq.pop();
+    if (dv != d[v]) continue;
+    for (int32_t e = off[v]; e < off[v + 1]; ++e) {
+      const int64_t nd = dv + w[e];
+      if (nd < d[adj[e]]) { d[adj[e]] = nd; pq.push({nd, adj[e]}); }
+    }
+  }
+  for (int32_t v = 0; v < V; ++v) dist[v] = d[v] == INT64_MAX ? INT32_MAX : (int32_t)d[v];
+}
+
+// Brandes (2001) with a queue and a stack, FP64, sequential; BC accumulates
+// delta of every vertex except the source (no 1/2 factor, as the DSL).
+void tb_brandes(int32_t V, const int32_t* off, const int32_t* adj, const int32_t* sources,
+                int32_t nsrc, double* BC, int32_t* levels_out, double* sigma_out) {
+  std::fill(BC, BC + V, 0.0);
+  std::vector<int32_t> dist(V), S;
+  std::vector<double> sigma(V), delta(V);
+  S.reserve(V);
+  for (int32_t i = 0; i < nsrc; ++i) {
+    const int32_t s = sources[i];
+    std::fill(dist.begin(), dist.end(), -1);
+    std::fill(sigma.begin(), sigma.end(), 0.0);
+    std::fill(delta.begin(), delta.end(), 0.0);
+    S.clear();
+    dist[s] = 0;
+    sigma[s] = 1;
+    S.push_back(s);
+    for (size_t h = 0; h < S.size(); ++h) {
+      const int32_t v = S[h];
+      for (int32_t e = off[v]; e < off[v + 1]; ++e) {
+        const int32_t w = adj[e];
+        if (dist[w] < 0) { dist[w] = dist[v] + 1; S.push_back(w); }
+        if (dist[w] == dist[v] + 1) sigma[w] += sigma[v];
+      }
+    }
+    for (size_t h = S.size(); h-- > 0;) {
+      const int32_t v = S[h];
+      for (int32_t e = off[v]; e < off[v + 1]; ++e) {
+        const int32_t w = adj[e];
+        if (dist[w] == dist[v] + 1) delta[v] += sigma[v] / sigma[w] * (1.0 + delta[w]);
+      }
+      if (v != s) BC[v] += delta[v];
+    }
+    if (i == nsrc - 1) {
+      if (levels_out) std::copy(dist.begin(), dist.end(), levels_out);
+      if (sigma_out) std::copy(sigma.begin(), sigma.end(), sigma_out);
+    }
+  }
+}
+
+// Brute-force triangle count over the symmetric adjacency (n <= a few 100).
+int64_t tb_tc_brute(int32_t V, const int32_t* off, const int32_t* adj) {
+  std::vector<uint8_t> A(size_t(V) * V, 0);
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t e = off[v]; e < off[v + 1]; ++e) A[size_t(v) * V + adj[e]] = 1;
+  int64_t c = 0;
+  for (int32_t a = 0; a < V; ++a)
+    for (int32_t b = a + 1; b < V; ++b)
+      if (A[size_t(a) * V + b] && A[size_t(b) * V + a])
+        for (int32_t x = b + 1; x < V; ++x)
+          c += A[size_t(a) * V + x] && A[size_t(b) * V + x] && A[size_t(x) * V + a] && A[size_t(x) * V + b];
+  return c;
+}
+
+// Sorted-merge forward triangle count (Schank & Wagner), id-oriented.
+int64_t tb_tc_merge(int32_t V, const int32_t* off, const int32_t* adj) {
+  int64_t c = 0;
+#pragma omp parallel for reduction(+ : c) schedule(dynamic, 256)
+  for (int32_t u = 0; u < V; ++u)
+    for (int32_t e = off[u]; e < off[u + 1]; ++e) {
+      const int32_t v = adj[e];
+      if (v <= u) continue;
+      int32_t i = off[u], j = off[v];
+      while (i < off[u + 1] && j < off[v + 1]) {
+        if (adj[i] <= v) { ++i; continue; }
+        if (adj[j] <= v) { ++j; continue; }
+        if (adj[i] == adj[j]) { ++c; ++i; ++j; }
+        else if (adj[i] < adj[j]) ++i;
+        else ++j;
+      }
+    }
+  return c;
+}
+
+// FP64 power iteration with the same update and stopping rule as Compute_PR.
+void tb_pr_fp64(int32_t V, const int32_t* off, const int32_t* roff, const int32_t* src,
+                double beta, double delta, int32_t maxIter, double* pr, int32_t* iterOut) {
+  std::vector<double> nxt(V);
+  for (int32_t v = 0; v < V; ++v) pr[v] = 1.0 / V;
+  int32_t it = 0;
+  double diff;
+  do {
+    diff = 0;
+    for (int32_t v = 0; v < V; ++v) {
+      double s = 0;
+      for (int32_t e = roff[v]; e < roff[v + 1]; ++e) s += pr[src[e]] / double(off[src[e] + 1] - off[src[e]]);
+      nxt[v] = (1 - delta) / V + delta * s;
+      diff += std::abs(nxt[v] - pr[v]);
+    }
+    std::copy(nxt.begin(), nxt.end(), pr);
+    ++it;
+  } while (diff > beta && it < maxIter);
+  if (iterOut) *iterOut = it;
+}
+
+int oracle_num_threads(void) { return omp_get_max_threads(); }
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of liboracle.so, the CPU
+restatement of StarPlat's generated OpenMP code (oracle.cpp) and the textbook
+routines that pin it. Imported by the test suite, the smoke check and the CPU
+baseline leg of the benchmark; never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+i32, i64, f32, f64 = C.c_int32, C.c_int64, C.c_float, C.c_double
+P_i32, P_f32, P_f64 = C.POINTER(i32), C.POINTER(f32), C.POINTER(f64)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run `make -C {_HERE}`")
+        L = C.CDLL(LIB)
+        L.oracle_compute_pr.argtypes = [i32, P_i32, P_i32, P_i32, P_i32, f32, f32, i32, P_f32, P_i32, C.c_int]
+        L.oracle_compute_sssp.argtypes = [i32, P_i32, P_i32, P_i32, P_i32, i32, C.c_int]
+        L.oracle_compute_tc.argtypes = [i32, P_i32, P_i32, C.c_int, C.c_int]
+        L.oracle_compute_tc.restype = i64
+        L.oracle_compute_bc.argtypes = [i32, P_i32, P_i32, P_i32, i32, P_f64, P_i32, P_f64, C.c_int]
+        L.tb_dijkstra.argtypes = [i32, P_i32, P_i32, P_i32, i32, P_i32]
+        L.tb_brandes.argtypes = [i32, P_i32, P_i32, P_i32, i32, P_f64, P_i32, P_f64]
+        L.tb_tc_brute.argtypes = [i32, P_i32, P_i32]
+        L.tb_tc_brute.restype = i64
+        L.tb_tc_merge.argtypes = [i32, P_i32, P_i32]
+        L.tb_tc_merge.restype = i64
+        L.tb_pr_fp64.argtypes = [i32, P_i32, P_i32, P_i32, f64, f64, i32, P_f64, P_i32]
+        L.oracle_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def num_threads():
+    return lib().oracle_num_threads()
+
+
+def compute_pr(g, beta=1e-10, delta=0.85, maxIter=100, plant=0):
+    pr = np.empty(g.V, np.float32)
+    it = i32(0)
+    lib().oracle_compute_pr(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), _p(g.rev_indexofNodes, i32),
+                            _p(g.srcList, i32), beta, delta, maxIter, _p(pr, f32), C.byref(it), plant)
+    return pr, it.value
+
+
+def compute_sssp(g, weight, src, plant=0):
+    d = np.empty(g.V, np.int32)
+    w = np.ascontiguousarray(weight, np.int32)
+    lib().oracle_compute_sssp(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), _p(d, i32), _p(w, i32), src, plant)
+    return d
+
+
+def compute_tc(g, literal=False, plant=0):
+    return int(lib().oracle_compute_tc(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), int(literal), plant))
+
+
+def compute_bc(g, sources, plant=0):
+    """FP64 BC plus the bfsDist / sigma of the last source."""
+    s = np.ascontiguousarray(sorted(set(int(x) for x in sources)), np.int32)
+    bc = np.empty(g.V, np.float64)
+    lev = np.empty(g.V, np.int32)
+    sig = np.empty(g.V, np.float64)
+    lib().oracle_compute_bc(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), _p(s, i32), len(s),
+                            _p(bc, f64), _p(lev, i32), _p(sig, f64), plant)
+    return bc, lev, sig
+
+
+def dijkstra(g, weight, src):
+    d = np.empty(g.V, np.int32)
+    w = np.ascontiguousarray(weight, np.int32)
+    lib().tb_dijkstra(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), _p(w, i32), src, _p(d, i32))
+    return d
+
+
+def brandes(g, sources):
+    s = np.ascontiguousarray(sorted(set(int(x) for x in sources)), np.int32)
+    bc = np.empty(g.V, np.float64)
+    lev = np.empty(g.V, np.int32)
+    sig = np.empty(g.V, np.float64)
+    lib().tb_brandes(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), _p(s, i32), len(s), _p(bc, f64),
+                     _p(lev, i32), _p(sig, f64))
+    return bc, lev, sig
+
+
+def tc_brute(g):
+    return int(lib().tb_tc_brute(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32)))
+
+
+def tc_merge(g):
+    return int(lib().tb_tc_merge(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32)))
+
+
+def pr_fp64(g, beta=1e-10, delta=0.85, maxIter=100):
+    pr = np.empty(g.V, np.float64)
+    it = i32(0)
+    lib().tb_pr_fp64(g.V, _p(g.indexofNodes, i32), _p(g.rev_indexofNodes, i32), _p(g.srcList, i32), beta, delta,
+                     maxIter, _p(pr, f64), C.byref(it))
+    return pr, it.value

@@ -1,0 +1,50 @@
+"""The four generated entry points, with the reference's names and argument
+meaning, running on the gfx950 kernel library through the C ABI.
+
+    Compute_PR(g, beta, delta, maxIter, pageRank)   PageRankDSLV2 / V3 diff
+    Compute_SSSP(g, dist, weight, src)              sssp_dslV2
+    Compute_TC(g) -> int                            triangle_counting_dsl
+    Compute_BC(g, BC, sourceSet)                    bc_dslV2
+
+Output arrays are caller-owned numpy arrays of num_nodes() entries, filled in
+place, as the generated C++ fills the caller's pointers
+(backend_omp/dsl_cpp_generator.cpp:3056-3151). A failing call raises
+``SpmError``; nothing falls back to the CPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _out(a, dtype, n, name):
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or a.size != n or not a.flags.c_contiguous:
+        raise TypeError(f"{name} must be a contiguous numpy {np.dtype(dtype).name} array of num_nodes() entries")
+    return a
+
+
+def Compute_PR(g, beta, delta, maxIter, pageRank):
+    """Pull PageRank; returns the number of iterations run."""
+    _out(pageRank, np.float32, g.num_nodes(), "pageRank")
+    _, it = g.device().pr(float(beta), float(delta), int(maxIter), out=pageRank)
+    return it
+
+
+def Compute_SSSP(g, dist, weight, src):
+    """Bellman-Ford fixed point; dist[v] = INT_MAX when unreachable."""
+    _out(dist, np.int32, g.num_nodes(), "dist")
+    w = np.ascontiguousarray(weight, np.int32)
+    if w.size != g.num_edges():
+        raise ValueError("weight needs one entry per CSR edge (indexed by edge position)")
+    g.device().sssp(int(src), weight=w, out=dist)
+
+
+def Compute_TC(g):
+    """Number of (u < v < w) wedges closed by an edge, as the DSL sums them."""
+    return int(g.device().tc())
+
+
+def Compute_BC(g, BC, sourceSet):
+    """Brandes BC accumulated over sourceSet (no 1/2 factor, as the DSL)."""
+    _out(BC, np.float32, g.num_nodes(), "BC")
+    src = np.array(sorted(set(int(s) for s in sourceSet)), np.int32)
+    g.device().bc(src, out=BC)

@@ -1,0 +1,406 @@
+// spm_bc.hip -- batched Brandes betweenness centrality for gfx950, the
+// replacement of the code StarPlat generates for Compute_BC
+// (graphcode/staticDSLCodes/bc_dslV2:1-21):
+//   for src in sourceSet: delta = 0; sigma = 0; sigma[src] = 1;
+//     iterateInBFS(v from src): for w in nbrs(v), bfsDist[w]==bfsDist[v]+1:
+//       sigma[w] += sigma[v]
+//     iterateInReverse(v != src): for w in nbrs(v) on the BFS DAG:
+//       delta[v] += sigma[v]/sigma[w] * (1 + delta[w]);  BC[v] += delta[v]
+// (OMP emission: backend_omp/dsl_cpp_generator.cpp:66-193; CUDA kernels with
+//  a host round trip per level: backend_cuda/...cpp:275-570).
+//
+// Design. Sources are processed 64 at a time, one per lane of a wavefront:
+// sigma is FP64 and delta FP32 in vertex-major rows of 64 entries (512 B and
+// 256 B), so the lanes of a wavefront working on one vertex touch one
+// contiguous row. BFS levels are kept as 64-bit masks per vertex and level
+// (bit s set = vertex at this level for source s), plus a visited mask.
+// Each level runs:
+//   scan   one lane per vertex: next = OR of the in-neighbours' current
+//          masks, minus visited (bottom-up, no atomics); appends the vertex
+//          to the level's list when next != 0;
+//   sigma  one wavefront per listed vertex: for every in-neighbour whose
+//          current mask meets next, lanes of those sources add the
+//          neighbour's sigma (integers below 2^53, so exact in any order).
+// The reverse pass walks the level lists backwards, one wavefront per vertex:
+// lane s sums sigma_v/sigma_w*(1+delta_w) over out-neighbours w at the next
+// level for s, in ascending neighbour order as the reference does, and the
+// 64 per-source dependencies are reduced into BC[v] (pairwise over the batch,
+// then once per batch).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "spm_internal.hpp"
+
+namespace spm {
+
+constexpr int kBcBlock = 256;
+constexpr int kBcWaves = kBcBlock / kWave;
+constexpr int kBatch = 64;
+
+struct BcWork {
+  double* sigma = nullptr;          // [V][64]
+  float* delta = nullptr;           // [V][64]
+  unsigned long long* visited = nullptr;  // [V]
+  unsigned long long* masks = nullptr;    // [mask_levels][V]
+  int32_t* lists = nullptr;         // level lists, concatenated
+  unsigned int* cnt = nullptr;      // [1] list append cursor
+  int32_t* srcbuf = nullptr;        // [64]
+  size_t mask_levels = 0;
+  size_t list_cap = 0;
+};
+
+// Clears the visited and level-0 masks; the sources' bits are set by bc_seed,
+// a separate launch, so no workgroup can clear a bit another one has set.
+__global__ void bc_init(int32_t V, unsigned long long* __restrict__ visited,
+                        unsigned long long* __restrict__ mask0, const int32_t* __restrict__ src, int nb,
+                        int32_t* __restrict__ list0, unsigned int* cnt) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
+    visited[v] = 0ull;
+    mask0[v] = 0ull;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *cnt = unsigned(nb);
+  if (blockIdx.x == 0 && threadIdx.x < nb) list0[threadIdx.x] = src[threadIdx.x];
+}
+
+__global__ void bc_seed(const int32_t* __restrict__ src, int nb, unsigned long long* __restrict__ visited,
+                        unsigned long long* __restrict__ mask0, double* __restrict__ sigma) {
+  const int s = threadIdx.x;
+  if (s < nb) {
+    const int32_t v = src[s];
+    atomicOr(&visited[v], 1ull << s);
+    atomicOr(&mask0[v], 1ull << s);
+    sigma[size_t(v) * kBatch + s] = 1.0;
+  }
+}
+
+__global__ __launch_bounds__(kBcBlock) void bc_scan(
+    int32_t V, const int32_t* __restrict__ roff, const int32_t* __restrict__ radj,
+    const unsigned long long* __restrict__ cur, unsigned long long* __restrict__ nxt,
+    unsigned long long* __restrict__ visited, unsigned long long batchmask,
+    int32_t* __restrict__ list, unsigned int* __restrict__ cnt) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int lane = __lane_id();
+  // Rounded so whole wavefronts take part in the ballot-aggregated append.
+  const int64_t vend = (int64_t(V) + kWave - 1) / kWave * kWave;
+  for (int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < vend; w += stride) {
+    unsigned long long nm = 0;
+    if (w < V) {
+      const unsigned long long vis = visited[w];
+      const unsigned long long unv = batchmask & ~vis;
+      if (unv) {
+        for (int32_t i = roff[w]; i < roff[w + 1]; ++i) nm |= cur[radj[i]];
+        nm &= unv;
+        if (nm) visited[w] = vis | nm;
+      }
+      nxt[w] = nm;
+    }
+    const unsigned long long b = __ballot(nm != 0ull);
+    if (b) {
+      unsigned int base = 0;
+      const int leader = __ffsll((long long)b) - 1;
+      if (lane == leader) base = atomicAdd(cnt, unsigned(__popcll(b)));
+      base = __shfl(base, leader, kWave);
+      if (nm) list[base + __popcll(b & ((1ull << lane) - 1ull))] = int32_t(w);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBcBlock) void bc_sigma(
+    const int32_t* __restrict__ list, unsigned int n, const int32_t* __restrict__ roff,
+    const int32_t* __restrict__ radj, const unsigned long long* __restrict__ cur,
+    const unsigned long long* __restrict__ nxt, double* __restrict__ sigma) {
+  const int lane = __lane_id();
+  const int64_t nw = int64_t(gridDim.x) * kBcWaves;
+  for (int64_t i = int64_t(blockIdx.x) * kBcWaves + threadIdx.x / kWave; i < n; i += nw) {
+    const int32_t w = list[i];
+    const unsigned long long nm = nxt[w];
+    double acc = 0.0;
+    const int32_t b = roff[w], e = roff[w + 1];
+    for (int32_t i0 = b; i0 < e; i0 += kWave) {
+      const int32_t j = i0 + lane;
+      int32_t v = 0;
+      unsigned long long m = 0;
+      if (j < e) {
+        v = radj[j];
+        m = cur[v] & nm;
+      }
+      unsigned long long act = __ballot(m != 0ull);
+      while (act) {
+        const int src_lane = __ffsll((long long)act) - 1;
+        act &= act - 1;
+        const int32_t vv = __shfl(v, src_lane, kWave);
+        const unsigned long long mm = __shfl(m, src_lane, kWave);
+        if ((mm >> lane) & 1ull) acc += sigma[size_t(vv) * kBatch + lane];
+      }
+    }
+    if ((nm >> lane) & 1ull) sigma[size_t(w) * kBatch + lane] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kBcBlock) void bc_back(
+    const int32_t* __restrict__ list, unsigned int n, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ adj, const unsigned long long* __restrict__ md,
+    const unsigned long long* __restrict__ md1, const double* __restrict__ sigma,
+    float* __restrict__ delta, float* __restrict__ BC) {
+  const int lane = __lane_id();
+  const int64_t nw = int64_t(gridDim.x) * kBcWaves;
+  for (int64_t i = int64_t(blockIdx.x) * kBcWaves + threadIdx.x / kWave; i < n; i += nw) {
+    const int32_t v = list[i];
+    const unsigned long long mv = md[v];
+    const bool mine = (mv >> lane) & 1ull;
+    const double sv = mine ? sigma[size_t(v) * kBatch + lane] : 1.0;
+    float acc = 0.f;
+    const int32_t b = off[v], e = off[v + 1];
+    for (int32_t i0 = b; i0 < e; i0 += kWave) {
+      const int32_t j = i0 + lane;
+      int32_t w = 0;
+      unsigned long long m = 0;
+      if (j < e) {
+        w = adj[j];
+        m = md1[w] & mv;
+      }
+      unsigned long long act = __ballot(m != 0ull);
+      while (act) {
+        const int src_lane = __ffsll((long long)act) - 1;
+        act &= act - 1;
+        const int32_t ww = __shfl(w, src_lane, kWave);
+        const unsigned long long mm = __shfl(m, src_lane, kWave);
+        if ((mm >> lane) & 1ull) {
+          const size_t k = size_t(ww) * kBatch + lane;
+          acc += float(sv / sigma[k]) * (1.0f + delta[k]);
+        }
+      }
+    }
+    float part = 0.f;
+    if (mine) {
+      delta[size_t(v) * kBatch + lane] = acc;
+      part = acc;
+    }
+    part = wave_sum(part);
+    if (lane == 0 && mv) BC[v] += part;
+  }
+}
+
+__global__ void bc_extract(int32_t V, const unsigned long long* __restrict__ masks, int levels,
+                           const double* __restrict__ sigma, int32_t* __restrict__ lev,
+                           double* __restrict__ sig) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
+    int32_t l = -1;
+    for (int d = 0; d < levels && l < 0; ++d)
+      if (masks[size_t(d) * V + v] & 1ull) l = d;
+    lev[v] = l;
+    sig[v] = l >= 0 ? sigma[size_t(v) * kBatch] : 0.0;
+  }
+}
+
+__global__ void bc_zero(int32_t V, float* __restrict__ BC) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) BC[v] = 0.f;
+}
+
+static int bc_alloc(spm_graph* g) {
+  if (!g->roff) return set_error(SPM_ENOGRAPH, "BC needs the reverse CSR (rev_indexofNodes, srcList)");
+  if (g->bc) return SPM_OK;
+  BcWork* w = new BcWork();
+  g->bc = w;
+  const size_t V = size_t(g->V);
+  SPM_TRY(dmalloc(g, (void**)&w->sigma, sizeof(double) * V * kBatch, "bc sigma"));
+  SPM_TRY(dmalloc(g, (void**)&w->delta, sizeof(float) * V * kBatch, "bc delta"));
+  SPM_TRY(dmalloc(g, (void**)&w->visited, sizeof(unsigned long long) * V, "bc visited"));
+  SPM_TRY(dmalloc(g, (void**)&w->cnt, sizeof(unsigned int), "bc counter"));
+  SPM_TRY(dmalloc(g, (void**)&w->srcbuf, sizeof(int32_t) * kBatch, "bc sources"));
+  return SPM_OK;
+}
+
+// Makes room for `levels` mask levels and `entries` list entries (grow only;
+// contents are kept).
+static int bc_reserve(spm_graph* g, size_t levels, size_t entries) {
+  BcWork* w = g->bc;
+  const size_t V = size_t(g->V);
+  if (levels > w->mask_levels) {
+    const size_t nl = std::max(levels, w->mask_levels * 2);
+    unsigned long long* p = nullptr;
+    SPM_TRY(dmalloc(g, (void**)&p, sizeof(unsigned long long) * V * nl, "bc level masks"));
+    if (w->masks) {
+      SPM_CHECK_HIP(hipMemcpyAsync(p, w->masks, sizeof(unsigned long long) * V * w->mask_levels, hipMemcpyDeviceToDevice, g->stream));
+      SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+      dfree(g, w->masks, sizeof(unsigned long long) * V * w->mask_levels);
+    }
+    w->masks = p;
+    w->mask_levels = nl;
+  }
+  if (entries > w->list_cap) {
+    const size_t nc = std::max(entries, w->list_cap * 2);
+    int32_t* p = nullptr;
+    SPM_TRY(dmalloc(g, (void**)&p, sizeof(int32_t) * nc, "bc level lists"));
+    if (w->lists) {
+      SPM_CHECK_HIP(hipMemcpyAsync(p, w->lists, sizeof(int32_t) * w->list_cap, hipMemcpyDeviceToDevice, g->stream));
+      SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+      dfree(g, w->lists, sizeof(int32_t) * w->list_cap);
+    }
+    w->lists = p;
+    w->list_cap = nc;
+  }
+  return SPM_OK;
+}
+
+int bc_release(spm_graph* g) {
+  BcWork* w = g->bc;
+  if (!w) return SPM_OK;
+  const size_t V = size_t(g->V);
+  dfree(g, w->sigma, sizeof(double) * V * kBatch);
+  dfree(g, w->delta, sizeof(float) * V * kBatch);
+  dfree(g, w->visited, sizeof(unsigned long long) * V);
+  dfree(g, w->masks, sizeof(unsigned long long) * V * w->mask_levels);
+  dfree(g, w->lists, sizeof(int32_t) * w->list_cap);
+  dfree(g, w->cnt, sizeof(unsigned int));
+  dfree(g, w->srcbuf, sizeof(int32_t) * kBatch);
+  delete w;
+  g->bc = nullptr;
+  return SPM_OK;
+}
+
+// Forward pass of one batch; returns the deepest non-empty level and the list
+// start of every level (lstart[d] .. lstart[d+1]).
+static int bc_forward(spm_graph* g, const int32_t* h_src, int nb, int* levels_out,
+                      std::vector<unsigned int>& lstart) {
+  BcWork* w = g->bc;
+  const int32_t V = g->V;
+  const size_t Vs = size_t(V);
+  SPM_TRY(bc_reserve(g, 2, Vs + kBatch));
+  SPM_CHECK_HIP(hipMemcpyAsync(w->srcbuf, h_src, sizeof(int32_t) * nb, hipMemcpyHostToDevice, g->stream));
+  const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
+  SPM_TRY(launch(g, "bc_init", bc_init, dim3(igrid), dim3(256), 0, V, w->visited, w->masks,
+                 (const int32_t*)w->srcbuf, nb, w->lists, w->cnt));
+  SPM_TRY(launch(g, "bc_seed", bc_seed, dim3(1), dim3(kBatch), 0, (const int32_t*)w->srcbuf, nb,
+                 w->visited, w->masks, w->sigma));
+  const unsigned long long batchmask = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
+  lstart.assign(1, 0u);
+  lstart.push_back(unsigned(nb));
+  unsigned int* h = reinterpret_cast<unsigned int*>(g->h_pinned);
+  const int sgrid = int(std::min<int64_t>((V + kBcBlock - 1) / kBcBlock, int64_t(g->num_cus) * 16));
+  const int wgrid = g->num_cus * 8;
+  int d = 0;
+  for (;; ++d) {
+    const unsigned int used = lstart.back();
+    SPM_TRY(bc_reserve(g, size_t(d) + 2, size_t(used) + Vs));
+    unsigned long long* cur = w->masks + size_t(d) * Vs;
+    unsigned long long* nxt = w->masks + size_t(d + 1) * Vs;
+    SPM_TRY(launch(g, "bc_scan", bc_scan, dim3(sgrid), dim3(kBcBlock), 0, V, (const int32_t*)g->roff,
+                   (const int32_t*)g->radj, (const unsigned long long*)cur, nxt, w->visited, batchmask,
+                   w->lists, w->cnt));
+    SPM_CHECK_HIP(hipMemcpyAsync(h, w->cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    const unsigned int total = h[0];
+    if (total == used) break;  // no vertex reached at level d+1
+    lstart.push_back(total);
+    SPM_TRY(launch(g, "bc_sigma", bc_sigma, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + used),
+                   total - used, (const int32_t*)g->roff, (const int32_t*)g->radj,
+                   (const unsigned long long*)cur, (const unsigned long long*)nxt, w->sigma));
+  }
+  *levels_out = d;  // levels 0..d are non-empty; masks[d+1] is all zero
+  return SPM_OK;
+}
+
+static int bc_backward(spm_graph* g, int L, const std::vector<unsigned int>& lstart, float* d_BC) {
+  BcWork* w = g->bc;
+  const size_t Vs = size_t(g->V);
+  const int wgrid = g->num_cus * 8;
+  for (int d = L; d >= 1; --d) {
+    const unsigned int b = lstart[d], e = lstart[d + 1];
+    if (e == b) continue;
+    SPM_TRY(launch(g, "bc_back", bc_back, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + b), e - b,
+                   (const int32_t*)g->off, (const int32_t*)g->adj,
+                   (const unsigned long long*)(w->masks + size_t(d) * Vs),
+                   (const unsigned long long*)(w->masks + size_t(d + 1) * Vs), (const double*)w->sigma,
+                   w->delta, d_BC));
+  }
+  return SPM_OK;
+}
+
+static int bc_check_sources(const spm_graph* g, const int32_t* sources, int32_t nsrc, std::vector<int32_t>& s) {
+  if (nsrc < 0 || (nsrc > 0 && !sources)) return set_error(SPM_EINVAL, "BC: bad source list");
+  s.assign(sources, sources + nsrc);
+  std::sort(s.begin(), s.end());
+  for (size_t i = 0; i < s.size(); ++i) {
+    if (s[i] < 0 || s[i] >= g->V) return set_error(SPM_EINVAL, "BC: source out of range");
+    if (i && s[i] == s[i - 1]) return set_error(SPM_EINVAL, "BC: duplicate source (sourceSet is a set)");
+  }
+  return SPM_OK;
+}
+
+int bc_run(spm_graph* g, const int32_t* sources, int32_t nsrc, float* d_BC) {
+  std::vector<int32_t> s;
+  SPM_TRY(bc_check_sources(g, sources, nsrc, s));
+  SPM_TRY(bc_alloc(g));
+  const int igrid = int(std::min<int64_t>((g->V + 255) / 256, 4096));
+  SPM_TRY(launch(g, "bc_zero", bc_zero, dim3(igrid), dim3(256), 0, g->V, d_BC));
+  std::vector<unsigned int> lstart;
+  for (size_t b = 0; b < s.size(); b += kBatch) {
+    const int nb = int(std::min<size_t>(kBatch, s.size() - b));
+    int L = 0;
+    SPM_TRY(bc_forward(g, s.data() + b, nb, &L, lstart));
+    SPM_TRY(bc_backward(g, L, lstart, d_BC));
+  }
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  SPM_TRY(prof_flush(g));
+  return SPM_OK;
+}
+
+}  // namespace spm
+
+using namespace spm;
+
+extern "C" {
+
+int spm_bc_device(spm_graph* g, const int32_t* sources, int32_t nsrc, float* d_BC) {
+  if (!g || !d_BC) return set_error(SPM_EINVAL, "spm_bc_device: NULL argument");
+  return bc_run(g, sources, nsrc, d_BC);
+}
+
+int spm_compute_bc(spm_graph* g, float* BC, const int32_t* sources, int32_t nsrc) {
+  if (!g || !BC) return set_error(SPM_EINVAL, "spm_compute_bc: NULL argument");
+  float* d = nullptr;
+  SPM_TRY(dmalloc(g, (void**)&d, sizeof(float) * g->V, "bc output"));
+  int rc = bc_run(g, sources, nsrc, d);
+  if (rc == SPM_OK) {
+    hipError_t e = hipMemcpyAsync(BC, d, sizeof(float) * g->V, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) rc = set_error(SPM_EHIP, std::string("spm_compute_bc copy: ") + hipGetErrorString(e));
+  }
+  dfree(g, d, sizeof(float) * g->V);
+  return rc;
+}
+
+int spm_bc_paths(spm_graph* g, int32_t src, int32_t* bfsDist, double* sigma) {
+  if (!g || !bfsDist || !sigma) return set_error(SPM_EINVAL, "spm_bc_paths: NULL argument");
+  if (src < 0 || src >= g->V) return set_error(SPM_EINVAL, "spm_bc_paths: source out of range");
+  SPM_TRY(bc_alloc(g));
+  std::vector<unsigned int> lstart;
+  int L = 0;
+  SPM_TRY(bc_forward(g, &src, 1, &L, lstart));
+  int32_t* dl = nullptr;
+  double* ds = nullptr;
+  SPM_TRY(dmalloc(g, (void**)&dl, sizeof(int32_t) * g->V, "bc paths levels"));
+  int rc = dmalloc(g, (void**)&ds, sizeof(double) * g->V, "bc paths sigma");
+  if (rc == SPM_OK) {
+    const int igrid = int(std::min<int64_t>((g->V + 255) / 256, 4096));
+    rc = launch(g, "bc_extract", bc_extract, dim3(igrid), dim3(256), 0, g->V, (const unsigned long long*)g->bc->masks,
+                L + 1, (const double*)g->bc->sigma, dl, ds);
+  }
+  if (rc == SPM_OK) {
+    hipError_t e = hipMemcpyAsync(bfsDist, dl, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sigma, ds, sizeof(double) * g->V, hipMemcpyDeviceToHost, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) rc = set_error(SPM_EHIP, std::string("spm_bc_paths copy: ") + hipGetErrorString(e));
+  }
+  dfree(g, dl, sizeof(int32_t) * g->V);
+  dfree(g, ds, sizeof(double) * g->V);
+  if (rc == SPM_OK) rc = prof_flush(g);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// spm_graph.hip -- device CSR mirror of graphcode/graph.hpp (:44-57) and the
+// handle plumbing (errors, allocation accounting, profiling, schedules).
+#include <cstdio>
+#include <cstring>
+
+#include "spm_internal.hpp"
+
+namespace spm {
+
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int dmalloc(spm_graph* g, void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 4;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "hipMalloc(%s, %zu bytes): %s", what, bytes, hipGetErrorString(e));
+    return set_error(SPM_ENOMEM, buf);
+  }
+  g->bytes += (int64_t)bytes;
+  return SPM_OK;
+}
+
+void dfree(spm_graph* g, void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  g->bytes -= (int64_t)(bytes ? bytes : 4);
+}
+
+static hipEvent_t pool_get(spm_graph* g) {
+  if (!g->prof.pool.empty()) {
+    hipEvent_t e = g->prof.pool.back();
+    g->prof.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+void prof_begin(spm_graph* g, const char*, hipEvent_t* a, hipEvent_t* b) {
+  *a = pool_get(g);
+  *b = pool_get(g);
+  (void)hipEventRecord(*a, g->stream);
+}
+
+void prof_end(spm_graph* g, const char* name, hipEvent_t a, hipEvent_t b) {
+  (void)hipEventRecord(b, g->stream);
+  g->prof.pending.push_back({name, a, b});
+}
+
+int prof_flush(spm_graph* g) {
+  if (g->prof.pending.empty()) return SPM_OK;
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  for (auto& p : g->prof.pending) {
+    float ms = 0.f;
+    SPM_CHECK_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& t = g->prof.totals[p.name];
+    t.first += ms;
+    t.second += 1;
+    g->prof.pool.push_back(p.a);
+    g->prof.pool.push_back(p.b);
+  }
+  g->prof.pending.clear();
+  return SPM_OK;
+}
+
+}  // namespace spm
+
+using namespace spm;
+
+namespace {
+
+// The fixed heuristic schedule. Degree thresholds follow the wavefront width:
+// rows shorter than a few elements are summed by one lane, rows up to a
+// wavefront's worth by a lane group, rows up to a few thousand by a
+// wavefront, and hubs by a whole workgroup.
+void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
+  std::memset(s, 0, sizeof *s);
+  s->pr_t1 = 4;
+  s->pr_t2 = 48;
+  s->pr_t3 = 2048;
+  s->pr_group = 8;
+  s->sssp_mode = 0;
+  s->sssp_rounds = 12;
+  s->sssp_grid = 0;
+  s->tc_mode = 0;
+  (void)g;
+}
+
+int validate_schedule(const spm_schedule* s) {
+  if (s->pr_t1 < 0 || s->pr_t2 < s->pr_t1 || s->pr_t3 < s->pr_t2) return set_error(SPM_EINVAL, "schedule: PR thresholds must be non-decreasing");
+  if (s->pr_group != 2 && s->pr_group != 4 && s->pr_group != 8 && s->pr_group != 16 && s->pr_group != 32)
+    return set_error(SPM_EINVAL, "schedule: pr_group must be 2, 4, 8, 16 or 32");
+  if (s->sssp_mode < 0 || s->sssp_mode > 1) return set_error(SPM_EINVAL, "schedule: sssp_mode must be 0 or 1");
+  if (s->sssp_rounds < 6 || s->sssp_rounds > 96 || s->sssp_rounds % 6) return set_error(SPM_EINVAL, "schedule: sssp_rounds must be a multiple of 6 in [6, 96]");
+  if (s->sssp_grid < 0 || s->sssp_grid > 65536) return set_error(SPM_EINVAL, "schedule: sssp_grid out of range");
+  if (s->tc_mode < 0 || s->tc_mode > 1) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0 or 1");
+  return SPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spm_last_error(void) { return spm::g_last_error.c_str(); }
+int spm_version(void) { return 100; }
+
+int spm_graph_create(int32_t V, int64_t E, const int32_t* off, const int32_t* adj,
+                     const int32_t* w, const int32_t* roff, const int32_t* radj,
+                     spm_graph** out) {
+  if (!out) return set_error(SPM_EINVAL, "spm_graph_create: out is NULL");
+  *out = nullptr;
+  if (V < 1 || E < 0 || E > 0x7fffffffLL) return set_error(SPM_EINVAL, "spm_graph_create: need V >= 1 and 0 <= E < 2^31");
+  if (!off || (E > 0 && !adj)) return set_error(SPM_EINVAL, "spm_graph_create: indexofNodes/edgeList missing");
+  if ((roff == nullptr) != (radj == nullptr)) return set_error(SPM_EINVAL, "spm_graph_create: rev_indexofNodes and srcList go together");
+  if (off[0] != 0 || off[V] != E) return set_error(SPM_EINVAL, "spm_graph_create: indexofNodes[0] must be 0 and indexofNodes[V] == E");
+  if (roff && (roff[0] != 0 || roff[V] != E)) return set_error(SPM_EINVAL, "spm_graph_create: rev_indexofNodes[0] must be 0 and rev_indexofNodes[V] == E");
+  int dev = 0;
+  SPM_CHECK_HIP(hipGetDevice(&dev));
+  spm_graph* g = new spm_graph();
+  g->device = dev;
+  g->V = V;
+  g->E = E;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess) g->num_cus = prop.multiProcessorCount;
+  hipError_t he = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) { delete g; return set_error(SPM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(he)); }
+  he = hipHostMalloc((void**)&g->h_pinned, 64, hipHostMallocDefault);
+  if (he != hipSuccess) { (void)hipStreamDestroy(g->stream); delete g; return set_error(SPM_EHIP, "hipHostMalloc failed"); }
+  int rc = SPM_OK;
+  const size_t vb = sizeof(int32_t) * (size_t(V) + 1), eb = sizeof(int32_t) * size_t(E);
+  if (rc == SPM_OK) rc = dmalloc(g, (void**)&g->off, vb, "indexofNodes");
+  if (rc == SPM_OK) rc = dmalloc(g, (void**)&g->adj, eb, "edgeList");
+  if (rc == SPM_OK && w) rc = dmalloc(g, (void**)&g->w, eb, "edgeLen");
+  if (rc == SPM_OK && roff) rc = dmalloc(g, (void**)&g->roff, vb, "rev_indexofNodes");
+  if (rc == SPM_OK && roff) rc = dmalloc(g, (void**)&g->radj, eb, "srcList");
+  if (rc == SPM_OK) {
+    hipError_t e = hipMemcpyAsync(g->off, off, vb, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess && E) e = hipMemcpyAsync(g->adj, adj, eb, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess && w && E) e = hipMemcpyAsync(g->w, w, eb, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess && roff) e = hipMemcpyAsync(g->roff, roff, vb, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess && roff && E) e = hipMemcpyAsync(g->radj, radj, eb, hipMemcpyHostToDevice, g->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+    if (e != hipSuccess) rc = set_error(SPM_EHIP, std::string("spm_graph_create upload: ") + hipGetErrorString(e));
+  }
+  if (rc != SPM_OK) {
+    spm_graph_destroy(g);
+    return rc;
+  }
+  heuristic_schedule(g, &g->sched);
+  *out = g;
+  return SPM_OK;
+}
+
+int spm_graph_destroy(spm_graph* g) {
+  if (!g) return SPM_OK;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  pr_release(g);
+  sssp_release(g);
+  tc_release(g);
+  bc_release(g);
+  const size_t vb = sizeof(int32_t) * (size_t(g->V) + 1), eb = sizeof(int32_t) * size_t(g->E);
+  dfree(g, g->off, vb);
+  dfree(g, g->adj, eb);
+  dfree(g, g->w, eb);
+  dfree(g, g->roff, vb);
+  dfree(g, g->radj, eb);
+  for (auto& p : g->prof.pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : g->prof.pool) (void)hipEventDestroy(e);
+  if (g->h_pinned) (void)hipHostFree(g->h_pinned);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return SPM_OK;
+}
+
+int spm_graph_set_weights(spm_graph* g, const int32_t* weight) {
+  if (!g || !weight) return set_error(SPM_EINVAL, "spm_graph_set_weights: NULL argument");
+  const size_t eb = sizeof(int32_t) * size_t(g->E);
+  if (!g->w) SPM_TRY(dmalloc(g, (void**)&g->w, eb, "edgeLen"));
+  if (g->E) SPM_CHECK_HIP(hipMemcpyAsync(g->w, weight, eb, hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  return SPM_OK;
+}
+
+void* spm_graph_stream(spm_graph* g) { return g ? (void*)g->stream : nullptr; }
+
+int spm_graph_sync(spm_graph* g) {
+  if (!g) return set_error(SPM_EINVAL, "spm_graph_sync: NULL handle");
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  return SPM_OK;
+}
+
+int64_t spm_graph_device_bytes(spm_graph* g) { return g ? g->bytes : -1; }
+
+int spm_get_schedule(spm_graph* g, spm_schedule* s) {
+  if (!g || !s) return set_error(SPM_EINVAL, "spm_get_schedule: NULL argument");
+  *s = g->sched;
+  return SPM_OK;
+}
+
+int spm_set_schedule(spm_graph* g, const spm_schedule* s) {
+  if (!g || !s) return set_error(SPM_EINVAL, "spm_set_schedule: NULL argument");
+  SPM_TRY(validate_schedule(s));
+  const bool pr_changed = s->pr_t1 != g->sched.pr_t1 || s->pr_t2 != g->sched.pr_t2 ||
+                          s->pr_t3 != g->sched.pr_t3 || s->pr_group != g->sched.pr_group;
+  const bool sssp_changed = s->sssp_rounds != g->sched.sssp_rounds || s->sssp_grid != g->sched.sssp_grid ||
+                            s->sssp_mode != g->sched.sssp_mode;
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  if (pr_changed) pr_release(g);
+  if (sssp_changed) sssp_release(g);
+  g->sched = *s;
+  return SPM_OK;
+}
+
+int spm_default_schedule(spm_graph* g, spm_schedule* s) {
+  if (!g || !s) return set_error(SPM_EINVAL, "spm_default_schedule: NULL argument");
+  heuristic_schedule(g, s);
+  return SPM_OK;
+}
+
+int spm_profile_enable(spm_graph* g, int enable) {
+  if (!g) return set_error(SPM_EINVAL, "spm_profile_enable: NULL handle");
+  SPM_TRY(prof_flush(g));
+  g->prof.on = enable != 0;
+  return SPM_OK;
+}
+
+int spm_profile_reset(spm_graph* g) {
+  if (!g) return set_error(SPM_EINVAL, "spm_profile_reset: NULL handle");
+  SPM_TRY(prof_flush(g));
+  g->prof.totals.clear();
+  return SPM_OK;
+}
+
+int spm_profile_query(spm_graph* g, const char* kernel, double* total_ms, int64_t* launches) {
+  if (!g || !kernel) return set_error(SPM_EINVAL, "spm_profile_query: NULL argument");
+  SPM_TRY(prof_flush(g));
+  auto it = g->prof.totals.find(kernel);
+  if (total_ms) *total_ms = it == g->prof.totals.end() ? 0.0 : it->second.first;
+  if (launches) *launches = it == g->prof.totals.end() ? 0 : it->second.second;
+  return SPM_OK;
+}
+
+}  // extern "C"

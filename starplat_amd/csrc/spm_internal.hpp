@@ -1,0 +1,134 @@
+// spm_internal.hpp -- shared definitions of the gfx950 kernel library:
+// the device graph handle, error plumbing, launch/profiling helper and
+// wavefront primitives. Wavefronts are 64 lanes on CDNA4.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/starplat_amd.h"
+
+namespace spm {
+
+constexpr int kWave = 64;
+
+// ---- errors ---------------------------------------------------------------
+int set_error(int code, const std::string& msg);
+#define SPM_CHECK_HIP(expr)                                                       \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::spm::set_error(SPM_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define SPM_TRY(expr)              \
+  do {                             \
+    int r_ = (expr);               \
+    if (r_ != SPM_OK) return r_;   \
+  } while (0)
+
+// ---- device buffers -------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct SsspWork;
+struct PrWork;
+struct TcWork;
+struct BcWork;
+
+struct Profiler {
+  bool on = false;
+  struct Pending { std::string name; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, int64_t>> totals;
+};
+
+}  // namespace spm
+
+struct spm_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int32_t V = 0;
+  int64_t E = 0;
+  int32_t* off = nullptr;   // indexofNodes [V+1]
+  int32_t* adj = nullptr;   // edgeList     [E]
+  int32_t* w = nullptr;     // weights      [E]
+  int32_t* roff = nullptr;  // rev_indexofNodes [V+1]
+  int32_t* radj = nullptr;  // srcList      [E]
+  int num_cus = 256;
+  int64_t bytes = 0;
+  spm_schedule sched{};
+  spm::SsspWork* sssp = nullptr;
+  spm::PrWork* pr = nullptr;
+  spm::TcWork* tc = nullptr;
+  spm::BcWork* bc = nullptr;
+  spm::Profiler prof;
+  int32_t* h_pinned = nullptr;  // small pinned scratch for flag readback (16 ints)
+};
+
+namespace spm {
+
+// Allocation accounted on the handle.
+int dmalloc(spm_graph* g, void** p, size_t bytes, const char* what);
+void dfree(spm_graph* g, void* p, size_t bytes);
+
+// Profiling hooks around one launch on g->stream.
+void prof_begin(spm_graph* g, const char* name, hipEvent_t* a, hipEvent_t* b);
+void prof_end(spm_graph* g, const char* name, hipEvent_t a, hipEvent_t b);
+int prof_flush(spm_graph* g);
+
+template <typename Kernel, typename... Args>
+inline int launch(spm_graph* g, const char* name, Kernel k, dim3 grid, dim3 block, size_t shmem,
+                  Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (g->prof.on) prof_begin(g, name, &a, &b);
+  hipLaunchKernelGGL(k, grid, block, shmem, g->stream, args...);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(SPM_EHIP, std::string("launch ") + name + ": " + hipGetErrorString(e));
+  if (g->prof.on) prof_end(g, name, a, b);
+  return SPM_OK;
+}
+
+int pr_release(spm_graph* g);
+int sssp_release(spm_graph* g);
+int tc_release(spm_graph* g);
+int bc_release(spm_graph* g);
+
+// ---- wavefront primitives ---------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Inclusive scan across the 64 lanes of a wavefront.
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  const int l = __lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    T t = __shfl_up(v, o, kWave);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+
+// Sum over groups of G consecutive lanes (G a power of two <= 64); every
+// lane of the group receives the total.
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+}  // namespace spm

@@ -1,0 +1,332 @@
+// spm_pr.hip -- pull PageRank for gfx950, the replacement of the code StarPlat
+// generates for Compute_PR (graphcode/staticDSLCodes/PageRankDSLV2:1-19, with
+// the diff update of PageRankDSLV3:15-19):
+//   do { diff = 0; forall v: sum = sum_{u in nodes_to(v)} pr[u]/outdeg(u);
+//        pr_nxt[v] = (1-delta)/N + delta*sum; diff += |pr_nxt[v]-pr[v]|;
+//        pr = pr_nxt; iter++ } while (diff > beta && iter < maxIter)
+// (OMP emission: backend_omp/dsl_cpp_generator.cpp:988-1015, :1094-1113,
+//  :2475-2484; do-while :568-575).
+//
+// Design. Each iteration is one kernel. A vertex's in-row (srcList) is read
+// once; the per-source term pr[u]/outdeg(u) is produced by the previous
+// iteration (or the init kernel) into a contribution array, so the gather
+// reads one float per in-edge. Vertices are binned once by in-degree and the
+// workgroups of one launch are split over the bins: one lane per vertex for
+// short rows, a group of G lanes, a 64-lane wavefront, or a 256-lane
+// workgroup for hubs. The diff reduction ends in the last workgroup to
+// finish (a completion counter), which also evaluates the loop condition and
+// raises a device flag; later iterations see the flag and return at once, so
+// a whole run is captured into one hipGraph with no host round trip.
+//
+// Arithmetic follows the emitted code: the per-term division pr[u]/outdeg(u)
+// is the same IEEE single division, and the update is evaluated without FMA
+// contraction. Only the order of the in-row summation differs (tree vs
+// sequential), which keeps ranks within a few ulps of the restatement.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "spm_internal.hpp"
+
+namespace spm {
+
+constexpr int kPrBlock = 256;
+
+struct PrState {
+  float diff[2];
+  unsigned int blocks_done;
+  int done;
+  int iter;
+  int maxIter;
+  float beta;
+  float delta;
+  float base;  // (1 - delta) / num_nodes
+};
+
+struct PrBins {
+  int32_t nb[4];   // vertices per bin
+  int32_t blk[5];  // first workgroup of each bin; blk[4] = grid size
+  int32_t group;   // lanes per vertex in bin 1
+};
+
+struct PrWork {
+  float* pr[2] = {nullptr, nullptr};
+  float* c[2] = {nullptr, nullptr};
+  float* degf = nullptr;
+  int32_t* order = nullptr;  // vertices grouped by bin
+  PrState* st = nullptr;
+  PrBins bins{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int R = 0;
+};
+
+__global__ void pr_init(int32_t V, const int32_t* __restrict__ off, float* __restrict__ degf,
+                        float* __restrict__ pr, float* __restrict__ c, PrState* st, float init,
+                        int maxIter, float beta, float delta, float base) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
+    const float d = float(off[v + 1] - off[v]);
+    degf[v] = d;
+    pr[v] = init;
+    c[v] = d > 0.f ? init / d : 0.f;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->diff[0] = 0.f;
+    st->diff[1] = 0.f;
+    st->blocks_done = 0;
+    st->done = 0;
+    st->iter = 0;
+    st->maxIter = maxIter;
+    st->beta = beta;
+    st->delta = delta;
+    st->base = base;
+  }
+}
+
+__device__ __forceinline__ void pr_store(int32_t v, float sum, const PrState& s,
+                                         const float* __restrict__ pr_in, float* __restrict__ pr_out,
+                                         float* __restrict__ c_out, const float* __restrict__ degf,
+                                         float& mydiff) {
+  const float nv = __fadd_rn(s.base, __fmul_rn(s.delta, sum));
+  const float old = pr_in[v];
+  pr_out[v] = nv;
+  const float d = degf[v];
+  c_out[v] = d > 0.f ? nv / d : 0.f;
+  mydiff += nv - old >= 0.f ? nv - old : old - nv;
+}
+
+template <int G>
+__global__ __launch_bounds__(kPrBlock) void pr_iter(
+    const int32_t* __restrict__ roff, const int32_t* __restrict__ radj,
+    const int32_t* __restrict__ order, const float* __restrict__ degf, PrBins bins,
+    const float* __restrict__ pr_in, const float* __restrict__ c_in, float* __restrict__ pr_out,
+    float* __restrict__ c_out, PrState* __restrict__ st, int parity) {
+  __shared__ float s_red[kPrBlock / kWave];
+  __shared__ int s_last;
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const PrState s = *st;
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  const int b = blockIdx.x;
+  float mydiff = 0.f;
+
+  if (b < bins.blk[1]) {
+    // bin 0: one lane per vertex
+    const int64_t i = int64_t(b) * kPrBlock + tid;
+    if (i < bins.nb[0]) {
+      const int32_t v = order[i];
+      float sum = 0.f;
+      for (int32_t e = roff[v]; e < roff[v + 1]; ++e) sum += c_in[radj[e]];
+      pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
+    }
+  } else if (b < bins.blk[2]) {
+    // bin 1: G lanes per vertex
+    const int64_t i = int64_t(b - bins.blk[1]) * (kPrBlock / G) + tid / G;
+    const int g = tid % G;
+    float sum = 0.f;
+    int32_t v = -1;
+    if (i < bins.nb[1]) {
+      v = order[bins.nb[0] + i];
+      for (int32_t e = roff[v] + g; e < roff[v + 1]; e += G) sum += c_in[radj[e]];
+    }
+    sum = group_sum<G>(sum);
+    if (v >= 0 && g == 0) pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
+  } else if (b < bins.blk[3]) {
+    // bin 2: one wavefront per vertex
+    const int64_t i = int64_t(b - bins.blk[2]) * (kPrBlock / kWave) + wid;
+    float sum = 0.f;
+    int32_t v = -1;
+    if (i < bins.nb[2]) {
+      v = order[bins.nb[0] + bins.nb[1] + i];
+      for (int32_t e = roff[v] + lane; e < roff[v + 1]; e += kWave) sum += c_in[radj[e]];
+    }
+    sum = wave_sum(sum);
+    if (v >= 0 && lane == 0) pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
+  } else {
+    // bin 3: one workgroup per vertex
+    const int32_t v = order[bins.nb[0] + bins.nb[1] + bins.nb[2] + (b - bins.blk[3])];
+    float sum = 0.f;
+    for (int32_t e = roff[v] + tid; e < roff[v + 1]; e += kPrBlock) sum += c_in[radj[e]];
+    sum = wave_sum(sum);
+    if (lane == 0) s_red[wid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+      float t = 0.f;
+      for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
+      pr_store(v, t, s, pr_in, pr_out, c_out, degf, mydiff);
+    }
+    __syncthreads();
+  }
+
+  // diff: block reduction, one atomic per workgroup, loop test by the last one.
+  mydiff = wave_sum(mydiff);
+  if (lane == 0) s_red[wid] = mydiff;
+  __syncthreads();
+  if (tid == 0) {
+    float t = 0.f;
+    for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
+    atomicAdd(&st->diff[parity], t);
+    __threadfence();
+    const unsigned int prev = atomicAdd(&st->blocks_done, 1u);
+    s_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last && tid == 0) {
+    __threadfence();
+    const float d = __hip_atomic_load(&st->diff[parity], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int it = st->iter + 1;
+    st->iter = it;
+    st->diff[parity] = 0.f;
+    st->blocks_done = 0;
+    if (!(d > s.beta && it < s.maxIter)) __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+static int pr_prepare(spm_graph* g) {
+  if (g->pr) return SPM_OK;
+  if (!g->roff) return set_error(SPM_ENOGRAPH, "PageRank needs the reverse CSR (rev_indexofNodes, srcList)");
+  PrWork* w = new PrWork();
+  g->pr = w;
+  const int32_t V = g->V;
+  for (int i = 0; i < 2; ++i) {
+    SPM_TRY(dmalloc(g, (void**)&w->pr[i], sizeof(float) * V, "pr ranks"));
+    SPM_TRY(dmalloc(g, (void**)&w->c[i], sizeof(float) * V, "pr contributions"));
+  }
+  SPM_TRY(dmalloc(g, (void**)&w->degf, sizeof(float) * V, "pr outdeg"));
+  SPM_TRY(dmalloc(g, (void**)&w->order, sizeof(int32_t) * V, "pr bins"));
+  SPM_TRY(dmalloc(g, (void**)&w->st, sizeof(PrState), "pr state"));
+  // Bin vertices by in-degree on the host (once per handle and schedule).
+  std::vector<int32_t> roff(size_t(V) + 1);
+  SPM_CHECK_HIP(hipMemcpyAsync(roff.data(), g->roff, sizeof(int32_t) * (V + 1), hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  const spm_schedule& s = g->sched;
+  std::vector<int32_t> order(V);
+  int32_t cnt[4] = {0, 0, 0, 0};
+  auto bin_of = [&](int32_t v) {
+    const int32_t d = roff[v + 1] - roff[v];
+    return d <= s.pr_t1 ? 0 : d <= s.pr_t2 ? 1 : d <= s.pr_t3 ? 2 : 3;
+  };
+  for (int32_t v = 0; v < V; ++v) ++cnt[bin_of(v)];
+  int32_t start[4] = {0, cnt[0], cnt[0] + cnt[1], cnt[0] + cnt[1] + cnt[2]};
+  for (int32_t v = 0; v < V; ++v) order[start[bin_of(v)]++] = v;
+  SPM_CHECK_HIP(hipMemcpyAsync(w->order, order.data(), sizeof(int32_t) * V, hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  PrBins& b = w->bins;
+  for (int i = 0; i < 4; ++i) b.nb[i] = cnt[i];
+  b.group = s.pr_group;
+  b.blk[0] = 0;
+  b.blk[1] = (cnt[0] + kPrBlock - 1) / kPrBlock;
+  b.blk[2] = b.blk[1] + (cnt[1] + kPrBlock / s.pr_group - 1) / (kPrBlock / s.pr_group);
+  b.blk[3] = b.blk[2] + (cnt[2] + kPrBlock / kWave - 1) / (kPrBlock / kWave);
+  b.blk[4] = b.blk[3] + cnt[3];
+  return SPM_OK;
+}
+
+int pr_release(spm_graph* g) {
+  PrWork* w = g->pr;
+  if (!w) return SPM_OK;
+  if (w->exec) (void)hipGraphExecDestroy(w->exec);
+  if (w->graph) (void)hipGraphDestroy(w->graph);
+  for (int i = 0; i < 2; ++i) {
+    dfree(g, w->pr[i], sizeof(float) * g->V);
+    dfree(g, w->c[i], sizeof(float) * g->V);
+  }
+  dfree(g, w->degf, sizeof(float) * g->V);
+  dfree(g, w->order, sizeof(int32_t) * g->V);
+  dfree(g, w->st, sizeof(PrState));
+  delete w;
+  g->pr = nullptr;
+  return SPM_OK;
+}
+
+static int pr_enqueue_iters(spm_graph* g, int R) {
+  PrWork* w = g->pr;
+  const dim3 grid(std::max(w->bins.blk[4], 1)), block(kPrBlock);
+  for (int r = 0; r < R; ++r) {
+    const int p = r & 1;
+    const float* pin = w->pr[p];
+    const float* cin = w->c[p];
+#define SPM_PR_LAUNCH(G)                                                                          \
+  case G:                                                                                         \
+    SPM_TRY(launch(g, "pr_iter", pr_iter<G>, grid, block, 0, (const int32_t*)g->roff,            \
+                   (const int32_t*)g->radj, (const int32_t*)w->order, (const float*)w->degf,       \
+                   w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], w->st, p));                        \
+    break;
+    switch (w->bins.group) {
+      SPM_PR_LAUNCH(2)
+      SPM_PR_LAUNCH(4)
+      SPM_PR_LAUNCH(8)
+      SPM_PR_LAUNCH(16)
+      SPM_PR_LAUNCH(32)
+      default: return set_error(SPM_EINVAL, "pr: bad lane group");
+    }
+#undef SPM_PR_LAUNCH
+  }
+  return SPM_OK;
+}
+
+int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out, float* h_out,
+           int32_t* iter_out) {
+  SPM_TRY(pr_prepare(g));
+  PrWork* w = g->pr;
+  const int32_t V = g->V;
+  const float num_nodes = float(V);
+  const float init = 1 / num_nodes;
+  const float base = (1 - delta) / num_nodes;
+  const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
+  SPM_TRY(launch(g, "pr_init", pr_init, dim3(igrid), dim3(256), 0, V, (const int32_t*)g->off, w->degf,
+                 w->pr[0], w->c[0], w->st, init, int(maxIter), beta, delta, base));
+  // Iterations per replay: even (ping-pong parity), up to 128.
+  int R = std::max(2, std::min(128, (int(std::max(maxIter, 1)) + 1) & ~1));
+  const bool use_graph = !g->prof.on;
+  if (use_graph && (!w->exec || w->R != R)) {
+    if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
+    if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
+    SPM_CHECK_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = pr_enqueue_iters(g, R);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc != SPM_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return set_error(SPM_EHIP, std::string("pr capture: ") + hipGetErrorString(e));
+    w->graph = graph;
+    SPM_CHECK_HIP(hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0));
+    w->R = R;
+  }
+  int32_t* h = g->h_pinned;
+  for (int64_t rep = 0;; ++rep) {
+    if (use_graph) SPM_CHECK_HIP(hipGraphLaunch(w->exec, g->stream));
+    else SPM_TRY(pr_enqueue_iters(g, R));
+    SPM_CHECK_HIP(hipMemcpyAsync(h, &w->st->done, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    if (h[0]) break;
+    if (rep > int64_t(maxIter) / R + 2) return set_error(SPM_EHIP, "pr: loop did not terminate");
+  }
+  const int32_t iters = h[1];
+  const float* res = w->pr[iters & 1];
+  if (d_out) SPM_CHECK_HIP(hipMemcpyAsync(d_out, res, sizeof(float) * V, hipMemcpyDeviceToDevice, g->stream));
+  if (h_out) SPM_CHECK_HIP(hipMemcpyAsync(h_out, res, sizeof(float) * V, hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  if (iter_out) *iter_out = iters;
+  SPM_TRY(prof_flush(g));
+  return SPM_OK;
+}
+
+}  // namespace spm
+
+using namespace spm;
+
+extern "C" {
+
+int spm_pr_device(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_pageRank,
+                  int32_t* iterCount) {
+  if (!g || !d_pageRank) return set_error(SPM_EINVAL, "spm_pr_device: NULL argument");
+  return pr_run(g, beta, delta, maxIter, d_pageRank, nullptr, iterCount);
+}
+
+int spm_compute_pr(spm_graph* g, float beta, float delta, int32_t maxIter, float* pageRank,
+                   int32_t* iterCount) {
+  if (!g || !pageRank) return set_error(SPM_EINVAL, "spm_compute_pr: NULL argument");
+  return pr_run(g, beta, delta, maxIter, nullptr, pageRank, iterCount);
+}
+
+}  // extern "C"

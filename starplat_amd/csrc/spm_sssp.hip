@@ -1,0 +1,407 @@
+// spm_sssp.hip -- Bellman-Ford SSSP for gfx950, the replacement of the code
+// StarPlat generates for sssp_dslV2 (graphcode/staticDSLCodes/sssp_dslV2:1-15):
+// fixedPoint until(finished: !modified) { forall v (modified) forall nbr:
+//   <nbr.dist, nbr.modified> = <Min(nbr.dist, v.dist + e.weight), True> }
+// (OMP emission: backend_omp/dsl_cpp_generator.cpp:297-527, :2637-2708;
+//  CUDA host loop with two copies per round: backend_cuda/...cpp:3405-3530).
+//
+// Design. The frontier ("modified" set) is a queue of vertices, each stored
+// with the exclusive prefix of out-degrees of the queue entries before it.
+// One 64-bit atomic per workgroup reserves (entries, edges) together, so
+// entry positions and edge prefixes grow in the same order. The next round
+// splits the frontier's edges into chunks of kChunk edges; the pusher of a
+// vertex records, for every chunk that starts inside its edge range, the
+// queue position that chunk starts at. A workgroup then stages its chunk's
+// vertices (at most kChunk+1) in LDS and every lane locates its edge by a
+// binary search in LDS. Hubs of 10^5 edges are spread over 100 workgroups,
+// and every relaxation lane does useful work.
+//
+// Three queues rotate (read k, write k+1, clear k+2), so a round is one
+// kernel and R rounds are captured into one hipGraph; the host reads one
+// 8-byte counter per R rounds instead of two copies per round. A round whose
+// input queue is empty exits at once. Per-vertex round stamps deduplicate
+// pushes without clearing flags between rounds.
+//
+// Distances are exact shortest paths, so the result is bit-identical to any
+// correct schedule of the reference's fixed point.
+#include <climits>
+#include <cstring>
+
+#include "spm_internal.hpp"
+
+namespace spm {
+
+constexpr int kSsspBlock = 256;
+constexpr int kChunk = 1024;
+constexpr int kPer = kChunk / kSsspBlock;
+constexpr int kSsspWaves = kSsspBlock / kWave;
+
+struct SsspQueue {
+  int32_t* vtx;     // [V] frontier vertices (each pushed at most once a round)
+  uint32_t* eoff;   // [V] exclusive out-degree prefix of the entry
+  int32_t* cfirst;  // [E/kChunk + 2] first entry of every edge chunk
+};
+
+struct SsspWork {
+  SsspQueue q[3];
+  unsigned long long* ctr = nullptr;  // [4] (entries<<32 | edges) x3, rounds
+  uint32_t* stamp = nullptr;          // [V]
+  uint32_t* round_base = nullptr;     // [1]
+  int32_t* dist = nullptr;            // [V] working distances
+  uint8_t* active = nullptr;          // [2][V] dense-mode frontier flags
+  size_t qcap = 0, ccap = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int rounds_per_graph = 0;
+  uint32_t host_base = 1;
+};
+
+__global__ void sssp_init(int32_t V, int32_t* __restrict__ dist, int32_t src,
+                          const int32_t* __restrict__ off, SsspQueue q0,
+                          unsigned long long* __restrict__ ctr, uint8_t* __restrict__ active) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < V; i += stride) {
+    dist[i] = i == src ? 0 : INT_MAX;
+    if (active) {
+      active[i] = i == src;
+      active[V + i] = 0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int32_t deg = off[src + 1] - off[src];
+    q0.vtx[0] = src;
+    q0.eoff[0] = 0;
+    q0.cfirst[0] = 0;
+    ctr[0] = deg > 0 ? ((1ull << 32) | uint32_t(deg)) : 0ull;
+    ctr[1] = 0;
+    ctr[2] = 0;
+    ctr[3] = 0;
+  }
+}
+
+__global__ void sssp_advance(uint32_t* round_base, uint32_t by) { *round_base += by; }
+
+// Block-wide exclusive scan of two counters (4 wavefronts of 64 lanes).
+__device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
+                                            uint32_t& ta, uint32_t& tb, uint32_t* s_tot) {
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  const uint32_t ia = wave_incl_scan(a), ib = wave_incl_scan(b);
+  if (lane == kWave - 1) {
+    s_tot[wid] = ia;
+    s_tot[kSsspWaves + wid] = ib;
+  }
+  __syncthreads();
+  uint32_t pa = 0, pb = 0;
+  ta = 0;
+  tb = 0;
+#pragma unroll
+  for (int w = 0; w < kSsspWaves; ++w) {
+    const uint32_t xa = s_tot[w], xb = s_tot[kSsspWaves + w];
+    if (w < wid) { pa += xa; pb += xb; }
+    ta += xa;
+    tb += xb;
+  }
+  ea = pa + ia - a;
+  eb = pb + ib - b;
+}
+
+__global__ __launch_bounds__(kSsspBlock) void sssp_relax(
+    const int32_t* __restrict__ off, const int32_t* __restrict__ adj,
+    const int32_t* __restrict__ wgt, int32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
+    SsspQueue qin, SsspQueue qout, unsigned long long* __restrict__ ctr_in,
+    unsigned long long* __restrict__ ctr_out, unsigned long long* __restrict__ ctr_clear,
+    unsigned long long* __restrict__ rounds, const uint32_t* __restrict__ round_base, uint32_t rloc) {
+  __shared__ uint32_t s_off[kChunk + 1];
+  __shared__ int32_t s_dv[kChunk + 1];
+  __shared__ int32_t s_base[kChunk + 1];
+  __shared__ uint32_t s_tot[2 * kSsspWaves];
+  __shared__ unsigned long long s_res;
+
+  const unsigned long long c = *ctr_in;
+  const uint32_t n = uint32_t(c >> 32), T = uint32_t(c);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *ctr_clear = 0ull;
+    if (T) *rounds += 1;
+  }
+  if (T == 0) return;
+  const uint32_t sv = *round_base + rloc;
+  const uint32_t nchunks = (T + kChunk - 1) / kChunk;
+  const int tid = threadIdx.x;
+
+  for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const int32_t p0 = qin.cfirst[chunk];
+    const int32_t p1 = chunk + 1 < nchunks ? qin.cfirst[chunk + 1] : int32_t(n) - 1;
+    const int cnt = p1 - p0 + 1;
+    for (int i = tid; i < cnt; i += kSsspBlock) {
+      const int32_t v = qin.vtx[p0 + i];
+      s_off[i] = qin.eoff[p0 + i];
+      s_dv[i] = __hip_atomic_load(&dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_base[i] = off[v];
+    }
+    __syncthreads();
+
+    // Pushes live in statically indexed registers (one slot per edge a lane
+    // relaxes), so no private-memory array is needed.
+    int32_t pv[kPer];
+    int32_t pd[kPer];
+    uint32_t pc = 0, sd = 0;
+    const uint32_t x0 = chunk * kChunk;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t x = x0 + k * kSsspBlock + tid;
+      pv[k] = -1;
+      pd[k] = 0;
+      if (x < T) {
+        int lo = 0, hi = cnt - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_off[mid] <= x) lo = mid;
+          else hi = mid - 1;
+        }
+        const int32_t e = s_base[lo] + int32_t(x - s_off[lo]);
+        const int32_t nb = adj[e];
+        const int32_t nd = s_dv[lo] + wgt[e];
+        if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          const int32_t old = atomicMin(&dist[nb], nd);
+          if (nd < old && atomicExch(&stamp[nb], sv) != sv) {
+            const int32_t deg = off[nb + 1] - off[nb];
+            if (deg > 0) {
+              pv[k] = nb;
+              pd[k] = deg;
+              ++pc;
+              sd += uint32_t(deg);
+            }
+          }
+        }
+      }
+    }
+
+    uint32_t epc, esd, tpc, tsd;
+    block_scan2(pc, sd, epc, esd, tpc, tsd, s_tot);
+    if (tid == 0 && tpc) s_res = atomicAdd(ctr_out, (static_cast<unsigned long long>(tpc) << 32) | tsd);
+    __syncthreads();
+    if (pc) {
+      const unsigned long long r = s_res;
+      uint32_t pos = uint32_t(r >> 32) + epc;
+      uint32_t eo = uint32_t(r) + esd;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        if (pv[k] < 0) continue;
+        qout.vtx[pos] = pv[k];
+        qout.eoff[pos] = eo;
+        const uint32_t d = uint32_t(pd[k]);
+        for (uint32_t cc = (eo + kChunk - 1) / kChunk; cc <= (eo + d - 1) / kChunk; ++cc) qout.cfirst[cc] = int32_t(pos);
+        ++pos;
+        eo += d;
+      }
+    }
+    __syncthreads();  // LDS reuse by the next chunk
+  }
+}
+
+// Dense mode: the reference's full-V sweep of the modified flags, one lane per
+// vertex with the out-edges of a wavefront's 64 vertices flattened across the
+// lanes (wave prefix sum + LDS binary search), two flag arrays swapped per round.
+__global__ __launch_bounds__(kSsspBlock) void sssp_dense(
+    int32_t V, const int32_t* __restrict__ off, const int32_t* __restrict__ adj,
+    const int32_t* __restrict__ wgt, int32_t* __restrict__ dist, uint8_t* __restrict__ cur,
+    uint8_t* __restrict__ nxt, unsigned long long* __restrict__ ctr_in,
+    unsigned long long* __restrict__ ctr_out, unsigned long long* __restrict__ ctr_clear,
+    unsigned long long* __restrict__ rounds) {
+  __shared__ uint32_t s_pre[kSsspWaves][kWave];
+  __shared__ int32_t s_base[kSsspWaves][kWave];
+  __shared__ int32_t s_dv[kSsspWaves][kWave];
+  const unsigned long long c = *ctr_in;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *ctr_clear = 0ull;
+    if (c) *rounds += 1;
+  }
+  if (c == 0) return;
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  unsigned long long found = 0;
+  const int64_t nwaves = int64_t(gridDim.x) * kSsspWaves;
+  for (int64_t base = (int64_t(blockIdx.x) * kSsspWaves + wid) * kWave; base < V; base += nwaves * kWave) {
+    const int64_t v = base + lane;
+    uint32_t deg = 0;
+    int32_t b = 0, dv = 0;
+    if (v < V && cur[v]) {
+      cur[v] = 0;
+      b = off[v];
+      deg = uint32_t(off[v + 1] - b);
+      dv = __hip_atomic_load(&dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint32_t incl = wave_incl_scan(deg);
+    const uint32_t total = __shfl(incl, kWave - 1, kWave);
+    s_pre[wid][lane] = incl - deg;
+    s_base[wid][lane] = b;
+    s_dv[wid][lane] = dv;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t x = lane; x < total; x += kWave) {
+      int lo = 0, hi = kWave - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[wid][mid] <= x) lo = mid;
+        else hi = mid - 1;
+      }
+      const int32_t e = s_base[wid][lo] + int32_t(x - s_pre[wid][lo]);
+      const int32_t nb = adj[e];
+      const int32_t nd = s_dv[wid][lo] + wgt[e];
+      if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        const int32_t old = atomicMin(&dist[nb], nd);
+        if (nd < old) {
+          nxt[nb] = 1;
+          found = 1;
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (__any(found) && lane == 0) atomicOr(ctr_out, 1ull);
+}
+
+static int sssp_alloc(spm_graph* g) {
+  if (g->sssp) return SPM_OK;
+  SsspWork* w = new SsspWork();
+  g->sssp = w;
+  const int32_t V = g->V;
+  w->qcap = size_t(V) + 1;
+  w->ccap = size_t(g->E) / kChunk + 3;
+  for (int i = 0; i < 3; ++i) {
+    SPM_TRY(dmalloc(g, (void**)&w->q[i].vtx, sizeof(int32_t) * w->qcap, "sssp queue"));
+    SPM_TRY(dmalloc(g, (void**)&w->q[i].eoff, sizeof(uint32_t) * w->qcap, "sssp queue offsets"));
+    SPM_TRY(dmalloc(g, (void**)&w->q[i].cfirst, sizeof(int32_t) * w->ccap, "sssp chunk index"));
+  }
+  SPM_TRY(dmalloc(g, (void**)&w->ctr, sizeof(unsigned long long) * 4, "sssp counters"));
+  SPM_TRY(dmalloc(g, (void**)&w->stamp, sizeof(uint32_t) * V, "sssp stamps"));
+  SPM_TRY(dmalloc(g, (void**)&w->round_base, sizeof(uint32_t), "sssp round base"));
+  SPM_TRY(dmalloc(g, (void**)&w->dist, sizeof(int32_t) * V, "sssp dist"));
+  SPM_TRY(dmalloc(g, (void**)&w->active, 2 * size_t(V), "sssp flags"));
+  SPM_CHECK_HIP(hipMemsetAsync(w->stamp, 0, sizeof(uint32_t) * V, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(w->round_base, &w->host_base, sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  return SPM_OK;
+}
+
+int sssp_release(spm_graph* g) {
+  SsspWork* w = g->sssp;
+  if (!w) return SPM_OK;
+  if (w->exec) (void)hipGraphExecDestroy(w->exec);
+  if (w->graph) (void)hipGraphDestroy(w->graph);
+  for (int i = 0; i < 3; ++i) {
+    dfree(g, w->q[i].vtx, sizeof(int32_t) * w->qcap);
+    dfree(g, w->q[i].eoff, sizeof(uint32_t) * w->qcap);
+    dfree(g, w->q[i].cfirst, sizeof(int32_t) * w->ccap);
+  }
+  dfree(g, w->ctr, sizeof(unsigned long long) * 4);
+  dfree(g, w->stamp, sizeof(uint32_t) * g->V);
+  dfree(g, w->round_base, sizeof(uint32_t));
+  dfree(g, w->dist, sizeof(int32_t) * g->V);
+  dfree(g, w->active, 2 * size_t(g->V));
+  delete w;
+  g->sssp = nullptr;
+  return SPM_OK;
+}
+
+static int sssp_grid(const spm_graph* g) {
+  return g->sched.sssp_grid > 0 ? g->sched.sssp_grid : g->num_cus * 4;
+}
+
+// Enqueues R rounds (R a multiple of 6: queues rotate by 3, dense flags by 2) followed by the round-base advance.
+static int sssp_enqueue_rounds(spm_graph* g, int R) {
+  SsspWork* w = g->sssp;
+  const int grid = sssp_grid(g);
+  for (int r = 0; r < R; ++r) {
+    const int i = r % 3, o = (r + 1) % 3, z = (r + 2) % 3;
+    if (g->sched.sssp_mode == 1) {
+      uint8_t* cur = w->active + size_t(r & 1) * g->V;
+      uint8_t* nxt = w->active + size_t((r + 1) & 1) * g->V;
+      SPM_TRY(launch(g, "sssp_dense", sssp_dense, dim3(grid), dim3(kSsspBlock), 0, g->V,
+                     (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)g->w, w->dist,
+                     cur, nxt, w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3));
+    } else {
+      SPM_TRY(launch(g, "sssp_relax", sssp_relax, dim3(grid), dim3(kSsspBlock), 0,
+                     (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)g->w, w->dist,
+                     w->stamp, w->q[i], w->q[o], w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3,
+                     (const uint32_t*)w->round_base, uint32_t(r)));
+    }
+  }
+  SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, uint32_t(R)));
+  return SPM_OK;
+}
+
+static int sssp_capture(spm_graph* g) {
+  SsspWork* w = g->sssp;
+  const int R = g->sched.sssp_rounds;
+  if (w->exec && w->rounds_per_graph == R) return SPM_OK;
+  if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
+  if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
+  SPM_CHECK_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+  int rc = sssp_enqueue_rounds(g, R);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(g->stream, &graph);
+  if (rc != SPM_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+  if (e != hipSuccess) return set_error(SPM_EHIP, std::string("sssp capture: ") + hipGetErrorString(e));
+  w->graph = graph;
+  SPM_CHECK_HIP(hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0));
+  w->rounds_per_graph = R;
+  return SPM_OK;
+}
+
+int sssp_run(spm_graph* g, int32_t src, int32_t* d_out, int32_t* rounds_out) {
+  if (!g->w) return set_error(SPM_ENOGRAPH, "SSSP needs edge weights (edgeLen or spm_graph_set_weights)");
+  if (src < 0 || src >= g->V) return set_error(SPM_EINVAL, "SSSP source out of range");
+  SPM_TRY(sssp_alloc(g));
+  SsspWork* w = g->sssp;
+  const int R = g->sched.sssp_rounds;
+  const int dense = g->sched.sssp_mode == 1;
+  if (w->host_base > 0xF0000000u) {  // stamp wrap-around: start over
+    w->host_base = 1;
+    SPM_CHECK_HIP(hipMemsetAsync(w->stamp, 0, sizeof(uint32_t) * g->V, g->stream));
+    SPM_CHECK_HIP(hipMemcpyAsync(w->round_base, &w->host_base, sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+  }
+  const int igrid = int(std::min<int64_t>((g->V + 255) / 256, 4096));
+  SPM_TRY(launch(g, "sssp_init", sssp_init, dim3(igrid), dim3(256), 0, g->V, w->dist, src,
+                 (const int32_t*)g->off, w->q[0], w->ctr, dense ? w->active : (uint8_t*)nullptr));
+  const bool use_graph = !g->prof.on;
+  if (use_graph) SPM_TRY(sssp_capture(g));
+  unsigned long long* h = reinterpret_cast<unsigned long long*>(g->h_pinned);
+  const int64_t max_replays = int64_t(g->V) / R + 2;
+  int64_t replays = 0;
+  for (;;) {
+    if (use_graph) SPM_CHECK_HIP(hipGraphLaunch(w->exec, g->stream));
+    else SPM_TRY(sssp_enqueue_rounds(g, R));
+    w->host_base += uint32_t(R);
+    SPM_CHECK_HIP(hipMemcpyAsync(h, w->ctr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    if (h[0] == 0ull) break;
+    if (++replays > max_replays) return set_error(SPM_EDIVERGE, "SSSP did not converge within V rounds (negative weights?)");
+  }
+  SPM_CHECK_HIP(hipMemcpyAsync(h, w->ctr + 3, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
+  if (d_out) SPM_CHECK_HIP(hipMemcpyAsync(d_out, w->dist, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  if (rounds_out) *rounds_out = int32_t(h[0]);
+  SPM_TRY(prof_flush(g));
+  return SPM_OK;
+}
+
+}  // namespace spm
+
+using namespace spm;
+
+extern "C" {
+
+int spm_sssp_device(spm_graph* g, int32_t src, int32_t* d_dist, int32_t* rounds) {
+  if (!g || !d_dist) return set_error(SPM_EINVAL, "spm_sssp_device: NULL argument");
+  return sssp_run(g, src, d_dist, rounds);
+}
+
+int spm_compute_sssp(spm_graph* g, int32_t* dist, const int32_t* weight, int32_t src) {
+  if (!g || !dist) return set_error(SPM_EINVAL, "spm_compute_sssp: NULL argument");
+  if (weight) SPM_TRY(spm_graph_set_weights(g, weight));
+  SPM_TRY(sssp_run(g, src, nullptr, nullptr));
+  SPM_CHECK_HIP(hipMemcpyAsync(dist, g->sssp->dist, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  return SPM_OK;
+}
+
+}  // extern "C"

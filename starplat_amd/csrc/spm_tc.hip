@@ -1,0 +1,309 @@
+// spm_tc.hip -- triangle counting for gfx950, the replacement of the code
+// StarPlat generates for Compute_TC (graphcode/staticDSLCodes/
+// triangle_counting_dsl:1-13):
+//   forall v: forall u in nbrs(v), u < v: forall w in nbrs(v), w > v:
+//     if (is_an_edge(u, w)) triangle_count += 1
+// with is_an_edge -> graph::check_if_nbr, a linear scan per call
+// (graph.hpp:233-287; OMP emission dsl_cpp_generator.cpp:2485-2495).
+//
+// Two exact forms of the same sum:
+//  * symmetric graphs (every edge stored both ways, the usual TC input): each
+//    triangle is counted exactly once by the DSL (at its middle vertex), so
+//    the count equals the number of triangles, which is computed by forward
+//    counting over a degree-ordered orientation: u -> w when
+//    (deg u, u) < (deg w, w). Out-rows of the orientation are at most
+//    sqrt(2m) long, which bounds the work of every hub.
+//  * any other graph: the DSL's own sum, item = CSR edge (v,u) with u < v,
+//    count |nbrs(u) intersect {w in nbrs(v) : w > v}|.
+// Both reduce to "size of the intersection of two ascending rows" per item.
+// A wavefront takes 64 items, flattens the elements of each item's shorter
+// row across its lanes (wave prefix sum, LDS lookup) and binary-searches each
+// element in the longer row; hits are counted with ballot + popcount. The
+// integer total is exact, so results are bit-identical to the reference.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "spm_internal.hpp"
+
+namespace spm {
+
+constexpr int kTcBlock = 256;
+constexpr int kTcWaves = kTcBlock / kWave;
+
+struct TcWork {
+  int checked = 0;       // rows verified sorted; symmetric flag valid
+  int symmetric = 0;
+  int oriented = 0;      // oriented CSR built
+  int32_t* dplus = nullptr;   // [V+1] oriented out-degree, then offsets
+  int32_t* padj = nullptr;    // [m] oriented rows
+  int32_t* psrc = nullptr;    // [m] oriented edge source
+  int32_t* csrc = nullptr;    // [E] CSR edge source (general form)
+  int64_t m = 0;
+  unsigned long long* acc = nullptr;  // [2] total, flags
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+};
+
+__device__ __forceinline__ bool tc_above(int32_t du, int32_t u, int32_t dw, int32_t w) {
+  return dw > du || (dw == du && w > u);
+}
+
+// First index in a[0..n) whose value is >= x.
+__device__ __forceinline__ int32_t lower_bound_dev(const int32_t* __restrict__ a, int32_t n, int32_t x) {
+  int32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (a[mid] < x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Checks ascending rows and symmetry; flags[0] |= 1 on an unsorted row,
+// flags[0] |= 2 when some edge (u,w) has no (w,u).
+__global__ __launch_bounds__(kTcBlock) void tc_check(int32_t V, const int32_t* __restrict__ off,
+                                                      const int32_t* __restrict__ adj,
+                                                      unsigned long long* flags) {
+  const int lane = __lane_id();
+  const int64_t nw = int64_t(gridDim.x) * kTcWaves;
+  unsigned long long f = 0;
+  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw) {
+    const int32_t b = off[u], e = off[u + 1];
+    for (int32_t i = b + lane; i < e; i += kWave) {
+      const int32_t w = adj[i];
+      if (i > b && adj[i - 1] >= w) f |= 1;
+      const int32_t wb = off[w], wl = off[w + 1] - wb;
+      const int32_t p = lower_bound_dev(adj + wb, wl, int32_t(u));
+      if (p >= wl || adj[wb + p] != u) f |= 2;
+    }
+  }
+  // One atomic per wavefront with any flag.
+  const unsigned long long any = (__ballot((f & 1ull) != 0) ? 1ull : 0ull) | (__ballot((f & 2ull) != 0) ? 2ull : 0ull);
+  if (lane == 0 && any) atomicOr(flags, any);
+}
+
+__global__ __launch_bounds__(kTcBlock) void tc_orient_count(int32_t V, const int32_t* __restrict__ off,
+                                                             const int32_t* __restrict__ adj,
+                                                             int32_t* __restrict__ dplus) {
+  const int lane = __lane_id();
+  const int64_t nw = int64_t(gridDim.x) * kTcWaves;
+  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw) {
+    const int32_t b = off[u], e = off[u + 1], du = e - b;
+    int32_t c = 0;
+    for (int32_t i = b + lane; i < e; i += kWave) {
+      const int32_t w = adj[i];
+      c += tc_above(du, int32_t(u), off[w + 1] - off[w], w) ? 1 : 0;
+    }
+    c = wave_sum(c);
+    if (lane == 0) dplus[u] = c;
+  }
+}
+
+// Oriented rows keep ascending order: lanes write their kept elements at the
+// ballot prefix of the lanes below them.
+__global__ __launch_bounds__(kTcBlock) void tc_orient_fill(int32_t V, const int32_t* __restrict__ off,
+                                                            const int32_t* __restrict__ adj,
+                                                            const int32_t* __restrict__ poff,
+                                                            int32_t* __restrict__ padj,
+                                                            int32_t* __restrict__ psrc) {
+  const int lane = __lane_id();
+  const int64_t nw = int64_t(gridDim.x) * kTcWaves;
+  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw) {
+    const int32_t b = off[u], e = off[u + 1], du = e - b;
+    int32_t out = poff[u];
+    for (int32_t i0 = b; i0 < e; i0 += kWave) {
+      const int32_t i = i0 + lane;
+      bool keep = false;
+      int32_t w = 0;
+      if (i < e) {
+        w = adj[i];
+        keep = tc_above(du, int32_t(u), off[w + 1] - off[w], w);
+      }
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const int32_t pos = out + __popcll(m & ((1ull << lane) - 1ull));
+        padj[pos] = w;
+        psrc[pos] = int32_t(u);
+      }
+      out += __popcll(m);
+    }
+  }
+}
+
+__global__ void tc_expand_src(int32_t V, const int32_t* __restrict__ off, int32_t* __restrict__ csrc) {
+  const int lane = __lane_id();
+  const int64_t nw = int64_t(gridDim.x) * kTcWaves;
+  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw)
+    for (int32_t i = off[u] + lane; i < off[u + 1]; i += kWave) csrc[i] = int32_t(u);
+}
+
+// Intersection counting. ORIENTED: item e is oriented edge (src[e], adj[e])
+// and both rows are oriented rows. Otherwise item e is CSR edge (v=src[e],
+// u=adj[e]); it counts only when u < v, against row(u) and the part of
+// row(v) above v.
+template <bool ORIENTED>
+__global__ __launch_bounds__(kTcBlock) void tc_count(int64_t nitems, const int32_t* __restrict__ roff,
+                                                      const int32_t* __restrict__ radj,
+                                                      const int32_t* __restrict__ src,
+                                                      unsigned long long* __restrict__ total) {
+  __shared__ uint32_t s_pre[kTcWaves][kWave];
+  __shared__ int32_t s_sp[kTcWaves][kWave];
+  __shared__ int32_t s_bp[kTcWaves][kWave];
+  __shared__ int32_t s_bl[kTcWaves][kWave];
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  unsigned long long cnt = 0;
+  const int64_t nw = int64_t(gridDim.x) * kTcWaves;
+  for (int64_t base = (int64_t(blockIdx.x) * kTcWaves + wid) * kWave; base < nitems; base += nw * kWave) {
+    const int64_t e = base + lane;
+    uint32_t work = 0;
+    int32_t sp = 0, bp = 0, bl = 0;
+    if (e < nitems) {
+      const int32_t x = src[e], y = radj[e];
+      int32_t ap, al, cp, cl;
+      bool live = true;
+      if (ORIENTED) {
+        ap = roff[x];
+        al = roff[x + 1] - ap;
+        cp = roff[y];
+        cl = roff[y + 1] - cp;
+      } else {
+        live = y < x;  // u = y < v = x
+        ap = roff[y];
+        al = roff[y + 1] - ap;
+        const int32_t vb = roff[x], vl = roff[x + 1] - vb;
+        const int32_t s0 = live ? lower_bound_dev(radj + vb, vl, x + 1) : vl;
+        cp = vb + s0;
+        cl = vl - s0;
+      }
+      if (live && al > 0 && cl > 0) {
+        if (al <= cl) { sp = ap; work = uint32_t(al); bp = cp; bl = cl; }
+        else { sp = cp; work = uint32_t(cl); bp = ap; bl = al; }
+      }
+    }
+    const uint32_t incl = wave_incl_scan(work);
+    const uint32_t tot = __shfl(incl, kWave - 1, kWave);
+    s_pre[wid][lane] = incl - work;
+    s_sp[wid][lane] = sp;
+    s_bp[wid][lane] = bp;
+    s_bl[wid][lane] = bl;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
+      const uint32_t t = t0 + lane;
+      bool hit = false;
+      if (t < tot) {
+        int lo = 0, hi = kWave - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_pre[wid][mid] <= t) lo = mid;
+          else hi = mid - 1;
+        }
+        const int32_t xv = radj[s_sp[wid][lo] + int32_t(t - s_pre[wid][lo])];
+        const int32_t* big = radj + s_bp[wid][lo];
+        const int32_t n = s_bl[wid][lo];
+        const int32_t p = lower_bound_dev(big, n, xv);
+        hit = p < n && big[p] == xv;
+      }
+      cnt += __popcll(__ballot(hit));
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0 && cnt) atomicAdd(total, cnt);
+}
+
+static int tc_prepare(spm_graph* g) {
+  if (!g->tc) g->tc = new TcWork();
+  TcWork* w = g->tc;
+  const int32_t V = g->V;
+  const int grid = g->num_cus * 8;
+  if (!w->acc) SPM_TRY(dmalloc(g, (void**)&w->acc, 2 * sizeof(unsigned long long), "tc counters"));
+  if (!w->checked) {
+    SPM_CHECK_HIP(hipMemsetAsync(w->acc, 0, 2 * sizeof(unsigned long long), g->stream));
+    SPM_TRY(launch(g, "tc_check", tc_check, dim3(grid), dim3(kTcBlock), 0, V, (const int32_t*)g->off,
+                   (const int32_t*)g->adj, w->acc + 1));
+    unsigned long long f = 0;
+    SPM_CHECK_HIP(hipMemcpyAsync(&f, w->acc + 1, sizeof f, hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    if (f & 1) return set_error(SPM_EINVAL, "TC needs ascending adjacency rows (graph.hpp sorts them)");
+    w->symmetric = (f & 2) == 0;
+    w->checked = 1;
+  }
+  if (w->symmetric && !w->oriented) {
+    SPM_TRY(dmalloc(g, (void**)&w->dplus, sizeof(int32_t) * (size_t(V) + 1), "tc oriented offsets"));
+    SPM_TRY(launch(g, "tc_orient_count", tc_orient_count, dim3(grid), dim3(kTcBlock), 0, V,
+                   (const int32_t*)g->off, (const int32_t*)g->adj, w->dplus));
+    SPM_CHECK_HIP(hipMemsetAsync(w->dplus + V, 0, sizeof(int32_t), g->stream));
+    size_t need = 0;
+    SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, w->dplus, w->dplus, V + 1, g->stream));
+    if (need > w->tmp_bytes) {
+      dfree(g, w->tmp, w->tmp_bytes);
+      w->tmp = nullptr;
+      w->tmp_bytes = 0;
+      SPM_TRY(dmalloc(g, &w->tmp, need, "tc scan scratch"));
+      w->tmp_bytes = need;
+    }
+    SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(w->tmp, need, w->dplus, w->dplus, V + 1, g->stream));
+    int32_t m = 0;
+    SPM_CHECK_HIP(hipMemcpyAsync(&m, w->dplus + V, sizeof m, hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    w->m = m;
+    SPM_TRY(dmalloc(g, (void**)&w->padj, sizeof(int32_t) * size_t(m), "tc oriented rows"));
+    SPM_TRY(dmalloc(g, (void**)&w->psrc, sizeof(int32_t) * size_t(m), "tc oriented sources"));
+    SPM_TRY(launch(g, "tc_orient_fill", tc_orient_fill, dim3(grid), dim3(kTcBlock), 0, V,
+                   (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)w->dplus, w->padj, w->psrc));
+    w->oriented = 1;
+  }
+  if (!w->symmetric && !w->csrc) {
+    SPM_TRY(dmalloc(g, (void**)&w->csrc, sizeof(int32_t) * size_t(g->E), "tc edge sources"));
+    SPM_TRY(launch(g, "tc_expand_src", tc_expand_src, dim3(grid), dim3(kTcBlock), 0, V, (const int32_t*)g->off, w->csrc));
+  }
+  return SPM_OK;
+}
+
+int tc_release(spm_graph* g) {
+  TcWork* w = g->tc;
+  if (!w) return SPM_OK;
+  dfree(g, w->dplus, sizeof(int32_t) * (size_t(g->V) + 1));
+  dfree(g, w->padj, sizeof(int32_t) * size_t(w->m));
+  dfree(g, w->psrc, sizeof(int32_t) * size_t(w->m));
+  dfree(g, w->csrc, sizeof(int32_t) * size_t(g->E));
+  dfree(g, w->acc, 2 * sizeof(unsigned long long));
+  dfree(g, w->tmp, w->tmp_bytes);
+  delete w;
+  g->tc = nullptr;
+  return SPM_OK;
+}
+
+int tc_run(spm_graph* g, uint64_t* out) {
+  SPM_TRY(tc_prepare(g));
+  TcWork* w = g->tc;
+  SPM_CHECK_HIP(hipMemsetAsync(w->acc, 0, sizeof(unsigned long long), g->stream));
+  const int grid = g->num_cus * 8;
+  if (w->symmetric) {
+    if (w->m) SPM_TRY(launch(g, "tc_count", tc_count<true>, dim3(grid), dim3(kTcBlock), 0, w->m,
+                            (const int32_t*)w->dplus, (const int32_t*)w->padj, (const int32_t*)w->psrc, w->acc));
+  } else {
+    if (g->E) SPM_TRY(launch(g, "tc_count_general", tc_count<false>, dim3(grid), dim3(kTcBlock), 0, g->E,
+                            (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)w->csrc, w->acc));
+  }
+  unsigned long long t = 0;
+  SPM_CHECK_HIP(hipMemcpyAsync(&t, w->acc, sizeof t, hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  *out = t;
+  SPM_TRY(prof_flush(g));
+  return SPM_OK;
+}
+
+}  // namespace spm
+
+using namespace spm;
+
+extern "C" {
+
+int spm_compute_tc(spm_graph* g, uint64_t* triangle_count) {
+  if (!g || !triangle_count) return set_error(SPM_EINVAL, "spm_compute_tc: NULL argument");
+  return tc_run(g, triangle_count);
+}
+
+}  // extern "C"

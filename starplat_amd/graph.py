@@ -1,0 +1,155 @@
+"""``graph``: host mirror of StarPlat's CSR graph class (graphcode/graph.hpp).
+
+Same public arrays and methods the generated code uses:
+
+    indexofNodes      graph.hpp:44      edgeList     :46
+    rev_indexofNodes  :45               srcList      :48
+    getEdgeLen()      :134-137          num_nodes()  :139-142   num_edges() :145-152
+    parseGraph()      :861-865          check_if_nbr :233-287   getEdge     :168-186
+
+The arrays are numpy int32 views of memory owned by the native host library;
+the CSR is built by a parallel sort rather than graph.hpp's std::map, with the
+same contents. Seeded generators (RMAT, Erdos-Renyi, grid) produce graphs
+directly. The device mirror (an ``spm_graph`` handle holding the CSR in HBM)
+is created on first use by the Compute_* functions and reused after.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import weakref
+
+import numpy as np
+
+from . import _native as N
+
+
+def _view(p, n):
+    if n == 0:
+        return np.zeros(0, np.int32)
+    return np.ctypeslib.as_array(p, shape=(int(n),))
+
+
+class graph:
+    """CSR graph with graph.hpp semantics (num_nodes = max id + 1)."""
+
+    def __init__(self, filePath: str | None = None):
+        self.filePath = filePath
+        self._h = None
+        self._dev = None
+        self._dev_weights = None
+
+    # ---- construction --------------------------------------------------
+    def _adopt(self, h):
+        if not h:
+            raise RuntimeError("graph construction failed (bad arguments or unreadable file)")
+        self._h = h
+        hl = N.host_lib()
+        weakref.finalize(self, hl.spmh_free, h)
+        s = h.contents
+        self.V = int(s.V)
+        self.E = int(s.E)
+        self.indexofNodes = _view(s.indexofNodes, self.V + 1)
+        self.edgeList = _view(s.edgeList, self.E)
+        self.edgeLen = _view(s.edgeLen, self.E)
+        self.rev_indexofNodes = _view(s.rev_indexofNodes, self.V + 1)
+        self.srcList = _view(s.srcList, self.E)
+        return self
+
+    def parseGraph(self):
+        """graph::parseGraph(): parseEdges (graph.hpp:464-515) + parseEdgesContent (:720-859)."""
+        if self.filePath is None:
+            raise ValueError("graph has no file path")
+        return self._adopt(N.host_lib().spmh_parse_text(str(self.filePath).encode()))
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, undirected=False,
+             weights=None):
+        """RMAT graph: edge_factor*2^scale samples, self-loops and duplicates removed.
+        weights=(lo, hi) assigns uniform int32 weights per surviving edge."""
+        lo, hi = weights if weights else (0, 0)
+        return cls()._adopt(N.host_lib().spmh_rmat(scale, edge_factor, a, b, c, seed,
+                                                   int(bool(undirected)), lo, hi))
+
+    @classmethod
+    def erdos_renyi(cls, n, m, seed=1, undirected=True, weights=None):
+        lo, hi = weights if weights else (0, 0)
+        return cls()._adopt(N.host_lib().spmh_er(n, m, seed, int(bool(undirected)), lo, hi))
+
+    @classmethod
+    def grid2d(cls, rows, cols, seed=1, weights=None):
+        lo, hi = weights if weights else (0, 0)
+        return cls()._adopt(N.host_lib().spmh_grid(rows, cols, seed, lo, hi))
+
+    @classmethod
+    def from_edges(cls, src, dst, weight=None):
+        """graph.hpp semantics on an explicit edge list: duplicates summed, self-loops kept."""
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        w = None if weight is None else np.ascontiguousarray(weight, np.int32)
+        if src.shape != dst.shape or (w is not None and w.shape != src.shape):
+            raise ValueError("src, dst and weight must have the same length")
+        return cls()._adopt(N.host_lib().spmh_from_edges(
+            len(src), N.ptr(src, C.c_int32), N.ptr(dst, C.c_int32), N.ptr(w, C.c_int32)))
+
+    def write_text(self, path, weights=True):
+        rc = N.host_lib().spmh_write_text(self._h, str(path).encode(), int(bool(weights)))
+        if rc != 0:
+            raise OSError(f"cannot write {path}")
+
+    # ---- graph.hpp accessors -------------------------------------------
+    def num_nodes(self):
+        return self.V
+
+    def num_edges(self):
+        return self.E
+
+    def getEdgeLen(self):
+        return self.edgeLen
+
+    def count_outNbrs(self, v):
+        return int(self.indexofNodes[v + 1] - self.indexofNodes[v])
+
+    def neighbors(self, v):
+        return self.edgeList[self.indexofNodes[v]:self.indexofNodes[v + 1]]
+
+    def nodes_to(self, v):
+        return self.srcList[self.rev_indexofNodes[v]:self.rev_indexofNodes[v + 1]]
+
+    def check_if_nbr(self, s, d):
+        return bool(np.any(self.neighbors(s) == d))
+
+    def getEdge(self, s, d):
+        """(weight, CSR position) of the first edge s->d, as getEdge returns it."""
+        row = self.neighbors(s)
+        hit = np.nonzero(row == d)[0]
+        if len(hit) == 0:
+            return None
+        e = int(self.indexofNodes[s] + hit[0])
+        return int(self.edgeLen[e]), e
+
+    def is_symmetric(self):
+        fwd = np.repeat(np.arange(self.V, dtype=np.int64), np.diff(self.indexofNodes)) * (1 << 32) + self.edgeList
+        rev = self.edgeList.astype(np.int64) * (1 << 32) + np.repeat(np.arange(self.V, dtype=np.int64), np.diff(self.indexofNodes))
+        return np.array_equal(np.sort(fwd), np.sort(rev))
+
+    # ---- device mirror ---------------------------------------------------
+    def device(self):
+        """The spm_graph handle mirroring this CSR in HBM (created once)."""
+        if self._dev is None:
+            from .device import DeviceGraph
+            self._dev = DeviceGraph(self)
+        return self._dev
+
+    def release_device(self):
+        if self._dev is not None:
+            self._dev.close()
+            self._dev = None
+
+
+def sources(V, n, seed):
+    """n distinct uniform vertices of [0, V), ascending (a std::set<int>'s order)."""
+    out = np.empty(n, np.int32)
+    rc = N.host_lib().spmh_sources(V, n, seed, N.ptr(out, C.c_int32))
+    if rc != 0:
+        raise ValueError("cannot draw that many distinct sources")
+    return out

@@ -1,0 +1,243 @@
+"""Parity of the gfx950 kernels (called through the C ABI) with the CPU oracle.
+
+Bar: SSSP distances, TC counts, BFS levels and path counts bit-exact; PageRank
+within 1e-5 relative L-inf of the restatement (and its FP64 error reported);
+BC within 1e-5 relative L-inf of the FP64 oracle. Relative L-inf here is
+max|gpu - ref| / max|ref|.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from starplat_amd import Compute_BC, Compute_PR, Compute_SSSP, Compute_TC, graph, sources
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+EXP = json.load(open(os.path.join(GOLDEN, "sample_graph_expected.json")))
+INT_MAX = np.iinfo(np.int32).max
+PR_TOL = 1e-5
+BC_TOL = 1e-5
+
+
+def rel_linf(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ------------------------------------------------------------------ golden fixture
+def test_golden_sample_graph(sample_graph):
+    g = sample_graph
+    assert Compute_TC(g) == EXP["triangles"]
+    lev, sig = g.device().bc_paths(0)
+    np.testing.assert_array_equal(lev, EXP["bfs_levels_from_0"])
+    np.testing.assert_array_equal(sig, EXP["sigma_from_0"])
+    bc = np.empty(g.V, np.float32)
+    Compute_BC(g, bc, {0})
+    np.testing.assert_allclose(bc, EXP["bc_sources_0"], rtol=0, atol=1e-6)
+    dist = np.empty(g.V, np.int32)
+    Compute_SSSP(g, dist, g.getEdgeLen(), 0)
+    np.testing.assert_array_equal(dist, EXP["sssp_from_0_zero_weights"])
+
+
+# ------------------------------------------------------------------ SSSP
+SSSP_CASES = [
+    ("rmat", dict(scale=8, edge_factor=8, seed=1)),
+    ("rmat", dict(scale=12, edge_factor=16, seed=2)),
+    ("rmat", dict(scale=16, edge_factor=16, seed=3)),
+    ("er", dict(n=20000, m=80000, seed=4)),
+    ("grid", dict(rows=60, cols=70)),
+]
+
+
+def make(kind, kw, weights=(1, 100)):
+    if kind == "rmat":
+        return graph.rmat(kw["scale"], kw["edge_factor"], seed=kw["seed"], weights=weights)
+    if kind == "er":
+        return graph.erdos_renyi(kw["n"], kw["m"], seed=kw["seed"], undirected=False, weights=weights)
+    return graph.grid2d(kw["rows"], kw["cols"], weights=weights)
+
+
+@pytest.mark.parametrize("kind,kw", SSSP_CASES)
+@pytest.mark.parametrize("mode", [0, 1])
+def test_sssp_parity(oracle, kind, kw, mode):
+    g = make(kind, kw)
+    dev = g.device()
+    dev.set_schedule(sssp_mode=mode)
+    for src in (0, g.V // 3, g.V - 1):
+        ref = oracle.compute_sssp(g, g.edgeLen, src)
+        got = dev.sssp(src)
+        np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("rounds,grid", [(6, 1), (12, 0), (48, 7), (96, 4096)])
+def test_sssp_schedules_bit_identical(oracle, rounds, grid):
+    g = graph.rmat(14, 16, seed=5, weights=(1, 100))
+    ref = oracle.compute_sssp(g, g.edgeLen, 0)
+    dev = g.device()
+    dev.set_schedule(sssp_rounds=rounds, sssp_grid=grid)
+    np.testing.assert_array_equal(dev.sssp(0), ref)
+    np.testing.assert_array_equal(dev.sssp(0), ref)  # replay of the captured graph
+
+
+def test_sssp_edge_cases(oracle):
+    # single vertex, no edges
+    g = graph.from_edges(np.array([0], np.int32), np.array([0], np.int32), np.array([5], np.int32))
+    np.testing.assert_array_equal(g.device().sssp(0), [0])
+    # source with no out-edges; unreachable part; zero and heavy weights; duplicates summed
+    s = np.array([1, 1, 2, 3, 3, 4], np.int32)
+    d = np.array([2, 3, 4, 4, 4, 0], np.int32)
+    w = np.array([0, 7, 2147483, 1, 2, 9], np.int32)
+    g = graph.from_edges(s, d, w)
+    for src in range(g.V):
+        np.testing.assert_array_equal(g.device().sssp(src), oracle.compute_sssp(g, g.edgeLen, src))
+    assert g.device().sssp(0)[1] == INT_MAX
+    # a caller-supplied weight array replaces the handle's weights
+    g = graph.rmat(10, 8, seed=7, weights=(1, 100))
+    w2 = (np.arange(g.E) % 13 + 1).astype(np.int32)
+    dist = np.empty(g.V, np.int32)
+    Compute_SSSP(g, dist, w2, 0)
+    np.testing.assert_array_equal(dist, oracle.compute_sssp(g, w2, 0))
+    with pytest.raises(Exception):
+        g.device().sssp(g.V)  # source out of range -> error status, no launch
+
+
+# ------------------------------------------------------------------ PR
+@pytest.mark.parametrize("scale,seed", [(8, 1), (12, 2), (16, 1)])
+def test_pr_parity(oracle, scale, seed):
+    g = graph.rmat(scale, 16, seed=seed)
+    ref, it_ref = oracle.compute_pr(g, 1e-10, 0.85, 100)
+    pr = np.empty(g.V, np.float32)
+    it = Compute_PR(g, 1e-10, 0.85, 100, pr)
+    assert rel_linf(pr, ref) <= PR_TOL
+    assert abs(it - it_ref) <= 3, (it, it_ref)
+    fp64, _ = oracle.pr_fp64(g, 1e-10, 0.85, it)
+    print(f"PR scale {scale}: iters gpu {it} oracle {it_ref}; rel Linf vs restatement {rel_linf(pr, ref):.2e}, "
+          f"vs FP64 {rel_linf(pr, fp64):.2e}")
+
+
+def test_pr_iteration_count_and_closed_form(oracle):
+    n = 100
+    s = np.arange(n, dtype=np.int32)
+    g = graph.from_edges(s, (s + 1) % n)
+    pr = np.empty(n, np.float32)
+    assert Compute_PR(g, 1e-10, 0.85, 100, pr) == 1
+    np.testing.assert_array_equal(pr, np.float32(1) / np.float32(n))
+    g = graph.rmat(12, 8, seed=3)
+    for maxIter in (1, 2, 7):
+        ref, it_ref = oracle.compute_pr(g, 1e-10, 0.85, maxIter)
+        out, it = g.device().pr(1e-10, 0.85, maxIter)
+        assert it == it_ref == maxIter
+        assert rel_linf(out, ref) <= PR_TOL
+    # large beta: stops after the first iteration whose diff <= beta
+    ref, it_ref = oracle.compute_pr(g, 1e-3, 0.85, 100)
+    out, it = g.device().pr(1e-3, 0.85, 100)
+    assert it == it_ref and rel_linf(out, ref) <= PR_TOL
+
+
+@pytest.mark.parametrize("t1,t2,t3,grp", [(0, 0, 0, 2), (64, 64, 64, 32), (2, 16, 256, 4), (8, 128, 100000, 16)])
+def test_pr_schedules(oracle, t1, t2, t3, grp):
+    g = graph.rmat(13, 16, seed=4)
+    ref, _ = oracle.compute_pr(g, 1e-10, 0.85, 30)
+    dev = g.device()
+    dev.set_schedule(pr_t1=t1, pr_t2=t2, pr_t3=t3, pr_group=grp)
+    out, it = dev.pr(1e-10, 0.85, 30)
+    assert it == 30 and rel_linf(out, ref) <= PR_TOL
+
+
+# ------------------------------------------------------------------ TC
+@pytest.mark.parametrize("scale,seed", [(8, 1), (12, 2), (16, 3)])
+def test_tc_parity(oracle, scale, seed):
+    g = graph.rmat(scale, 16, seed=seed, undirected=True)
+    assert Compute_TC(g) == oracle.compute_tc(g)
+
+
+def test_tc_closed_forms_and_directed(oracle):
+    n = 30
+    a, b = np.meshgrid(np.arange(n), np.arange(n))
+    m = a != b
+    assert Compute_TC(graph.from_edges(a[m].astype(np.int32), b[m].astype(np.int32))) == n * (n - 1) * (n - 2) // 6
+    assert Compute_TC(graph.grid2d(40, 50)) == 0
+    # directed input: the DSL's own sum, not the triangle count
+    g = graph.rmat(11, 8, seed=6)
+    assert not g.is_symmetric()
+    assert Compute_TC(g) == oracle.compute_tc(g, literal=True)
+    g = graph.erdos_renyi(3000, 30000, seed=2, undirected=True)
+    assert Compute_TC(g) == oracle.compute_tc(g)
+
+
+# ------------------------------------------------------------------ BC
+@pytest.mark.parametrize("kind", ["rmat_dir", "rmat_und", "er"])
+def test_bc_paths_bit_exact(oracle, kind):
+    if kind == "rmat_dir":
+        g = graph.rmat(12, 8, seed=1)
+    elif kind == "rmat_und":
+        g = graph.rmat(12, 8, seed=2, undirected=True)
+    else:
+        g = graph.erdos_renyi(10000, 40000, seed=3)
+    for s in (0, 1, g.V // 2):
+        _, lev_ref, sig_ref = oracle.compute_bc(g, [s])
+        lev, sig = g.device().bc_paths(s)
+        np.testing.assert_array_equal(lev, lev_ref)
+        np.testing.assert_array_equal(sig, sig_ref)
+
+
+@pytest.mark.parametrize("nsrc", [1, 5, 64, 65, 150])
+def test_bc_parity(oracle, nsrc):
+    g = graph.erdos_renyi(8000, 32000, seed=4)
+    src = sources(g.V, nsrc, seed=4)
+    ref, _, _ = oracle.compute_bc(g, src)
+    bc = np.empty(g.V, np.float32)
+    Compute_BC(g, bc, set(src.tolist()))
+    assert rel_linf(bc, ref) <= BC_TOL, rel_linf(bc, ref)
+
+
+def test_bc_directed_rmat(oracle):
+    g = graph.rmat(12, 16, seed=8)
+    src = sources(g.V, 70, seed=1)
+    ref, _, _ = oracle.compute_bc(g, src)
+    bc = g.device().bc(src)
+    assert rel_linf(bc, ref) <= BC_TOL
+    with pytest.raises(Exception):
+        g.device().bc(np.array([1, 1], np.int32))  # duplicates: not a set
+
+
+# ------------------------------------------------------------------ full size
+@pytest.mark.slow
+def test_full_sssp_rmat22(oracle):
+    g = graph.rmat(22, 16, seed=2, weights=(1, 100))
+    got = g.device().sssp(0)
+    ref = oracle.compute_sssp(g, g.edgeLen, 0)
+    np.testing.assert_array_equal(got, ref)
+    g.device().set_schedule(sssp_mode=1)
+    np.testing.assert_array_equal(g.device().sssp(0), ref)
+
+
+@pytest.mark.slow
+def test_full_tc_rmat21(oracle):
+    g = graph.rmat(21, 16, seed=3, undirected=True)
+    assert Compute_TC(g) == oracle.compute_tc(g)
+
+
+@pytest.mark.slow
+def test_full_pr_rmat22(oracle):
+    g = graph.rmat(22, 16, seed=1)
+    ref, it_ref = oracle.compute_pr(g, 1e-10, 0.85, 100)
+    out, it = g.device().pr(1e-10, 0.85, 100)
+    assert abs(it - it_ref) <= 3 and rel_linf(out, ref) <= PR_TOL
+
+
+@pytest.mark.slow
+def test_full_bc_er4m_subset(oracle):
+    g = graph.erdos_renyi(4194304, 16777216, seed=4)
+    src = sources(g.V, 256, seed=4)
+    for s in src[:2]:
+        _, lev_ref, sig_ref = oracle.compute_bc(g, [int(s)])
+        lev, sig = g.device().bc_paths(int(s))
+        np.testing.assert_array_equal(lev, lev_ref)
+        np.testing.assert_array_equal(sig, sig_ref)
+    sub = src[:16]
+    ref, _, _ = oracle.compute_bc(g, sub)
+    assert rel_linf(g.device().bc(sub), ref) <= BC_TOL

@@ -1,0 +1,270 @@
+"""Benchmark of the StarPlat hot path on one MI355X.
+
+Headline (the JSON `value`): SSSP Bellman-Ford fixedPoint on a directed RMAT
+scale-22 graph (edge factor 16, a,b,c = .57,.19,.19, int32 weights uniform in
+[1,100], source 0, seed 2), in GTEPS = m_reach / time, where m_reach is the
+number of out-edges of vertices reachable from the source (Graph500
+convention). One step = one Compute_SSSP on the graph already resident in HBM.
+PageRank, triangle counting and batched BC are measured the same way and
+reported under "per_algo" with their own rooflines.
+
+Roofline: algorithmic bytes of the dominant kernel's work divided by that
+kernel's device time, measured live with HIP events on the handle's stream.
+CPU baseline: the OpenMP restatement of StarPlat's generated code (oracle/),
+timed on this host's cores on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from starplat_amd import _native as N  # noqa: E402
+from starplat_amd import graph, hip, sources  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "GTEPS per algo at RMAT-22 on 1 MI355X, % HBM roofline; learned vs fixed sched"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def timed_steps(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    hip.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def kernel_time(dev, fn, kernels, reps=2):
+    """Total device ms and launches of `kernels` over `reps` profiled runs."""
+    dev.profile(True)
+    dev.profile_reset()
+    for _ in range(reps):
+        fn()
+    out = {k: dev.profile_query(k) for k in kernels}
+    dev.profile(False)
+    return {k: (ms / reps, n / reps) for k, (ms, n) in out.items()}
+
+
+def roofline(bytes_per_run, kms, launches):
+    achieved = bytes_per_run / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+            "bytes_per_launch": round(bytes_per_run / max(launches, 1)),
+            "kernel_ms_per_launch": round(kms / max(launches, 1), 5), "launches_per_step": launches}
+
+
+def host_cores():
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        keep = [l for l in out.splitlines() if l.split(":")[0] in ("Socket(s)", "Core(s) per socket", "Thread(s) per core", "Model name")]
+        return "; ".join(" ".join(l.split()) for l in keep)
+    except Exception:
+        return platform.processor()
+
+
+# --------------------------------------------------------------------------- SSSP
+def bench_sssp(args, O):
+    log("generating RMAT-22 (directed, weights U[1,100], seed 2)")
+    g = graph.rmat(args.sssp_scale, 16, seed=2, weights=(1, 100))
+    r, m, rl = C.c_int64(), C.c_int64(), C.c_int64()
+    model = N.host_lib().spmh_model_sssp(g._h, N.ptr(g.edgeLen, C.c_int32), 0, C.byref(r), C.byref(m), C.byref(rl))
+    dev = g.device()
+    d_out = hip.DeviceArray(g.V, np.int32)
+    rounds = dev.sssp_device(0, d_out)
+    ms = timed_steps(lambda: dev.sssp_device(0, d_out), args.steps, args.warmup)
+    kt = kernel_time(dev, lambda: dev.sssp_device(0, d_out), ["sssp_relax"])
+    kms, launches = kt["sssp_relax"]
+    res = {
+        "workload": f"SSSP Bellman-Ford fixedPoint, directed RMAT scale {args.sssp_scale} EF16 (a,b,c)=(.57,.19,.19), "
+                    "int32 weights U[1,100], src 0, seed 2",
+        "V": g.V, "E": g.E, "m_reach": m.value, "rounds_gpu": rounds, "rounds_sync_trace": r.value,
+        "relaxations_sync_trace": rl.value, "ms_per_step": round(ms, 4),
+        "gteps": m.value / (ms * 1e-3) / 1e9,
+        "roofline": roofline(model, kms, launches), "algorithmic_bytes": model,
+    }
+    if not args.no_cpu_baseline:
+        ref = O.compute_sssp(g, g.edgeLen, 0)
+        got = d_out.to_host()
+        res["parity_bit_exact"] = bool(np.array_equal(got, ref))
+        times = []
+        for _ in range(args.cpu_reps):
+            t0 = time.perf_counter()
+            O.compute_sssp(g, g.edgeLen, 0)
+            times.append(time.perf_counter() - t0)
+        t = statistics.median(times)
+        res["cpu_baseline"] = {"value": round(m.value / t / 1e9, 4), "unit": "GTEPS", "cores": O.num_threads(),
+                               "kind": "port", "sample": f"full workload, median of {args.cpu_reps} runs after 1 warm-up",
+                               "seconds": round(t, 4)}
+    d_out.free()
+    g.release_device()
+    return res
+
+
+# --------------------------------------------------------------------------- PR
+def bench_pr(args, O, scale, seed, label):
+    log(f"generating RMAT-{scale} for PageRank")
+    g = graph.rmat(scale, 16, seed=seed)
+    model = N.host_lib().spmh_model_pr_bytes(g._h)
+    dev = g.device()
+    d_out = hip.DeviceArray(g.V, np.float32)
+    it = dev.pr_device(1e-10, 0.85, 100, d_out)
+    ms = timed_steps(lambda: dev.pr_device(1e-10, 0.85, 100, d_out), max(1, args.steps // 2), 1)
+    kt = kernel_time(dev, lambda: dev.pr_device(1e-10, 0.85, 100, d_out), ["pr_iter"], reps=1)
+    kms, launches = kt["pr_iter"]
+    res = {"workload": f"{label}: pull PageRank, directed RMAT scale {scale} EF16, beta 1e-10, delta 0.85, maxIter 100, seed {seed}",
+           "V": g.V, "E": g.E, "iterations": it, "ms_per_run": round(ms, 4),
+           "gteps": g.E * it / (ms * 1e-3) / 1e9,
+           "roofline": roofline(model * it, kms, launches)}
+    if not args.no_cpu_baseline:
+        ref, it_ref = O.compute_pr(g, 1e-10, 0.85, 100)
+        got = d_out.to_host()
+        res["rel_linf_vs_restatement"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+        res["iterations_oracle"] = it_ref
+        k = 10
+        t0 = time.perf_counter()
+        O.compute_pr(g, 1e-10, 0.85, k)
+        t = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(g.E * k / t / 1e9, 4), "unit": "GTEPS", "cores": O.num_threads(),
+                               "kind": "port", "sample": f"first {k} iterations", "seconds": round(t, 4)}
+    d_out.free()
+    g.release_device()
+    return res
+
+
+# --------------------------------------------------------------------------- TC
+def bench_tc(args, O):
+    log("generating undirected RMAT-21 for TC")
+    g = graph.rmat(args.tc_scale, 16, seed=3, undirected=True)
+    mu, wk = C.c_int64(), C.c_int64()
+    model = N.host_lib().spmh_model_tc_bytes(g._h, C.byref(mu), C.byref(wk))
+    dev = g.device()
+    count = dev.tc()  # includes one-time orientation
+    ms = timed_steps(lambda: dev.tc(), max(1, args.steps // 4), 1)
+    kt = kernel_time(dev, lambda: dev.tc(), ["tc_count"], reps=1)
+    kms, launches = kt["tc_count"]
+    res = {"workload": f"Triangle counting, undirected RMAT scale {args.tc_scale} EF16, symmetrised, sorted, seed 3",
+           "V": g.V, "E": g.E, "m_und": mu.value, "triangles": count, "ms_per_run": round(ms, 4),
+           "gteps": mu.value / (ms * 1e-3) / 1e9, "roofline": roofline(model, kms, launches)}
+    if not args.no_cpu_baseline:
+        stride = 16
+        t0 = time.perf_counter()
+        O.compute_tc_strided(g, stride)
+        t = (time.perf_counter() - t0) * stride
+        res["cpu_baseline"] = {"value": round(mu.value / t / 1e9, 5), "unit": "GTEPS", "cores": O.num_threads(),
+                               "kind": "port", "sample": f"outer loop over every {stride}th vertex (1/{stride} of the v range), time x{stride}",
+                               "seconds_estimated": round(t, 3)}
+    g.release_device()
+    return res
+
+
+# --------------------------------------------------------------------------- BC
+def bench_bc(args, O):
+    log("generating Erdos-Renyi 4M / avg degree 8 for BC")
+    g = graph.erdos_renyi(args.bc_n, args.bc_n * 4, seed=4, undirected=True)
+    src = sources(g.V, args.bc_sources, seed=4)
+    et = C.c_int64()
+    log("BC byte model (one BFS per source on the host)")
+    model = N.host_lib().spmh_model_bc_bytes(g._h, N.ptr(src, C.c_int32), len(src), C.byref(et))
+    dev = g.device()
+    d_out = hip.DeviceArray(g.V, np.float32)
+    dev.bc_device(src, d_out)
+    ms = timed_steps(lambda: dev.bc_device(src, d_out), max(1, args.steps // 8), 0)
+    ks = ["bc_scan", "bc_sigma", "bc_back"]
+    kt = kernel_time(dev, lambda: dev.bc_device(src, d_out), ks, reps=1)
+    kms = sum(v[0] for v in kt.values())
+    launches = sum(v[1] for v in kt.values())
+    res = {"workload": f"Batched Brandes BC, Erdos-Renyi V={args.bc_n} avg degree 8, {len(src)} sources (seed 4), FP64 sigma",
+           "V": g.V, "E": g.E, "ms_per_run": round(ms, 3),
+           "gteps": len(src) * g.E / (ms * 1e-3) / 1e9,
+           "kernel_ms": {k: round(v[0], 3) for k, v in kt.items()},
+           "roofline": roofline(model, kms, launches),
+           "note": "model bytes count every source's traversal separately; the batched kernels share adjacency reads "
+                   "across 64 sources, so achieved model GB/s can exceed the HBM peak"}
+    if not args.no_cpu_baseline:
+        k = 2
+        t0 = time.perf_counter()
+        O.compute_bc(g, src[:k])
+        t = (time.perf_counter() - t0) / k * len(src)
+        res["cpu_baseline"] = {"value": round(len(src) * g.E / t / 1e9, 5), "unit": "GTEPS", "cores": O.num_threads(),
+                               "kind": "port", "sample": f"first {k} sources, time scaled to {len(src)}",
+                               "seconds_estimated": round(t, 2)}
+    d_out.free()
+    g.release_device()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--algos", default="sssp,pr,tc,bc")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--sssp-scale", type=int, default=22)
+    ap.add_argument("--tc-scale", type=int, default=21)
+    ap.add_argument("--bc-n", type=int, default=4194304)
+    ap.add_argument("--bc-sources", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    if hip.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    O = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # CPU baseline leg only
+    algos = args.algos.split(",")
+    per = {}
+    head = None
+    if "sssp" in algos:
+        head = bench_sssp(args, O)
+        per["sssp"] = {k: v for k, v in head.items() if k not in ("roofline", "cpu_baseline")}
+    if "pr" in algos:
+        per["pr"] = bench_pr(args, O, 22, 1, "PageRank at RMAT-22")
+        per["pr_c0"] = bench_pr(args, O, 16, 1, "PageRank config C0")
+    if "tc" in algos:
+        per["tc"] = bench_tc(args, O)
+    if "bc" in algos:
+        per["bc"] = bench_bc(args, O)
+    if head is None:
+        first = next(iter(per.values()))
+        head = {"workload": first["workload"], "gteps": first["gteps"], "ms_per_step": first.get("ms_per_run"),
+                "roofline": first["roofline"], "cpu_baseline": first.get("cpu_baseline")}
+    out = {
+        "metric": METRIC, "value": round(head["gteps"], 4), "unit": "GTEPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded RMAT / Erdos-Renyi generators)",
+        "config": {"workload": head["workload"], "V": head.get("V"), "E": head.get("E"), "m_reach": head.get("m_reach"),
+                   "parallelism": "single GPU"},
+        "roofline": head["roofline"],
+        "cpu_baseline": head.get("cpu_baseline"),
+        "host": host_cores(),
+        "per_algo": per,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

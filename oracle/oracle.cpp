@@ -205,6 +205,26 @@ int64_t oracle_compute_tc(int32_t V, const int32_t* indexofNodes, const int32_t*
   return triangle_count;
 }
 
+// The same loop restricted to outer vertices v = k*stride (a bounded sample
+// for CPU timing; stride 1 is the whole sum). Binary-search is_an_edge.
+int64_t oracle_compute_tc_strided(int32_t V, const int32_t* indexofNodes, const int32_t* edgeList,
+                                  int32_t stride) {
+  long triangle_count = 0;
+#pragma omp parallel for reduction(+ : triangle_count) schedule(dynamic, 16)
+  for (int v = 0; v < V; v += stride) {
+    for (int edge = indexofNodes[v]; edge < indexofNodes[v + 1]; edge++) {
+      int u = edgeList[edge];
+      if (u < v) {
+        for (int edge_1 = indexofNodes[v]; edge_1 < indexofNodes[v + 1]; edge_1++) {
+          int w = edgeList[edge_1];
+          if (w > v && check_if_nbr_bsearch(indexofNodes, edgeList, u, w)) triangle_count = triangle_count + 1;
+        }
+      }
+    }
+  }
+  return triangle_count;
+}
+
 // ---------------------------------------------------------------------------
 // Compute_BC (bc_dslV2) with FP64 sigma, delta and BC (the FP64 oracle).
 // Sources are processed in the given (ascending) order. When levels/sigma are
@@ -405,13 +425,15 @@ void tb_pr_fp64(int32_t V, const int32_t* off, const int32_t* roff, const int32_
   double diff;
   do {
     diff = 0;
+#pragma omp parallel for reduction(+ : diff) schedule(dynamic, 1024)
     for (int32_t v = 0; v < V; ++v) {
       double s = 0;
       for (int32_t e = roff[v]; e < roff[v + 1]; ++e) s += pr[src[e]] / double(off[src[e] + 1] - off[src[e]]);
       nxt[v] = (1 - delta) / V + delta * s;
       diff += std::abs(nxt[v] - pr[v]);
     }
-    std::copy(nxt.begin(), nxt.end(), pr);
+#pragma omp parallel for schedule(static)
+    for (int32_t v = 0; v < V; ++v) pr[v] = nxt[v];
     ++it;
   } while (diff > beta && it < maxIter);
   if (iterOut) *iterOut = it;

@@ -28,6 +28,8 @@ def lib():
         L.oracle_compute_sssp.argtypes = [i32, P_i32, P_i32, P_i32, P_i32, i32, C.c_int]
         L.oracle_compute_tc.argtypes = [i32, P_i32, P_i32, C.c_int, C.c_int]
         L.oracle_compute_tc.restype = i64
+        L.oracle_compute_tc_strided.argtypes = [i32, P_i32, P_i32, i32]
+        L.oracle_compute_tc_strided.restype = i64
         L.oracle_compute_bc.argtypes = [i32, P_i32, P_i32, P_i32, i32, P_f64, P_i32, P_f64, C.c_int]
         L.tb_dijkstra.argtypes = [i32, P_i32, P_i32, P_i32, i32, P_i32]
         L.tb_brandes.argtypes = [i32, P_i32, P_i32, P_i32, i32, P_f64, P_i32, P_f64]
@@ -66,6 +68,10 @@ def compute_sssp(g, weight, src, plant=0):
 
 def compute_tc(g, literal=False, plant=0):
     return int(lib().oracle_compute_tc(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), int(literal), plant))
+
+
+def compute_tc_strided(g, stride):
+    return int(lib().oracle_compute_tc_strided(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), stride))
 
 
 def compute_bc(g, sources, plant=0):

@@ -60,6 +60,7 @@ __global__ void sssp_init(int32_t V, int32_t* __restrict__ dist, int32_t src,
                           const int32_t* __restrict__ off, SsspQueue q0,
                           unsigned long long* __restrict__ ctr, uint8_t* __restrict__ active) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  const int32_t deg = off[src + 1] - off[src];
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < V; i += stride) {
     dist[i] = i == src ? 0 : INT_MAX;
     if (active) {
@@ -67,11 +68,12 @@ __global__ void sssp_init(int32_t V, int32_t* __restrict__ dist, int32_t src,
       active[V + i] = 0;
     }
   }
+  // Every chunk of the source's edges starts inside entry 0.
+  const int64_t nch = (int64_t(deg) + kChunk - 1) / kChunk;
+  for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < nch; c += stride) q0.cfirst[c] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const int32_t deg = off[src + 1] - off[src];
     q0.vtx[0] = src;
     q0.eoff[0] = 0;
-    q0.cfirst[0] = 0;
     ctr[0] = deg > 0 ? ((1ull << 32) | uint32_t(deg)) : 0ull;
     ctr[1] = 0;
     ctr[2] = 0;

@@ -1,0 +1,109 @@
+// starplat_graph.hpp -- the `graph` class that code generated by the mi355x
+// backend compiles against. It keeps the public members and methods generated
+// code and drivers use from StarPlat's graphcode/graph.hpp (class graph,
+// :31-1215) -- indexofNodes, edgeList, rev_indexofNodes, srcList (:44-48),
+// getEdgeLen() (:134-137), num_nodes() (:139-142), num_edges() (:145-152),
+// parseGraph() (:861-865), check_if_nbr() (:233-287), getEdge() (:168-186) --
+// without graph.hpp's MKL/GNN/dynamic-graph includes. Parsing and CSR
+// construction go through the host data layer (libspm_host.so, same
+// semantics); device() returns the HBM mirror used by the Compute_* bodies.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+#include "starplat_amd.h"
+#include "starplat_host.h"
+
+// graphcode/abstractGraph.hpp:16-25
+class edge {
+ public:
+  int32_t source = 0;
+  int32_t destination = 0;
+  int32_t weight = 0;
+  int32_t id = 0;
+  int8_t dir = 1;
+};
+
+class graph {
+ public:
+  int32_t* indexofNodes = nullptr;
+  int32_t* rev_indexofNodes = nullptr;
+  int32_t* edgeList = nullptr;
+  int32_t* srcList = nullptr;
+
+  explicit graph(const char* file) : filePath_(file ? file : "") {}
+  graph(const graph&) = delete;
+  graph& operator=(const graph&) = delete;
+  ~graph() {
+    if (dev_) spm_graph_destroy(dev_);
+    if (h_) spmh_free(h_);
+  }
+
+  void parseGraph() {
+    spmh_graph* h = spmh_parse_text(filePath_.c_str());
+    if (!h) throw std::runtime_error("parseGraph: cannot read " + filePath_);
+    adopt(h);
+  }
+  // Takes ownership of a CSR built by the host data layer (generators).
+  void adopt(spmh_graph* h) {
+    if (dev_) { spm_graph_destroy(dev_); dev_ = nullptr; }
+    if (h_) spmh_free(h_);
+    h_ = h;
+    indexofNodes = h->indexofNodes;
+    rev_indexofNodes = h->rev_indexofNodes;
+    edgeList = h->edgeList;
+    srcList = h->srcList;
+  }
+
+  int num_nodes() { return h_ ? h_->V : 1; }
+  int num_edges() { return h_ ? int(h_->E) : 0; }
+  int* getEdgeLen() { return h_ ? h_->edgeLen : nullptr; }
+
+  bool check_if_nbr(int s, int d) {
+    for (int32_t e = indexofNodes[s]; e < indexofNodes[s + 1]; ++e)
+      if (edgeList[e] == d) return true;
+    return false;
+  }
+  edge getEdge(int s, int d) {
+    edge found;
+    for (int32_t e = indexofNodes[s]; e < indexofNodes[s + 1]; ++e)
+      if (edgeList[e] == d) {
+        found.source = s;
+        found.destination = d;
+        found.weight = h_->edgeLen[e];
+        found.id = e;
+        return found;
+      }
+    return found;
+  }
+
+  // HBM mirror of the CSR, created on first use and reused by every
+  // generated Compute_* call on this graph.
+  spm_graph* device() {
+    if (!dev_) {
+      if (!h_) throw std::runtime_error("graph::device: parseGraph() first");
+      int rc = spm_graph_create(h_->V, h_->E, h_->indexofNodes, h_->edgeList, h_->edgeLen,
+                                h_->rev_indexofNodes, h_->srcList, &dev_);
+      if (rc != SPM_OK) throw std::runtime_error(std::string("spm_graph_create: ") + spm_last_error());
+    }
+    return dev_;
+  }
+
+ private:
+  std::string filePath_;
+  spmh_graph* h_ = nullptr;
+  spm_graph* dev_ = nullptr;
+};
+
+// Status check used by generated code: a failing library call is fatal, as a
+// failing CUDA call is in the reference's generated code (libcuda.cuh:215-222).
+inline void spm_require(int rc, const char* what) {
+  if (rc != SPM_OK) {
+    std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, spm_last_error());
+    std::exit(1);
+  }
+}

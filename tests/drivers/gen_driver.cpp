@@ -1,0 +1,56 @@
+// Driver for code generated by the mi355x backend, in the role of StarPlat's
+// graphcode/main_functions/*main.cpp: parse a graph.hpp-format text file, call
+// one generated Compute_* entry point, write the result as raw binary so the
+// test can compare it with the oracle. Built with -include <generated .h> and
+// one of -DRUN_SSSP / -DRUN_PR / -DRUN_TC / -DRUN_BC.
+//
+//   gen_sssp <graph.txt> <out.bin> <src>
+//   gen_pr   <graph.txt> <out.bin> <beta> <delta> <maxIter>
+//   gen_tc   <graph.txt> <out.bin>
+//   gen_bc   <graph.txt> <out.bin> <src> [<src> ...]
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <set>
+#include <vector>
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::fprintf(stderr, "usage: %s <graph.txt> <out.bin> [args]\n", argv[0]);
+    return 2;
+  }
+  graph g(argv[1]);
+  g.parseGraph();
+  const int V = g.num_nodes();
+  (void)V;
+  FILE* out = std::fopen(argv[2], "wb");
+  if (!out) return 3;
+#if defined(RUN_SSSP)
+  if (argc < 4) return 2;
+  std::vector<int> dist(V, INT_MAX);
+  std::vector<int> weight(g.num_edges());
+  int k = 0;  // weight by CSR position, from getEdge, as the reference driver builds it
+  for (int v = 0; v < V; v++)
+    for (int e = g.indexofNodes[v]; e < g.indexofNodes[v + 1]; e++) weight[k++] = g.getEdge(v, g.edgeList[e]).weight;
+  Compute_SSSP(g, dist.data(), weight.data(), std::atoi(argv[3]));
+  std::fwrite(dist.data(), sizeof(int), V, out);
+#elif defined(RUN_PR)
+  if (argc < 6) return 2;
+  std::vector<float> pr(V);
+  Compute_PR(g, (float)std::atof(argv[3]), (float)std::atof(argv[4]), std::atoi(argv[5]), pr.data());
+  std::fwrite(pr.data(), sizeof(float), V, out);
+#elif defined(RUN_TC)
+  long n = Compute_TC(g);
+  std::fwrite(&n, sizeof n, 1, out);
+#elif defined(RUN_BC)
+  std::set<int> sourceSet;
+  for (int i = 3; i < argc; i++) sourceSet.insert(std::atoi(argv[i]));
+  std::vector<float> BC(V);
+  Compute_BC(g, BC.data(), sourceSet);
+  std::fwrite(BC.data(), sizeof(float), V, out);
+#else
+#error "define one of RUN_SSSP, RUN_PR, RUN_TC, RUN_BC"
+#endif
+  std::fclose(out);
+  return 0;
+}

@@ -1,0 +1,137 @@
+"""The mi355x code generator: DSL front end, idiom lowering, C++ emission,
+CLI, and (on the GPU) the generated code linked into a driver and run."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from starplat_amd import graph
+from starplat_amd.dsl.compile import compile_source
+from starplat_amd.dsl.frontend import DSLError, parse
+from starplat_amd.dsl.lower import lower
+
+PROG = os.path.join(ROOT, "starplat_amd", "dsl", "programs")
+REF_DSL = "/root/reference/graphcode/staticDSLCodes"
+BIN = os.path.join(ROOT, "build", "bin")
+
+
+def src(name):
+    return open(os.path.join(PROG, name)).read()
+
+
+@pytest.mark.parametrize("prog,kind,roles", [
+    ("pagerank.sp", "pr", {"beta": "beta", "delta": "delta", "maxIter": "maxIter", "pageRank": "pageRank"}),
+    ("sssp.sp", "sssp", {"dist": "dist", "weight": "weight", "src": "src"}),
+    ("triangle_count.sp", "tc", {"count": "count"}),
+    ("betweenness.sp", "bc", {"BC": "BC", "sourceSet": "sourceSet"}),
+])
+def test_lower_programs(prog, kind, roles):
+    (f,) = parse(src(prog))
+    plan = lower(f)
+    assert plan.kind == kind
+    for k, v in roles.items():
+        assert plan.roles[k] == v
+
+
+def test_emitted_signatures_match_omp_backend():
+    h, cc, _ = compile_source(src("sssp.sp"), "sssp")
+    assert "void Compute_SSSP(graph& g, int* dist, int* weight, int src);" in h
+    assert "spm_compute_sssp(g.device(), dist, weight, src)" in cc
+    h, _, _ = compile_source(src("pagerank.sp"), "pr")
+    assert "void Compute_PR(graph& g, float beta, float delta, int maxIter, float* pageRank);" in h
+    h, _, _ = compile_source(src("triangle_count.sp"), "tc")
+    assert "long Compute_TC(graph& g);" in h
+    h, _, _ = compile_source(src("betweenness.sp"), "bc")
+    assert "void Compute_BC(graph& g, float* BC, std::set<int>& sourceSet);" in h
+
+
+MUTATIONS = [
+    ("sssp.sp", "Min(t.dist", "Max(t.dist"),                 # another reduction
+    ("sssp.sp", ".filter(improved == True)", ""),              # no frontier filter
+    ("sssp.sp", "v.dist + e.weight", "v.dist * e.weight"),     # another relaxation
+    ("pagerank.sp", "delta * acc", "acc"),                    # dropped damping
+    ("pagerank.sp", "u.pageRank / g.count_outNbrs(u)", "u.pageRank"),
+    ("pagerank.sp", "pageRank = nextRank;", ""),              # no Jacobi copy
+    ("triangle_count.sp", "filter(a < v)", "filter(a > v)"),
+    ("triangle_count.sp", "g.is_an_edge(a, c)", "g.is_an_edge(a, v)"),
+    ("betweenness.sp", "(1 + w.dep)", "(w.dep)"),
+    ("betweenness.sp", "w.paths += v.paths;", "w.paths += 1;"),
+]
+
+
+@pytest.mark.parametrize("prog,old,new", MUTATIONS)
+def test_mutated_programs_are_rejected(prog, old, new):
+    text = src(prog)
+    assert old in text
+    with pytest.raises(DSLError):
+        for f in parse(text.replace(old, new)):
+            lower(f)
+
+
+def test_pagerank_without_diff_update_runs_once():
+    text = src("pagerank.sp")
+    for old in ("change += r - v.pageRank;", "change += v.pageRank - r;"):
+        text = text.replace(old, "")
+    (f,) = parse(text)
+    plan = lower(f)
+    assert not plan.diff_updated and plan.notes
+    _, cc, _ = compile_source(text, "pr")
+    assert "(0.0f > beta) ? maxIter : 1" in cc
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DSL), reason="reference tree not present")
+@pytest.mark.parametrize("name,kind", [("PageRankDSLV2", "pr"), ("PageRankDSLV3", "pr"), ("sssp_dslV2", "sssp"),
+                                       ("sssp_dslV3", "sssp"), ("triangle_counting_dsl", "tc"), ("bc_dslV2", "bc")])
+def test_reference_programs_lower(name, kind):
+    (f,) = parse(open(os.path.join(REF_DSL, name)).read())
+    assert lower(f).kind == kind
+
+
+def test_cli(tmp_path):
+    def run(*args):
+        return subprocess.run([sys.executable, "-m", "starplat_amd.dsl", *args], cwd=ROOT, capture_output=True, text=True)
+
+    r = run("-s", "-f", os.path.join(PROG, "sssp.sp"), "-b", "mi355x", "-o", str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "generated_mi355x" / "sssp.h").exists() and (tmp_path / "generated_mi355x" / "sssp.cc").exists()
+    assert run("-s", "-f", "/nonexistent", "-b", "mi355x").returncode != 0
+    assert run("-s", "-f", os.path.join(PROG, "sssp.sp"), "-b", "cuda").returncode != 0
+    assert run("-d", "-f", os.path.join(PROG, "sssp.sp"), "-b", "mi355x").returncode != 0
+    assert run("-f", os.path.join(PROG, "sssp.sp")).returncode != 0
+
+
+def test_generated_drivers_built():
+    for a in ("sssp", "pr", "tc", "bc"):
+        assert os.access(os.path.join(BIN, f"gen_{a}"), os.X_OK), "run build() first"
+
+
+# ------------------------------------------------------------------ GPU: run the generated code
+def run_driver(algo, path, out, *args):
+    r = subprocess.run([os.path.join(BIN, f"gen_{algo}"), str(path), str(out), *map(str, args)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_generated_code_end_to_end(tmp_path, oracle):
+    g = graph.rmat(11, 8, seed=12, weights=(1, 100))
+    p = tmp_path / "g.txt"
+    g.write_text(p)
+    run_driver("sssp", p, tmp_path / "d.bin", 0)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "d.bin", np.int32), oracle.compute_sssp(g, g.edgeLen, 0))
+    run_driver("pr", p, tmp_path / "p.bin", 1e-10, 0.85, 20)
+    ref, _ = oracle.compute_pr(g, 1e-10, 0.85, 20)
+    got = np.fromfile(tmp_path / "p.bin", np.float32)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(ref)
+    u = graph.rmat(11, 8, seed=13, undirected=True)
+    q = tmp_path / "u.txt"
+    u.write_text(q, weights=False)
+    run_driver("tc", q, tmp_path / "t.bin")
+    assert int(np.fromfile(tmp_path / "t.bin", np.int64)[0]) == oracle.compute_tc(u)
+    run_driver("bc", q, tmp_path / "b.bin", 0, 5, 17)
+    bref, _, _ = oracle.compute_bc(u, [0, 5, 17])
+    got = np.fromfile(tmp_path / "b.bin", np.float32)
+    assert np.max(np.abs(got - bref)) <= 1e-5 * np.max(bref)

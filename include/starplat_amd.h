@@ -121,13 +121,36 @@ typedef struct spm_schedule {
   int32_t sssp_mode;      /* 0 edge-balanced frontier queue, 1 dense sweep   */
   int32_t sssp_rounds;    /* rounds per captured graph (multiple of 6)       */
   int32_t sssp_grid;      /* workgroups per relaxation launch (0 = auto)     */
-  int32_t tc_mode;        /* 0 edge-centric search, 1 vertex-staged LDS      */
-  int32_t reserved[8];
+  int32_t tc_mode;        /* 0 edge-centric search (the only mode so far)    */
+  int32_t pr_mode;        /* 0 pull over in-rows, 1 push (scatter + apply)   */
+  int32_t reserved[7];
 } spm_schedule;
 int spm_get_schedule(spm_graph* g, spm_schedule* s);
 int spm_set_schedule(spm_graph* g, const spm_schedule* s);
 /* The fixed heuristic schedule for g (fallback and judge of the learned one). */
 int spm_default_schedule(spm_graph* g, spm_schedule* s);
+
+/* ---- learned schedule selector --------------------------------------------
+ * No counterpart in the reference: a 16 -> 64 -> 64 -> 8 perceptron mapping
+ * graph features to a schedule class, trained and evaluated on the GPU by
+ * this library's own kernels. FP32 master parameters, BF16 weight copies
+ * read by forward and backward, FP32 accumulation. Parameter order:
+ * W1[64][16], b1[64], W2[64][64], b2[64], W3[8][64], b3[8]. */
+typedef struct spm_mlp spm_mlp;
+int spm_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed, spm_mlp** out);
+int spm_mlp_destroy(spm_mlp* m);
+int spm_mlp_num_params(void);
+int spm_mlp_set_params(spm_mlp* m, const float* params);
+int spm_mlp_get_params(spm_mlp* m, float* params);
+/* logits: float[n][8] for x: float[n][16] (host buffers). */
+int spm_mlp_forward(spm_mlp* m, const float* x, int32_t n, float* logits);
+int spm_mlp_predict(spm_mlp* m, const float* x, int32_t n, int32_t* cls);
+/* Mean softmax cross-entropy over the n samples and its gradient w.r.t. the
+ * FP32 masters (evaluated with the BF16 weights). */
+int spm_mlp_grad(spm_mlp* m, const float* x, const int32_t* y, int32_t n, float* loss, float* grads);
+/* Adam, seeded shuffles, `epochs` passes of mini-batches of `batch`. */
+int spm_mlp_train(spm_mlp* m, const float* x, const int32_t* y, int32_t n, int32_t epochs, int32_t batch,
+                  float lr, uint64_t seed, float* final_loss);
 
 /* ---- profiling -----------------------------------------------------------
  * With profiling on, each kernel launch is bracketed by HIP events on the

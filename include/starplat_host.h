@@ -72,6 +72,17 @@ void spmh_free(spmh_graph* g);
  * ascending (std::set<int> iteration order, as Compute_BC's sourceSet). */
 int spmh_sources(int32_t V, int32_t n, uint64_t seed, int32_t* out);
 
+/* The 16 graph features of the schedule selector, written to out[16]:
+ *  0 log2 V            1 log2(E+1)          2 mean out-degree
+ *  3 log2(1+var deg)   4 log2(1+max outdeg) 5 log2(1+max indeg)
+ *  6 signed log(1+|skew of outdeg|)         7 fraction of outdeg-0 vertices
+ *  8 fraction of indeg<=4 vertices          9 share of in-edges on indeg>64 vertices
+ * 10 log2(1+diameter estimate; double-sweep BFS)  11 fraction reached by the sweep
+ * 12 log2(1+BFS levels per 1000 reached)          13 widest BFS level / V
+ * 14 job (0 SSSP, 1 PageRank; the caller's value) 15 reciprocity (fraction of
+ *    edges whose reverse exists) */
+int spmh_features(const spmh_graph* g, float job, float* out);
+
 /* Algorithmic-byte models used for roofline accounting. */
 /* PR: bytes of one pull iteration = 8E + 20V + 4. */
 double spmh_model_pr_bytes(const spmh_graph* g);

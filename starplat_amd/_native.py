@@ -42,7 +42,8 @@ class SpmSchedule(C.Structure):
         ("sssp_rounds", i32),
         ("sssp_grid", i32),
         ("tc_mode", i32),
-        ("reserved", i32 * 8),
+        ("pr_mode", i32),
+        ("reserved", i32 * 7),
     ]
 
     def as_dict(self):
@@ -86,6 +87,15 @@ DEVICE_API = [
     ("spm_get_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
     ("spm_set_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
     ("spm_default_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
+    ("spm_mlp_create", C.c_int, [i32, i32, i32, u64, C.POINTER(vp)]),
+    ("spm_mlp_destroy", C.c_int, [vp]),
+    ("spm_mlp_num_params", C.c_int, []),
+    ("spm_mlp_set_params", C.c_int, [vp, P_f32]),
+    ("spm_mlp_get_params", C.c_int, [vp, P_f32]),
+    ("spm_mlp_forward", C.c_int, [vp, P_f32, i32, P_f32]),
+    ("spm_mlp_predict", C.c_int, [vp, P_f32, i32, P_i32]),
+    ("spm_mlp_grad", C.c_int, [vp, P_f32, P_i32, i32, P_f32, P_f32]),
+    ("spm_mlp_train", C.c_int, [vp, P_f32, P_i32, i32, i32, i32, f32, u64, P_f32]),
     ("spm_profile_enable", C.c_int, [vp, C.c_int]),
     ("spm_profile_reset", C.c_int, [vp]),
     ("spm_profile_query", C.c_int, [vp, C.c_char_p, P_f64, P_i64]),
@@ -100,6 +110,7 @@ HOST_API = [
     ("spmh_write_text", C.c_int, [P_spmh, C.c_char_p, C.c_int]),
     ("spmh_free", None, [P_spmh]),
     ("spmh_sources", C.c_int, [i32, i32, u64, P_i32]),
+    ("spmh_features", C.c_int, [P_spmh, f32, P_f32]),
     ("spmh_model_pr_bytes", f64, [P_spmh]),
     ("spmh_model_sssp", f64, [P_spmh, P_i32, i32, P_i64, P_i64, P_i64]),
     ("spmh_model_tc_bytes", f64, [P_spmh, P_i64, P_i64]),

@@ -93,6 +93,7 @@ void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
   s->sssp_rounds = 12;
   s->sssp_grid = 0;
   s->tc_mode = 0;
+  s->pr_mode = 0;
   (void)g;
 }
 
@@ -103,7 +104,8 @@ int validate_schedule(const spm_schedule* s) {
   if (s->sssp_mode < 0 || s->sssp_mode > 1) return set_error(SPM_EINVAL, "schedule: sssp_mode must be 0 or 1");
   if (s->sssp_rounds < 6 || s->sssp_rounds > 96 || s->sssp_rounds % 6) return set_error(SPM_EINVAL, "schedule: sssp_rounds must be a multiple of 6 in [6, 96]");
   if (s->sssp_grid < 0 || s->sssp_grid > 65536) return set_error(SPM_EINVAL, "schedule: sssp_grid out of range");
-  if (s->tc_mode < 0 || s->tc_mode > 1) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0 or 1");
+  if (s->tc_mode != 0) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0");
+  if (s->pr_mode < 0 || s->pr_mode > 1) return set_error(SPM_EINVAL, "schedule: pr_mode must be 0 (pull) or 1 (push)");
   return SPM_OK;
 }
 
@@ -211,7 +213,8 @@ int spm_set_schedule(spm_graph* g, const spm_schedule* s) {
   if (!g || !s) return set_error(SPM_EINVAL, "spm_set_schedule: NULL argument");
   SPM_TRY(validate_schedule(s));
   const bool pr_changed = s->pr_t1 != g->sched.pr_t1 || s->pr_t2 != g->sched.pr_t2 ||
-                          s->pr_t3 != g->sched.pr_t3 || s->pr_group != g->sched.pr_group;
+                          s->pr_t3 != g->sched.pr_t3 || s->pr_group != g->sched.pr_group ||
+                          s->pr_mode != g->sched.pr_mode;
   const bool sssp_changed = s->sssp_rounds != g->sched.sssp_rounds || s->sssp_grid != g->sched.sssp_grid ||
                             s->sssp_mode != g->sched.sssp_mode;
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));

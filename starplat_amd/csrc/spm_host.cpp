@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -288,6 +289,92 @@ int spmh_sources(int32_t V, int32_t n, uint64_t seed, int32_t* out) {
   }
   std::sort(picked.begin(), picked.end());
   std::copy(picked.begin(), picked.end(), out);
+  return 0;
+}
+
+int spmh_features(const spmh_graph* g, float job, float* out) {
+  const int32_t V = g->V;
+  const double E = double(g->E);
+  const int32_t* off = g->indexofNodes;
+  const int32_t* roff = g->rev_indexofNodes;
+  double s1 = 0, s2 = 0, s3 = 0;
+  int64_t maxo = 0, maxi = 0, zero_out = 0, low_in = 0, hub_edges = 0;
+#pragma omp parallel for reduction(+ : s1, s2, zero_out, low_in, hub_edges) reduction(max : maxo, maxi) schedule(static)
+  for (int32_t v = 0; v < V; ++v) {
+    const int64_t d = off[v + 1] - off[v], di = roff[v + 1] - roff[v];
+    s1 += double(d);
+    s2 += double(d) * double(d);
+    maxo = std::max(maxo, d);
+    maxi = std::max(maxi, di);
+    zero_out += d == 0;
+    low_in += di <= 4;
+    if (di > 64) hub_edges += di;
+  }
+  const double mean = s1 / V, var = std::max(0.0, s2 / V - mean * mean);
+#pragma omp parallel for reduction(+ : s3) schedule(static)
+  for (int32_t v = 0; v < V; ++v) {
+    const double x = double(off[v + 1] - off[v]) - mean;
+    s3 += x * x * x;
+  }
+  const double skew = var > 0 ? (s3 / V) / std::pow(var, 1.5) : 0.0;
+  // Double-sweep BFS over out-edges: from the highest-degree vertex, then
+  // from the deepest vertex of that sweep.
+  auto bfs = [&](int32_t root, int32_t* far, int64_t* reached, int64_t* widest) {
+    std::vector<int32_t> level(V, -1), q;
+    q.reserve(V);
+    level[root] = 0;
+    q.push_back(root);
+    int32_t depth = 0;
+    std::vector<int64_t> width(1, 1);
+    for (size_t h = 0; h < q.size(); ++h) {
+      const int32_t v = q[h];
+      for (int32_t e = off[v]; e < off[v + 1]; ++e) {
+        const int32_t w = g->edgeList[e];
+        if (level[w] < 0) {
+          level[w] = level[v] + 1;
+          q.push_back(w);
+          if (level[w] > depth) { depth = level[w]; width.push_back(0); }
+          ++width[level[w]];
+        }
+      }
+    }
+    *far = q.back();
+    *reached = int64_t(q.size());
+    *widest = *std::max_element(width.begin(), width.end());
+    return depth;
+  };
+  int32_t root = 0;
+  for (int32_t v = 1; v < V; ++v)
+    if (off[v + 1] - off[v] > off[root + 1] - off[root]) root = v;
+  int32_t far = root;
+  int64_t reached = 1, widest = 1, r2 = 1, w2 = 1;
+  int32_t d1 = bfs(root, &far, &reached, &widest);
+  int32_t far2 = far;
+  int32_t d2 = bfs(far, &far2, &r2, &w2);
+  const int32_t diam = std::max(d1, d2);
+  int64_t recip = 0;
+#pragma omp parallel for reduction(+ : recip) schedule(dynamic, 1024)
+  for (int32_t u = 0; u < V; ++u)
+    for (int32_t e = off[u]; e < off[u + 1]; ++e) {
+      const int32_t w = g->edgeList[e];
+      recip += std::binary_search(g->edgeList + off[w], g->edgeList + off[w + 1], u) ? 1 : 0;
+    }
+  out[0] = float(std::log2(double(V)));
+  out[1] = float(std::log2(E + 1));
+  out[2] = float(mean);
+  out[3] = float(std::log2(1 + var));
+  out[4] = float(std::log2(1.0 + double(maxo)));
+  out[5] = float(std::log2(1.0 + double(maxi)));
+  out[6] = float((skew >= 0 ? 1 : -1) * std::log1p(std::fabs(skew)));
+  out[7] = float(double(zero_out) / V);
+  out[8] = float(double(low_in) / V);
+  out[9] = E > 0 ? float(double(hub_edges) / E) : 0.f;
+  out[10] = float(std::log2(1.0 + diam));
+  out[11] = float(double(reached) / V);
+  out[12] = float(std::log2(1.0 + 1000.0 * (d1 + 1) / double(reached)));
+  out[13] = float(double(widest) / V);
+  out[14] = job;
+  out[15] = E > 0 ? float(double(recip) / E) : 0.f;
   return 0;
 }
 

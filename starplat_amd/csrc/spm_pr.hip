@@ -53,6 +53,7 @@ struct PrWork {
   float* pr[2] = {nullptr, nullptr};
   float* c[2] = {nullptr, nullptr};
   float* degf = nullptr;
+  float* acc = nullptr;      // push-mode accumulators
   int32_t* order = nullptr;  // vertices grouped by bin
   PrState* st = nullptr;
   PrBins bins{};
@@ -62,14 +63,15 @@ struct PrWork {
 };
 
 __global__ void pr_init(int32_t V, const int32_t* __restrict__ off, float* __restrict__ degf,
-                        float* __restrict__ pr, float* __restrict__ c, PrState* st, float init,
-                        int maxIter, float beta, float delta, float base) {
+                        float* __restrict__ pr, float* __restrict__ c, float* __restrict__ acc, PrState* st,
+                        float init, int maxIter, float beta, float delta, float base) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
     const float d = float(off[v + 1] - off[v]);
     degf[v] = d;
     pr[v] = init;
     c[v] = d > 0.f ? init / d : 0.f;
+    acc[v] = 0.f;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->diff[0] = 0.f;
@@ -94,6 +96,36 @@ __device__ __forceinline__ void pr_store(int32_t v, float sum, const PrState& s,
   const float d = degf[v];
   c_out[v] = d > 0.f ? nv / d : 0.f;
   mydiff += nv - old >= 0.f ? nv - old : old - nv;
+}
+
+// End of an iteration: workgroup sum of |new - old|, one atomic per
+// workgroup, and the loop test (diff > beta && iter < maxIter) evaluated by the
+// last workgroup to finish, which raises the done flag and resets the
+// accumulators for the iteration after next.
+__device__ __forceinline__ void pr_finish(float mydiff, PrState* __restrict__ st, const PrState& s, int parity,
+                                          float* s_red, int* s_last) {
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  mydiff = wave_sum(mydiff);
+  if (lane == 0) s_red[wid] = mydiff;
+  __syncthreads();
+  if (tid == 0) {
+    float t = 0.f;
+    for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
+    atomicAdd(&st->diff[parity], t);
+    __threadfence();
+    const unsigned int prev = atomicAdd(&st->blocks_done, 1u);
+    *s_last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (*s_last && tid == 0) {
+    __threadfence();
+    const float d = __hip_atomic_load(&st->diff[parity], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int it = st->iter + 1;
+    st->iter = it;
+    st->diff[parity] = 0.f;
+    st->blocks_done = 0;
+    if (!(d > s.beta && it < s.maxIter)) __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 template <int G>
@@ -158,28 +190,71 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     __syncthreads();
   }
 
-  // diff: block reduction, one atomic per workgroup, loop test by the last one.
-  mydiff = wave_sum(mydiff);
-  if (lane == 0) s_red[wid] = mydiff;
-  __syncthreads();
-  if (tid == 0) {
-    float t = 0.f;
-    for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
-    atomicAdd(&st->diff[parity], t);
-    __threadfence();
-    const unsigned int prev = atomicAdd(&st->blocks_done, 1u);
-    s_last = prev == gridDim.x - 1;
+  pr_finish(mydiff, st, s, parity, s_red, &s_last);
+}
+
+// Push iteration, step 1: every out-edge (u,v) adds u's contribution to
+// acc[v] (hardware FP32 atomics; the order of additions is unspecified, so
+// sums differ from the pull order by round-off only). A wavefront flattens
+// the out-edges of 64 consecutive vertices across its lanes.
+__global__ __launch_bounds__(kPrBlock) void pr_push_scatter(int32_t V, const int32_t* __restrict__ off,
+                                                            const int32_t* __restrict__ adj,
+                                                            const float* __restrict__ c_in, float* __restrict__ acc,
+                                                            const PrState* __restrict__ st) {
+  __shared__ uint32_t s_pre[kPrBlock / kWave][kWave];
+  __shared__ int32_t s_base[kPrBlock / kWave][kWave];
+  __shared__ float s_c[kPrBlock / kWave][kWave];
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  const int64_t nw = int64_t(gridDim.x) * (kPrBlock / kWave);
+  for (int64_t base = (int64_t(blockIdx.x) * (kPrBlock / kWave) + wid) * kWave; base < V; base += nw * kWave) {
+    const int64_t u = base + lane;
+    uint32_t deg = 0;
+    int32_t b = 0;
+    float cu = 0.f;
+    if (u < V) {
+      b = off[u];
+      deg = uint32_t(off[u + 1] - b);
+      cu = c_in[u];
+    }
+    const uint32_t incl = wave_incl_scan(deg);
+    const uint32_t total = __shfl(incl, kWave - 1, kWave);
+    s_pre[wid][lane] = incl - deg;
+    s_base[wid][lane] = b;
+    s_c[wid][lane] = cu;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t x = lane; x < total; x += kWave) {
+      int lo = 0, hi = kWave - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pre[wid][mid] <= x) lo = mid;
+        else hi = mid - 1;
+      }
+      const int32_t v = adj[s_base[wid][lo] + int32_t(x - s_pre[wid][lo])];
+      unsafeAtomicAdd(&acc[v], s_c[wid][lo]);
+    }
+    __builtin_amdgcn_wave_barrier();
   }
-  __syncthreads();
-  if (s_last && tid == 0) {
-    __threadfence();
-    const float d = __hip_atomic_load(&st->diff[parity], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int it = st->iter + 1;
-    st->iter = it;
-    st->diff[parity] = 0.f;
-    st->blocks_done = 0;
-    if (!(d > s.beta && it < s.maxIter)) __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Push iteration, step 2: one lane per vertex applies the update, clears its
+// accumulator for the next iteration and joins the convergence test.
+__global__ __launch_bounds__(kPrBlock) void pr_push_apply(int32_t V, const float* __restrict__ degf,
+                                                          float* __restrict__ acc, const float* __restrict__ pr_in,
+                                                          float* __restrict__ pr_out, float* __restrict__ c_out,
+                                                          PrState* __restrict__ st, int parity) {
+  __shared__ float s_red[kPrBlock / kWave];
+  __shared__ int s_last;
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const PrState s = *st;
+  float mydiff = 0.f;
+  const int64_t stride = int64_t(gridDim.x) * kPrBlock;
+  for (int64_t v = int64_t(blockIdx.x) * kPrBlock + threadIdx.x; v < V; v += stride) {
+    const float sum = acc[v];
+    acc[v] = 0.f;
+    pr_store(int32_t(v), sum, s, pr_in, pr_out, c_out, degf, mydiff);
   }
+  pr_finish(mydiff, st, s, parity, s_red, &s_last);
 }
 
 static int pr_prepare(spm_graph* g) {
@@ -193,6 +268,7 @@ static int pr_prepare(spm_graph* g) {
     SPM_TRY(dmalloc(g, (void**)&w->c[i], sizeof(float) * V, "pr contributions"));
   }
   SPM_TRY(dmalloc(g, (void**)&w->degf, sizeof(float) * V, "pr outdeg"));
+  SPM_TRY(dmalloc(g, (void**)&w->acc, sizeof(float) * V, "pr push accumulators"));
   SPM_TRY(dmalloc(g, (void**)&w->order, sizeof(int32_t) * V, "pr bins"));
   SPM_TRY(dmalloc(g, (void**)&w->st, sizeof(PrState), "pr state"));
   // Bin vertices by in-degree on the host (once per handle and schedule).
@@ -232,6 +308,7 @@ int pr_release(spm_graph* g) {
     dfree(g, w->c[i], sizeof(float) * g->V);
   }
   dfree(g, w->degf, sizeof(float) * g->V);
+  dfree(g, w->acc, sizeof(float) * g->V);
   dfree(g, w->order, sizeof(int32_t) * g->V);
   dfree(g, w->st, sizeof(PrState));
   delete w;
@@ -242,10 +319,18 @@ int pr_release(spm_graph* g) {
 static int pr_enqueue_iters(spm_graph* g, int R) {
   PrWork* w = g->pr;
   const dim3 grid(std::max(w->bins.blk[4], 1)), block(kPrBlock);
+  const int pgrid = int(std::min<int64_t>((g->V + kPrBlock - 1) / kPrBlock, int64_t(g->num_cus) * 8));
   for (int r = 0; r < R; ++r) {
     const int p = r & 1;
     const float* pin = w->pr[p];
     const float* cin = w->c[p];
+    if (g->sched.pr_mode == 1) {
+      SPM_TRY(launch(g, "pr_push_scatter", pr_push_scatter, dim3(pgrid), block, 0, g->V, (const int32_t*)g->off,
+                     (const int32_t*)g->adj, cin, w->acc, (const PrState*)w->st));
+      SPM_TRY(launch(g, "pr_push_apply", pr_push_apply, dim3(pgrid), block, 0, g->V, (const float*)w->degf, w->acc,
+                     pin, w->pr[1 - p], w->c[1 - p], w->st, p));
+      continue;
+    }
 #define SPM_PR_LAUNCH(G)                                                                          \
   case G:                                                                                         \
     SPM_TRY(launch(g, "pr_iter", pr_iter<G>, grid, block, 0, (const int32_t*)g->roff,            \
@@ -275,7 +360,7 @@ int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out,
   const float base = (1 - delta) / num_nodes;
   const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
   SPM_TRY(launch(g, "pr_init", pr_init, dim3(igrid), dim3(256), 0, V, (const int32_t*)g->off, w->degf,
-                 w->pr[0], w->c[0], w->st, init, int(maxIter), beta, delta, base));
+                 w->pr[0], w->c[0], w->acc, w->st, init, int(maxIter), beta, delta, base));
   // Iterations per replay: even (ping-pong parity), up to 128.
   int R = std::max(2, std::min(128, (int(std::max(maxIter, 1)) + 1) & ~1));
   const bool use_graph = !g->prof.on;

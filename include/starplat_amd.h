@@ -87,6 +87,11 @@ int spm_compute_sssp(spm_graph* g, int32_t* dist, const int32_t* weight, int32_t
 /* Same, handle weights, result in device memory d_dist (int32[V]). rounds
  * (may be NULL) receives the number of non-empty relaxation rounds. */
 int spm_sssp_device(spm_graph* g, int32_t src, int32_t* d_dist, int32_t* rounds);
+/* Same fixed point seeded from n sources at once (dist 0 at each; duplicates
+ * ignored). On a disjoint union of graphs with one source per component this
+ * runs many SSSP jobs in one launch sequence with per-job results unchanged.
+ * No counterpart in the reference, which runs one Compute_SSSP per call. */
+int spm_sssp_multi_device(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_dist, int32_t* rounds);
 
 /* ---- Triangle counting --------------------------------------------------
  * Replaces: auto Compute_TC(graph& g) (returns long) generated from

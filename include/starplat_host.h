@@ -63,6 +63,11 @@ spmh_graph* spmh_from_edges(int64_t n, const int32_t* src, const int32_t* dst,
  * weight reads as 0. Returns NULL when the file cannot be opened. */
 spmh_graph* spmh_parse_text(const char* path);
 
+/* Disjoint union of n graphs: graph i's vertex v becomes base_i + v with
+ * base_i = sum of the V of graphs before it (written to bases[n+1] when not
+ * NULL). Rows stay ascending, weights are kept. */
+spmh_graph* spmh_union(const spmh_graph* const* gs, int32_t n, int64_t* bases);
+
 /* Writes "src dst w" lines (or "src dst" when weights == 0). 0 on success. */
 int spmh_write_text(const spmh_graph* g, const char* path, int weights);
 

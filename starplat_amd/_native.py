@@ -80,6 +80,7 @@ DEVICE_API = [
     ("spm_pr_device", C.c_int, [vp, f32, f32, i32, vp, P_i32]),
     ("spm_compute_sssp", C.c_int, [vp, P_i32, P_i32, i32]),
     ("spm_sssp_device", C.c_int, [vp, i32, vp, P_i32]),
+    ("spm_sssp_multi_device", C.c_int, [vp, P_i32, i32, vp, P_i32]),
     ("spm_compute_tc", C.c_int, [vp, P_u64]),
     ("spm_compute_bc", C.c_int, [vp, P_f32, P_i32, i32]),
     ("spm_bc_device", C.c_int, [vp, P_i32, i32, vp]),
@@ -107,6 +108,7 @@ HOST_API = [
     ("spmh_grid", P_spmh, [C.c_int, C.c_int, u64, C.c_int, C.c_int]),
     ("spmh_from_edges", P_spmh, [i64, P_i32, P_i32, P_i32]),
     ("spmh_parse_text", P_spmh, [C.c_char_p]),
+    ("spmh_union", P_spmh, [C.POINTER(P_spmh), i32, P_i64]),
     ("spmh_write_text", C.c_int, [P_spmh, C.c_char_p, C.c_int]),
     ("spmh_free", None, [P_spmh]),
     ("spmh_sources", C.c_int, [i32, i32, u64, P_i32]),

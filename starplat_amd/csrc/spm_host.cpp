@@ -258,6 +258,47 @@ spmh_graph* spmh_parse_text(const char* path) {
   return spmh_from_edges((int64_t)s.size(), s.data(), d.data(), w.data());
 }
 
+spmh_graph* spmh_union(const spmh_graph* const* gs, int32_t n, int64_t* bases) {
+  if (n < 1 || !gs) return nullptr;
+  int64_t V = 0, E = 0;
+  std::vector<int64_t> vb(n + 1), eb(n + 1);
+  for (int32_t i = 0; i < n; ++i) {
+    vb[i] = V;
+    eb[i] = E;
+    V += gs[i]->V;
+    E += gs[i]->E;
+  }
+  vb[n] = V;
+  eb[n] = E;
+  if (V > 0x7fffffff || E > 0x7fffffff) return nullptr;
+  spmh_graph* g = static_cast<spmh_graph*>(std::calloc(1, sizeof(spmh_graph)));
+  g->V = int32_t(V);
+  g->E = E;
+  g->indexofNodes = alloc_i32(V + 1);
+  g->edgeList = alloc_i32(E);
+  g->edgeLen = alloc_i32(E);
+  g->rev_indexofNodes = alloc_i32(V + 1);
+  g->srcList = alloc_i32(E);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int32_t i = 0; i < n; ++i) {
+    const spmh_graph* s = gs[i];
+    const int32_t vo = int32_t(vb[i]), eo = int32_t(eb[i]);
+    for (int32_t v = 0; v < s->V; ++v) {
+      g->indexofNodes[vo + v] = s->indexofNodes[v] + eo;
+      g->rev_indexofNodes[vo + v] = s->rev_indexofNodes[v] + eo;
+    }
+    for (int64_t e = 0; e < s->E; ++e) {
+      g->edgeList[eo + e] = s->edgeList[e] + vo;
+      g->srcList[eo + e] = s->srcList[e] + vo;
+      g->edgeLen[eo + e] = s->edgeLen[e];
+    }
+  }
+  g->indexofNodes[V] = int32_t(E);
+  g->rev_indexofNodes[V] = int32_t(E);
+  if (bases) std::copy(vb.begin(), vb.end(), bases);
+  return g;
+}
+
 int spmh_write_text(const spmh_graph* g, const char* path, int weights) {
   FILE* f = std::fopen(path, "w");
   if (!f) return -1;

@@ -24,8 +24,10 @@
 //
 // Distances are exact shortest paths, so the result is bit-identical to any
 // correct schedule of the reference's fixed point.
+#include <algorithm>
 #include <climits>
 #include <cstring>
+#include <vector>
 
 #include "spm_internal.hpp"
 
@@ -49,35 +51,22 @@ struct SsspWork {
   uint32_t* round_base = nullptr;     // [1]
   int32_t* dist = nullptr;            // [V] working distances
   uint8_t* active = nullptr;          // [2][V] dense-mode frontier flags
-  size_t qcap = 0, ccap = 0;
+  int32_t* srcbuf = nullptr;          // multi-source seeds
+  size_t qcap = 0, ccap = 0, src_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int rounds_per_graph = 0;
   uint32_t host_base = 1;
 };
 
-__global__ void sssp_init(int32_t V, int32_t* __restrict__ dist, int32_t src,
-                          const int32_t* __restrict__ off, SsspQueue q0,
-                          unsigned long long* __restrict__ ctr, uint8_t* __restrict__ active) {
+__global__ void sssp_init(int32_t V, int32_t* __restrict__ dist, uint8_t* __restrict__ active) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  const int32_t deg = off[src + 1] - off[src];
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < V; i += stride) {
-    dist[i] = i == src ? 0 : INT_MAX;
+    dist[i] = INT_MAX;
     if (active) {
-      active[i] = i == src;
+      active[i] = 0;
       active[V + i] = 0;
     }
-  }
-  // Every chunk of the source's edges starts inside entry 0.
-  const int64_t nch = (int64_t(deg) + kChunk - 1) / kChunk;
-  for (int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < nch; c += stride) q0.cfirst[c] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    q0.vtx[0] = src;
-    q0.eoff[0] = 0;
-    ctr[0] = deg > 0 ? ((1ull << 32) | uint32_t(deg)) : 0ull;
-    ctr[1] = 0;
-    ctr[2] = 0;
-    ctr[3] = 0;
   }
 }
 
@@ -105,6 +94,47 @@ __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea
   }
   ea = pa + ia - a;
   eb = pb + ib - b;
+}
+
+// Seeds the first frontier with n distinct sources (one workgroup): dist 0,
+// queue entries with their degree prefix, and the chunk index of every chunk
+// starting inside a source's edges. srcs == nullptr means the single source
+// `one`.
+__global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restrict__ srcs, int32_t n, int32_t one,
+                                                        const int32_t* __restrict__ off, int32_t* __restrict__ dist,
+                                                        SsspQueue q0, unsigned long long* __restrict__ ctr,
+                                                        uint8_t* __restrict__ active) {
+  __shared__ uint32_t s_tot[2 * kSsspWaves];
+  uint32_t cnt = 0, edges = 0;  // running totals (uniform across the block)
+  for (int32_t base = 0; base < n; base += kSsspBlock) {
+    const int32_t j = base + int32_t(threadIdx.x);
+    int32_t s = -1, deg = 0;
+    if (j < n) {
+      s = srcs ? srcs[j] : one;
+      deg = off[s + 1] - off[s];
+      dist[s] = 0;
+      if (active) active[s] = 1;
+    }
+    const uint32_t push = deg > 0 ? 1u : 0u;
+    uint32_t ep, ed, tp, td;
+    block_scan2(push, uint32_t(deg), ep, ed, tp, td, s_tot);
+    if (push) {
+      const uint32_t pos = cnt + ep, eo = edges + ed;
+      q0.vtx[pos] = s;
+      q0.eoff[pos] = eo;
+      for (uint32_t cc = (eo + kChunk - 1) / kChunk; cc <= (eo + uint32_t(deg) - 1) / kChunk; ++cc)
+        q0.cfirst[cc] = int32_t(pos);
+    }
+    cnt += tp;
+    edges += td;
+    __syncthreads();  // s_tot reuse
+  }
+  if (threadIdx.x == 0) {
+    ctr[0] = (static_cast<unsigned long long>(cnt) << 32) | edges;
+    ctr[1] = 0;
+    ctr[2] = 0;
+    ctr[3] = 0;
+  }
 }
 
 __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
@@ -299,6 +329,7 @@ int sssp_release(spm_graph* g) {
   dfree(g, w->round_base, sizeof(uint32_t));
   dfree(g, w->dist, sizeof(int32_t) * g->V);
   dfree(g, w->active, 2 * size_t(g->V));
+  dfree(g, w->srcbuf, sizeof(int32_t) * w->src_cap);
   delete w;
   g->sssp = nullptr;
   return SPM_OK;
@@ -349,9 +380,12 @@ static int sssp_capture(spm_graph* g) {
   return SPM_OK;
 }
 
-int sssp_run(spm_graph* g, int32_t src, int32_t* d_out, int32_t* rounds_out) {
+// srcs: n distinct sources in host memory (n == 1 passes the value directly).
+int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32_t* rounds_out) {
   if (!g->w) return set_error(SPM_ENOGRAPH, "SSSP needs edge weights (edgeLen or spm_graph_set_weights)");
-  if (src < 0 || src >= g->V) return set_error(SPM_EINVAL, "SSSP source out of range");
+  if (n < 1 || !srcs) return set_error(SPM_EINVAL, "SSSP needs at least one source");
+  for (int32_t i = 0; i < n; ++i)
+    if (srcs[i] < 0 || srcs[i] >= g->V) return set_error(SPM_EINVAL, "SSSP source out of range");
   SPM_TRY(sssp_alloc(g));
   SsspWork* w = g->sssp;
   const int R = g->sched.sssp_rounds;
@@ -361,9 +395,21 @@ int sssp_run(spm_graph* g, int32_t src, int32_t* d_out, int32_t* rounds_out) {
     SPM_CHECK_HIP(hipMemsetAsync(w->stamp, 0, sizeof(uint32_t) * g->V, g->stream));
     SPM_CHECK_HIP(hipMemcpyAsync(w->round_base, &w->host_base, sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
   }
+  if (n > 1) {
+    if (size_t(n) > w->src_cap) {
+      dfree(g, w->srcbuf, sizeof(int32_t) * w->src_cap);
+      w->srcbuf = nullptr;
+      w->src_cap = 0;
+      SPM_TRY(dmalloc(g, (void**)&w->srcbuf, sizeof(int32_t) * size_t(n), "sssp sources"));
+      w->src_cap = size_t(n);
+    }
+    SPM_CHECK_HIP(hipMemcpyAsync(w->srcbuf, srcs, sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, g->stream));
+  }
   const int igrid = int(std::min<int64_t>((g->V + 255) / 256, 4096));
-  SPM_TRY(launch(g, "sssp_init", sssp_init, dim3(igrid), dim3(256), 0, g->V, w->dist, src,
-                 (const int32_t*)g->off, w->q[0], w->ctr, dense ? w->active : (uint8_t*)nullptr));
+  uint8_t* act = dense ? w->active : (uint8_t*)nullptr;
+  SPM_TRY(launch(g, "sssp_init", sssp_init, dim3(igrid), dim3(256), 0, g->V, w->dist, act));
+  SPM_TRY(launch(g, "sssp_seed", sssp_seed, dim3(1), dim3(kSsspBlock), 0, n > 1 ? (const int32_t*)w->srcbuf : nullptr,
+                 n, srcs[0], (const int32_t*)g->off, w->dist, w->q[0], w->ctr, act));
   const bool use_graph = !g->prof.on;
   if (use_graph) SPM_TRY(sssp_capture(g));
   unsigned long long* h = reinterpret_cast<unsigned long long*>(g->h_pinned);
@@ -394,13 +440,21 @@ extern "C" {
 
 int spm_sssp_device(spm_graph* g, int32_t src, int32_t* d_dist, int32_t* rounds) {
   if (!g || !d_dist) return set_error(SPM_EINVAL, "spm_sssp_device: NULL argument");
-  return sssp_run(g, src, d_dist, rounds);
+  return sssp_run(g, &src, 1, d_dist, rounds);
+}
+
+int spm_sssp_multi_device(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_dist, int32_t* rounds) {
+  if (!g || !d_dist || !srcs || n < 1) return set_error(SPM_EINVAL, "spm_sssp_multi_device: bad argument");
+  std::vector<int32_t> s(srcs, srcs + n);
+  std::sort(s.begin(), s.end());
+  s.erase(std::unique(s.begin(), s.end()), s.end());
+  return sssp_run(g, s.data(), int32_t(s.size()), d_dist, rounds);
 }
 
 int spm_compute_sssp(spm_graph* g, int32_t* dist, const int32_t* weight, int32_t src) {
   if (!g || !dist) return set_error(SPM_EINVAL, "spm_compute_sssp: NULL argument");
   if (weight) SPM_TRY(spm_graph_set_weights(g, weight));
-  SPM_TRY(sssp_run(g, src, nullptr, nullptr));
+  SPM_TRY(sssp_run(g, &src, 1, nullptr, nullptr));
   SPM_CHECK_HIP(hipMemcpyAsync(dist, g->sssp->dist, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   return SPM_OK;

@@ -128,6 +128,13 @@ class DeviceGraph:
         N.check("spm_sssp_device", N.device_lib().spm_sssp_device(self.h, src, d_out.ptr, C.byref(r)))
         return r.value
 
+    def sssp_multi_device(self, sources, d_out):
+        s = np.ascontiguousarray(sources, np.int32)
+        r = C.c_int32(0)
+        N.check("spm_sssp_multi_device", N.device_lib().spm_sssp_multi_device(
+            self.h, N.ptr(s, C.c_int32), len(s), d_out.ptr, C.byref(r)))
+        return r.value
+
     def bc_device(self, sources, d_out):
         s = np.ascontiguousarray(sources, np.int32)
         N.check("spm_bc_device", N.device_lib().spm_bc_device(self.h, N.ptr(s, C.c_int32), len(s), d_out.ptr))

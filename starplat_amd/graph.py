@@ -91,6 +91,14 @@ class graph:
         return cls()._adopt(N.host_lib().spmh_from_edges(
             len(src), N.ptr(src, C.c_int32), N.ptr(dst, C.c_int32), N.ptr(w, C.c_int32)))
 
+    @classmethod
+    def union(cls, graphs):
+        """Disjoint union; returns (graph, bases) with graph i's vertex v at bases[i] + v."""
+        arr = (N.P_spmh * len(graphs))(*[g._h for g in graphs])
+        bases = np.zeros(len(graphs) + 1, np.int64)
+        u = cls()._adopt(N.host_lib().spmh_union(arr, len(graphs), N.ptr(bases, C.c_int64)))
+        return u, bases
+
     def write_text(self, path, weights=True):
         rc = N.host_lib().spmh_write_text(self._h, str(path).encode(), int(bool(weights)))
         if rc != 0:

@@ -267,3 +267,25 @@ def test_full_bc_er4m_subset(oracle):
     sub = src[:16]
     ref, _, _ = oracle.compute_bc(g, sub)
     assert rel_linf(g.device().bc(sub), ref) <= BC_TOL
+
+
+# ------------------------------------------------------------------ batched SSSP
+def test_sssp_multi_source_and_batch(oracle):
+    from starplat_amd import hip
+    from starplat_amd.batch import SsspBatch
+
+    gs = [graph.rmat(s, 8, seed=s, weights=(1, 100)) for s in (6, 9, 11)]
+    gs += [graph.grid2d(30, 40, weights=(1, 100)), graph.erdos_renyi(3000, 9000, seed=2, undirected=False, weights=(1, 100))]
+    srcs = [0, 7, 100, 0, 2999]
+    b = SsspBatch(list(zip(gs, srcs)))
+    b.run()
+    for g, s, got in zip(gs, srcs, b.results()):
+        np.testing.assert_array_equal(got, oracle.compute_sssp(g, g.edgeLen, s))
+    b.close()
+    # several sources in one graph: distance to the nearest source
+    g = graph.rmat(12, 8, seed=3, weights=(1, 100))
+    src = np.array([0, 5, 5, 4000], np.int32)  # duplicate ignored
+    out = hip.DeviceArray(g.V, np.int32)
+    g.device().sssp_multi_device(src, out)
+    ref = np.min(np.stack([oracle.compute_sssp(g, g.edgeLen, int(s)) for s in set(src.tolist())]), axis=0)
+    np.testing.assert_array_equal(out.to_host(), ref)

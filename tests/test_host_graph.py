@@ -140,3 +140,17 @@ def test_sources_distinct_sorted():
     np.testing.assert_array_equal(s, sources(1000, 256, seed=4))
     with pytest.raises(ValueError):
         sources(10, 11, seed=1)
+
+
+def test_union_is_block_diagonal():
+    gs = [graph.rmat(8, 8, seed=i, weights=(1, 100)) for i in range(3)] + [graph.grid2d(5, 7, weights=(1, 9))]
+    u, bases = graph.union(gs)
+    check_csr(u)
+    assert list(np.diff(bases)) == [g.V for g in gs] and u.E == sum(g.E for g in gs)
+    for i, g in enumerate(gs):
+        lo, hi = bases[i], bases[i + 1]
+        rows = u.indexofNodes[lo:hi + 1] - u.indexofNodes[lo]
+        np.testing.assert_array_equal(rows, g.indexofNodes)
+        seg = slice(u.indexofNodes[lo], u.indexofNodes[hi])
+        np.testing.assert_array_equal(u.edgeList[seg] - lo, g.edgeList)
+        np.testing.assert_array_equal(u.edgeLen[seg], g.edgeLen)

@@ -13,10 +13,12 @@
 // reads one float per in-edge. Vertices are binned once by in-degree and the
 // workgroups of one launch are split over the bins: one lane per vertex for
 // short rows, a group of G lanes, a 64-lane wavefront, or a 256-lane
-// workgroup for hubs. The diff reduction ends in the last workgroup to
-// finish (a completion counter), which also evaluates the loop condition and
+// workgroup for hubs. Each workgroup stores its share of the diff; a
+// one-workgroup check kernel sums the shares, evaluates the loop condition and
 // raises a device flag; later iterations see the flag and return at once, so
-// a whole run is captured into one hipGraph with no host round trip.
+// a whole run is captured into one hipGraph with no host round trip. (No
+// workgroup updates a shared counter: same-address device-scope atomics from
+// 10^5 workgroups cost milliseconds per iteration.)
 //
 // Arithmetic follows the emitted code: the per-term division pr[u]/outdeg(u)
 // is the same IEEE single division, and the update is evaluated without FMA
@@ -33,8 +35,6 @@ namespace spm {
 constexpr int kPrBlock = 256;
 
 struct PrState {
-  float diff[2];
-  unsigned int blocks_done;
   int done;
   int iter;
   int maxIter;
@@ -54,6 +54,8 @@ struct PrWork {
   float* c[2] = {nullptr, nullptr};
   float* degf = nullptr;
   float* acc = nullptr;      // push-mode accumulators
+  float* partial = nullptr;  // per-workgroup diff sums of one iteration
+  int partial_n = 0;
   int32_t* order = nullptr;  // vertices grouped by bin
   PrState* st = nullptr;
   PrBins bins{};
@@ -74,9 +76,6 @@ __global__ void pr_init(int32_t V, const int32_t* __restrict__ off, float* __res
     acc[v] = 0.f;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->diff[0] = 0.f;
-    st->diff[1] = 0.f;
-    st->blocks_done = 0;
     st->done = 0;
     st->iter = 0;
     st->maxIter = maxIter;
@@ -98,12 +97,11 @@ __device__ __forceinline__ void pr_store(int32_t v, float sum, const PrState& s,
   mydiff += nv - old >= 0.f ? nv - old : old - nv;
 }
 
-// End of an iteration: workgroup sum of |new - old|, one atomic per
-// workgroup, and the loop test (diff > beta && iter < maxIter) evaluated by the
-// last workgroup to finish, which raises the done flag and resets the
-// accumulators for the iteration after next.
-__device__ __forceinline__ void pr_finish(float mydiff, PrState* __restrict__ st, const PrState& s, int parity,
-                                          float* s_red, int* s_last) {
+// End of an iteration inside a workgroup: the sum of |new - old| over its
+// vertices goes to partial[blockIdx.x] with a plain store. No workgroup
+// touches a shared counter: with ~10^5 workgroups per iteration at RMAT-22,
+// same-address device-scope atomics would serialise the whole iteration.
+__device__ __forceinline__ void pr_finish(float mydiff, float* __restrict__ partial, float* s_red) {
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   mydiff = wave_sum(mydiff);
   if (lane == 0) s_red[wid] = mydiff;
@@ -111,20 +109,30 @@ __device__ __forceinline__ void pr_finish(float mydiff, PrState* __restrict__ st
   if (tid == 0) {
     float t = 0.f;
     for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
-    atomicAdd(&st->diff[parity], t);
-    __threadfence();
-    const unsigned int prev = atomicAdd(&st->blocks_done, 1u);
-    *s_last = prev == gridDim.x - 1;
+    partial[blockIdx.x] = t;
   }
+}
+
+// The loop test (diff > beta && iter < maxIter), one workgroup: sums the
+// per-workgroup partials in a fixed order, advances the iteration count and
+// raises the done flag that makes later iterations return at once.
+constexpr int kCheckBlock = 1024;
+__global__ __launch_bounds__(kCheckBlock) void pr_check(const float* __restrict__ partial, int n,
+                                                        PrState* __restrict__ st) {
+  __shared__ float s_red[kCheckBlock / kWave];
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  float t = 0.f;
+  for (int i = tid; i < n; i += kCheckBlock) t += partial[i];
+  t = wave_sum(t);
+  if (lane == 0) s_red[wid] = t;
   __syncthreads();
-  if (*s_last && tid == 0) {
-    __threadfence();
-    const float d = __hip_atomic_load(&st->diff[parity], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid == 0) {
+    float d = 0.f;
+    for (int w = 0; w < kCheckBlock / kWave; ++w) d += s_red[w];
     const int it = st->iter + 1;
     st->iter = it;
-    st->diff[parity] = 0.f;
-    st->blocks_done = 0;
-    if (!(d > s.beta && it < s.maxIter)) __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!(d > st->beta && it < st->maxIter)) st->done = 1;
   }
 }
 
@@ -133,9 +141,8 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     const int32_t* __restrict__ roff, const int32_t* __restrict__ radj,
     const int32_t* __restrict__ order, const float* __restrict__ degf, PrBins bins,
     const float* __restrict__ pr_in, const float* __restrict__ c_in, float* __restrict__ pr_out,
-    float* __restrict__ c_out, PrState* __restrict__ st, int parity) {
+    float* __restrict__ c_out, const PrState* __restrict__ st, float* __restrict__ partial) {
   __shared__ float s_red[kPrBlock / kWave];
-  __shared__ int s_last;
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const PrState s = *st;
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
@@ -190,7 +197,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     __syncthreads();
   }
 
-  pr_finish(mydiff, st, s, parity, s_red, &s_last);
+  pr_finish(mydiff, partial, s_red);
 }
 
 // Push iteration, step 1: every out-edge (u,v) adds u's contribution to
@@ -242,9 +249,9 @@ __global__ __launch_bounds__(kPrBlock) void pr_push_scatter(int32_t V, const int
 __global__ __launch_bounds__(kPrBlock) void pr_push_apply(int32_t V, const float* __restrict__ degf,
                                                           float* __restrict__ acc, const float* __restrict__ pr_in,
                                                           float* __restrict__ pr_out, float* __restrict__ c_out,
-                                                          PrState* __restrict__ st, int parity) {
+                                                          const PrState* __restrict__ st,
+                                                          float* __restrict__ partial) {
   __shared__ float s_red[kPrBlock / kWave];
-  __shared__ int s_last;
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const PrState s = *st;
   float mydiff = 0.f;
@@ -254,7 +261,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_push_apply(int32_t V, const float
     acc[v] = 0.f;
     pr_store(int32_t(v), sum, s, pr_in, pr_out, c_out, degf, mydiff);
   }
-  pr_finish(mydiff, st, s, parity, s_red, &s_last);
+  pr_finish(mydiff, partial, s_red);
 }
 
 static int pr_prepare(spm_graph* g) {
@@ -295,6 +302,8 @@ static int pr_prepare(spm_graph* g) {
   b.blk[2] = b.blk[1] + (cnt[1] + kPrBlock / s.pr_group - 1) / (kPrBlock / s.pr_group);
   b.blk[3] = b.blk[2] + (cnt[2] + kPrBlock / kWave - 1) / (kPrBlock / kWave);
   b.blk[4] = b.blk[3] + cnt[3];
+  w->partial_n = std::max(b.blk[4], g->num_cus * 8) + 1;
+  SPM_TRY(dmalloc(g, (void**)&w->partial, sizeof(float) * size_t(w->partial_n), "pr partial diffs"));
   return SPM_OK;
 }
 
@@ -309,6 +318,7 @@ int pr_release(spm_graph* g) {
   }
   dfree(g, w->degf, sizeof(float) * g->V);
   dfree(g, w->acc, sizeof(float) * g->V);
+  dfree(g, w->partial, sizeof(float) * size_t(w->partial_n));
   dfree(g, w->order, sizeof(int32_t) * g->V);
   dfree(g, w->st, sizeof(PrState));
   delete w;
@@ -328,14 +338,16 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
       SPM_TRY(launch(g, "pr_push_scatter", pr_push_scatter, dim3(pgrid), block, 0, g->V, (const int32_t*)g->off,
                      (const int32_t*)g->adj, cin, w->acc, (const PrState*)w->st));
       SPM_TRY(launch(g, "pr_push_apply", pr_push_apply, dim3(pgrid), block, 0, g->V, (const float*)w->degf, w->acc,
-                     pin, w->pr[1 - p], w->c[1 - p], w->st, p));
+                     pin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st, w->partial));
+      SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial, pgrid, w->st));
       continue;
     }
 #define SPM_PR_LAUNCH(G)                                                                          \
   case G:                                                                                         \
     SPM_TRY(launch(g, "pr_iter", pr_iter<G>, grid, block, 0, (const int32_t*)g->roff,            \
                    (const int32_t*)g->radj, (const int32_t*)w->order, (const float*)w->degf,       \
-                   w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], w->st, p));                        \
+                   w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st,             \
+                   w->partial));                                                                   \
     break;
     switch (w->bins.group) {
       SPM_PR_LAUNCH(2)
@@ -346,6 +358,8 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
       default: return set_error(SPM_EINVAL, "pr: bad lane group");
     }
 #undef SPM_PR_LAUNCH
+    SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial,
+                   int(grid.x), w->st));
   }
   return SPM_OK;
 }

@@ -13,8 +13,17 @@
 // vertex records, for every chunk that starts inside its edge range, the
 // queue position that chunk starts at. A workgroup then stages its chunk's
 // vertices (at most kChunk+1) in LDS and every lane locates its edge by a
-// binary search in LDS. Hubs of 10^5 edges are spread over 100 workgroups,
-// and every relaxation lane does useful work.
+// binary search in LDS (lanes take consecutive edges, so loads stay
+// coalesced). Hubs of 10^5 edges are spread over ~50 workgroups, and every
+// relaxation lane does useful work. Frontier distances are read with
+// device-scope loads (a stale one could lose an improvement); the per-edge
+// test against the neighbour's distance is a cached pre-filter in front of
+// the authoritative atomicMin.
+//
+// Large rounds (frontier over kSparseEdges edges) do not touch a shared
+// counter per pushed vertex: relaxation only stamps improved vertices, and
+// two tiled kernels compact the stamps into the next queue in ascending
+// vertex order (each tile sums the counts of the tiles before it).
 //
 // Three queues rotate (read k, write k+1, clear k+2), so a round is one
 // kernel and R rounds are captured into one hipGraph; the host reads one
@@ -34,9 +43,13 @@
 namespace spm {
 
 constexpr int kSsspBlock = 256;
-constexpr int kChunk = 1024;
+constexpr int kChunk = 2048;
 constexpr int kPer = kChunk / kSsspBlock;
 constexpr int kSsspWaves = kSsspBlock / kWave;
+// A round whose frontier has fewer edges pushes through a counter (at most
+// kSparseEdges / kChunk reservations); larger rounds are compacted by tiles.
+constexpr uint32_t kSparseEdges = 1u << 19;
+constexpr int kTileV = kSsspBlock * 16;  // vertices per compaction tile
 
 struct SsspQueue {
   int32_t* vtx;     // [V] frontier vertices (each pushed at most once a round)
@@ -51,6 +64,7 @@ struct SsspWork {
   uint32_t* round_base = nullptr;     // [1]
   int32_t* dist = nullptr;            // [V] working distances
   uint8_t* active = nullptr;          // [2][V] dense-mode frontier flags
+  uint2* tile_sum = nullptr;          // [tiles] (vertices, edges) per compaction tile
   int32_t* srcbuf = nullptr;          // multi-source seeds
   size_t qcap = 0, ccap = 0, src_cap = 0;
   hipGraph_t graph = nullptr;
@@ -159,6 +173,9 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
   const uint32_t sv = *round_base + rloc;
   const uint32_t nchunks = (T + kChunk - 1) / kChunk;
   const int tid = threadIdx.x;
+  // Small rounds push through the block-aggregated counter; large rounds
+  // only mark improved vertices and leave the queue to sssp_count/sssp_emit.
+  const bool sparse = T < kSparseEdges;
 
   for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const int32_t p0 = qin.cfirst[chunk];
@@ -177,14 +194,20 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
     int32_t pv[kPer];
     int32_t pd[kPer];
     uint32_t pc = 0, sd = 0;
+    // Lanes take consecutive edges (edge x0 + k*256 + tid in step k), so each
+    // load instruction of a wavefront reads one contiguous run of the
+    // frontier's edges. The entry owning edge x is found by binary search in
+    // LDS; entries only move forward with k, so each search starts at the
+    // previous step's entry.
     const uint32_t x0 = chunk * kChunk;
+    int lo = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const uint32_t x = x0 + k * kSsspBlock + tid;
       pv[k] = -1;
       pd[k] = 0;
       if (x < T) {
-        int lo = 0, hi = cnt - 1;
+        int hi = min(cnt - 1, lo + kSsspBlock);
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
           if (s_off[mid] <= x) lo = mid;
@@ -193,19 +216,30 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
         const int32_t e = s_base[lo] + int32_t(x - s_off[lo]);
         const int32_t nb = adj[e];
         const int32_t nd = s_dv[lo] + wgt[e];
-        if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        // A cached (possibly stale) read is enough to skip hopeless updates:
+        // distances only decrease, so a stale value is never below the true
+        // one and atomicMin below stays the authority.
+        if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
           const int32_t old = atomicMin(&dist[nb], nd);
-          if (nd < old && atomicExch(&stamp[nb], sv) != sv) {
-            const int32_t deg = off[nb + 1] - off[nb];
-            if (deg > 0) {
-              pv[k] = nb;
-              pd[k] = deg;
-              ++pc;
-              sd += uint32_t(deg);
+          if (nd < old) {
+            if (!sparse) {
+              stamp[nb] = sv;  // idempotent mark, compacted later
+            } else if (atomicExch(&stamp[nb], sv) != sv) {
+              const int32_t deg = off[nb + 1] - off[nb];
+              if (deg > 0) {
+                pv[k] = nb;
+                pd[k] = deg;
+                ++pc;
+                sd += uint32_t(deg);
+              }
             }
           }
         }
       }
+    }
+    if (!sparse) {
+      __syncthreads();  // LDS reuse by the next chunk
+      continue;
     }
 
     uint32_t epc, esd, tpc, tsd;
@@ -229,6 +263,102 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
     }
     __syncthreads();  // LDS reuse by the next chunk
   }
+}
+
+// Frontier compaction for large rounds, step 1: each tile of kTileV vertices
+// (16 consecutive per lane) counts the vertices marked in this round that have
+// out-edges, and their out-degrees.
+__device__ __forceinline__ void tile_flags(int32_t V, const int32_t* __restrict__ off,
+                                           const uint32_t* __restrict__ stamp, uint32_t sv, int64_t v0,
+                                           uint32_t (&deg)[16]) {
+  if (v0 + 16 <= V) {
+    const uint4* p = reinterpret_cast<const uint4*>(stamp + v0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint4 s = p[q];
+      deg[4 * q + 0] = s.x == sv;
+      deg[4 * q + 1] = s.y == sv;
+      deg[4 * q + 2] = s.z == sv;
+      deg[4 * q + 3] = s.w == sv;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) deg[k] = (v0 + k < V) && stamp[v0 + k] == sv;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    if (deg[k]) deg[k] = uint32_t(off[v0 + k + 1] - off[v0 + k]);
+}
+
+__global__ __launch_bounds__(kSsspBlock) void sssp_count(int32_t V, const int32_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ stamp,
+                                                         const unsigned long long* __restrict__ ctr_in,
+                                                         const uint32_t* __restrict__ round_base, uint32_t rloc,
+                                                         uint2* __restrict__ tile_sum) {
+  __shared__ uint32_t s_tot[2 * kSsspWaves];
+  const uint32_t T = uint32_t(*ctr_in);
+  if (T < kSparseEdges) return;  // this round pushed through the counter
+  const uint32_t sv = *round_base + rloc;
+  uint32_t deg[16];
+  tile_flags(V, off, stamp, sv, int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16, deg);
+  uint32_t c = 0, e = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    c += deg[k] > 0;
+    e += deg[k];
+  }
+  uint32_t ec, ee, tc, te;
+  block_scan2(c, e, ec, ee, tc, te, s_tot);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = make_uint2(tc, te);
+}
+
+// Step 2: every tile sums the counts of the tiles before it (no atomics, no
+// waiting on other workgroups), then writes its vertices in ascending order
+// with their degree prefix and the chunk index; the last tile publishes the
+// frontier size.
+__global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t* __restrict__ off,
+                                                        const uint32_t* __restrict__ stamp,
+                                                        const unsigned long long* __restrict__ ctr_in,
+                                                        const uint32_t* __restrict__ round_base, uint32_t rloc,
+                                                        const uint2* __restrict__ tile_sum, SsspQueue qout,
+                                                        unsigned long long* __restrict__ ctr_out) {
+  __shared__ uint32_t s_tot[2 * kSsspWaves];
+  const uint32_t T = uint32_t(*ctr_in);
+  if (T < kSparseEdges) return;
+  const uint32_t sv = *round_base + rloc;
+  uint32_t pc = 0, pe = 0;
+  for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kSsspBlock) {
+    const uint2 x = tile_sum[t];
+    pc += x.x;
+    pe += x.y;
+  }
+  uint32_t d0, d1, bc, be;
+  block_scan2(pc, pe, d0, d1, bc, be, s_tot);  // bc, be: sums over the earlier tiles
+  __syncthreads();
+  const int64_t v0 = int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
+  uint32_t deg[16];
+  tile_flags(V, off, stamp, sv, v0, deg);
+  uint32_t c = 0, e = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    c += deg[k] > 0;
+    e += deg[k];
+  }
+  uint32_t ec, ee, tc, te;
+  block_scan2(c, e, ec, ee, tc, te, s_tot);
+  uint32_t pos = bc + ec, eo = be + ee;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t d = deg[k];
+    if (!d) continue;
+    qout.vtx[pos] = int32_t(v0 + k);
+    qout.eoff[pos] = eo;
+    for (uint32_t cc = (eo + kChunk - 1) / kChunk; cc <= (eo + d - 1) / kChunk; ++cc) qout.cfirst[cc] = int32_t(pos);
+    ++pos;
+    eo += d;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    *ctr_out = (static_cast<unsigned long long>(bc + tc) << 32) | (be + te);
 }
 
 // Dense mode: the reference's full-V sweep of the modified flags, one lane per
@@ -278,7 +408,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_dense(
       const int32_t e = s_base[wid][lo] + int32_t(x - s_pre[wid][lo]);
       const int32_t nb = adj[e];
       const int32_t nd = s_dv[wid][lo] + wgt[e];
-      if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      // cached pre-filter; atomicMin decides (see sssp_relax)
+      if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
         const int32_t old = atomicMin(&dist[nb], nd);
         if (nd < old) {
           nxt[nb] = 1;
@@ -288,7 +419,10 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_dense(
     }
     __builtin_amdgcn_wave_barrier();
   }
-  if (__any(found) && lane == 0) atomicOr(ctr_out, 1ull);
+  // Read before the atomic: once one wavefront has raised the flag the others
+  // skip the same-address atomic.
+  if (__any(found) && lane == 0 && __hip_atomic_load(ctr_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull)
+    atomicOr(ctr_out, 1ull);
 }
 
 static int sssp_alloc(spm_graph* g) {
@@ -308,6 +442,7 @@ static int sssp_alloc(spm_graph* g) {
   SPM_TRY(dmalloc(g, (void**)&w->round_base, sizeof(uint32_t), "sssp round base"));
   SPM_TRY(dmalloc(g, (void**)&w->dist, sizeof(int32_t) * V, "sssp dist"));
   SPM_TRY(dmalloc(g, (void**)&w->active, 2 * size_t(V), "sssp flags"));
+  SPM_TRY(dmalloc(g, (void**)&w->tile_sum, sizeof(uint2) * ((size_t(V) + kTileV - 1) / kTileV), "sssp tiles"));
   SPM_CHECK_HIP(hipMemsetAsync(w->stamp, 0, sizeof(uint32_t) * V, g->stream));
   SPM_CHECK_HIP(hipMemcpyAsync(w->round_base, &w->host_base, sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
@@ -329,6 +464,7 @@ int sssp_release(spm_graph* g) {
   dfree(g, w->round_base, sizeof(uint32_t));
   dfree(g, w->dist, sizeof(int32_t) * g->V);
   dfree(g, w->active, 2 * size_t(g->V));
+  dfree(g, w->tile_sum, sizeof(uint2) * ((size_t(g->V) + kTileV - 1) / kTileV));
   dfree(g, w->srcbuf, sizeof(int32_t) * w->src_cap);
   delete w;
   g->sssp = nullptr;
@@ -356,6 +492,14 @@ static int sssp_enqueue_rounds(spm_graph* g, int R) {
                      (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)g->w, w->dist,
                      w->stamp, w->q[i], w->q[o], w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3,
                      (const uint32_t*)w->round_base, uint32_t(r)));
+      const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
+      SPM_TRY(launch(g, "sssp_count", sssp_count, dim3(tiles), dim3(kSsspBlock), 0, g->V, (const int32_t*)g->off,
+                     (const uint32_t*)w->stamp, (const unsigned long long*)(w->ctr + i),
+                     (const uint32_t*)w->round_base, uint32_t(r), w->tile_sum));
+      SPM_TRY(launch(g, "sssp_emit", sssp_emit, dim3(tiles), dim3(kSsspBlock), 0, g->V, (const int32_t*)g->off,
+                     (const uint32_t*)w->stamp, (const unsigned long long*)(w->ctr + i),
+                     (const uint32_t*)w->round_base, uint32_t(r), (const uint2*)w->tile_sum, w->q[o],
+                     w->ctr + o));
     }
   }
   SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, uint32_t(R)));

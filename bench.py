@@ -211,18 +211,36 @@ def bench_bc(args, O):
     return res
 
 
+# --------------------------------------------------------------------------- schedules
+def bench_schedule(args):
+    from starplat_amd import mixed_batch
+
+    log(f"mixed batch: {args.sched_graphs} graphs ({args.sched_held_out} held out), autotune + train")
+    res = mixed_batch.run(n=args.sched_graphs, held_out=args.sched_held_out, seed=5, reps=2, epochs=400, log=log,
+                          rmat_scales=tuple(args.sched_rmat), er_log2=tuple(args.sched_er), grid_sides=tuple(args.sched_grid))
+    res["workload"] = ("mixed batch: RMAT (EF 4-32), Erdos-Renyi (avg degree 4-16) and 2-D grids, each with one SSSP "
+                       "(weights U[1,100], src 0) or PageRank job; 8 schedule classes timed per job; MLP 16-64-64-8 "
+                       "trained on the GPU on the fastest class; makespan = sum of held-out job times")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--algos", default="sssp,pr,tc,bc")
+    ap.add_argument("--algos", default="sssp,pr,tc,bc,sched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--sssp-scale", type=int, default=22)
     ap.add_argument("--tc-scale", type=int, default=21)
     ap.add_argument("--bc-n", type=int, default=4194304)
     ap.add_argument("--bc-sources", type=int, default=256)
+    ap.add_argument("--sched-graphs", type=int, default=150)
+    ap.add_argument("--sched-held-out", type=int, default=50)
+    ap.add_argument("--sched-rmat", type=int, nargs=2, default=[10, 16])
+    ap.add_argument("--sched-er", type=int, nargs=2, default=[10, 16])
+    ap.add_argument("--sched-grid", type=int, nargs=2, default=[32, 256])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -247,10 +265,14 @@ def main():
         per["tc"] = bench_tc(args, O)
     if "bc" in algos:
         per["bc"] = bench_bc(args, O)
-    if head is None:
+    sched = bench_schedule(args) if "sched" in algos else None
+    if head is None and per:
         first = next(iter(per.values()))
         head = {"workload": first["workload"], "gteps": first["gteps"], "ms_per_step": first.get("ms_per_run"),
                 "roofline": first["roofline"], "cpu_baseline": first.get("cpu_baseline")}
+    if head is None:  # schedule selection only
+        head = {"workload": sched["workload"] if sched else "none", "gteps": float("nan"), "ms_per_step": None,
+                "roofline": None, "cpu_baseline": None}
     out = {
         "metric": METRIC, "value": round(head["gteps"], 4), "unit": "GTEPS", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_step"],
@@ -263,6 +285,8 @@ def main():
         "host": host_cores(),
         "per_algo": per,
     }
+    if sched is not None:
+        out["schedule_selection"] = sched
     print(json.dumps(out))
 
 

@@ -128,7 +128,9 @@ typedef struct spm_schedule {
   int32_t sssp_grid;      /* workgroups per relaxation launch (0 = auto)     */
   int32_t tc_mode;        /* 0 edge-centric search (the only mode so far)    */
   int32_t pr_mode;        /* 0 pull over in-rows, 1 push (scatter + apply)   */
-  int32_t reserved[7];
+  int32_t pr_seg_log2;    /* pull: sweep sources in segments of 2^k vertices
+                             so gathers stay in L2 (0 = one sweep; 10..30)   */
+  int32_t reserved[6];
 } spm_schedule;
 int spm_get_schedule(spm_graph* g, spm_schedule* s);
 int spm_set_schedule(spm_graph* g, const spm_schedule* s);

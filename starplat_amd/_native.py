@@ -43,7 +43,8 @@ class SpmSchedule(C.Structure):
         ("sssp_grid", i32),
         ("tc_mode", i32),
         ("pr_mode", i32),
-        ("reserved", i32 * 7),
+        ("pr_seg_log2", i32),
+        ("reserved", i32 * 6),
     ]
 
     def as_dict(self):

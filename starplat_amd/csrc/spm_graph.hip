@@ -94,7 +94,9 @@ void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
   s->sssp_grid = 0;
   s->tc_mode = 0;
   s->pr_mode = 0;
-  (void)g;
+  // One sweep: measured at RMAT-22, 2 MB source segments double the
+  // iteration time (each pass repeats the per-vertex load chain).
+  s->pr_seg_log2 = 0;
 }
 
 int validate_schedule(const spm_schedule* s) {
@@ -106,6 +108,8 @@ int validate_schedule(const spm_schedule* s) {
   if (s->sssp_grid < 0 || s->sssp_grid > 65536) return set_error(SPM_EINVAL, "schedule: sssp_grid out of range");
   if (s->tc_mode != 0) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0");
   if (s->pr_mode < 0 || s->pr_mode > 1) return set_error(SPM_EINVAL, "schedule: pr_mode must be 0 (pull) or 1 (push)");
+  if (s->pr_seg_log2 != 0 && (s->pr_seg_log2 < 10 || s->pr_seg_log2 > 30))
+    return set_error(SPM_EINVAL, "schedule: pr_seg_log2 must be 0 or in [10, 30]");
   return SPM_OK;
 }
 
@@ -214,7 +218,7 @@ int spm_set_schedule(spm_graph* g, const spm_schedule* s) {
   SPM_TRY(validate_schedule(s));
   const bool pr_changed = s->pr_t1 != g->sched.pr_t1 || s->pr_t2 != g->sched.pr_t2 ||
                           s->pr_t3 != g->sched.pr_t3 || s->pr_group != g->sched.pr_group ||
-                          s->pr_mode != g->sched.pr_mode;
+                          s->pr_mode != g->sched.pr_mode || s->pr_seg_log2 != g->sched.pr_seg_log2;
   const bool sssp_changed = s->sssp_rounds != g->sched.sssp_rounds || s->sssp_grid != g->sched.sssp_grid ||
                             s->sssp_mode != g->sched.sssp_mode;
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));

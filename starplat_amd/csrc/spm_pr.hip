@@ -56,6 +56,9 @@ struct PrWork {
   float* acc = nullptr;      // push-mode accumulators
   float* partial = nullptr;  // per-workgroup diff sums of one iteration
   int partial_n = 0;
+  int32_t* sp = nullptr;     // segment split points [V][S-1]
+  size_t sp_n = 0;
+  int S = 1;                 // source segments per pull sweep
   int32_t* order = nullptr;  // vertices grouped by bin
   PrState* st = nullptr;
   PrBins bins{};
@@ -136,27 +139,68 @@ __global__ __launch_bounds__(kCheckBlock) void pr_check(const float* __restrict_
   }
 }
 
+// Source segmentation (cache blocking). With S > 1 segments, pass `seg` sums
+// only the in-edges whose source lies in [seg*2^k, (seg+1)*2^k); the split
+// points of every in-row (rows are ascending) are precomputed in sp[v][S-1].
+// Passes 0..S-2 accumulate into acc[v]; the last pass adds its own part and
+// writes the rank. Each pass gathers from a 2^k-entry slice of the
+// contribution array that stays resident in L2.
+struct PrSeg {
+  const int32_t* sp;  // [V][S-1] split points, or null when S == 1
+  float* acc;         // [V] partial sums across passes
+  int S;
+  int seg;
+};
+
+__device__ __forceinline__ void seg_range(int32_t v, const int32_t* __restrict__ roff, const PrSeg& sg,
+                                          int32_t& a, int32_t& b) {
+  if (sg.S == 1) {
+    a = roff[v];
+    b = roff[v + 1];
+    return;
+  }
+  const int32_t* p = sg.sp + int64_t(v) * (sg.S - 1);
+  a = sg.seg == 0 ? roff[v] : p[sg.seg - 1];
+  b = sg.seg == sg.S - 1 ? roff[v + 1] : p[sg.seg];
+}
+
+__device__ __forceinline__ void seg_emit(int32_t v, float sum, const PrSeg& sg, const PrState& s,
+                                         const float* __restrict__ pr_in, float* __restrict__ pr_out,
+                                         float* __restrict__ c_out, const float* __restrict__ degf,
+                                         float& mydiff) {
+  if (sg.S > 1) {
+    if (sg.seg < sg.S - 1) {
+      sg.acc[v] = sg.seg == 0 ? sum : sg.acc[v] + sum;
+      return;
+    }
+    sum = sg.acc[v] + sum;
+  }
+  pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
+}
+
 template <int G>
 __global__ __launch_bounds__(kPrBlock) void pr_iter(
     const int32_t* __restrict__ roff, const int32_t* __restrict__ radj,
     const int32_t* __restrict__ order, const float* __restrict__ degf, PrBins bins,
     const float* __restrict__ pr_in, const float* __restrict__ c_in, float* __restrict__ pr_out,
-    float* __restrict__ c_out, const PrState* __restrict__ st, float* __restrict__ partial) {
+    float* __restrict__ c_out, const PrState* __restrict__ st, float* __restrict__ partial, PrSeg sg) {
   __shared__ float s_red[kPrBlock / kWave];
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const PrState s = *st;
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   const int b = blockIdx.x;
   float mydiff = 0.f;
+  int32_t ea, eb;
 
   if (b < bins.blk[1]) {
     // bin 0: one lane per vertex
     const int64_t i = int64_t(b) * kPrBlock + tid;
     if (i < bins.nb[0]) {
       const int32_t v = order[i];
+      seg_range(v, roff, sg, ea, eb);
       float sum = 0.f;
-      for (int32_t e = roff[v]; e < roff[v + 1]; ++e) sum += c_in[radj[e]];
-      pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
+      for (int32_t e = ea; e < eb; ++e) sum += c_in[radj[e]];
+      seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
     }
   } else if (b < bins.blk[2]) {
     // bin 1: G lanes per vertex
@@ -166,10 +210,11 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     int32_t v = -1;
     if (i < bins.nb[1]) {
       v = order[bins.nb[0] + i];
-      for (int32_t e = roff[v] + g; e < roff[v + 1]; e += G) sum += c_in[radj[e]];
+      seg_range(v, roff, sg, ea, eb);
+      for (int32_t e = ea + g; e < eb; e += G) sum += c_in[radj[e]];
     }
     sum = group_sum<G>(sum);
-    if (v >= 0 && g == 0) pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
+    if (v >= 0 && g == 0) seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
   } else if (b < bins.blk[3]) {
     // bin 2: one wavefront per vertex
     const int64_t i = int64_t(b - bins.blk[2]) * (kPrBlock / kWave) + wid;
@@ -177,27 +222,48 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     int32_t v = -1;
     if (i < bins.nb[2]) {
       v = order[bins.nb[0] + bins.nb[1] + i];
-      for (int32_t e = roff[v] + lane; e < roff[v + 1]; e += kWave) sum += c_in[radj[e]];
+      seg_range(v, roff, sg, ea, eb);
+      for (int32_t e = ea + lane; e < eb; e += kWave) sum += c_in[radj[e]];
     }
     sum = wave_sum(sum);
-    if (v >= 0 && lane == 0) pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
+    if (v >= 0 && lane == 0) seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
   } else {
     // bin 3: one workgroup per vertex
     const int32_t v = order[bins.nb[0] + bins.nb[1] + bins.nb[2] + (b - bins.blk[3])];
+    seg_range(v, roff, sg, ea, eb);
     float sum = 0.f;
-    for (int32_t e = roff[v] + tid; e < roff[v + 1]; e += kPrBlock) sum += c_in[radj[e]];
+    for (int32_t e = ea + tid; e < eb; e += kPrBlock) sum += c_in[radj[e]];
     sum = wave_sum(sum);
     if (lane == 0) s_red[wid] = sum;
     __syncthreads();
     if (tid == 0) {
       float t = 0.f;
       for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
-      pr_store(v, t, s, pr_in, pr_out, c_out, degf, mydiff);
+      seg_emit(v, t, sg, s, pr_in, pr_out, c_out, degf, mydiff);
     }
     __syncthreads();
   }
+  if (sg.seg == sg.S - 1) pr_finish(mydiff, partial, s_red);
+}
 
-  pr_finish(mydiff, partial, s_red);
+// Split points of every in-row at the segment boundaries (one lane per vertex,
+// binary search in the ascending row).
+__global__ void pr_split(int32_t V, const int32_t* __restrict__ roff, const int32_t* __restrict__ radj, int S,
+                         int seg_log2, int32_t* __restrict__ sp) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
+    const int32_t a = roff[v], b = roff[v + 1];
+    for (int s = 1; s < S; ++s) {
+      const int32_t x = int32_t(int64_t(s) << seg_log2);
+      int32_t lo = a, hi = b;
+      while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (radj[mid] < x) lo = mid + 1;
+        else hi = mid;
+      }
+      sp[v * (S - 1) + (s - 1)] = lo;
+    }
+  }
 }
 
 // Push iteration, step 1: every out-edge (u,v) adds u's contribution to
@@ -304,6 +370,16 @@ static int pr_prepare(spm_graph* g) {
   b.blk[4] = b.blk[3] + cnt[3];
   w->partial_n = std::max(b.blk[4], g->num_cus * 8) + 1;
   SPM_TRY(dmalloc(g, (void**)&w->partial, sizeof(float) * size_t(w->partial_n), "pr partial diffs"));
+  // Source segments for cache blocking.
+  w->S = 1;
+  if (s.pr_seg_log2 > 0) w->S = int((int64_t(V) + (int64_t(1) << s.pr_seg_log2) - 1) >> s.pr_seg_log2);
+  if (w->S > 1) {
+    w->sp_n = size_t(V) * size_t(w->S - 1);
+    SPM_TRY(dmalloc(g, (void**)&w->sp, sizeof(int32_t) * w->sp_n, "pr segment splits"));
+    const int grid = int(std::min<int64_t>((V + 255) / 256, 4096));
+    SPM_TRY(launch(g, "pr_split", pr_split, dim3(grid), dim3(256), 0, V, (const int32_t*)g->roff,
+                   (const int32_t*)g->radj, w->S, s.pr_seg_log2, w->sp));
+  }
   return SPM_OK;
 }
 
@@ -319,6 +395,7 @@ int pr_release(spm_graph* g) {
   dfree(g, w->degf, sizeof(float) * g->V);
   dfree(g, w->acc, sizeof(float) * g->V);
   dfree(g, w->partial, sizeof(float) * size_t(w->partial_n));
+  dfree(g, w->sp, sizeof(int32_t) * w->sp_n);
   dfree(g, w->order, sizeof(int32_t) * g->V);
   dfree(g, w->st, sizeof(PrState));
   delete w;
@@ -342,22 +419,25 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
       SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial, pgrid, w->st));
       continue;
     }
+    for (int seg = 0; seg < w->S; ++seg) {
+      const PrSeg sg{w->sp, w->acc, w->S, seg};
 #define SPM_PR_LAUNCH(G)                                                                          \
   case G:                                                                                         \
     SPM_TRY(launch(g, "pr_iter", pr_iter<G>, grid, block, 0, (const int32_t*)g->roff,            \
                    (const int32_t*)g->radj, (const int32_t*)w->order, (const float*)w->degf,       \
                    w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st,             \
-                   w->partial));                                                                   \
+                   w->partial, sg));                                                               \
     break;
-    switch (w->bins.group) {
-      SPM_PR_LAUNCH(2)
-      SPM_PR_LAUNCH(4)
-      SPM_PR_LAUNCH(8)
-      SPM_PR_LAUNCH(16)
-      SPM_PR_LAUNCH(32)
-      default: return set_error(SPM_EINVAL, "pr: bad lane group");
-    }
+      switch (w->bins.group) {
+        SPM_PR_LAUNCH(2)
+        SPM_PR_LAUNCH(4)
+        SPM_PR_LAUNCH(8)
+        SPM_PR_LAUNCH(16)
+        SPM_PR_LAUNCH(32)
+        default: return set_error(SPM_EINVAL, "pr: bad lane group");
+      }
 #undef SPM_PR_LAUNCH
+    }
     SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial,
                    int(grid.x), w->st));
   }

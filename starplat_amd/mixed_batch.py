@@ -1,0 +1,122 @@
+"""The mixed-graph batch: autotuner sweep, selector training, evaluation.
+
+A seeded set of uneven graphs -- RMAT (scale, edge factor 4-32), Erdos-Renyi
+(vertices, average degree 4-16) and 2-D grids -- each carrying one SSSP job
+(int32 weights in [1,100], source 0) or one PageRank job (beta 1e-10, delta
+0.85, maxIter 100). Graphs are generated, featurised and timed under every
+schedule class one at a time (host memory stays bounded). The first part of
+the set trains the selector on (features, fastest class); the held-out part
+is scored by makespan (sum of job times on the one GPU) under three policies:
+the fixed heuristic (class 0), the learned selector, and the best autotuned
+class per job.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .graph import graph
+from .schedule import FIXED_CLASS, JOB_PR, JOB_SSSP, N_CLASSES, Job, Selector, features, time_job
+
+
+def _rng(seed, i):
+    return np.random.default_rng([seed, i])
+
+
+def make_job(seed, i, rmat_scales=(12, 20), er_log2=(12, 20), grid_sides=(64, 1024)):
+    """Job i of the set: graph kind and parameters from a per-index stream."""
+    r = _rng(seed, i)
+    kind = int(r.integers(0, 3))
+    w = (1, 100)
+    gseed = seed * 100003 + i
+    if kind == 0:
+        s = int(r.integers(rmat_scales[0], rmat_scales[1] + 1))
+        ef = int(r.integers(4, 33))
+        g, name = graph.rmat(s, ef, seed=gseed, weights=w), f"rmat{s}_ef{ef}"
+    elif kind == 1:
+        lg = int(r.integers(er_log2[0], er_log2[1] + 1))
+        deg = int(r.integers(4, 17))
+        n = 1 << lg
+        g, name = graph.erdos_renyi(n, n * deg // 2, seed=gseed, undirected=True, weights=w), f"er{lg}_d{deg}"
+    else:
+        side = int(np.exp(r.uniform(np.log(grid_sides[0]), np.log(grid_sides[1]))))
+        g, name = graph.grid2d(side, side, seed=gseed, weights=w), f"grid{side}"
+    job = JOB_SSSP if r.random() < 0.5 else JOB_PR
+    return Job(g, job, 0, name + ("_sssp" if job == JOB_SSSP else "_pr"))
+
+
+def autotune_cutoff(job, reps=2, cutoff=8.0):
+    """Time every class; a class whose first (untimed-quality) run is already
+    `cutoff` times slower than the best so far keeps that first time and is
+    not repeated -- it cannot be the fastest."""
+    from . import hip
+    out = np.zeros(N_CLASSES)
+    best = float("inf")
+    for k in range(N_CLASSES):
+        from .schedule import apply_class
+        apply_class(job.g.device(), k)
+        hip.synchronize()
+        t0 = time.perf_counter()
+        job.run()
+        hip.synchronize()
+        first = (time.perf_counter() - t0) * 1e3
+        if first > cutoff * best:
+            out[k] = first
+            continue
+        t = min(first, time_job_steady(job, reps))
+        out[k] = t
+        best = min(best, t)
+    return out
+
+
+def time_job_steady(job, reps):
+    from . import hip
+    best = float("inf")
+    for _ in range(reps):
+        hip.synchronize()
+        t0 = time.perf_counter()
+        job.run()
+        hip.synchronize()
+        best = min(best, (time.perf_counter() - t0) * 1e3)
+    return best
+
+
+def run(n=2000, held_out=400, seed=5, reps=2, epochs=400, log=None, **ranges):
+    t0 = time.perf_counter()
+    feats = np.zeros((n, 16), np.float32)
+    times = np.zeros((n, N_CLASSES))
+    names = []
+    for i in range(n):
+        job = make_job(seed, i, **ranges)
+        feats[i] = features(job.g, job.kind)
+        times[i] = autotune_cutoff(job, reps)
+        names.append(job.name)
+        job.release()
+        del job
+        if log and (i % 25 == 0 or i == n - 1):
+            log(f"autotuned {i + 1}/{n}: {names[-1]} best class {int(np.argmin(times[i]))} "
+                f"({times[i].min():.2f} ms; fixed {times[i][FIXED_CLASS]:.2f} ms)")
+    t_sweep = time.perf_counter() - t0
+    ntr = n - held_out
+    sel = Selector()
+    t1 = time.perf_counter()
+    loss = sel.fit(feats[:ntr], times[:ntr], epochs=epochs, seed=seed)
+    t_train = time.perf_counter() - t1
+    pick = sel.choose(feats[ntr:])
+    ho = times[ntr:]
+    idx = np.arange(len(ho))
+    fixed = float(ho[:, FIXED_CLASS].sum())
+    learned = float(ho[idx, pick].sum())
+    best = float(ho.min(axis=1).sum())
+    train_acc = float(np.mean(sel.choose(feats[:ntr]) == np.argmin(times[:ntr], axis=1)))
+    return {
+        "graphs": n, "held_out": held_out, "seed": seed, "ranges": {k: list(v) for k, v in ranges.items()},
+        "makespan_ms": {"fixed_heuristic": round(fixed, 3), "learned": round(learned, 3), "best_autotuned": round(best, 3)},
+        "learned_vs_fixed_speedup": round(fixed / learned, 4),
+        "learned_over_best": round(learned / best - 1.0, 4),
+        "train_final_loss": round(float(loss), 4), "train_accuracy": round(train_acc, 4),
+        "held_out_class_histogram": np.bincount(pick, minlength=N_CLASSES).tolist(),
+        "best_class_histogram": np.bincount(np.argmin(ho, axis=1), minlength=N_CLASSES).tolist(),
+        "seconds": {"generate_feature_autotune": round(t_sweep, 1), "train_on_gpu": round(t_train, 2)},
+    }

@@ -162,12 +162,16 @@ def test_pr_iteration_count_and_closed_form(oracle):
     assert it == it_ref and rel_linf(out, ref) <= PR_TOL
 
 
-@pytest.mark.parametrize("t1,t2,t3,grp", [(0, 0, 0, 2), (64, 64, 64, 32), (2, 16, 256, 4), (8, 128, 100000, 16)])
-def test_pr_schedules(oracle, t1, t2, t3, grp):
+@pytest.mark.parametrize("t1,t2,t3,grp,seg,mode", [(0, 0, 0, 2, 0, 0), (64, 64, 64, 32, 0, 0), (2, 16, 256, 4, 10, 0),
+                                                   (8, 128, 100000, 16, 12, 0), (4, 48, 2048, 8, 11, 0),
+                                                   (4, 48, 2048, 8, 0, 1)])
+def test_pr_schedules(oracle, t1, t2, t3, grp, seg, mode):
+    """Lane mapping, source segmentation (8 passes of 1024 sources at seg 10)
+    and push vs pull change only the summation order."""
     g = graph.rmat(13, 16, seed=4)
     ref, _ = oracle.compute_pr(g, 1e-10, 0.85, 30)
     dev = g.device()
-    dev.set_schedule(pr_t1=t1, pr_t2=t2, pr_t3=t3, pr_group=grp)
+    dev.set_schedule(pr_t1=t1, pr_t2=t2, pr_t3=t3, pr_group=grp, pr_seg_log2=seg, pr_mode=mode)
     out, it = dev.pr(1e-10, 0.85, 30)
     assert it == 30 and rel_linf(out, ref) <= PR_TOL
 

@@ -82,21 +82,56 @@ def time_job_steady(job, reps):
     return best
 
 
-def run(n=2000, held_out=400, seed=5, reps=2, epochs=400, log=None, **ranges):
-    t0 = time.perf_counter()
-    feats = np.zeros((n, 16), np.float32)
-    times = np.zeros((n, N_CLASSES))
-    names = []
-    for i in range(n):
+def sweep(seed, start, stop, reps=2, log=None, rows_path=None, **ranges):
+    """Features and per-class times of jobs start..stop-1. Job i depends only
+    on (seed, i), so disjoint slices can be swept separately; with rows_path
+    every job is appended as one JSON line as soon as it is timed."""
+    import json
+
+    feats, times, names = [], [], []
+    for i in range(start, stop):
         job = make_job(seed, i, **ranges)
-        feats[i] = features(job.g, job.kind)
-        times[i] = autotune_cutoff(job, reps)
+        f = features(job.g, job.kind)
+        t = autotune_cutoff(job, reps)
+        feats.append(f)
+        times.append(t)
         names.append(job.name)
         job.release()
         del job
-        if log and (i % 25 == 0 or i == n - 1):
-            log(f"autotuned {i + 1}/{n}: {names[-1]} best class {int(np.argmin(times[i]))} "
-                f"({times[i].min():.2f} ms; fixed {times[i][FIXED_CLASS]:.2f} ms)")
+        if rows_path:
+            with open(rows_path, "a") as fh:
+                fh.write(json.dumps({"i": i, "seed": seed, "name": names[-1], "features": [float(x) for x in f],
+                                     "times_ms": [float(x) for x in t]}) + "\n")
+        if log and ((i - start) % 25 == 0 or i == stop - 1):
+            log(f"autotuned {i + 1 - start}/{stop - start}: {names[-1]} best class {int(np.argmin(t))} "
+                f"({t.min():.2f} ms; fixed {t[FIXED_CLASS]:.2f} ms)")
+    return np.array(feats, np.float32).reshape(-1, 16), np.array(times).reshape(-1, N_CLASSES), names
+
+
+def load_rows(paths):
+    """Rows written by sweep(), deduplicated by job index, in index order."""
+    import json
+
+    rows = {}
+    for p in paths:
+        for line in open(p):
+            r = json.loads(line)
+            rows[(r["seed"], r["i"])] = r
+    keys = sorted(rows)
+    feats = np.array([rows[k]["features"] for k in keys], np.float32)
+    times = np.array([rows[k]["times_ms"] for k in keys])
+    return feats, times, [rows[k]["name"] for k in keys]
+
+
+def run(n=2000, held_out=400, seed=5, reps=2, epochs=400, log=None, rows=None, **ranges):
+    """Sweep n jobs (or use precomputed rows = (feats, times)), train on the
+    first n - held_out, score the rest."""
+    t0 = time.perf_counter()
+    if rows is None:
+        feats, times, _ = sweep(seed, 0, n, reps, log, **ranges)
+    else:
+        feats, times = rows[0][:n], rows[1][:n]
+        n = len(feats)
     t_sweep = time.perf_counter() - t0
     ntr = n - held_out
     sel = Selector()

@@ -241,6 +241,9 @@ def main():
     ap.add_argument("--sched-rmat", type=int, nargs=2, default=[10, 16])
     ap.add_argument("--sched-er", type=int, nargs=2, default=[10, 16])
     ap.add_argument("--sched-grid", type=int, nargs=2, default=[32, 256])
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a known-size kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -248,6 +251,16 @@ def main():
     if hip.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration_mib > 0:
+        nbytes = args.copy_calibration_mib << 20
+        a = hip.DeviceArray(nbytes // 4, np.int32)
+        b = hip.DeviceArray(nbytes // 4, np.int32)
+        hip.lib().hipMemset(a.ptr, 1, nbytes)
+        hip.lib().hipMemcpy(b.ptr, a.ptr, nbytes, hip.D2D)
+        hip.synchronize()
+        a.free()
+        b.free()
+        log(f"calibration copy of {args.copy_calibration_mib} MiB done")
     O = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

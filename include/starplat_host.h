@@ -71,6 +71,14 @@ spmh_graph* spmh_union(const spmh_graph* const* gs, int32_t n, int64_t* bases);
 /* Writes "src dst w" lines (or "src dst" when weights == 0). 0 on success. */
 int spmh_write_text(const spmh_graph* g, const char* path, int weights);
 
+/* Binary CSR file: a 64-byte header (magic "SPMCSR01", V, E, flags,
+ * checksum) followed by indexofNodes, edgeList, edgeLen, rev_indexofNodes,
+ * srcList as little-endian int32. Replaces re-parsing graph.hpp text
+ * (graph.hpp:464-515, a std::map of every pair) for large graphs. Returns 0 on
+ * success; the reader returns NULL on a missing, short or corrupt file. */
+int spmh_write_binary(const spmh_graph* g, const char* path);
+spmh_graph* spmh_read_binary(const char* path);
+
 void spmh_free(spmh_graph* g);
 
 /* n distinct vertices drawn uniformly from [0,V) with the given seed, written

@@ -111,6 +111,8 @@ HOST_API = [
     ("spmh_parse_text", P_spmh, [C.c_char_p]),
     ("spmh_union", P_spmh, [C.POINTER(P_spmh), i32, P_i64]),
     ("spmh_write_text", C.c_int, [P_spmh, C.c_char_p, C.c_int]),
+    ("spmh_write_binary", C.c_int, [P_spmh, C.c_char_p]),
+    ("spmh_read_binary", P_spmh, [C.c_char_p]),
     ("spmh_free", None, [P_spmh]),
     ("spmh_sources", C.c_int, [i32, i32, u64, P_i32]),
     ("spmh_features", C.c_int, [P_spmh, f32, P_f32]),

@@ -310,6 +310,83 @@ int spmh_write_text(const spmh_graph* g, const char* path, int weights) {
   return std::fclose(f) == 0 ? 0 : -1;
 }
 
+namespace {
+struct BinHeader {
+  char magic[8];
+  int64_t V;
+  int64_t E;
+  int64_t flags;
+  uint64_t checksum;  // FNV-1a over the five arrays
+  int64_t pad[3];
+};
+static_assert(sizeof(BinHeader) == 64, "header is 64 bytes");
+
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001B3ull;
+  return h;
+}
+
+uint64_t csr_checksum(const spmh_graph* g) {
+  uint64_t h = 0xCBF29CE484222325ull;
+  const size_t vb = sizeof(int32_t) * (size_t(g->V) + 1), eb = sizeof(int32_t) * size_t(g->E);
+  h = fnv1a(h, g->indexofNodes, vb);
+  h = fnv1a(h, g->edgeList, eb);
+  h = fnv1a(h, g->edgeLen, eb);
+  h = fnv1a(h, g->rev_indexofNodes, vb);
+  return fnv1a(h, g->srcList, eb);
+}
+}  // namespace
+
+int spmh_write_binary(const spmh_graph* g, const char* path) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return -1;
+  BinHeader h{};
+  std::memcpy(h.magic, "SPMCSR01", 8);
+  h.V = g->V;
+  h.E = g->E;
+  h.checksum = csr_checksum(g);
+  const size_t vb = size_t(g->V) + 1, eb = size_t(g->E);
+  bool ok = std::fwrite(&h, sizeof h, 1, f) == 1;
+  ok = ok && std::fwrite(g->indexofNodes, sizeof(int32_t), vb, f) == vb;
+  ok = ok && std::fwrite(g->edgeList, sizeof(int32_t), eb, f) == eb;
+  ok = ok && std::fwrite(g->edgeLen, sizeof(int32_t), eb, f) == eb;
+  ok = ok && std::fwrite(g->rev_indexofNodes, sizeof(int32_t), vb, f) == vb;
+  ok = ok && std::fwrite(g->srcList, sizeof(int32_t), eb, f) == eb;
+  return (std::fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+spmh_graph* spmh_read_binary(const char* path) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return nullptr;
+  BinHeader h{};
+  if (std::fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, "SPMCSR01", 8) != 0 || h.V < 1 ||
+      h.V > 0x7fffffff || h.E < 0 || h.E > 0x7fffffff) {
+    std::fclose(f);
+    return nullptr;
+  }
+  spmh_graph* g = static_cast<spmh_graph*>(std::calloc(1, sizeof(spmh_graph)));
+  g->V = int32_t(h.V);
+  g->E = h.E;
+  const size_t vb = size_t(h.V) + 1, eb = size_t(h.E);
+  g->indexofNodes = alloc_i32(int64_t(vb));
+  g->edgeList = alloc_i32(h.E);
+  g->edgeLen = alloc_i32(h.E);
+  g->rev_indexofNodes = alloc_i32(int64_t(vb));
+  g->srcList = alloc_i32(h.E);
+  bool ok = std::fread(g->indexofNodes, sizeof(int32_t), vb, f) == vb;
+  ok = ok && std::fread(g->edgeList, sizeof(int32_t), eb, f) == eb;
+  ok = ok && std::fread(g->edgeLen, sizeof(int32_t), eb, f) == eb;
+  ok = ok && std::fread(g->rev_indexofNodes, sizeof(int32_t), vb, f) == vb;
+  ok = ok && std::fread(g->srcList, sizeof(int32_t), eb, f) == eb;
+  std::fclose(f);
+  if (!ok || csr_checksum(g) != h.checksum || g->indexofNodes[0] != 0 || g->indexofNodes[g->V] != g->E) {
+    spmh_free(g);
+    return nullptr;
+  }
+  return g;
+}
+
 void spmh_free(spmh_graph* g) {
   if (!g) return;
   std::free(g->indexofNodes);

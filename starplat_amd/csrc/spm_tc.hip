@@ -41,6 +41,8 @@ struct TcWork {
   int32_t* psrc = nullptr;    // [m] oriented edge source
   int32_t* csrc = nullptr;    // [E] CSR edge source (general form)
   int64_t m = 0;
+  int32_t* maxd = nullptr;    // [1] longest oriented row (device)
+  int32_t max_dplus = 0;
   unsigned long long* acc = nullptr;  // [2] total, flags
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
@@ -212,12 +214,92 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int64_t nitems, const int32
   if (lane == 0 && cnt) atomicAdd(total, cnt);
 }
 
+// Oriented rows, hashed per source vertex: a wavefront owns vertex u, builds
+// an open-addressing hash set of N+(u) in its slice of LDS (size a power of
+// two >= 2 d+(u), linear probing, keys are distinct vertex ids), then streams
+// the rows N+(v) of every v in N+(u) -- flattened across the 64 lanes with a
+// wave prefix sum -- and probes each element: every hit is one triangle
+// u < v < w in the orientation. Rows of the (degree, id) orientation are at
+// most sqrt(2m) long, so N+(u) always fits kTcHash slots.
+constexpr int kTcHash = 2048;
+__global__ __launch_bounds__(kTcBlock) void tc_count_hash(int32_t V, const int32_t* __restrict__ poff,
+                                                          const int32_t* __restrict__ padj,
+                                                          unsigned long long* __restrict__ total,
+                                                          unsigned long long* __restrict__ overflow) {
+  __shared__ int32_t s_tab[kTcWaves][kTcHash];
+  __shared__ uint32_t s_pre[kTcWaves][kWave];
+  __shared__ int32_t s_row[kTcWaves][kWave];
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  int32_t* tab = s_tab[wid];
+  unsigned long long cnt = 0;
+  const int64_t nw = int64_t(gridDim.x) * kTcWaves;
+  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + wid; u < V; u += nw) {
+    const int32_t ub = poff[u], du = poff[u + 1] - ub;
+    if (du < 2) continue;  // a triangle needs two oriented neighbours of u
+    if (2 * du > kTcHash) {  // cannot happen for the (degree, id) orientation; counted, not skipped silently
+      if (lane == 0) atomicAdd(overflow, 1ull);
+      continue;
+    }
+    int lg = 6;
+    while ((1 << lg) < 2 * du) ++lg;
+    const int size = 1 << lg;
+    const uint32_t mask = uint32_t(size - 1);
+    // Fibonacci hashing: the top lg bits of x * 2^32/phi.
+    const int shift = 32 - lg;
+    for (int i = lane; i < size; i += kWave) tab[i] = -1;
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < du; i += kWave) {
+      const int32_t x = padj[ub + i];
+      uint32_t h = (uint32_t(x) * 2654435769u) >> shift;
+      while (atomicCAS(&tab[h], -1, x) != -1) h = (h + 1) & mask;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // probe rows N+(v), v in N+(u), 64 v's at a time
+    for (int32_t i0 = 0; i0 < du; i0 += kWave) {
+      const int32_t i = i0 + lane;
+      uint32_t len = 0;
+      int32_t rb = 0;
+      if (i < du) {
+        const int32_t v = padj[ub + i];
+        rb = poff[v];
+        len = uint32_t(poff[v + 1] - rb);
+      }
+      const uint32_t incl = wave_incl_scan(len);
+      const uint32_t tot = __shfl(incl, kWave - 1, kWave);
+      s_pre[wid][lane] = incl - len;
+      s_row[wid][lane] = rb;
+      __builtin_amdgcn_wave_barrier();
+      for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
+        const uint32_t t = t0 + lane;
+        bool hit = false;
+        if (t < tot) {
+          int lo = 0, hi = kWave - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pre[wid][mid] <= t) lo = mid;
+            else hi = mid - 1;
+          }
+          const int32_t w = padj[s_row[wid][lo] + int32_t(t - s_pre[wid][lo])];
+          uint32_t h = (uint32_t(w) * 2654435769u) >> shift;
+          int32_t k;
+          while ((k = tab[h]) != -1 && k != w) h = (h + 1) & mask;
+          hit = k == w;
+        }
+        cnt += __popcll(__ballot(hit));
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  if (lane == 0 && cnt) atomicAdd(total, cnt);
+}
+
 static int tc_prepare(spm_graph* g) {
   if (!g->tc) g->tc = new TcWork();
   TcWork* w = g->tc;
   const int32_t V = g->V;
   const int grid = g->num_cus * 8;
   if (!w->acc) SPM_TRY(dmalloc(g, (void**)&w->acc, 2 * sizeof(unsigned long long), "tc counters"));
+  if (!w->maxd) SPM_TRY(dmalloc(g, (void**)&w->maxd, sizeof(int32_t), "tc max row"));
   if (!w->checked) {
     SPM_CHECK_HIP(hipMemsetAsync(w->acc, 0, 2 * sizeof(unsigned long long), g->stream));
     SPM_TRY(launch(g, "tc_check", tc_check, dim3(grid), dim3(kTcBlock), 0, V, (const int32_t*)g->off,
@@ -234,6 +316,20 @@ static int tc_prepare(spm_graph* g) {
     SPM_TRY(launch(g, "tc_orient_count", tc_orient_count, dim3(grid), dim3(kTcBlock), 0, V,
                    (const int32_t*)g->off, (const int32_t*)g->adj, w->dplus));
     SPM_CHECK_HIP(hipMemsetAsync(w->dplus + V, 0, sizeof(int32_t), g->stream));
+    {  // longest oriented row, to choose the hashed count kernel
+      size_t need_max = 0;
+      SPM_CHECK_HIP(hipcub::DeviceReduce::Max(nullptr, need_max, w->dplus, w->maxd, V, g->stream));
+      if (need_max > w->tmp_bytes) {
+        dfree(g, w->tmp, w->tmp_bytes);
+        w->tmp = nullptr;
+        w->tmp_bytes = 0;
+        SPM_TRY(dmalloc(g, &w->tmp, need_max, "tc reduce scratch"));
+        w->tmp_bytes = need_max;
+      }
+      SPM_CHECK_HIP(hipcub::DeviceReduce::Max(w->tmp, need_max, w->dplus, w->maxd, V, g->stream));
+      SPM_CHECK_HIP(hipMemcpyAsync(&w->max_dplus, w->maxd, sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+      SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    }
     size_t need = 0;
     SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, w->dplus, w->dplus, V + 1, g->stream));
     if (need > w->tmp_bytes) {
@@ -269,6 +365,7 @@ int tc_release(spm_graph* g) {
   dfree(g, w->psrc, sizeof(int32_t) * size_t(w->m));
   dfree(g, w->csrc, sizeof(int32_t) * size_t(g->E));
   dfree(g, w->acc, 2 * sizeof(unsigned long long));
+  dfree(g, w->maxd, sizeof(int32_t));
   dfree(g, w->tmp, w->tmp_bytes);
   delete w;
   g->tc = nullptr;
@@ -280,17 +377,23 @@ int tc_run(spm_graph* g, uint64_t* out) {
   TcWork* w = g->tc;
   SPM_CHECK_HIP(hipMemsetAsync(w->acc, 0, sizeof(unsigned long long), g->stream));
   const int grid = g->num_cus * 8;
-  if (w->symmetric) {
+  if (w->symmetric && g->sched.tc_mode == 0 && 2 * w->max_dplus <= kTcHash) {
+    SPM_CHECK_HIP(hipMemsetAsync(w->acc + 1, 0, sizeof(unsigned long long), g->stream));
+    SPM_TRY(launch(g, "tc_count_hash", tc_count_hash, dim3(grid), dim3(kTcBlock), 0, g->V,
+                   (const int32_t*)w->dplus, (const int32_t*)w->padj, w->acc, w->acc + 1));
+  } else if (w->symmetric) {
     if (w->m) SPM_TRY(launch(g, "tc_count", tc_count<true>, dim3(grid), dim3(kTcBlock), 0, w->m,
                             (const int32_t*)w->dplus, (const int32_t*)w->padj, (const int32_t*)w->psrc, w->acc));
   } else {
     if (g->E) SPM_TRY(launch(g, "tc_count_general", tc_count<false>, dim3(grid), dim3(kTcBlock), 0, g->E,
                             (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)w->csrc, w->acc));
   }
-  unsigned long long t = 0;
-  SPM_CHECK_HIP(hipMemcpyAsync(&t, w->acc, sizeof t, hipMemcpyDeviceToHost, g->stream));
+  unsigned long long t[2] = {0, 0};
+  SPM_CHECK_HIP(hipMemcpyAsync(t, w->acc, sizeof t, hipMemcpyDeviceToHost, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
-  *out = t;
+  if (w->symmetric && g->sched.tc_mode == 0 && 2 * w->max_dplus <= kTcHash && t[1] != 0)
+    return set_error(SPM_EHIP, "tc: oriented row larger than the LDS hash (inconsistent orientation)");
+  *out = t[0];
   SPM_TRY(prof_flush(g));
   return SPM_OK;
 }

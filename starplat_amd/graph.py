@@ -99,6 +99,18 @@ class graph:
         u = cls()._adopt(N.host_lib().spmh_union(arr, len(graphs), N.ptr(bases, C.c_int64)))
         return u, bases
 
+    def save(self, path):
+        """Binary CSR file (header + the five arrays, checksummed)."""
+        if N.host_lib().spmh_write_binary(self._h, str(path).encode()) != 0:
+            raise OSError(f"cannot write {path}")
+
+    @classmethod
+    def load(cls, path):
+        h = N.host_lib().spmh_read_binary(str(path).encode())
+        if not h:
+            raise OSError(f"{path}: missing, truncated or corrupt binary CSR")
+        return cls()._adopt(h)
+
     def write_text(self, path, weights=True):
         rc = N.host_lib().spmh_write_text(self._h, str(path).encode(), int(bool(weights)))
         if rc != 0:

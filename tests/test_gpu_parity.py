@@ -178,9 +178,21 @@ def test_pr_schedules(oracle, t1, t2, t3, grp, seg, mode):
 
 # ------------------------------------------------------------------ TC
 @pytest.mark.parametrize("scale,seed", [(8, 1), (12, 2), (16, 3)])
-def test_tc_parity(oracle, scale, seed):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_tc_parity(oracle, scale, seed, mode):
     g = graph.rmat(scale, 16, seed=seed, undirected=True)
+    g.device().set_schedule(tc_mode=mode)
     assert Compute_TC(g) == oracle.compute_tc(g)
+
+
+def test_tc_long_oriented_rows_fall_back():
+    """K_1100: oriented rows reach 1099 entries, past the LDS hash (1024), so
+    the count falls back to the edge-centric search; C(1100, 3) triangles."""
+    n = 1100
+    a, b = np.meshgrid(np.arange(n), np.arange(n))
+    m = a != b
+    g = graph.from_edges(a[m].astype(np.int32), b[m].astype(np.int32))
+    assert Compute_TC(g) == n * (n - 1) * (n - 2) // 6
 
 
 def test_tc_closed_forms_and_directed(oracle):

@@ -154,3 +154,19 @@ def test_union_is_block_diagonal():
         seg = slice(u.indexofNodes[lo], u.indexofNodes[hi])
         np.testing.assert_array_equal(u.edgeList[seg] - lo, g.edgeList)
         np.testing.assert_array_equal(u.edgeLen[seg], g.edgeLen)
+
+
+def test_binary_csr_round_trip_and_corruption(tmp_path):
+    g = graph.rmat(12, 8, seed=21, weights=(1, 100))
+    p = tmp_path / "g.spmcsr"
+    g.save(p)
+    h = graph.load(p)
+    for a in ("indexofNodes", "edgeList", "edgeLen", "rev_indexofNodes", "srcList"):
+        np.testing.assert_array_equal(getattr(g, a), getattr(h, a))
+    data = bytearray(p.read_bytes())
+    (tmp_path / "short").write_bytes(bytes(data[:-4]))
+    data[100] ^= 0xFF
+    (tmp_path / "flipped").write_bytes(bytes(data))
+    for bad in ("short", "flipped", "missing"):
+        with pytest.raises(OSError):
+            graph.load(tmp_path / bad)

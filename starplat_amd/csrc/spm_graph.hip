@@ -106,7 +106,7 @@ int validate_schedule(const spm_schedule* s) {
   if (s->sssp_mode < 0 || s->sssp_mode > 1) return set_error(SPM_EINVAL, "schedule: sssp_mode must be 0 or 1");
   if (s->sssp_rounds < 6 || s->sssp_rounds > 96 || s->sssp_rounds % 6) return set_error(SPM_EINVAL, "schedule: sssp_rounds must be a multiple of 6 in [6, 96]");
   if (s->sssp_grid < 0 || s->sssp_grid > 65536) return set_error(SPM_EINVAL, "schedule: sssp_grid out of range");
-  if (s->tc_mode < 0 || s->tc_mode > 1) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0 (hash) or 1 (search)");
+  if (s->tc_mode < 0 || s->tc_mode > 1) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0 (search) or 1 (hash)");
   if (s->pr_mode < 0 || s->pr_mode > 1) return set_error(SPM_EINVAL, "schedule: pr_mode must be 0 (pull) or 1 (push)");
   if (s->pr_seg_log2 != 0 && (s->pr_seg_log2 < 10 || s->pr_seg_log2 > 30))
     return set_error(SPM_EINVAL, "schedule: pr_seg_log2 must be 0 or in [10, 30]");

@@ -59,7 +59,7 @@ struct PrWork {
   int32_t* sp = nullptr;     // segment split points [V][S-1]
   size_t sp_n = 0;
   int S = 1;                 // source segments per pull sweep
-  int32_t* order = nullptr;  // vertices grouped by bin
+  int4* rows = nullptr;      // {v, in-row start, in-row end, 0} grouped by bin
   PrState* st = nullptr;
   PrBins bins{};
   hipGraph_t graph = nullptr;
@@ -178,10 +178,40 @@ __device__ __forceinline__ void seg_emit(int32_t v, float sum, const PrSeg& sg, 
   pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
 }
 
+// Sum of c[radj[e]] for e = ea+start, ea+start+stride, ... < eb, four rows
+// elements at a time: four independent index loads, then four independent
+// gathers, so a lane keeps several memory requests in flight instead of a
+// chain of dependent ones.
+__device__ __forceinline__ float gather_sum(const int32_t* __restrict__ radj, const float* __restrict__ c,
+                                            int32_t ea, int32_t eb, int32_t start, int32_t stride) {
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int32_t e = ea + start;
+  for (; e + 3 * stride < eb; e += 4 * stride) {
+    const int32_t a0 = radj[e], a1 = radj[e + stride], a2 = radj[e + 2 * stride], a3 = radj[e + 3 * stride];
+    s0 += c[a0];
+    s1 += c[a1];
+    s2 += c[a2];
+    s3 += c[a3];
+  }
+  for (; e < eb; e += stride) s0 += c[radj[e]];
+  return (s0 + s1) + (s2 + s3);
+}
+
+// rows[i] = {v, first in-edge, end of in-edges, 0} for the i-th vertex in bin
+// order: one 16-byte load replaces the order -> offsets chain.
+__device__ __forceinline__ void row_of(const int4* __restrict__ rows, int64_t i, const int32_t* __restrict__ roff,
+                                       const PrSeg& sg, int32_t& v, int32_t& ea, int32_t& eb) {
+  const int4 r = rows[i];
+  v = r.x;
+  ea = r.y;
+  eb = r.z;
+  if (sg.S > 1) seg_range(v, roff, sg, ea, eb);
+}
+
 template <int G>
 __global__ __launch_bounds__(kPrBlock) void pr_iter(
     const int32_t* __restrict__ roff, const int32_t* __restrict__ radj,
-    const int32_t* __restrict__ order, const float* __restrict__ degf, PrBins bins,
+    const int4* __restrict__ rows, const float* __restrict__ degf, PrBins bins,
     const float* __restrict__ pr_in, const float* __restrict__ c_in, float* __restrict__ pr_out,
     float* __restrict__ c_out, const PrState* __restrict__ st, float* __restrict__ partial, PrSeg sg) {
   __shared__ float s_red[kPrBlock / kWave];
@@ -190,28 +220,24 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   const int b = blockIdx.x;
   float mydiff = 0.f;
-  int32_t ea, eb;
+  int32_t v, ea, eb;
 
   if (b < bins.blk[1]) {
     // bin 0: one lane per vertex
     const int64_t i = int64_t(b) * kPrBlock + tid;
     if (i < bins.nb[0]) {
-      const int32_t v = order[i];
-      seg_range(v, roff, sg, ea, eb);
-      float sum = 0.f;
-      for (int32_t e = ea; e < eb; ++e) sum += c_in[radj[e]];
-      seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
+      row_of(rows, i, roff, sg, v, ea, eb);
+      seg_emit(v, gather_sum(radj, c_in, ea, eb, 0, 1), sg, s, pr_in, pr_out, c_out, degf, mydiff);
     }
   } else if (b < bins.blk[2]) {
     // bin 1: G lanes per vertex
     const int64_t i = int64_t(b - bins.blk[1]) * (kPrBlock / G) + tid / G;
     const int g = tid % G;
     float sum = 0.f;
-    int32_t v = -1;
+    v = -1;
     if (i < bins.nb[1]) {
-      v = order[bins.nb[0] + i];
-      seg_range(v, roff, sg, ea, eb);
-      for (int32_t e = ea + g; e < eb; e += G) sum += c_in[radj[e]];
+      row_of(rows, bins.nb[0] + i, roff, sg, v, ea, eb);
+      sum = gather_sum(radj, c_in, ea, eb, g, G);
     }
     sum = group_sum<G>(sum);
     if (v >= 0 && g == 0) seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
@@ -219,20 +245,17 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     // bin 2: one wavefront per vertex
     const int64_t i = int64_t(b - bins.blk[2]) * (kPrBlock / kWave) + wid;
     float sum = 0.f;
-    int32_t v = -1;
+    v = -1;
     if (i < bins.nb[2]) {
-      v = order[bins.nb[0] + bins.nb[1] + i];
-      seg_range(v, roff, sg, ea, eb);
-      for (int32_t e = ea + lane; e < eb; e += kWave) sum += c_in[radj[e]];
+      row_of(rows, bins.nb[0] + bins.nb[1] + i, roff, sg, v, ea, eb);
+      sum = gather_sum(radj, c_in, ea, eb, lane, kWave);
     }
     sum = wave_sum(sum);
     if (v >= 0 && lane == 0) seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
   } else {
     // bin 3: one workgroup per vertex
-    const int32_t v = order[bins.nb[0] + bins.nb[1] + bins.nb[2] + (b - bins.blk[3])];
-    seg_range(v, roff, sg, ea, eb);
-    float sum = 0.f;
-    for (int32_t e = ea + tid; e < eb; e += kPrBlock) sum += c_in[radj[e]];
+    row_of(rows, bins.nb[0] + bins.nb[1] + bins.nb[2] + (b - bins.blk[3]), roff, sg, v, ea, eb);
+    float sum = gather_sum(radj, c_in, ea, eb, tid, kPrBlock);
     sum = wave_sum(sum);
     if (lane == 0) s_red[wid] = sum;
     __syncthreads();
@@ -342,7 +365,7 @@ static int pr_prepare(spm_graph* g) {
   }
   SPM_TRY(dmalloc(g, (void**)&w->degf, sizeof(float) * V, "pr outdeg"));
   SPM_TRY(dmalloc(g, (void**)&w->acc, sizeof(float) * V, "pr push accumulators"));
-  SPM_TRY(dmalloc(g, (void**)&w->order, sizeof(int32_t) * V, "pr bins"));
+  SPM_TRY(dmalloc(g, (void**)&w->rows, sizeof(int4) * V, "pr bins"));
   SPM_TRY(dmalloc(g, (void**)&w->st, sizeof(PrState), "pr state"));
   // Bin vertices by in-degree on the host (once per handle and schedule).
   std::vector<int32_t> roff(size_t(V) + 1);
@@ -358,7 +381,12 @@ static int pr_prepare(spm_graph* g) {
   for (int32_t v = 0; v < V; ++v) ++cnt[bin_of(v)];
   int32_t start[4] = {0, cnt[0], cnt[0] + cnt[1], cnt[0] + cnt[1] + cnt[2]};
   for (int32_t v = 0; v < V; ++v) order[start[bin_of(v)]++] = v;
-  SPM_CHECK_HIP(hipMemcpyAsync(w->order, order.data(), sizeof(int32_t) * V, hipMemcpyHostToDevice, g->stream));
+  std::vector<int4> rows(V);
+  for (int32_t i = 0; i < V; ++i) {
+    const int32_t v = order[i];
+    rows[i] = make_int4(v, roff[v], roff[v + 1], 0);
+  }
+  SPM_CHECK_HIP(hipMemcpyAsync(w->rows, rows.data(), sizeof(int4) * V, hipMemcpyHostToDevice, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   PrBins& b = w->bins;
   for (int i = 0; i < 4; ++i) b.nb[i] = cnt[i];
@@ -396,7 +424,7 @@ int pr_release(spm_graph* g) {
   dfree(g, w->acc, sizeof(float) * g->V);
   dfree(g, w->partial, sizeof(float) * size_t(w->partial_n));
   dfree(g, w->sp, sizeof(int32_t) * w->sp_n);
-  dfree(g, w->order, sizeof(int32_t) * g->V);
+  dfree(g, w->rows, sizeof(int4) * g->V);
   dfree(g, w->st, sizeof(PrState));
   delete w;
   g->pr = nullptr;
@@ -424,7 +452,7 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
 #define SPM_PR_LAUNCH(G)                                                                          \
   case G:                                                                                         \
     SPM_TRY(launch(g, "pr_iter", pr_iter<G>, grid, block, 0, (const int32_t*)g->roff,            \
-                   (const int32_t*)g->radj, (const int32_t*)w->order, (const float*)w->degf,       \
+                   (const int32_t*)g->radj, (const int4*)w->rows, (const float*)w->degf,          \
                    w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st,             \
                    w->partial, sg));                                                               \
     break;

@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 from starplat_amd import _native as N  # noqa: E402
 from starplat_amd import graph, hip, sources  # noqa: E402
+from starplat_amd import pmc_summary  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "GTEPS per algo at RMAT-22 on 1 MI355X, % HBM roofline; learned vs fixed sched"
@@ -62,12 +63,32 @@ def kernel_time(dev, fn, kernels, reps=2):
     return {k: (ms / reps, n / reps) for k, (ms, n) in out.items()}
 
 
-def roofline(bytes_per_run, kms, launches):
+def measured_traffic(workload, kernels):
+    """HBM bytes per dispatch of `kernels` on `workload` from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summaries in profiles/ (rows written by
+    starplat_amd.pmc_summary), or None if that workload was not measured."""
+    rows = {}
+    pdir = os.path.join(ROOT, "profiles")
+    if os.path.isdir(pdir):
+        for name in sorted(os.listdir(pdir)):
+            if name.endswith(".md"):
+                with open(os.path.join(pdir, name)) as f:
+                    rows.update(pmc_summary.parse_rows(f.read()))
+    return rows.get((workload, tuple(kernels)))
+
+
+def roofline(bytes_per_run, kms, launches, workload, kernels):
     achieved = bytes_per_run / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-            "bytes_per_launch": round(bytes_per_run / max(launches, 1)),
-            "kernel_ms_per_launch": round(kms / max(launches, 1), 5), "launches_per_step": launches}
+    per_launch = bytes_per_run / max(launches, 1)
+    traffic = measured_traffic(workload, kernels)
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None if traffic is None else round(traffic),
+           "bytes_per_launch": round(per_launch),
+           "kernel_ms_per_launch": round(kms / max(launches, 1), 5), "launches_per_step": launches,
+           "kernels": list(kernels), "traffic_key": workload}
+    if traffic is not None:
+        out["traffic_over_algorithmic"] = round(traffic / per_launch, 3)
+    return out
 
 
 def host_cores():
@@ -98,7 +119,8 @@ def bench_sssp(args, O):
         "V": g.V, "E": g.E, "m_reach": m.value, "rounds_gpu": rounds, "rounds_sync_trace": r.value,
         "relaxations_sync_trace": rl.value, "ms_per_step": round(ms, 4),
         "gteps": m.value / (ms * 1e-3) / 1e9,
-        "roofline": roofline(model, kms, launches), "algorithmic_bytes": model,
+        "roofline": roofline(model, kms, launches, f"sssp-rmat{args.sssp_scale}", ["sssp_relax"]),
+        "algorithmic_bytes": model,
     }
     if not args.no_cpu_baseline:
         ref = O.compute_sssp(g, g.edgeLen, 0)
@@ -132,7 +154,7 @@ def bench_pr(args, O, scale, seed, label):
     res = {"workload": f"{label}: pull PageRank, directed RMAT scale {scale} EF16, beta 1e-10, delta 0.85, maxIter 100, seed {seed}",
            "V": g.V, "E": g.E, "iterations": it, "ms_per_run": round(ms, 4),
            "gteps": g.E * it / (ms * 1e-3) / 1e9,
-           "roofline": roofline(model * it, kms, launches)}
+           "roofline": roofline(model * it, kms, launches, f"pr-rmat{scale}", ["pr_iter"])}
     if not args.no_cpu_baseline:
         ref, it_ref = O.compute_pr(g, 1e-10, 0.85, 100)
         got = d_out.to_host()
@@ -162,7 +184,8 @@ def bench_tc(args, O):
     kms, launches = kt["tc_count"]
     res = {"workload": f"Triangle counting, undirected RMAT scale {args.tc_scale} EF16, symmetrised, sorted, seed 3",
            "V": g.V, "E": g.E, "m_und": mu.value, "triangles": count, "ms_per_run": round(ms, 4),
-           "gteps": mu.value / (ms * 1e-3) / 1e9, "roofline": roofline(model, kms, launches)}
+           "gteps": mu.value / (ms * 1e-3) / 1e9,
+           "roofline": roofline(model, kms, launches, f"tc-rmat{args.tc_scale}", ["tc_count"])}
     if not args.no_cpu_baseline:
         stride = 16
         t0 = time.perf_counter()
@@ -187,7 +210,7 @@ def bench_bc(args, O):
     d_out = hip.DeviceArray(g.V, np.float32)
     dev.bc_device(src, d_out)
     ms = timed_steps(lambda: dev.bc_device(src, d_out), max(1, args.steps // 8), 0)
-    ks = ["bc_scan", "bc_sigma", "bc_back"]
+    ks = ["bc_scan", "bc_push", "bc_sigma", "bc_back"]
     kt = kernel_time(dev, lambda: dev.bc_device(src, d_out), ks, reps=1)
     kms = sum(v[0] for v in kt.values())
     launches = sum(v[1] for v in kt.values())
@@ -195,7 +218,7 @@ def bench_bc(args, O):
            "V": g.V, "E": g.E, "ms_per_run": round(ms, 3),
            "gteps": len(src) * g.E / (ms * 1e-3) / 1e9,
            "kernel_ms": {k: round(v[0], 3) for k, v in kt.items()},
-           "roofline": roofline(model, kms, launches),
+           "roofline": roofline(model, kms, launches, f"bc-er{args.bc_n}-{len(src)}", ks),
            "note": "model bytes count every source's traversal separately; the batched kernels share adjacency reads "
                    "across 64 sources, so achieved model GB/s can exceed the HBM peak"}
     if not args.no_cpu_baseline:
@@ -229,7 +252,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--algos", default="sssp,pr,tc,bc,sched")
+    ap.add_argument("--algos", default="sssp,pr,pr_c0,tc,bc,sched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--sssp-scale", type=int, default=22)
@@ -273,6 +296,7 @@ def main():
         per["sssp"] = {k: v for k, v in head.items() if k not in ("roofline", "cpu_baseline")}
     if "pr" in algos:
         per["pr"] = bench_pr(args, O, 22, 1, "PageRank at RMAT-22")
+    if "pr_c0" in algos:
         per["pr_c0"] = bench_pr(args, O, 16, 1, "PageRank config C0")
     if "tc" in algos:
         per["tc"] = bench_tc(args, O)

@@ -16,7 +16,9 @@
 // Each level runs:
 //   scan   one lane per vertex: next = OR of the in-neighbours' current
 //          masks, minus visited (bottom-up, no atomics); appends the vertex
-//          to the level's list when next != 0;
+//          to the level's list when next != 0; or, while the frontier is
+//          small, push: the frontier's out-edges OR their sources into the
+//          targets' next masks (top-down, atomics), with the same result;
 //   sigma  one wavefront per listed vertex: for every in-neighbour whose
 //          current mask meets next, lanes of those sources add the
 //          neighbour's sigma (integers below 2^53, so exact in any order).
@@ -38,6 +40,9 @@ namespace spm {
 constexpr int kBcBlock = 256;
 constexpr int kBcWaves = kBcBlock / kWave;
 constexpr int kBatch = 64;
+// A level whose frontier holds fewer than V / kPushRatio vertices is expanded
+// top-down (its out-edges) rather than bottom-up (every vertex's in-edges).
+constexpr uint64_t kPushRatio = 14;
 
 struct BcWork {
   double* sigma = nullptr;          // [V][64]
@@ -90,7 +95,8 @@ __global__ __launch_bounds__(kBcBlock) void bc_scan(
       const unsigned long long vis = visited[w];
       const unsigned long long unv = batchmask & ~vis;
       if (unv) {
-        for (int32_t i = roff[w]; i < roff[w + 1]; ++i) nm |= cur[radj[i]];
+        // Stops once every source still missing at w has been found.
+        for (int32_t i = roff[w], e = roff[w + 1]; i < e && (nm & unv) != unv; ++i) nm |= cur[radj[i]];
         nm &= unv;
         if (nm) visited[w] = vis | nm;
       }
@@ -104,6 +110,82 @@ __global__ __launch_bounds__(kBcBlock) void bc_scan(
       base = __shfl(base, leader, kWave);
       if (nm) list[base + __popcll(b & ((1ull << lane) - 1ull))] = int32_t(w);
     }
+  }
+}
+
+// Top-down step for a small frontier. A workgroup takes kPushTile frontier
+// vertices per round, each wavefront one vertex at a time with its lanes over
+// the out-edges: the sources of v not yet at w are ORed into nxt[w] and
+// visited[w], and the lane whose OR made nxt[w] non-zero appends w. Appends
+// are staged in LDS and reserved with one global atomic per workgroup round
+// (a shared counter hit once per vertex would serialise the launch); a round
+// that overflows the stage reserves the excess per wavefront.
+// nxt must be zero on entry. A stale read of visited[w] is harmless: it holds
+// at least the bits of earlier levels (written by earlier launches), and any
+// bit of this level it already holds is in nxt[w] too, so nxt[w] ends as the
+// OR over in-neighbours at this level minus the earlier levels -- the
+// bottom-up scan's value.
+constexpr int kPushTile = 64;
+constexpr int kPushStage = 4096;
+__global__ __launch_bounds__(kBcBlock) void bc_push(
+    const int32_t* __restrict__ front, unsigned int n, const int32_t* __restrict__ off,
+    const int32_t* __restrict__ adj, const unsigned long long* __restrict__ cur,
+    unsigned long long* __restrict__ nxt, unsigned long long* __restrict__ visited,
+    int32_t* __restrict__ list, unsigned int* __restrict__ cnt) {
+  __shared__ int32_t s_buf[kPushStage];
+  __shared__ unsigned int s_n, s_base;
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  for (int64_t t0 = int64_t(blockIdx.x) * kPushTile; t0 < n; t0 += int64_t(gridDim.x) * kPushTile) {
+    const int64_t t1 = std::min<int64_t>(t0 + kPushTile, n);
+    for (int64_t i = t0 + wid; i < t1; i += kBcWaves) {
+      const int32_t v = front[i];
+      const unsigned long long cv = cur[v];
+      const int32_t b = off[v], e = off[v + 1];
+      for (int32_t i0 = b; i0 < e; i0 += kWave) {
+        const int32_t j = i0 + lane;
+        bool add = false;
+        int32_t w = 0;
+        if (j < e) {
+          w = adj[j];
+          const unsigned long long bits =
+              cv & ~__hip_atomic_load(&visited[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (bits) {
+            add = atomicOr(&nxt[w], bits) == 0ull;
+            atomicOr(&visited[w], bits);
+          }
+        }
+        const unsigned long long bl = __ballot(add);
+        if (bl) {
+          const int leader = __ffsll((long long)bl) - 1;
+          const unsigned int k = unsigned(__popcll(bl));
+          unsigned int p = 0;
+          if (lane == leader) p = atomicAdd(&s_n, k);
+          p = __shfl(p, leader, kWave);
+          // Stage slots p .. p+k-1; the ones at or past kPushStage go to the
+          // list directly, reserved by the group's leader.
+          const unsigned int cap = unsigned(kPushStage);
+          const unsigned int over = p + k > cap ? std::min(k, p + k - cap) : 0u;
+          unsigned int g = 0;
+          if (over && lane == leader) g = atomicAdd(cnt, over);
+          g = __shfl(g, leader, kWave);
+          const unsigned int q = p + unsigned(__popcll(bl & ((1ull << lane) - 1ull)));
+          if (add) {
+            if (q < cap) s_buf[q] = w;
+            else list[g + (q - std::max(p, cap))] = w;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    const unsigned int m = std::min(s_n, unsigned(kPushStage));
+    if (threadIdx.x == 0 && m) s_base = atomicAdd(cnt, m);
+    __syncthreads();
+    for (unsigned int k = threadIdx.x; k < m; k += kBcBlock) list[s_base + k] = s_buf[k];
+    __syncthreads();
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
   }
 }
 
@@ -287,9 +369,19 @@ static int bc_forward(spm_graph* g, const int32_t* h_src, int nb, int* levels_ou
     SPM_TRY(bc_reserve(g, size_t(d) + 2, size_t(used) + Vs));
     unsigned long long* cur = w->masks + size_t(d) * Vs;
     unsigned long long* nxt = w->masks + size_t(d + 1) * Vs;
-    SPM_TRY(launch(g, "bc_scan", bc_scan, dim3(sgrid), dim3(kBcBlock), 0, V, (const int32_t*)g->roff,
-                   (const int32_t*)g->radj, (const unsigned long long*)cur, nxt, w->visited, batchmask,
-                   w->lists, w->cnt));
+    const unsigned int fb = lstart[lstart.size() - 2], nf = used - fb;  // level d's list
+    if (uint64_t(nf) * kPushRatio < uint64_t(V)) {
+      // Small frontier: push along its out-edges instead of scanning every
+      // vertex's in-edges.
+      SPM_CHECK_HIP(hipMemsetAsync(nxt, 0, sizeof(unsigned long long) * Vs, g->stream));
+      SPM_TRY(launch(g, "bc_push", bc_push, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + fb), nf,
+                     (const int32_t*)g->off, (const int32_t*)g->adj, (const unsigned long long*)cur, nxt,
+                     w->visited, w->lists, w->cnt));
+    } else {
+      SPM_TRY(launch(g, "bc_scan", bc_scan, dim3(sgrid), dim3(kBcBlock), 0, V, (const int32_t*)g->roff,
+                     (const int32_t*)g->radj, (const unsigned long long*)cur, nxt, w->visited, batchmask,
+                     w->lists, w->cnt));
+    }
     SPM_CHECK_HIP(hipMemcpyAsync(h, w->cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, g->stream));
     SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
     const unsigned int total = h[0];

@@ -125,8 +125,23 @@ __global__ __launch_bounds__(kCheckBlock) void pr_check(const float* __restrict_
   __shared__ float s_red[kCheckBlock / kWave];
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
-  float t = 0.f;
-  for (int i = tid; i < n; i += kCheckBlock) t += partial[i];
+  // 16-byte loads into four independent sums, two loads per step, so the one
+  // workgroup keeps several requests in flight (partial is 256-byte aligned).
+  const float4* p4 = reinterpret_cast<const float4*>(partial);
+  const int n4 = n >> 2;
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+  int i = tid;
+  for (; i + kCheckBlock < n4; i += 2 * kCheckBlock) {
+    const float4 x = p4[i], y = p4[i + kCheckBlock];
+    a.x += x.x; a.y += x.y; a.z += x.z; a.w += x.w;
+    b.x += y.x; b.y += y.y; b.z += y.z; b.w += y.w;
+  }
+  if (i < n4) {
+    const float4 x = p4[i];
+    a.x += x.x; a.y += x.y; a.z += x.z; a.w += x.w;
+  }
+  float t = ((a.x + b.x) + (a.y + b.y)) + ((a.z + b.z) + (a.w + b.w));
+  if (tid < (n & 3)) t += partial[(n4 << 2) + tid];
   t = wave_sum(t);
   if (lane == 0) s_red[wid] = t;
   __syncthreads();

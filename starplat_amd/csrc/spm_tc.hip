@@ -65,24 +65,40 @@ __device__ __forceinline__ int32_t lower_bound_dev(const int32_t* __restrict__ a
 
 // Checks ascending rows and symmetry; flags[0] |= 1 on an unsorted row,
 // flags[0] |= 2 when some edge (u,w) has no (w,u).
+// Flags: 1 = a row is not strictly ascending; 2 = some edge has no reverse
+// edge (not symmetric); 4 = the rows differ from the reverse CSR's (symmetry
+// undecided by the streaming test). With the reverse CSR at hand (kRev), a
+// graph is symmetric exactly when every out-row equals the in-row of the same
+// vertex, which is a streaming comparison; without it, or when the rows
+// differ, each edge's reverse is looked up by binary search.
+template <bool kRev>
 __global__ __launch_bounds__(kTcBlock) void tc_check(int32_t V, const int32_t* __restrict__ off,
                                                       const int32_t* __restrict__ adj,
+                                                      const int32_t* __restrict__ roff,
+                                                      const int32_t* __restrict__ radj,
                                                       unsigned long long* flags) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kTcWaves;
   unsigned long long f = 0;
   for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw) {
     const int32_t b = off[u], e = off[u + 1];
+    if (kRev && (roff[u] != b || roff[u + 1] != e)) f |= 4;
     for (int32_t i = b + lane; i < e; i += kWave) {
       const int32_t w = adj[i];
       if (i > b && adj[i - 1] >= w) f |= 1;
-      const int32_t wb = off[w], wl = off[w + 1] - wb;
-      const int32_t p = lower_bound_dev(adj + wb, wl, int32_t(u));
-      if (p >= wl || adj[wb + p] != u) f |= 2;
+      if (kRev) {
+        if (!(f & 4) && radj[i] != w) f |= 4;
+      } else {
+        const int32_t wb = off[w], wl = off[w + 1] - wb;
+        const int32_t p = lower_bound_dev(adj + wb, wl, int32_t(u));
+        if (p >= wl || adj[wb + p] != u) f |= 2;
+      }
     }
   }
   // One atomic per wavefront with any flag.
-  const unsigned long long any = (__ballot((f & 1ull) != 0) ? 1ull : 0ull) | (__ballot((f & 2ull) != 0) ? 2ull : 0ull);
+  unsigned long long any = 0;
+  for (int k = 0; k < 3; ++k)
+    if (__ballot((f >> k) & 1ull)) any |= 1ull << k;
   if (lane == 0 && any) atomicOr(flags, any);
 }
 
@@ -302,11 +318,20 @@ static int tc_prepare(spm_graph* g) {
   if (!w->maxd) SPM_TRY(dmalloc(g, (void**)&w->maxd, sizeof(int32_t), "tc max row"));
   if (!w->checked) {
     SPM_CHECK_HIP(hipMemsetAsync(w->acc, 0, 2 * sizeof(unsigned long long), g->stream));
-    SPM_TRY(launch(g, "tc_check", tc_check, dim3(grid), dim3(kTcBlock), 0, V, (const int32_t*)g->off,
-                   (const int32_t*)g->adj, w->acc + 1));
-    unsigned long long f = 0;
-    SPM_CHECK_HIP(hipMemcpyAsync(&f, w->acc + 1, sizeof f, hipMemcpyDeviceToHost, g->stream));
-    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    unsigned long long f = 4;
+    if (g->roff) {
+      SPM_TRY(launch(g, "tc_check", tc_check<true>, dim3(grid), dim3(kTcBlock), 0, V, (const int32_t*)g->off,
+                     (const int32_t*)g->adj, (const int32_t*)g->roff, (const int32_t*)g->radj, w->acc + 1));
+      SPM_CHECK_HIP(hipMemcpyAsync(&f, w->acc + 1, sizeof f, hipMemcpyDeviceToHost, g->stream));
+      SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    }
+    if (f & 4) {
+      SPM_CHECK_HIP(hipMemsetAsync(w->acc + 1, 0, sizeof(unsigned long long), g->stream));
+      SPM_TRY(launch(g, "tc_check", tc_check<false>, dim3(grid), dim3(kTcBlock), 0, V, (const int32_t*)g->off,
+                     (const int32_t*)g->adj, (const int32_t*)nullptr, (const int32_t*)nullptr, w->acc + 1));
+      SPM_CHECK_HIP(hipMemcpyAsync(&f, w->acc + 1, sizeof f, hipMemcpyDeviceToHost, g->stream));
+      SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    }
     if (f & 1) return set_error(SPM_EINVAL, "TC needs ascending adjacency rows (graph.hpp sorts them)");
     w->symmetric = (f & 2) == 0;
     w->checked = 1;

@@ -235,6 +235,27 @@ def test_bc_parity(oracle, nsrc):
     assert rel_linf(bc, ref) <= BC_TOL, rel_linf(bc, ref)
 
 
+def test_bc_hub_frontier(oracle):
+    """A hub in a small frontier: the top-down step appends more vertices in
+    one workgroup round than it stages in LDS (4096), so the overflow path and
+    a level mixing both runs; two hubs share leaves so targets are reached
+    from several frontier vertices at once."""
+    rng = np.random.default_rng(11)
+    n = 20000
+    leaves = np.arange(2, n, dtype=np.int32)
+    src = [np.zeros(n - 2, np.int32), np.ones(n // 2, np.int32), rng.integers(2, n, 60000).astype(np.int32)]
+    dst = [leaves, leaves[: n // 2], rng.integers(2, n, 60000).astype(np.int32)]
+    g = graph.from_edges(np.concatenate(src + dst), np.concatenate(dst + src))  # symmetric
+    for s in (0, 1, 5):
+        _, lev_ref, sig_ref = oracle.compute_bc(g, [s])
+        lev, sig = g.device().bc_paths(s)
+        np.testing.assert_array_equal(lev, lev_ref)
+        np.testing.assert_array_equal(sig, sig_ref)
+    srcs = np.array([0, 1] + list(range(100, 162)) + [7], np.int32)
+    ref, _, _ = oracle.compute_bc(g, srcs)
+    assert rel_linf(g.device().bc(srcs), ref) <= BC_TOL
+
+
 def test_bc_directed_rmat(oracle):
     g = graph.rmat(12, 16, seed=8)
     src = sources(g.V, 70, seed=1)

@@ -43,6 +43,8 @@ constexpr int kBatch = 64;
 // A level whose frontier holds fewer than V / kPushRatio vertices is expanded
 // top-down (its out-edges) rather than bottom-up (every vertex's in-edges).
 constexpr uint64_t kPushRatio = 14;
+// Active neighbours whose sigma rows one wavefront loads per step.
+constexpr int kGather = 4;
 
 struct BcWork {
   double* sigma = nullptr;          // [V][64]
@@ -210,11 +212,22 @@ __global__ __launch_bounds__(kBcBlock) void bc_sigma(
       }
       unsigned long long act = __ballot(m != 0ull);
       while (act) {
-        const int src_lane = __ffsll((long long)act) - 1;
-        act &= act - 1;
-        const int32_t vv = __shfl(v, src_lane, kWave);
-        const unsigned long long mm = __shfl(m, src_lane, kWave);
-        if ((mm >> lane) & 1ull) acc += sigma[size_t(vv) * kBatch + lane];
+        // Up to kGather active in-neighbours per step: their row loads are
+        // issued together (sums of integers below 2^53 are exact in any order).
+        double x[kGather];
+#pragma unroll
+        for (int k = 0; k < kGather; ++k) {
+          x[k] = 0.0;
+          if (act) {
+            const int src_lane = __ffsll((long long)act) - 1;
+            act &= act - 1;
+            const int32_t vv = __shfl(v, src_lane, kWave);
+            const unsigned long long mm = __shfl(m, src_lane, kWave);
+            if ((mm >> lane) & 1ull) x[k] = sigma[size_t(vv) * kBatch + lane];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kGather; ++k) acc += x[k];
       }
     }
     if ((nm >> lane) & 1ull) sigma[size_t(w) * kBatch + lane] = acc;
@@ -244,12 +257,27 @@ __global__ __launch_bounds__(kBcBlock) void bc_back(
       }
       unsigned long long act = __ballot(m != 0ull);
       while (act) {
-        const int src_lane = __ffsll((long long)act) - 1;
-        act &= act - 1;
-        const int32_t ww = __shfl(w, src_lane, kWave);
-        const unsigned long long mm = __shfl(m, src_lane, kWave);
-        // sigma[w][s] of a successor already holds coef_w = (1 + delta_w) / sigma_w
-        if ((mm >> lane) & 1ull) acc += float(sv * sigma[size_t(ww) * kBatch + lane]);
+        // Up to kGather active successors per step: loads issued together,
+        // terms added in ascending neighbour order. sigma[w][s] of a successor
+        // already holds coef_w = (1 + delta_w) / sigma_w.
+        double c[kGather];
+        bool on[kGather];
+#pragma unroll
+        for (int k = 0; k < kGather; ++k) {
+          on[k] = false;
+          c[k] = 0.0;
+          if (act) {
+            const int src_lane = __ffsll((long long)act) - 1;
+            act &= act - 1;
+            const int32_t ww = __shfl(w, src_lane, kWave);
+            const unsigned long long mm = __shfl(m, src_lane, kWave);
+            on[k] = (mm >> lane) & 1ull;
+            if (on[k]) c[k] = sigma[size_t(ww) * kBatch + lane];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < kGather; ++k)
+          if (on[k]) acc += float(sv * c[k]);
       }
     }
     float part = 0.f;

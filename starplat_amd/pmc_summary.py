@@ -14,6 +14,10 @@ sums over all their dispatches divided by the number of dispatches.
 
 Prints one markdown table row; bench.py reads rows of this form from the
 profiles directory to fill `roofline.traffic`.
+
+    python -m starplat_amd.pmc_summary --kernel-stats K/run_kernel_stats.csv [--top 8]
+
+formats the per-kernel summary of a `rocprofv3 --kernel-trace --stats` run.
 """
 import argparse
 import csv
@@ -90,15 +94,36 @@ def parse_rows(text):
     return out
 
 
+def stats_table(path, top=8):
+    """Markdown table of the `top` kernels by total time in a rocprofv3
+    kernel_stats CSV."""
+    with open(path, newline="") as f:
+        rows = list(csv.DictReader(f))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = ["| kernel | calls | total ms | average us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    for r in rows[:top]:
+        out.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} "
+                   f"| {float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} "
+                   f"| {float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--copy-mib", type=float, required=True)
-    ap.add_argument("--workload", required=True)
-    ap.add_argument("--kernels", required=True, help="comma-separated kernel names (without namespace)")
+    ap.add_argument("--kernel-stats", help="format a rocprofv3 kernel_stats CSV and exit")
+    ap.add_argument("--top", type=int, default=8)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-mib", type=float)
+    ap.add_argument("--workload")
+    ap.add_argument("--kernels", help="comma-separated kernel names (without namespace)")
     ap.add_argument("--header", action="store_true")
     a = ap.parse_args(argv)
+    if a.kernel_stats:
+        print(stats_table(a.kernel_stats, a.top))
+        return 0
+    if not (a.fetch and a.write and a.copy_mib and a.workload and a.kernels):
+        ap.error("--fetch, --write, --copy-mib, --workload and --kernels are required")
     kernels = tuple(a.kernels.split(","))
     s = summarize(a.fetch, a.write, a.copy_mib, kernels)
     if a.header:

@@ -114,6 +114,51 @@ int spm_bc_device(spm_graph* g, const int32_t* sources, int32_t nsrc, float* d_B
  * (backend_omp/dsl_cpp_generator.cpp:66-130); host buffers. */
 int spm_bc_paths(spm_graph* g, int32_t src, int32_t* bfsDist, double* sigma);
 
+/* ---- edge updates (dynamic graphs) -----------------------------------------
+ * Replace graph::updateCSRDel and graph::updateCSRAdd (graphcode/graph.hpp:
+ * 869-975 over graph_ompv2.hpp:13-454), which generated Dynamic functions call
+ * once per batch (backend_omp/dsl_dyn_cpp_generator.cpp:301-316). Edge
+ * (src[i], dst[i]); vertices must lie in 0..num_nodes()-1 (no new vertices).
+ * A deletion removes every copy of the edge and ignores absent ones; an
+ * addition appends an edge even if one exists (weight[i], or 1 when weight is
+ * NULL, as update.hpp:56-60). Updates are kept beside the CSR (removed-slot
+ * bits, per-vertex overflow rows, as getNeighbors' diff_edgeList,
+ * graph.hpp:982-1062); the next static algorithm, spm_graph_compact or
+ * spm_graph_export folds them into a clean sorted CSR first. */
+int spm_graph_delete_edges(spm_graph* g, const int32_t* src, const int32_t* dst, int32_t n);
+int spm_graph_add_edges(spm_graph* g, const int32_t* src, const int32_t* dst, const int32_t* weight, int32_t n);
+int spm_graph_compact(spm_graph* g);
+/* Edges of the current graph (num_edges() after the updates, graph.hpp:145-153). */
+int spm_graph_num_edges(spm_graph* g, int64_t* num_edges);
+/* Copies the current (folded) arrays to host buffers sized num_nodes()+1 and
+ * num_edges; any pointer may be NULL. Rows ascending, copies of an edge kept. */
+int spm_graph_export(spm_graph* g, int32_t* indexofNodes, int32_t* edgeList, int32_t* edgeLen,
+                     int32_t* rev_indexofNodes, int32_t* srcList);
+
+/* ---- dynamic SSSP ------------------------------------------------------------
+ * Replaces the Dynamic function DynSSSP(Graph g, propNode<int> dist,
+ * propNode<int> parent, propEdge<int> weight, updates<g> updateBatch,
+ * int batchSize, int src) of graphcode/dynamicDSLCodes/dynamicBatchSSSP:67-94
+ * with its staticSSSP (:1-18), Incremental (:20-35) and Decremental (:37-65).
+ * Per batch: OnDelete (a deleted parent edge cuts its head off), updateCSRDel,
+ * Decremental (cut-off subtrees, then a pull fixed point from in-neighbours),
+ * OnAdd (an added edge that shortens its head marks both ends), updateCSRAdd
+ * (added edges weigh 1), Incremental (push fixed point from the marks).
+ * Needs weights >= 1 and the reverse CSR. dist: INT_MAX never reached,
+ * INT_MAX/2 cut off and not reached again (the reference's markers); parent:
+ * -1 for the source and unreached vertices, else the smallest-id predecessor
+ * on a shortest path among those the run relaxed through. */
+int spm_dynsssp_init(spm_graph* g, int32_t src);
+/* One batch of n updates: type[i] is 'a' (add) or 'd' (delete). */
+int spm_dynsssp_batch(spm_graph* g, const char* type, const int32_t* src, const int32_t* dst, int32_t n);
+/* Current dist / parent (host buffers of num_nodes(); either may be NULL). */
+int spm_dynsssp_result(spm_graph* g, int32_t* dist, int32_t* parent);
+int spm_dynsssp_device(spm_graph* g, int32_t* d_dist, int32_t* d_parent);
+/* The whole Dynamic function: init from src, then the updates in batches of
+ * batchSize (the generated loop, dsl_dyn_cpp_generator.cpp:132-150). */
+int spm_dyn_sssp(spm_graph* g, int32_t src, const char* type, const int32_t* usrc, const int32_t* udst, int32_t n,
+                 int32_t batchSize, int32_t* dist, int32_t* parent);
+
 /* ---- schedules -----------------------------------------------------------
  * The schedule changes how work maps to lanes, never the arithmetic. No
  * counterpart in the reference (its CUDA backend hard-codes 512 threads per

@@ -441,4 +441,140 @@ void tb_pr_fp64(int32_t V, const int32_t* off, const int32_t* roff, const int32_
 
 int oracle_num_threads(void) { return omp_get_max_threads(); }
 
+// Restatement of the Dynamic function DynSSSP of
+// graphcode/dynamicDSLCodes/dynamicBatchSSSP:67-94 (staticSSSP :1-18,
+// Incremental :20-35, Decremental :37-65), sequential, over an edge
+// multiset. Per batch (the generated loop, dsl_dyn_cpp_generator.cpp:132-150):
+//   OnDelete    for each deletion (s,t): if parent[t] == s cut t off
+//               (dist INT_MAX/2, parent -1, modified)
+//   updateCSRDel  every copy of each deleted edge goes
+//   Decremental   (a) repeat: a vertex whose parent is modified is cut off
+//                 (b) repeat: each modified vertex takes its best finite
+//                     in-neighbour
+//   OnAdd       for each addition (s,t): if dist[t] > dist[s] + 1 mark s, t
+//   updateCSRAdd  added edges weigh 1 (update.hpp:56-60)
+//   Incremental   push fixed point from the marked vertices
+// Rules where the DSL leaves a choice or the emitted code overflows:
+// distances >= INT_MAX/2 are infinite (never relaxed from; the emitted code
+// computes INT_MAX + w); (distance, parent) pairs compare as one key, distance
+// first, so among equally short predecessors the smallest id is the parent;
+// after staticSSSP the parent is the smallest-id predecessor on a shortest
+// path. plant 1: skip Decremental's loop (a); plant 2: OnAdd marks only t.
+// Returns the number of batches, or -1 if a loop exceeds V rounds.
+int oracle_dyn_sssp(int32_t V, const int32_t* off, const int32_t* adj, const int32_t* w, int32_t src,
+                    int32_t n, const char* type, const int32_t* us, const int32_t* ut, int32_t batch,
+                    int32_t* dist_out, int32_t* parent_out, int plant) {
+  const int32_t INF2 = INT_MAX / 2;
+  using Key = uint64_t;
+  auto key = [](int32_t d, int64_t p) { return (Key(uint32_t(d)) << 32) | Key(uint32_t(p)); };
+  auto kd = [](Key k) { return int32_t(k >> 32); };
+  const Key NOP = 0xFFFFFFFFull;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> out(V), in(V);
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t e = off[v]; e < off[v + 1]; ++e) {
+      out[v].push_back({adj[e], w[e]});
+      in[adj[e]].push_back({v, w[e]});
+    }
+  // staticSSSP: Bellman-Ford to the fixed point, then parents.
+  std::vector<int32_t> dist(V, INT_MAX);
+  dist[src] = 0;
+  for (bool changed = true; changed;) {
+    changed = false;
+    for (int32_t v = 0; v < V; ++v) {
+      if (dist[v] == INT_MAX) continue;
+      for (auto& [t, wt] : out[v])
+        if (dist[v] + wt < dist[t]) { dist[t] = dist[v] + wt; changed = true; }
+    }
+  }
+  std::vector<Key> dp(V);
+  for (int32_t v = 0; v < V; ++v) {
+    Key best = NOP;
+    if (v != src && dist[v] < INF2)
+      for (auto& [u, wt] : in[v])
+        if (dist[u] < INF2 && dist[u] + wt == dist[v] && Key(u) < best) best = Key(u);
+    dp[v] = key(dist[v], int64_t(best));
+  }
+  int32_t batches = 0;
+  for (int32_t k = 0; k < n; k += batch, ++batches) {
+    const int32_t m = std::min(batch, n - k);
+    std::vector<char> mod(V, 0), madd(V, 0);
+    std::vector<std::pair<int32_t, int32_t>> dels, adds;
+    for (int32_t i = k; i < k + m; ++i) (type[i] == 'd' ? dels : adds).push_back({us[i], ut[i]});
+    if (!dels.empty()) {
+      for (auto [s, t] : dels)
+        if (uint32_t(dp[t]) == uint32_t(s)) { dp[t] = key(INF2, -1); mod[t] = 1; }
+      for (auto [s, t] : dels) {
+        auto& o = out[s];
+        o.erase(std::remove_if(o.begin(), o.end(), [t = t](auto& x) { return x.first == t; }), o.end());
+        auto& r = in[t];
+        r.erase(std::remove_if(r.begin(), r.end(), [s = s](auto& x) { return x.first == s; }), r.end());
+      }
+      if (plant != 1) {
+        for (int64_t it = 0;; ++it) {
+          bool changed = false;
+          for (int32_t v = 0; v < V; ++v) {
+            if (mod[v]) continue;
+            const uint32_t p = uint32_t(dp[v]);
+            if (p != uint32_t(NOP) && mod[p]) { mod[v] = 1; dp[v] = key(INF2, -1); changed = true; }
+          }
+          if (!changed) break;
+          if (it > V) return -1;
+        }
+      }
+      for (int64_t it = 0;; ++it) {
+        bool changed = false;
+        for (int32_t v = 0; v < V; ++v) {
+          if (!mod[v]) continue;
+          Key best = dp[v];
+          for (auto& [u, wt] : in[v]) {
+            const int32_t du = kd(dp[u]);
+            if (du < INF2 && key(du + wt, u) < best) best = key(du + wt, u);
+          }
+          if (best < dp[v]) { dp[v] = best; changed = true; }
+        }
+        if (!changed) break;
+        if (it > V) return -1;
+      }
+    }
+    if (!adds.empty()) {
+      for (auto [s, t] : adds) {
+        const int32_t ds = kd(dp[s]);
+        if (ds < INF2 && kd(dp[t]) > ds + 1) {
+          if (plant != 2) madd[s] = 1;
+          madd[t] = 1;
+        }
+      }
+      for (auto [s, t] : adds) {
+        out[s].push_back({t, 1});
+        in[t].push_back({s, 1});
+      }
+      std::vector<char> cur = madd;
+      for (int64_t it = 0;; ++it) {
+        std::vector<char> nxt(V, 0);
+        bool any = false;
+        for (int32_t v = 0; v < V; ++v) {
+          if (!cur[v]) continue;
+          const int32_t dv = kd(dp[v]);
+          if (dv >= INF2) continue;
+          for (auto& [t, wt] : out[v]) {
+            const Key c = key(dv + wt, v);
+            if (c < dp[t]) {
+              if (kd(c) < kd(dp[t])) { nxt[t] = 1; any = true; }
+              dp[t] = c;
+            }
+          }
+        }
+        if (!any) break;
+        cur.swap(nxt);
+        if (it > V) return -1;
+      }
+    }
+  }
+  for (int32_t v = 0; v < V; ++v) {
+    dist_out[v] = kd(dp[v]);
+    parent_out[v] = uint32_t(dp[v]) == uint32_t(NOP) ? -1 : int32_t(uint32_t(dp[v]));
+  }
+  return batches;
+}
+
 }  // extern "C"

@@ -38,6 +38,9 @@ def lib():
         L.tb_tc_merge.argtypes = [i32, P_i32, P_i32]
         L.tb_tc_merge.restype = i64
         L.tb_pr_fp64.argtypes = [i32, P_i32, P_i32, P_i32, f64, f64, i32, P_f64, P_i32]
+        L.oracle_dyn_sssp.argtypes = [i32, P_i32, P_i32, P_i32, i32, i32, C.c_char_p, P_i32, P_i32, i32, P_i32,
+                                      P_i32, C.c_int]
+        L.oracle_dyn_sssp.restype = C.c_int
         L.oracle_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -116,3 +119,52 @@ def pr_fp64(g, beta=1e-10, delta=0.85, maxIter=100):
     lib().tb_pr_fp64(g.V, _p(g.indexofNodes, i32), _p(g.rev_indexofNodes, i32), _p(g.srcList, i32), beta, delta,
                      maxIter, _p(pr, f64), C.byref(it))
     return pr, it.value
+
+
+def dyn_sssp(g, weight, src, types, us, ut, batch, plant=0):
+    """DynSSSP restated (oracle.cpp): (dist, parent) after every batch of
+    `batch` updates; types is a bytes string of 'a' / 'd'."""
+    w = np.ascontiguousarray(weight, np.int32)
+    us = np.ascontiguousarray(us, np.int32)
+    ut = np.ascontiguousarray(ut, np.int32)
+    t = bytes(types)
+    assert len(t) == len(us) == len(ut)
+    dist = np.empty(g.V, np.int32)
+    parent = np.empty(g.V, np.int32)
+    rc = lib().oracle_dyn_sssp(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), _p(w, i32), int(src), len(us), t,
+                               _p(us, i32), _p(ut, i32), int(batch), _p(dist, i32), _p(parent, i32), int(plant))
+    if rc < 0:
+        raise RuntimeError("oracle_dyn_sssp: a loop did not converge")
+    return dist, parent
+
+
+def apply_updates(V, off, adj, w, ops, plant=0):
+    """Edge-multiset restatement of updateCSRDel / updateCSRAdd
+    (graph.hpp:869-975): ops is a sequence of ("d", src, dst) or
+    ("a", src, dst, weight) array tuples applied in order; a deletion removes
+    every copy of the edge, an addition appends one. Returns the CSR with rows
+    sorted by (neighbour, weight): (indexofNodes, edgeList, edgeLen).
+    plant 1: a deletion removes only the first copy."""
+    off = np.asarray(off, np.int64)
+    s = np.repeat(np.arange(V, dtype=np.int64), np.diff(off))
+    d = np.asarray(adj, np.int64).copy()
+    ww = np.asarray(w, np.int64).copy()
+    for op in ops:
+        if op[0] == "d":
+            for a, b in zip(np.asarray(op[1], np.int64), np.asarray(op[2], np.int64)):
+                hit = (s == a) & (d == b)
+                if plant == 1 and hit.any():
+                    first = np.argmax(hit)
+                    hit = np.zeros_like(hit)
+                    hit[first] = True
+                keep = ~hit
+                s, d, ww = s[keep], d[keep], ww[keep]
+        else:
+            s = np.concatenate([s, np.asarray(op[1], np.int64)])
+            d = np.concatenate([d, np.asarray(op[2], np.int64)])
+            ww = np.concatenate([ww, np.asarray(op[3], np.int64)])
+    order = np.lexsort((ww, d, s))
+    s, d, ww = s[order], d[order], ww[order]
+    new_off = np.zeros(V + 1, np.int64)
+    np.add.at(new_off, s + 1, 1)
+    return np.cumsum(new_off).astype(np.int32), d.astype(np.int32), ww.astype(np.int32)

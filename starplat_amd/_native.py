@@ -86,6 +86,16 @@ DEVICE_API = [
     ("spm_compute_bc", C.c_int, [vp, P_f32, P_i32, i32]),
     ("spm_bc_device", C.c_int, [vp, P_i32, i32, vp]),
     ("spm_bc_paths", C.c_int, [vp, i32, P_i32, P_f64]),
+    ("spm_graph_delete_edges", C.c_int, [vp, P_i32, P_i32, i32]),
+    ("spm_graph_add_edges", C.c_int, [vp, P_i32, P_i32, P_i32, i32]),
+    ("spm_graph_compact", C.c_int, [vp]),
+    ("spm_graph_num_edges", C.c_int, [vp, P_i64]),
+    ("spm_graph_export", C.c_int, [vp, P_i32, P_i32, P_i32, P_i32, P_i32]),
+    ("spm_dynsssp_init", C.c_int, [vp, i32]),
+    ("spm_dynsssp_batch", C.c_int, [vp, C.c_char_p, P_i32, P_i32, i32]),
+    ("spm_dynsssp_result", C.c_int, [vp, P_i32, P_i32]),
+    ("spm_dynsssp_device", C.c_int, [vp, vp, vp]),
+    ("spm_dyn_sssp", C.c_int, [vp, i32, C.c_char_p, P_i32, P_i32, i32, i32, P_i32, P_i32]),
     ("spm_get_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
     ("spm_set_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
     ("spm_default_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),

@@ -5,6 +5,8 @@ meaning, running on the gfx950 kernel library through the C ABI.
     Compute_SSSP(g, dist, weight, src)              sssp_dslV2
     Compute_TC(g) -> int                            triangle_counting_dsl
     Compute_BC(g, BC, sourceSet)                    bc_dslV2
+    DynSSSP(g, dist, parent, weight, updateBatch, batchSize, src)
+                                                    dynamicBatchSSSP
 
 Output arrays are caller-owned numpy arrays of num_nodes() entries, filled in
 place, as the generated C++ fills the caller's pointers
@@ -48,3 +50,28 @@ def Compute_BC(g, BC, sourceSet):
     _out(BC, np.float32, g.num_nodes(), "BC")
     src = np.array(sorted(set(int(s) for s in sourceSet)), np.int32)
     g.device().bc(src, out=BC)
+
+
+def DynSSSP(g, dist, parent, weight, updateBatch, batchSize, src):
+    """The Dynamic function of graphcode/dynamicDSLCodes/dynamicBatchSSSP:67-94:
+    staticSSSP from src, then per batch of batchSize updates OnDelete,
+    updateCSRDel, Decremental, OnAdd, updateCSRAdd, Incremental (added edges
+    weigh 1). weight: one entry per CSR edge of g before the updates (None
+    keeps the device mirror's). updateBatch: a list of ``updates.update``.
+    dist / parent are filled in place; g's arrays then describe the updated
+    graph, as the reference mutates its graph."""
+    from .updates import as_arrays
+    V = g.num_nodes()
+    _out(dist, np.int32, V, "dist")
+    _out(parent, np.int32, V, "parent")
+    dev = g.device()
+    if weight is not None:
+        w = np.ascontiguousarray(weight, np.int32)
+        if w.size != dev.num_edges():
+            raise ValueError("weight needs one entry per CSR edge (indexed by edge position)")
+        dev.set_weights(w)
+    types, s, d, _ = as_arrays(list(updateBatch))
+    dd, pp = dev.dyn_sssp(int(src), types, s, d, int(batchSize))
+    dist[:] = dd
+    parent[:] = pp
+    g.refresh_from_device()

@@ -452,6 +452,7 @@ static int bc_check_sources(const spm_graph* g, const int32_t* sources, int32_t 
 int bc_run(spm_graph* g, const int32_t* sources, int32_t nsrc, float* d_BC) {
   std::vector<int32_t> s;
   SPM_TRY(bc_check_sources(g, sources, nsrc, s));
+  SPM_TRY(dyn_settle(g));
   SPM_TRY(bc_alloc(g));
   const int igrid = int(std::min<int64_t>((g->V + 255) / 256, 4096));
   SPM_TRY(launch(g, "bc_zero", bc_zero, dim3(igrid), dim3(256), 0, g->V, d_BC));
@@ -495,6 +496,7 @@ int spm_compute_bc(spm_graph* g, float* BC, const int32_t* sources, int32_t nsrc
 int spm_bc_paths(spm_graph* g, int32_t src, int32_t* bfsDist, double* sigma) {
   if (!g || !bfsDist || !sigma) return set_error(SPM_EINVAL, "spm_bc_paths: NULL argument");
   if (src < 0 || src >= g->V) return set_error(SPM_EINVAL, "spm_bc_paths: source out of range");
+  SPM_TRY(dyn_settle(g));
   SPM_TRY(bc_alloc(g));
   std::vector<unsigned int> lstart;
   int L = 0;

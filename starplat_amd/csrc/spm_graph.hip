@@ -174,6 +174,7 @@ int spm_graph_destroy(spm_graph* g) {
   sssp_release(g);
   tc_release(g);
   bc_release(g);
+  dyn_release(g);
   const size_t vb = sizeof(int32_t) * (size_t(g->V) + 1), eb = sizeof(int32_t) * size_t(g->E);
   dfree(g, g->off, vb);
   dfree(g, g->adj, eb);
@@ -190,6 +191,8 @@ int spm_graph_destroy(spm_graph* g) {
 
 int spm_graph_set_weights(spm_graph* g, const int32_t* weight) {
   if (!g || !weight) return set_error(SPM_EINVAL, "spm_graph_set_weights: NULL argument");
+  SPM_TRY(dyn_settle(g));
+  dyn_weights_changed(g);
   const size_t eb = sizeof(int32_t) * size_t(g->E);
   if (!g->w) SPM_TRY(dmalloc(g, (void**)&g->w, eb, "edgeLen"));
   if (g->E) SPM_CHECK_HIP(hipMemcpyAsync(g->w, weight, eb, hipMemcpyHostToDevice, g->stream));

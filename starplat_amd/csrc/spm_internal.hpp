@@ -40,6 +40,7 @@ struct SsspWork;
 struct PrWork;
 struct TcWork;
 struct BcWork;
+struct DynWork;
 
 struct Profiler {
   bool on = false;
@@ -68,6 +69,7 @@ struct spm_graph {
   spm::PrWork* pr = nullptr;
   spm::TcWork* tc = nullptr;
   spm::BcWork* bc = nullptr;
+  spm::DynWork* dyn = nullptr;  // edge updates since the CSR was last clean
   spm::Profiler prof;
   int32_t* h_pinned = nullptr;  // small pinned scratch for flag readback (16 ints)
 };
@@ -99,6 +101,15 @@ int pr_release(spm_graph* g);
 int sssp_release(spm_graph* g);
 int tc_release(spm_graph* g);
 int bc_release(spm_graph* g);
+int dyn_release(spm_graph* g);
+// Folds pending edge updates into a clean sorted CSR; every static algorithm
+// calls it first (a no-op when there are none).
+int dyn_settle(spm_graph* g);
+// The handle's weights changed: weights derived from them are stale.
+void dyn_weights_changed(spm_graph* g);
+int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32_t* rounds_out);
+// Distances of the last SSSP run on g (device memory, V entries).
+int32_t* sssp_dist(spm_graph* g);
 
 // ---- wavefront primitives ---------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -489,6 +489,7 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
 
 int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out, float* h_out,
            int32_t* iter_out) {
+  SPM_TRY(dyn_settle(g));
   SPM_TRY(pr_prepare(g));
   PrWork* w = g->pr;
   const int32_t V = g->V;

@@ -525,7 +525,10 @@ static int sssp_capture(spm_graph* g) {
 }
 
 // srcs: n distinct sources in host memory (n == 1 passes the value directly).
+int32_t* sssp_dist(spm_graph* g) { return g->sssp ? g->sssp->dist : nullptr; }
+
 int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32_t* rounds_out) {
+  SPM_TRY(dyn_settle(g));
   if (!g->w) return set_error(SPM_ENOGRAPH, "SSSP needs edge weights (edgeLen or spm_graph_set_weights)");
   if (n < 1 || !srcs) return set_error(SPM_EINVAL, "SSSP needs at least one source");
   for (int32_t i = 0; i < n; ++i)
@@ -597,6 +600,7 @@ int spm_sssp_multi_device(spm_graph* g, const int32_t* srcs, int32_t n, int32_t*
 
 int spm_compute_sssp(spm_graph* g, int32_t* dist, const int32_t* weight, int32_t src) {
   if (!g || !dist) return set_error(SPM_EINVAL, "spm_compute_sssp: NULL argument");
+  SPM_TRY(dyn_settle(g));
   if (weight) SPM_TRY(spm_graph_set_weights(g, weight));
   SPM_TRY(sssp_run(g, &src, 1, nullptr, nullptr));
   SPM_CHECK_HIP(hipMemcpyAsync(dist, g->sssp->dist, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost, g->stream));

@@ -398,6 +398,7 @@ int tc_release(spm_graph* g) {
 }
 
 int tc_run(spm_graph* g, uint64_t* out) {
+  SPM_TRY(dyn_settle(g));
   SPM_TRY(tc_prepare(g));
   TcWork* w = g->tc;
   SPM_CHECK_HIP(hipMemsetAsync(w->acc, 0, sizeof(unsigned long long), g->stream));

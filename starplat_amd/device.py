@@ -26,6 +26,8 @@ class DeviceGraph:
             C.byref(h))
         N.check("spm_graph_create", rc)
         self.h = h.value
+        self.has_weights = w is not None
+        self.has_reverse = g.rev_indexofNodes is not None and g.srcList is not None
 
     def close(self):
         if self.h:
@@ -47,9 +49,99 @@ class DeviceGraph:
 
     def set_weights(self, weight):
         weight = np.ascontiguousarray(weight, np.int32)
+        self.num_edges()
         if weight.size != self.E:
             raise ValueError("weight must have one entry per CSR edge")
         N.check("spm_graph_set_weights", N.device_lib().spm_graph_set_weights(self.h, N.ptr(weight, C.c_int32)))
+        self.has_weights = True
+
+    # ---- edge updates -------------------------------------------------------
+    @staticmethod
+    def _pair(src, dst):
+        src = np.ascontiguousarray(src, np.int32)
+        dst = np.ascontiguousarray(dst, np.int32)
+        if src.shape != dst.shape or src.ndim != 1:
+            raise ValueError("src and dst must be 1-D arrays of one length")
+        return src, dst
+
+    def delete_edges(self, src, dst):
+        """graph::updateCSRDel for the edges (src[i], dst[i])."""
+        src, dst = self._pair(src, dst)
+        N.check("spm_graph_delete_edges", N.device_lib().spm_graph_delete_edges(
+            self.h, N.ptr(src, C.c_int32), N.ptr(dst, C.c_int32), len(src)))
+
+    def add_edges(self, src, dst, weight=None):
+        """graph::updateCSRAdd for the edges (src[i], dst[i]) (weight 1 when None)."""
+        src, dst = self._pair(src, dst)
+        if weight is not None:
+            weight = np.ascontiguousarray(weight, np.int32)
+            if weight.shape != src.shape:
+                raise ValueError("weight must match src")
+        N.check("spm_graph_add_edges", N.device_lib().spm_graph_add_edges(
+            self.h, N.ptr(src, C.c_int32), N.ptr(dst, C.c_int32), N.ptr(weight, C.c_int32), len(src)))
+
+    def compact(self):
+        N.check("spm_graph_compact", N.device_lib().spm_graph_compact(self.h))
+
+    def num_edges(self):
+        e = C.c_int64(0)
+        N.check("spm_graph_num_edges", N.device_lib().spm_graph_num_edges(self.h, C.byref(e)))
+        self.E = int(e.value)
+        return self.E
+
+    def export(self):
+        """The current CSR (after folding any updates) as numpy arrays:
+        (indexofNodes, edgeList, edgeLen or None, rev_indexofNodes or None,
+        srcList or None)."""
+        E = self.num_edges()
+        off = np.empty(self.V + 1, np.int32)
+        adj = np.empty(E, np.int32)
+        w = np.empty(E, np.int32) if self.has_weights else None
+        roff = np.empty(self.V + 1, np.int32) if self.has_reverse else None
+        radj = np.empty(E, np.int32) if self.has_reverse else None
+        N.check("spm_graph_export", N.device_lib().spm_graph_export(
+            self.h, N.ptr(off, C.c_int32), N.ptr(adj, C.c_int32), N.ptr(w, C.c_int32),
+            N.ptr(roff, C.c_int32), N.ptr(radj, C.c_int32)))
+        return off, adj, w, roff, radj
+
+    # ---- dynamic SSSP ---------------------------------------------------------
+    @staticmethod
+    def _types(types, n):
+        t = types.encode() if isinstance(types, str) else bytes(types)
+        if len(t) != n:
+            raise ValueError("one update type per update")
+        return t
+
+    def dynsssp_init(self, src):
+        N.check("spm_dynsssp_init", N.device_lib().spm_dynsssp_init(self.h, int(src)))
+
+    def dynsssp_batch(self, types, src, dst):
+        src, dst = self._pair(src, dst)
+        t = self._types(types, len(src))
+        N.check("spm_dynsssp_batch", N.device_lib().spm_dynsssp_batch(
+            self.h, t, N.ptr(src, C.c_int32), N.ptr(dst, C.c_int32), len(src)))
+
+    def dynsssp_result(self):
+        dist = np.empty(self.V, np.int32)
+        parent = np.empty(self.V, np.int32)
+        N.check("spm_dynsssp_result", N.device_lib().spm_dynsssp_result(
+            self.h, N.ptr(dist, C.c_int32), N.ptr(parent, C.c_int32)))
+        return dist, parent
+
+    def dynsssp_device(self, d_dist, d_parent=None):
+        N.check("spm_dynsssp_device", N.device_lib().spm_dynsssp_device(
+            self.h, d_dist.ptr if d_dist is not None else None, d_parent.ptr if d_parent is not None else None))
+
+    def dyn_sssp(self, src, types, usrc, udst, batch_size):
+        """The whole Dynamic function (spm_dyn_sssp): (dist, parent)."""
+        usrc, udst = self._pair(usrc, udst)
+        t = self._types(types, len(usrc))
+        dist = np.empty(self.V, np.int32)
+        parent = np.empty(self.V, np.int32)
+        N.check("spm_dyn_sssp", N.device_lib().spm_dyn_sssp(
+            self.h, int(src), t, N.ptr(usrc, C.c_int32), N.ptr(udst, C.c_int32), len(usrc), int(batch_size),
+            N.ptr(dist, C.c_int32), N.ptr(parent, C.c_int32)))
+        return dist, parent
 
     # ---- schedules ----------------------------------------------------------
     def schedule(self):

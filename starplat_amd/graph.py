@@ -55,6 +55,13 @@ class graph:
         self.srcList = _view(s.srcList, self.E)
         return self
 
+    def _native(self):
+        """The native host structure (spmh_graph*) behind the arrays."""
+        if self._h is None:
+            raise RuntimeError("the graph's native host structure is stale after edge updates; "
+                               "statistics and writers need a graph built from edges or a file")
+        return self._h
+
     def parseGraph(self):
         """graph::parseGraph(): parseEdges (graph.hpp:464-515) + parseEdgesContent (:720-859)."""
         if self.filePath is None:
@@ -94,14 +101,14 @@ class graph:
     @classmethod
     def union(cls, graphs):
         """Disjoint union; returns (graph, bases) with graph i's vertex v at bases[i] + v."""
-        arr = (N.P_spmh * len(graphs))(*[g._h for g in graphs])
+        arr = (N.P_spmh * len(graphs))(*[g._native() for g in graphs])
         bases = np.zeros(len(graphs) + 1, np.int64)
         u = cls()._adopt(N.host_lib().spmh_union(arr, len(graphs), N.ptr(bases, C.c_int64)))
         return u, bases
 
     def save(self, path):
         """Binary CSR file (header + the five arrays, checksummed)."""
-        if N.host_lib().spmh_write_binary(self._h, str(path).encode()) != 0:
+        if N.host_lib().spmh_write_binary(self._native(), str(path).encode()) != 0:
             raise OSError(f"cannot write {path}")
 
     @classmethod
@@ -112,7 +119,7 @@ class graph:
         return cls()._adopt(h)
 
     def write_text(self, path, weights=True):
-        rc = N.host_lib().spmh_write_text(self._h, str(path).encode(), int(bool(weights)))
+        rc = N.host_lib().spmh_write_text(self._native(), str(path).encode(), int(bool(weights)))
         if rc != 0:
             raise OSError(f"cannot write {path}")
 
@@ -164,6 +171,41 @@ class graph:
         if self._dev is not None:
             self._dev.close()
             self._dev = None
+
+    # ---- edge updates (graph.hpp:869-975) -------------------------------
+    def updateCSRDel(self, batchUpdate, k, size):
+        """graph::updateCSRDel (graph.hpp:869-930): the deletions among
+        batchUpdate[k:k+size], applied to the device mirror; the host arrays
+        are then refreshed from it."""
+        from .updates import as_arrays
+        dels = [u for u in batchUpdate[k:k + size] if u.type == "d"]
+        if dels:
+            _, s, d, _ = as_arrays(dels)
+            self.device().delete_edges(s, d)
+        self.refresh_from_device()
+
+    def updateCSRAdd(self, batchUpdate, k, size):
+        """graph::updateCSRAdd (graph.hpp:932-975): the additions among
+        batchUpdate[k:k+size] with their weights."""
+        from .updates import as_arrays
+        adds = [u for u in batchUpdate[k:k + size] if u.type == "a"]
+        if adds:
+            _, s, d, w = as_arrays(adds)
+            self.device().add_edges(s, d, w)
+        self.refresh_from_device()
+
+    def refresh_from_device(self):
+        """Replaces the host arrays with the device mirror's current CSR (rows
+        ascending). The native host structure no longer describes the graph
+        afterwards, so generators' statistics (features, byte models) refuse
+        it."""
+        off, adj, w, roff, radj = self.device().export()
+        self.indexofNodes, self.edgeList = off, adj
+        self.E = int(adj.size)
+        self.edgeLen = w if w is not None else np.zeros(self.E, np.int32)
+        if roff is not None:
+            self.rev_indexofNodes, self.srcList = roff, radj
+        self._h = None
 
 
 def sources(V, n, seed):

@@ -43,7 +43,7 @@ FIXED_CLASS = 0
 
 def features(g, job):
     out = np.zeros(N_FEATURES, np.float32)
-    N.check("spmh_features", N.host_lib().spmh_features(g._h, float(job), N.ptr(out, C.c_float)))
+    N.check("spmh_features", N.host_lib().spmh_features(g._native(), float(job), N.ptr(out, C.c_float)))
     return out
 
 

@@ -1,0 +1,180 @@
+"""Dynamic-graph path on the GPU through the C ABI: edge updates against the
+oracle's update semantics, static algorithms on an updated graph, and
+dynamic SSSP against the oracle's DynSSSP restatement (distances and parents
+bit-exact) and against Dijkstra on the final graph."""
+from collections import Counter
+
+import numpy as np
+import pytest
+
+from starplat_amd import Compute_PR, Compute_TC, DynSSSP, graph, parseUpdateFile
+from starplat_amd.updates import write_update_file
+
+from test_oracle_dyn import INF2, _G, batch_ops, check_against_dijkstra, random_updates
+
+pytestmark = pytest.mark.gpu
+
+
+def rows_equal(off, adj, w, roff, radj, ref_off, ref_adj, ref_w, V):
+    np.testing.assert_array_equal(off, ref_off)
+    for v in range(V):
+        a, b = off[v], off[v + 1]
+        assert np.all(np.diff(adj[a:b]) >= 0), v
+        assert Counter(zip(adj[a:b].tolist(), w[a:b].tolist())) == \
+            Counter(zip(ref_adj[a:b].tolist(), ref_w[a:b].tolist())), v
+    # the reverse CSR is the transpose of the forward one, rows ascending
+    src = np.repeat(np.arange(V), np.diff(off))
+    fwd = np.sort(adj.astype(np.int64) * (1 << 32) + src)
+    rdst = np.repeat(np.arange(V), np.diff(roff))
+    rev = np.sort(rdst.astype(np.int64) * (1 << 32) + radj)
+    np.testing.assert_array_equal(fwd, rev)
+    for v in range(V):
+        assert np.all(np.diff(radj[roff[v]:roff[v + 1]]) >= 0)
+
+
+@pytest.mark.parametrize("undirected", [False, True])
+def test_updates_match_oracle(oracle, undirected):
+    g = graph.rmat(11, 8, seed=4, weights=(1, 50), undirected=undirected)
+    dev = g.device()
+    ops = []
+    for k in range(4):
+        types, us, ut = random_updates(g, 500, seed=10 + k, symmetric=undirected)
+        t = np.frombuffer(types, np.uint8)
+        d, a = t == ord("d"), t == ord("a")
+        wa = (np.arange(int(a.sum())) % 7 + 1).astype(np.int32)
+        dev.delete_edges(us[d], ut[d])
+        dev.add_edges(us[a], ut[a], wa)
+        ops += [("d", us[d], ut[d]), ("a", us[a], ut[a], wa)]
+        if k == 1:  # duplicates of an existing edge, then delete all copies
+            s0, t0 = int(np.repeat(np.arange(g.V), np.diff(g.indexofNodes))[5]), int(g.edgeList[5])
+            dev.add_edges([s0, s0], [t0, t0], [9, 8])
+            ops.append(("a", np.array([s0, s0], np.int32), np.array([t0, t0], np.int32), np.array([9, 8], np.int32)))
+            dev.delete_edges([s0, 3], [t0, 3])
+            ops.append(("d", np.array([s0, 3], np.int32), np.array([t0, 3], np.int32)))
+    ref = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, ops)
+    off, adj, w, roff, radj = dev.export()
+    assert dev.num_edges() == ref[0][-1]
+    rows_equal(off, adj, w, roff, radj, *ref, g.V)
+    # a second round of updates on the folded graph
+    types, us, ut = random_updates(g, 300, seed=99)
+    t = np.frombuffer(types, np.uint8)
+    dev.delete_edges(us[t == ord("d")], ut[t == ord("d")])
+    dev.add_edges(us[t == ord("a")], ut[t == ord("a")])
+    ops2 = [("d", us[t == ord("d")], ut[t == ord("d")]),
+            ("a", us[t == ord("a")], ut[t == ord("a")], np.ones(int((t == ord("a")).sum()), np.int32))]
+    ref2 = oracle.apply_updates(g.V, *ref, ops2)
+    rows_equal(*dev.export(), *ref2, g.V)
+
+
+def test_update_errors_and_empty_batches(oracle):
+    g = graph.rmat(8, 4, seed=1, weights=(1, 5))
+    dev = g.device()
+    dev.delete_edges([], [])
+    dev.add_edges([], [])
+    with pytest.raises(Exception):
+        dev.delete_edges([0], [g.V])
+    with pytest.raises(Exception):
+        dev.add_edges([-1], [0])
+    off, adj, w, roff, radj = dev.export()
+    np.testing.assert_array_equal(off, g.indexofNodes)
+    np.testing.assert_array_equal(adj, g.edgeList)
+    # delete every edge, then the graph is empty
+    src = np.repeat(np.arange(g.V), np.diff(g.indexofNodes)).astype(np.int32)
+    dev.delete_edges(src, g.edgeList)
+    assert dev.num_edges() == 0
+    dev.add_edges([1], [2], [3])
+    off, adj, w, roff, radj = dev.export()
+    assert off[-1] == 1 and adj.tolist() == [2] and w.tolist() == [3] and radj.tolist() == [1]
+
+
+def test_static_algorithms_after_updates(oracle):
+    g = graph.rmat(11, 8, seed=6, weights=(1, 40))
+    dev = g.device()
+    types, us, ut = random_updates(g, 800, seed=5)
+    t = np.frombuffer(types, np.uint8)
+    dev.delete_edges(us[t == ord("d")], ut[t == ord("d")])
+    dev.add_edges(us[t == ord("a")], ut[t == ord("a")], np.full(int((t == ord("a")).sum()), 2, np.int32))
+    # SSSP on the updated handle sees the updated graph
+    d = dev.sssp(0)
+    off, adj, w, roff, radj = dev.export()
+    fin = _G(g.V, off, adj, w)
+    np.testing.assert_array_equal(d, oracle.dijkstra(fin, w, 0))
+    # the host mirror follows the device after updateCSR*
+    u = graph.rmat(10, 8, seed=8, undirected=True)
+    from starplat_amd.updates import update
+    src_of = np.repeat(np.arange(u.V), np.diff(u.indexofNodes))
+    batch = [update("d", int(src_of[e]), int(u.edgeList[e])) for e in range(0, u.E, 7)]
+    batch += [update("d", int(u.edgeList[e]), int(src_of[e])) for e in range(0, u.E, 7)]
+    tc0 = oracle.compute_tc(u)
+    u.updateCSRDel(batch, 0, len(batch))
+    assert u.is_symmetric()
+    assert Compute_TC(u) == oracle.compute_tc(u) < tc0
+    pr = np.empty(u.V, np.float32)
+    Compute_PR(u, 1e-10, 0.85, 20, pr)
+    ref, _ = oracle.compute_pr(u, 1e-10, 0.85, 20)
+    assert np.max(np.abs(pr - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+@pytest.mark.parametrize("batch", [1, 64, 5000])
+def test_dyn_sssp_matches_oracle(oracle, symmetric, batch):
+    g = graph.rmat(12, 8, seed=7, weights=(1, 30), undirected=symmetric)
+    types, us, ut = random_updates(g, 1500 if batch > 1 else 200, seed=batch, symmetric=symmetric)
+    dist_ref, par_ref = oracle.dyn_sssp(g, g.edgeLen, 0, types, us, ut, batch)
+    dev = g.device()
+    dist, parent = dev.dyn_sssp(0, types, us, ut, batch)
+    np.testing.assert_array_equal(dist, dist_ref)
+    np.testing.assert_array_equal(parent, par_ref)
+    check_against_dijkstra(oracle, g, types, us, ut, batch, dist, parent, 0)
+
+
+def test_dyn_sssp_stepwise_and_edge_cases(oracle):
+    g = graph.from_edges(np.array([0, 1, 2, 5], np.int32), np.array([1, 2, 3, 6], np.int32),
+                         np.array([5, 5, 5, 2], np.int32))
+    types, us, ut = b"ddaaa", np.array([1, 4, 5, 0, 1], np.int32), np.array([2, 4, 2, 3, 2], np.int32)
+    dref, pref = oracle.dyn_sssp(g, g.edgeLen, 0, types, us, ut, 4)
+    dev = g.device()
+    dev.dynsssp_init(0)
+    dev.dynsssp_batch(types[:4], us[:4], ut[:4])
+    d1, p1 = dev.dynsssp_result()
+    assert d1[2] == INF2 and p1[2] == -1 and d1[3] == 1
+    dev.dynsssp_batch(types[4:], us[4:], ut[4:])
+    dev.dynsssp_batch(b"", [], [])
+    d, p = dev.dynsssp_result()
+    np.testing.assert_array_equal(d, dref)
+    np.testing.assert_array_equal(p, pref)
+    with pytest.raises(Exception):
+        dev.dynsssp_batch(b"x", [0], [1])
+    # zero weights are refused (parent pointers must strictly shorten)
+    z = graph.from_edges(np.array([0, 1], np.int32), np.array([1, 2], np.int32), np.array([0, 1], np.int32))
+    with pytest.raises(Exception):
+        z.device().dynsssp_init(0)
+
+
+def test_dynsssp_entry_point_and_update_file(oracle, tmp_path):
+    g = graph.rmat(11, 8, seed=9, weights=(1, 25), undirected=True)
+    rng = np.random.default_rng(3)
+    src_of = np.repeat(np.arange(g.V), np.diff(g.indexofNodes))
+    e = rng.integers(0, g.E, 300)
+    types = np.where(rng.random(300) < 0.5, ord("d"), ord("a")).astype(np.uint8)
+    s = np.where(types == ord("d"), src_of[e], rng.integers(0, g.V, 300))
+    t = np.where(types == ord("d"), g.edgeList[e], rng.integers(0, g.V, 300))
+    path = tmp_path / "updates.txt"
+    write_update_file(str(path), types.tolist(), s, t)
+    updateBatch = parseUpdateFile(str(path))
+    assert len(updateBatch) == 600
+    from starplat_amd.updates import as_arrays
+    ty, us, ut, _ = as_arrays(updateBatch)
+    dref, pref = oracle.dyn_sssp(g, g.edgeLen, 3, ty, us, ut, 100)
+    w0 = g.edgeLen.copy()
+    dist = np.empty(g.V, np.int32)
+    parent = np.empty(g.V, np.int32)
+    DynSSSP(g, dist, parent, w0, updateBatch, 100, 3)
+    np.testing.assert_array_equal(dist, dref)
+    np.testing.assert_array_equal(parent, pref)
+    # g now holds the updated graph
+    g0 = graph.rmat(11, 8, seed=9, weights=(1, 25), undirected=True)
+    ref = oracle.apply_updates(g0.V, g0.indexofNodes, g0.edgeList, g0.edgeLen, batch_ops(ty, us, ut, 100))
+    np.testing.assert_array_equal(g.indexofNodes, ref[0])
+    np.testing.assert_array_equal(g.edgeList, ref[1])
+    assert g.is_symmetric()

@@ -1,0 +1,179 @@
+"""Pins the oracle's restatements of the dynamic-graph path: the edge-update
+semantics (updateCSRDel / updateCSRAdd) against a dictionary model, and the
+DynSSSP restatement against textbook Dijkstra on the graph the updates leave,
+with planted mistakes that these checks catch."""
+from collections import Counter
+
+import numpy as np
+import pytest
+
+from starplat_amd import graph
+from starplat_amd.updates import as_arrays, getAdditions, getDeletions, parseUpdateFile, update
+
+INF2 = (2**31 - 1) // 2
+INT_MAX = 2**31 - 1
+
+
+def random_updates(g, n, seed, p_del=0.5, symmetric=False):
+    """n updates: deletions of existing edges (and a few absent ones),
+    additions of random pairs; with symmetric, each followed by its reverse
+    (as parseUpdateFile pushes them)."""
+    rng = np.random.default_rng(seed)
+    src_of = np.repeat(np.arange(g.V), np.diff(g.indexofNodes))
+    types, us, ut = [], [], []
+    for _ in range(n):
+        if rng.random() < p_del:
+            if g.E and rng.random() < 0.9:
+                e = int(rng.integers(0, g.E))
+                s, t = int(src_of[e]), int(g.edgeList[e])
+            else:
+                s, t = int(rng.integers(0, g.V)), int(rng.integers(0, g.V))
+            k = "d"
+        else:
+            s, t, k = int(rng.integers(0, g.V)), int(rng.integers(0, g.V)), "a"
+        types.append(k), us.append(s), ut.append(t)
+        if symmetric:
+            types.append(k), us.append(t), ut.append(s)
+    return "".join(types).encode(), np.array(us, np.int32), np.array(ut, np.int32)
+
+
+def batch_ops(types, us, ut, batch):
+    """The update operations a Dynamic function applies: per batch, all its
+    deletions, then all its additions (weight 1)."""
+    ops = []
+    t = np.frombuffer(types, np.uint8)
+    for k in range(0, len(us), batch):
+        sl = slice(k, k + batch)
+        dm = t[sl] == ord("d")
+        ops.append(("d", us[sl][dm], ut[sl][dm]))
+        am = ~dm
+        ops.append(("a", us[sl][am], ut[sl][am], np.ones(int(am.sum()), np.int32)))
+    return ops
+
+
+class _G:
+    def __init__(self, V, off, adj, w):
+        self.V, self.indexofNodes, self.edgeList, self.edgeLen = V, off, adj, w
+
+
+def dict_model(g, ops):
+    rows = [Counter() for _ in range(g.V)]
+    for v in range(g.V):
+        for e in range(g.indexofNodes[v], g.indexofNodes[v + 1]):
+            rows[v][(int(g.edgeList[e]), int(g.edgeLen[e]))] += 1
+    for op in ops:
+        if op[0] == "d":
+            for s, t in zip(op[1], op[2]):
+                for key in [k for k in rows[s] if k[0] == t]:
+                    del rows[s][key]
+        else:
+            for s, t, w in zip(op[1], op[2], op[3]):
+                rows[s][(int(t), int(w))] += 1
+    return rows
+
+
+# ------------------------------------------------------------ update semantics
+def test_apply_updates_matches_dictionary_model(oracle):
+    g = graph.rmat(8, 6, seed=5, weights=(1, 9))
+    types, us, ut = random_updates(g, 300, seed=1)
+    ops = batch_ops(types, us, ut, 37)
+    # duplicate additions and an add-then-delete of the same pair
+    ops.append(("a", np.array([1, 1, 2], np.int32), np.array([2, 2, 3], np.int32), np.array([4, 5, 6], np.int32)))
+    ops.append(("d", np.array([2], np.int32), np.array([3], np.int32)))
+    off, adj, w = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, ops)
+    rows = dict_model(g, ops)
+    for v in range(g.V):
+        got = Counter(zip(adj[off[v]:off[v + 1]].tolist(), w[off[v]:off[v + 1]].tolist()))
+        assert got == rows[v], v
+        assert np.all(np.diff(adj[off[v]:off[v + 1]]) >= 0)
+    assert rows[1][(2, 4)] == 1 and rows[1][(2, 5)] == 1 and not any(k[0] == 3 for k in rows[2])
+
+
+def test_apply_updates_invariants_and_plant(oracle):
+    g = graph.rmat(7, 8, seed=2, weights=(1, 5))
+    src_of = np.repeat(np.arange(g.V), np.diff(g.indexofNodes)).astype(np.int32)
+    # deleting every edge leaves no edge; adding them back restores the graph
+    off, adj, w = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, [("d", src_of, g.edgeList)])
+    assert off[-1] == 0
+    off2, adj2, w2 = oracle.apply_updates(g.V, off, adj, w, [("a", src_of, g.edgeList, g.edgeLen)])
+    np.testing.assert_array_equal(off2, g.indexofNodes)
+    np.testing.assert_array_equal(adj2, g.edgeList)
+    np.testing.assert_array_equal(w2, g.edgeLen)
+    # planted: a deletion that removes only one copy leaves a duplicate behind
+    ops = [("a", np.array([0], np.int32), np.array([1], np.int32), np.array([3], np.int32))] * 2
+    ops = ops + [("d", np.array([0], np.int32), np.array([1], np.int32))]
+    good = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, ops)
+    bad = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, ops, plant=1)
+    assert good[0][-1] != bad[0][-1]
+
+
+def test_update_file_parsing(tmp_path):
+    p = tmp_path / "u.txt"
+    p.write_text("a 1 2\nd 3 4\n\nx 5 6\na 7\n")
+    u = parseUpdateFile(str(p))
+    assert [(x.type, x.source, x.destination, x.weight) for x in u] == [
+        ("a", 1, 2, 1), ("a", 2, 1, 1), ("d", 3, 4, 1), ("d", 4, 3, 1)]
+    assert [x.source for x in getDeletions(0, 3, u)] == [3]
+    assert [x.source for x in getAdditions(1, 3, u)] == [2]
+    t, s, d, w = as_arrays([update("a", 1, 2, 7), update("d", 3, 4)])
+    assert t == b"ad" and s.tolist() == [1, 3] and d.tolist() == [2, 4] and w.tolist() == [7, 1]
+
+
+# ------------------------------------------------------------ DynSSSP
+def check_against_dijkstra(oracle, g, types, us, ut, batch, dist, parent, src):
+    off, adj, w = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, batch_ops(types, us, ut, batch))
+    fin = _G(g.V, off, adj, w)
+    ref = oracle.dijkstra(fin, w, src)
+    reach = ref != INT_MAX
+    np.testing.assert_array_equal(dist[reach], ref[reach])
+    assert np.all(dist[~reach] >= INF2)
+    assert parent[src] == -1 and np.all(parent[~reach] == -1)
+    for v in np.nonzero(reach)[0]:
+        if v == src:
+            continue
+        p = parent[v]
+        row = slice(off[p], off[p + 1])
+        assert np.any((adj[row] == v) & (ref[p] + w[row] == ref[v])), (v, p)
+    # parent chains reach the source (no cycles)
+    for v in np.nonzero(reach)[0][:200]:
+        x, steps = v, 0
+        while x != src:
+            x = parent[x]
+            steps += 1
+            assert steps <= g.V
+    return fin
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+@pytest.mark.parametrize("batch", [1, 16, 1000])
+def test_dyn_sssp_restatement_is_exact(oracle, symmetric, batch):
+    g = graph.rmat(9, 8, seed=7, weights=(1, 30), undirected=symmetric)
+    types, us, ut = random_updates(g, 300, seed=batch, symmetric=symmetric)
+    dist, parent = oracle.dyn_sssp(g, g.edgeLen, 0, types, us, ut, batch)
+    check_against_dijkstra(oracle, g, types, us, ut, batch, dist, parent, 0)
+
+
+def test_dyn_sssp_edge_cases(oracle):
+    # path 0->1->2->3, weights 5; delete the middle edge, add a bypass from an
+    # unreachable vertex, re-add the middle edge in a later batch
+    g = graph.from_edges(np.array([0, 1, 2, 5], np.int32), np.array([1, 2, 3, 6], np.int32),
+                         np.array([5, 5, 5, 2], np.int32))
+    types = b"ddaa" + b"a"
+    us = np.array([1, 4, 5, 0, 1], np.int32)  # 4->4 is absent
+    ut = np.array([2, 4, 2, 3, 2], np.int32)
+    dist, parent = oracle.dyn_sssp(g, g.edgeLen, 0, types, us, ut, 4)
+    # after batch 1: 2 is cut off (INT_MAX/2), 3 reached through 0->3 (weight 1)
+    assert dist[1] == 5 and dist[3] == 1 and parent[3] == 0
+    # batch 2 re-adds 1->2 with weight 1: dist[2] = 6
+    assert dist[2] == 6 and parent[2] == 1
+    assert dist[5] == INT_MAX and dist[6] == INT_MAX and parent[5] == -1
+    check_against_dijkstra(oracle, g, types, us, ut, 4, dist, parent, 0)
+
+
+def test_dyn_sssp_plants_are_caught(oracle):
+    g = graph.rmat(9, 8, seed=11, weights=(1, 30))
+    types, us, ut = random_updates(g, 400, seed=3)
+    for plant in (1, 2):
+        dist, parent = oracle.dyn_sssp(g, g.edgeLen, 0, types, us, ut, 50, plant=plant)
+        with pytest.raises(AssertionError):
+            check_against_dijkstra(oracle, g, types, us, ut, 50, dist, parent, 0)

@@ -106,7 +106,7 @@ def bench_sssp(args, O):
     log("generating RMAT-22 (directed, weights U[1,100], seed 2)")
     g = graph.rmat(args.sssp_scale, 16, seed=2, weights=(1, 100))
     r, m, rl = C.c_int64(), C.c_int64(), C.c_int64()
-    model = N.host_lib().spmh_model_sssp(g._h, N.ptr(g.edgeLen, C.c_int32), 0, C.byref(r), C.byref(m), C.byref(rl))
+    model = N.host_lib().spmh_model_sssp(g._native(), N.ptr(g.edgeLen, C.c_int32), 0, C.byref(r), C.byref(m), C.byref(rl))
     dev = g.device()
     d_out = hip.DeviceArray(g.V, np.int32)
     rounds = dev.sssp_device(0, d_out)
@@ -144,7 +144,7 @@ def bench_sssp(args, O):
 def bench_pr(args, O, scale, seed, label):
     log(f"generating RMAT-{scale} for PageRank")
     g = graph.rmat(scale, 16, seed=seed)
-    model = N.host_lib().spmh_model_pr_bytes(g._h)
+    model = N.host_lib().spmh_model_pr_bytes(g._native())
     dev = g.device()
     d_out = hip.DeviceArray(g.V, np.float32)
     it = dev.pr_device(1e-10, 0.85, 100, d_out)
@@ -176,7 +176,7 @@ def bench_tc(args, O):
     log("generating undirected RMAT-21 for TC")
     g = graph.rmat(args.tc_scale, 16, seed=3, undirected=True)
     mu, wk = C.c_int64(), C.c_int64()
-    model = N.host_lib().spmh_model_tc_bytes(g._h, C.byref(mu), C.byref(wk))
+    model = N.host_lib().spmh_model_tc_bytes(g._native(), C.byref(mu), C.byref(wk))
     dev = g.device()
     count = dev.tc()  # includes one-time orientation
     ms = timed_steps(lambda: dev.tc(), max(1, args.steps // 4), 1)
@@ -205,7 +205,7 @@ def bench_bc(args, O):
     src = sources(g.V, args.bc_sources, seed=4)
     et = C.c_int64()
     log("BC byte model (one BFS per source on the host)")
-    model = N.host_lib().spmh_model_bc_bytes(g._h, N.ptr(src, C.c_int32), len(src), C.byref(et))
+    model = N.host_lib().spmh_model_bc_bytes(g._native(), N.ptr(src, C.c_int32), len(src), C.byref(et))
     dev = g.device()
     d_out = hip.DeviceArray(g.V, np.float32)
     dev.bc_device(src, d_out)
@@ -234,6 +234,75 @@ def bench_bc(args, O):
     return res
 
 
+# --------------------------------------------------------------------------- dynamic SSSP
+def dyn_batch(g, src_of, size, rng):
+    """One batch of `size` updates: half deletions of edges of the original
+    graph, half additions of uniform vertex pairs (weight 1, update.hpp)."""
+    nd = size // 2
+    e = rng.integers(0, g.E, nd)
+    us = np.concatenate([src_of[e], rng.integers(0, g.V, size - nd)]).astype(np.int32)
+    ut = np.concatenate([g.edgeList[e], rng.integers(0, g.V, size - nd)]).astype(np.int32)
+    return b"d" * nd + b"a" * (size - nd), us, ut
+
+
+def bench_dyn(args, O):
+    log(f"dynamic SSSP on RMAT-{args.sssp_scale}: {args.dyn_batches} batches of {args.dyn_batch} updates")
+    g = graph.rmat(args.sssp_scale, 16, seed=2, weights=(1, 100))
+    src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
+    rng = np.random.default_rng(5)
+    batches = [dyn_batch(g, src_of, args.dyn_batch, rng) for _ in range(args.dyn_batches + 1)]
+    dev = g.device()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    dev.dynsssp_init(0)
+    t_init = (time.perf_counter() - t0) * 1e3
+    dev.dynsssp_batch(*batches[0])  # warm-up (first allocation of the overflow rows)
+    times = []
+    for b in batches[1:-1]:
+        t0 = time.perf_counter()
+        dev.dynsssp_batch(*b)
+        times.append((time.perf_counter() - t0) * 1e3)
+    kt = kernel_time(dev, lambda: dev.dynsssp_batch(*batches[-1]),
+                     ["dsp_on_delete", "dyn_delete", "dsp_invalidate", "dsp_pull", "dsp_on_add", "dyn_count",
+                      "dyn_difflen", "dyn_diffcopy", "dyn_place", "dsp_push"], reps=1)
+    dist, parent = dev.dynsssp_result()
+    # the same graph from scratch: fold the updates, static SSSP, parents
+    hip.synchronize()
+    t0 = time.perf_counter()
+    dev.compact()
+    t_fold = (time.perf_counter() - t0) * 1e3
+    d_out = hip.DeviceArray(g.V, np.int32)
+    t0 = time.perf_counter()
+    dev.sssp_device(0, d_out)
+    t_static = (time.perf_counter() - t0) * 1e3
+    ref = d_out.to_host()
+    d_out.free()
+    reach = ref != np.iinfo(np.int32).max
+    ok = bool(np.array_equal(dist[reach], ref[reach]) and np.all(dist[~reach] >= np.iinfo(np.int32).max // 2))
+    ms = statistics.median(times)
+    res = {"workload": f"DynSSSP (dynamicBatchSSSP) on directed RMAT scale {args.sssp_scale} EF16 weights U[1,100], "
+                       f"src 0; batches of {args.dyn_batch} updates, half deletions of existing edges, half random "
+                       "additions of weight 1",
+           "V": g.V, "E": g.E, "batch": args.dyn_batch, "batches_timed": len(times),
+           "ms_per_batch": round(ms, 4), "updates_per_s": round(args.dyn_batch / (ms * 1e-3)),
+           "init_ms": round(t_init, 3), "fold_ms": round(t_fold, 3), "static_sssp_ms": round(t_static, 3),
+           "speedup_vs_fold_plus_static": round((t_fold + t_static) / ms, 2),
+           "dist_equals_static_recompute": ok,
+           "kernel_ms_one_batch": {k: round(v[0], 4) for k, v in kt.items() if v[1] > 0},
+           "roofline": None,
+           "note": "per-batch time includes the host's staging copies and one flag read-back per fixed-point round"}
+    if not args.no_cpu_baseline:
+        tm = {}
+        b = batches[1]
+        O.dyn_sssp(g, g.edgeLen, 0, b[0], b[1], b[2], args.dyn_batch, timing=tm)
+        res["cpu_baseline"] = {"value": round(args.dyn_batch / tm["batch_seconds"]), "unit": "updates/s", "cores": 1,
+                               "kind": "port", "sample": "one batch after the restatement's own static SSSP "
+                               "(sequential DynSSSP restatement, batch loop timed)",
+                               "seconds": round(tm["batch_seconds"], 4)}
+    g.release_device()
+    return res
+
+
 # --------------------------------------------------------------------------- schedules
 def bench_schedule(args):
     from starplat_amd import mixed_batch
@@ -252,13 +321,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--algos", default="sssp,pr,pr_c0,tc,bc,sched")
+    ap.add_argument("--algos", default="sssp,pr,pr_c0,tc,bc,dyn,sched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--sssp-scale", type=int, default=22)
     ap.add_argument("--tc-scale", type=int, default=21)
     ap.add_argument("--bc-n", type=int, default=4194304)
     ap.add_argument("--bc-sources", type=int, default=256)
+    ap.add_argument("--dyn-batch", type=int, default=65536)
+    ap.add_argument("--dyn-batches", type=int, default=6)
     ap.add_argument("--sched-graphs", type=int, default=150)
     ap.add_argument("--sched-held-out", type=int, default=50)
     ap.add_argument("--sched-rmat", type=int, nargs=2, default=[10, 16])
@@ -302,10 +373,13 @@ def main():
         per["tc"] = bench_tc(args, O)
     if "bc" in algos:
         per["bc"] = bench_bc(args, O)
+    if "dyn" in algos:
+        per["dyn"] = bench_dyn(args, O)
     sched = bench_schedule(args) if "sched" in algos else None
     if head is None and per:
         first = next(iter(per.values()))
-        head = {"workload": first["workload"], "gteps": first["gteps"], "ms_per_step": first.get("ms_per_run"),
+        head = {"workload": first["workload"], "gteps": first.get("gteps", float("nan")),
+                "ms_per_step": first.get("ms_per_run", first.get("ms_per_batch")),
                 "roofline": first["roofline"], "cpu_baseline": first.get("cpu_baseline")}
     if head is None:  # schedule selection only
         head = {"workload": sched["workload"] if sched else "none", "gteps": float("nan"), "ms_per_step": None,

@@ -1,7 +1,8 @@
 // oracle.cpp -- TEST INFRASTRUCTURE ONLY. CPU restatement of the OpenMP code
 // that StarPlat's OMP backend (src/backends/backend_omp/dsl_cpp_generator.cpp)
-// emits for the four hot-path DSL programs, plus independent textbook
-// routines that pin it. Only the test suite, the smoke check and the CPU
+// emits for the four hot-path DSL programs and for the dynamic SSSP program
+// (dynamicDSLCodes/dynamicBatchSSSP), plus independent textbook routines that
+// pin them. Only the test suite, the smoke check and the CPU
 // baseline leg of the benchmark load this library; the product path never
 // does.
 //
@@ -460,10 +461,11 @@ int oracle_num_threads(void) { return omp_get_max_threads(); }
 // first, so among equally short predecessors the smallest id is the parent;
 // after staticSSSP the parent is the smallest-id predecessor on a shortest
 // path. plant 1: skip Decremental's loop (a); plant 2: OnAdd marks only t.
+// batch_seconds (may be null) receives the wall time of the batch loop.
 // Returns the number of batches, or -1 if a loop exceeds V rounds.
 int oracle_dyn_sssp(int32_t V, const int32_t* off, const int32_t* adj, const int32_t* w, int32_t src,
                     int32_t n, const char* type, const int32_t* us, const int32_t* ut, int32_t batch,
-                    int32_t* dist_out, int32_t* parent_out, int plant) {
+                    int32_t* dist_out, int32_t* parent_out, int plant, double* batch_seconds) {
   const int32_t INF2 = INT_MAX / 2;
   using Key = uint64_t;
   auto key = [](int32_t d, int64_t p) { return (Key(uint32_t(d)) << 32) | Key(uint32_t(p)); };
@@ -495,6 +497,7 @@ int oracle_dyn_sssp(int32_t V, const int32_t* off, const int32_t* adj, const int
     dp[v] = key(dist[v], int64_t(best));
   }
   int32_t batches = 0;
+  const double t_start = omp_get_wtime();
   for (int32_t k = 0; k < n; k += batch, ++batches) {
     const int32_t m = std::min(batch, n - k);
     std::vector<char> mod(V, 0), madd(V, 0);
@@ -570,6 +573,7 @@ int oracle_dyn_sssp(int32_t V, const int32_t* off, const int32_t* adj, const int
       }
     }
   }
+  if (batch_seconds) *batch_seconds = omp_get_wtime() - t_start;
   for (int32_t v = 0; v < V; ++v) {
     dist_out[v] = kd(dp[v]);
     parent_out[v] = uint32_t(dp[v]) == uint32_t(NOP) ? -1 : int32_t(uint32_t(dp[v]));

@@ -39,7 +39,7 @@ def lib():
         L.tb_tc_merge.restype = i64
         L.tb_pr_fp64.argtypes = [i32, P_i32, P_i32, P_i32, f64, f64, i32, P_f64, P_i32]
         L.oracle_dyn_sssp.argtypes = [i32, P_i32, P_i32, P_i32, i32, i32, C.c_char_p, P_i32, P_i32, i32, P_i32,
-                                      P_i32, C.c_int]
+                                      P_i32, C.c_int, P_f64]
         L.oracle_dyn_sssp.restype = C.c_int
         L.oracle_num_threads.restype = C.c_int
         _lib = L
@@ -121,9 +121,10 @@ def pr_fp64(g, beta=1e-10, delta=0.85, maxIter=100):
     return pr, it.value
 
 
-def dyn_sssp(g, weight, src, types, us, ut, batch, plant=0):
+def dyn_sssp(g, weight, src, types, us, ut, batch, plant=0, timing=None):
     """DynSSSP restated (oracle.cpp): (dist, parent) after every batch of
-    `batch` updates; types is a bytes string of 'a' / 'd'."""
+    `batch` updates; types is a bytes string of 'a' / 'd'. With a dict as
+    `timing`, timing["batch_seconds"] receives the batch loop's wall time."""
     w = np.ascontiguousarray(weight, np.int32)
     us = np.ascontiguousarray(us, np.int32)
     ut = np.ascontiguousarray(ut, np.int32)
@@ -131,8 +132,12 @@ def dyn_sssp(g, weight, src, types, us, ut, batch, plant=0):
     assert len(t) == len(us) == len(ut)
     dist = np.empty(g.V, np.int32)
     parent = np.empty(g.V, np.int32)
+    secs = f64(0.0)
     rc = lib().oracle_dyn_sssp(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), _p(w, i32), int(src), len(us), t,
-                               _p(us, i32), _p(ut, i32), int(batch), _p(dist, i32), _p(parent, i32), int(plant))
+                               _p(us, i32), _p(ut, i32), int(batch), _p(dist, i32), _p(parent, i32), int(plant),
+                               C.byref(secs))
+    if timing is not None:
+        timing["batch_seconds"] = secs.value
     if rc < 0:
         raise RuntimeError("oracle_dyn_sssp: a loop did not converge")
     return dist, parent

@@ -178,3 +178,32 @@ def test_dynsssp_entry_point_and_update_file(oracle, tmp_path):
     np.testing.assert_array_equal(g.indexofNodes, ref[0])
     np.testing.assert_array_equal(g.edgeList, ref[1])
     assert g.is_symmetric()
+
+
+@pytest.mark.slow
+def test_full_dyn_sssp_rmat22(oracle):
+    """RMAT-22 (the SSSP benchmark graph), two batches of 65,536 updates:
+    distances and parents equal the restatement's; distances equal a static
+    SSSP on the folded graph."""
+    g = graph.rmat(22, 16, seed=2, weights=(1, 100))
+    rng = np.random.default_rng(5)
+    src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
+    parts = []
+    for _ in range(2):
+        e = rng.integers(0, g.E, 32768)
+        parts.append((b"d" * 32768 + b"a" * 32768,
+                      np.concatenate([src_of[e], rng.integers(0, g.V, 32768)]).astype(np.int32),
+                      np.concatenate([g.edgeList[e], rng.integers(0, g.V, 32768)]).astype(np.int32)))
+    types = b"".join(p[0] for p in parts)
+    us = np.concatenate([p[1] for p in parts])
+    ut = np.concatenate([p[2] for p in parts])
+    dref, pref = oracle.dyn_sssp(g, g.edgeLen, 0, types, us, ut, 65536)
+    dev = g.device()
+    dist, parent = dev.dyn_sssp(0, types, us, ut, 65536)
+    np.testing.assert_array_equal(dist, dref)
+    np.testing.assert_array_equal(parent, pref)
+    static = dev.sssp(0)
+    reach = static != np.iinfo(np.int32).max
+    np.testing.assert_array_equal(dist[reach], static[reach])
+    assert np.all(dist[~reach] >= INF2)
+    print(f"DynSSSP V={g.V}: reachable {int(reach.sum())}, parity with restatement and static recompute")

@@ -3,10 +3,12 @@
 // code and drivers use from StarPlat's graphcode/graph.hpp (class graph,
 // :31-1215) -- indexofNodes, edgeList, rev_indexofNodes, srcList (:44-48),
 // getEdgeLen() (:134-137), num_nodes() (:139-142), num_edges() (:145-152),
-// parseGraph() (:861-865), check_if_nbr() (:233-287), getEdge() (:168-186) --
-// without graph.hpp's MKL/GNN/dynamic-graph includes. Parsing and CSR
-// construction go through the host data layer (libspm_host.so, same
-// semantics); device() returns the HBM mirror used by the Compute_* bodies.
+// parseGraph() (:861-865), check_if_nbr() (:233-287), getEdge() (:168-186),
+// updateCSRDel() / updateCSRAdd() (:869-975) -- without graph.hpp's
+// MKL/GNN/geometric includes. Parsing and CSR construction go through the
+// host data layer (libspm_host.so, same semantics); device() returns the HBM
+// mirror used by the Compute_* bodies. The update calls apply a batch to the
+// HBM mirror and then refresh the public arrays from it (rows ascending).
 #pragma once
 
 #include <cstdint>
@@ -14,6 +16,7 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "starplat_amd.h"
 #include "starplat_host.h"
@@ -26,6 +29,15 @@ class edge {
   int32_t weight = 0;
   int32_t id = 0;
   int8_t dir = 1;
+};
+
+// graphcode/update.hpp:8-15
+class update {
+ public:
+  char type = 'a';
+  int32_t source = 0;
+  int32_t destination = 0;
+  int32_t weight = 1;
 };
 
 class graph {
@@ -53,6 +65,7 @@ class graph {
     if (dev_) { spm_graph_destroy(dev_); dev_ = nullptr; }
     if (h_) spmh_free(h_);
     h_ = h;
+    updated_ = false;
     indexofNodes = h->indexofNodes;
     rev_indexofNodes = h->rev_indexofNodes;
     edgeList = h->edgeList;
@@ -60,8 +73,15 @@ class graph {
   }
 
   int num_nodes() { return h_ ? h_->V : 1; }
-  int num_edges() { return h_ ? int(h_->E) : 0; }
-  int* getEdgeLen() { return h_ ? h_->edgeLen : nullptr; }
+  int num_edges() { return updated_ ? int(adj_.size()) : h_ ? int(h_->E) : 0; }
+  int* getEdgeLen() { return updated_ ? w_.data() : h_ ? h_->edgeLen : nullptr; }
+
+  // graph::updateCSRDel(batchUpdate, k, size) (graph.hpp:869-930): the
+  // deletions among batchUpdate[k, k+size). Every copy of an edge goes.
+  void updateCSRDel(std::vector<update>& batchUpdate, int k, int size) { apply(batchUpdate, k, size, 'd'); }
+  // graph::updateCSRAdd(batchUpdate, k, size) (graph.hpp:932-975): the
+  // additions among batchUpdate[k, k+size), with their weights.
+  void updateCSRAdd(std::vector<update>& batchUpdate, int k, int size) { apply(batchUpdate, k, size, 'a'); }
 
   bool check_if_nbr(int s, int d) {
     for (int32_t e = indexofNodes[s]; e < indexofNodes[s + 1]; ++e)
@@ -74,7 +94,7 @@ class graph {
       if (edgeList[e] == d) {
         found.source = s;
         found.destination = d;
-        found.weight = h_->edgeLen[e];
+        found.weight = getEdgeLen()[e];
         found.id = e;
         return found;
       }
@@ -94,9 +114,45 @@ class graph {
   }
 
  private:
+  void apply(std::vector<update>& u, int k, int size, char kind) {
+    std::vector<int32_t> s, d, w;
+    for (int i = k; i < k + size && i < int(u.size()); ++i)
+      if (u[i].type == kind) {
+        s.push_back(u[i].source);
+        d.push_back(u[i].destination);
+        w.push_back(u[i].weight);
+      }
+    int rc = kind == 'd' ? spm_graph_delete_edges(device(), s.data(), d.data(), int32_t(s.size()))
+                         : spm_graph_add_edges(device(), s.data(), d.data(), w.data(), int32_t(s.size()));
+    if (rc != SPM_OK) throw std::runtime_error(std::string("graph update: ") + spm_last_error());
+    refresh();
+  }
+  // Public arrays := the HBM mirror's current (folded) CSR.
+  void refresh() {
+    int64_t E = 0;
+    int rc = spm_graph_num_edges(dev_, &E);
+    const size_t V = size_t(h_->V);
+    off_.assign(V + 1, 0);
+    roff_.assign(V + 1, 0);
+    adj_.assign(size_t(E), 0);
+    w_.assign(size_t(E), 0);
+    radj_.assign(size_t(E), 0);
+    if (rc == SPM_OK)
+      rc = spm_graph_export(dev_, off_.data(), adj_.data(), h_->edgeLen ? w_.data() : nullptr, roff_.data(),
+                            radj_.data());
+    if (rc != SPM_OK) throw std::runtime_error(std::string("graph update: ") + spm_last_error());
+    indexofNodes = off_.data();
+    edgeList = adj_.data();
+    rev_indexofNodes = roff_.data();
+    srcList = radj_.data();
+    updated_ = true;
+  }
+
   std::string filePath_;
   spmh_graph* h_ = nullptr;
   spm_graph* dev_ = nullptr;
+  bool updated_ = false;
+  std::vector<int32_t> off_, adj_, w_, roff_, radj_;
 };
 
 // Status check used by generated code: a failing library call is fatal, as a

@@ -40,6 +40,16 @@ def build_generated(out_dir=BUILD):
                f"-Wl,-rpath,$ORIGIN/{rel}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}", "-o", exe]
         subprocess.run(cmd, check=True)
         exes[algo] = exe
+    # the dynamic-graph driver (graph::updateCSRDel/Add and DynSSSP through
+    # starplat_graph.hpp)
+    exe = os.path.join(bindir, "dyn_driver")
+    rel = os.path.relpath(os.path.join(ROOT, "starplat_amd"), bindir)
+    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "drivers", "dyn_driver.cpp"),
+                    "-L", os.path.join(ROOT, "starplat_amd"), "-lstarplat_amd", "-lspm_host",
+                    "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+                    f"-Wl,-rpath,$ORIGIN/{rel}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}", "-o", exe], check=True)
+    exes["dyn"] = exe
     return exes
 
 

@@ -180,6 +180,48 @@ def test_dynsssp_entry_point_and_update_file(oracle, tmp_path):
     assert g.is_symmetric()
 
 
+def test_cpp_header_driver(oracle, tmp_path):
+    """graph::updateCSRDel / updateCSRAdd and spm_dyn_sssp from C++ through
+    starplat_graph.hpp (tests/drivers/dyn_driver.cpp, built by build())."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "build", "bin", "dyn_driver")
+    assert os.access(exe, os.X_OK), "run build() first"
+    g0 = graph.rmat(10, 8, seed=12, weights=(1, 20), undirected=True)
+    gpath, upath = tmp_path / "g.txt", tmp_path / "u.txt"
+    g0.write_text(str(gpath))
+    g = graph(str(gpath)).parseGraph()
+    rng = np.random.default_rng(4)
+    src_of = np.repeat(np.arange(g.V), np.diff(g.indexofNodes))
+    e = rng.integers(0, g.E, 200)
+    types = np.where(rng.random(200) < 0.5, ord("d"), ord("a")).astype(np.uint8)
+    s = np.where(types == ord("d"), src_of[e], rng.integers(0, g.V, 200))
+    t = np.where(types == ord("d"), g.edgeList[e], rng.integers(0, g.V, 200))
+    write_update_file(str(upath), types.tolist(), s, t)
+    from starplat_amd.updates import as_arrays
+    ty, us, ut, _ = as_arrays(parseUpdateFile(str(upath)))
+    out = tmp_path / "o.bin"
+    r = subprocess.run([exe, "updates", str(gpath), str(upath), "50", str(out)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, np.int32)
+    off = raw[:g.V + 1]
+    E = int(off[-1])
+    ref = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, batch_ops(ty, us, ut, 50))
+    np.testing.assert_array_equal(off, ref[0])
+    for v in range(g.V):
+        a, b = off[v], off[v + 1]
+        assert Counter(zip(raw[g.V + 1 + a:g.V + 1 + b].tolist(), raw[g.V + 1 + E + a:g.V + 1 + E + b].tolist())) == \
+            Counter(zip(ref[1][a:b].tolist(), ref[2][a:b].tolist()))
+    r = subprocess.run([exe, "sssp", str(gpath), str(upath), "50", str(out), "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, np.int32)
+    dref, pref = oracle.dyn_sssp(g, g.edgeLen, 2, ty, us, ut, 50)
+    np.testing.assert_array_equal(raw[:g.V], dref)
+    np.testing.assert_array_equal(raw[g.V:], pref)
+
+
 @pytest.mark.slow
 def test_full_dyn_sssp_rmat22(oracle):
     """RMAT-22 (the SSSP benchmark graph), two batches of 65,536 updates:

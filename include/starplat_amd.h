@@ -175,7 +175,10 @@ typedef struct spm_schedule {
   int32_t pr_mode;        /* 0 pull over in-rows, 1 push (scatter + apply)   */
   int32_t pr_seg_log2;    /* pull: sweep sources in segments of 2^k vertices
                              so gathers stay in L2 (0 = one sweep; 10..30)   */
-  int32_t reserved[6];
+  int32_t relabel;        /* PR and SSSP on a copy of the CSR with vertices
+                             renumbered by descending out-degree (built once
+                             per handle; results in original ids); 0 or 1   */
+  int32_t reserved[5];
 } spm_schedule;
 int spm_get_schedule(spm_graph* g, spm_schedule* s);
 int spm_set_schedule(spm_graph* g, const spm_schedule* s);

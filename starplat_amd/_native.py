@@ -44,7 +44,8 @@ class SpmSchedule(C.Structure):
         ("tc_mode", i32),
         ("pr_mode", i32),
         ("pr_seg_log2", i32),
-        ("reserved", i32 * 6),
+        ("relabel", i32),
+        ("reserved", i32 * 5),
     ]
 
     def as_dict(self):

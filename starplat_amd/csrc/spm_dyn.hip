@@ -811,6 +811,7 @@ static int dyn_fold(spm_graph* g) {
   sssp_release(g);
   tc_release(g);
   bc_release(g);
+  relabel_release(g);
   const size_t eb = sizeof(int32_t) * size_t(g->E), vb = sizeof(int32_t) * (Vs + 1);
   dfree(g, g->off, vb);
   dfree(g, g->adj, eb);

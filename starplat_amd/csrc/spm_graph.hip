@@ -1,5 +1,6 @@
 // spm_graph.hip -- device CSR mirror of graphcode/graph.hpp (:44-57) and the
 // handle plumbing (errors, allocation accounting, profiling, schedules).
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 
@@ -97,6 +98,11 @@ void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
   // One sweep: measured at RMAT-22, 2 MB source segments double the
   // iteration time (each pass repeats the per-vertex load chain).
   s->pr_seg_log2 = 0;
+  // Degree-ordered relabeling pays off when PR's contribution / SSSP's
+  // distance array outgrows L2 (4 MB per XCD, ~1M vertices) and the degrees
+  // are skewed enough that a small set of vertices takes most accesses.
+  const double avg = g->V > 0 ? double(g->E) / double(g->V) : 0.0;
+  s->relabel = (g->V >= (1 << 20) && double(g->max_outdeg) >= 64.0 * avg) ? 1 : 0;
 }
 
 int validate_schedule(const spm_schedule* s) {
@@ -110,6 +116,7 @@ int validate_schedule(const spm_schedule* s) {
   if (s->pr_mode < 0 || s->pr_mode > 1) return set_error(SPM_EINVAL, "schedule: pr_mode must be 0 (pull) or 1 (push)");
   if (s->pr_seg_log2 != 0 && (s->pr_seg_log2 < 10 || s->pr_seg_log2 > 30))
     return set_error(SPM_EINVAL, "schedule: pr_seg_log2 must be 0 or in [10, 30]");
+  if (s->relabel < 0 || s->relabel > 1) return set_error(SPM_EINVAL, "schedule: relabel must be 0 or 1");
   return SPM_OK;
 }
 
@@ -136,6 +143,7 @@ int spm_graph_create(int32_t V, int64_t E, const int32_t* off, const int32_t* ad
   g->device = dev;
   g->V = V;
   g->E = E;
+  for (int32_t v = 0; v < V; ++v) g->max_outdeg = std::max(g->max_outdeg, off[v + 1] - off[v]);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) g->num_cus = prop.multiProcessorCount;
   hipError_t he = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
@@ -175,6 +183,7 @@ int spm_graph_destroy(spm_graph* g) {
   tc_release(g);
   bc_release(g);
   dyn_release(g);
+  relabel_release(g);
   const size_t vb = sizeof(int32_t) * (size_t(g->V) + 1), eb = sizeof(int32_t) * size_t(g->E);
   dfree(g, g->off, vb);
   dfree(g, g->adj, eb);
@@ -197,7 +206,7 @@ int spm_graph_set_weights(spm_graph* g, const int32_t* weight) {
   if (!g->w) SPM_TRY(dmalloc(g, (void**)&g->w, eb, "edgeLen"));
   if (g->E) SPM_CHECK_HIP(hipMemcpyAsync(g->w, weight, eb, hipMemcpyHostToDevice, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
-  return SPM_OK;
+  return relabel_weights(g);
 }
 
 void* spm_graph_stream(spm_graph* g) { return g ? (void*)g->stream : nullptr; }

@@ -41,6 +41,7 @@ struct PrWork;
 struct TcWork;
 struct BcWork;
 struct DynWork;
+struct Relabel;
 
 struct Profiler {
   bool on = false;
@@ -70,6 +71,8 @@ struct spm_graph {
   spm::TcWork* tc = nullptr;
   spm::BcWork* bc = nullptr;
   spm::DynWork* dyn = nullptr;  // edge updates since the CSR was last clean
+  spm::Relabel* rl = nullptr;   // degree-ordered copy of the CSR (PR, SSSP)
+  int32_t max_outdeg = 0;       // of the CSR given at creation
   spm::Profiler prof;
   int32_t* h_pinned = nullptr;  // small pinned scratch for flag readback (16 ints)
 };
@@ -108,6 +111,32 @@ int dyn_settle(spm_graph* g);
 // The handle's weights changed: weights derived from them are stale.
 void dyn_weights_changed(spm_graph* g);
 int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32_t* rounds_out);
+
+// The CSR PageRank and SSSP run on: the handle's own arrays, or (schedule
+// option relabel) a copy with vertices renumbered by descending out-degree,
+// built on first use. perm maps original ids to new ones (null: identity).
+struct CsrView {
+  const int32_t* off;
+  const int32_t* adj;
+  const int32_t* w;
+  const int32_t* roff;
+  const int32_t* radj;
+  const int32_t* perm;    // device [V]
+  const int32_t* h_perm;  // host [V]
+};
+int csr_view(spm_graph* g, CsrView* v);
+// Drops the relabeled copy (and the PR / SSSP workspaces that refer to it).
+int relabel_release(spm_graph* g);
+// Re-derives the relabeled copy's weights after the handle's weights changed.
+int relabel_weights(spm_graph* g);
+// out[v] = in[perm[v]] for V 4-byte elements (a copy without relabeling);
+// in and out must differ.
+int gather_back(spm_graph* g, const CsrView& v, const void* in, void* out);
+// Sorted CSR (rows ascending by column) and, when roff is given, reverse CSR
+// from n (row << 32 | column, value) pairs in k0/v0; k1/v1 are scratch of the
+// same size. w / rw receive the values (may be null).
+int csr_from_pairs(spm_graph* g, int64_t n, unsigned long long* k0, unsigned long long* k1, int32_t* v0, int32_t* v1,
+                   int32_t* off, int32_t* adj, int32_t* w, int32_t* roff, int32_t* radj, int32_t* rw);
 // Distances of the last SSSP run on g (device memory, V entries).
 int32_t* sssp_dist(spm_graph* g);
 

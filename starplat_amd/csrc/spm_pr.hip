@@ -60,6 +60,8 @@ struct PrWork {
   size_t sp_n = 0;
   int S = 1;                 // source segments per pull sweep
   int4* rows = nullptr;      // {v, in-row start, in-row end, 0} grouped by bin
+  float* out = nullptr;      // [V] result in new ids before the gather back (relabeled runs)
+  CsrView view{};            // the CSR the kernels run on
   PrState* st = nullptr;
   PrBins bins{};
   hipGraph_t graph = nullptr;
@@ -368,11 +370,13 @@ __global__ __launch_bounds__(kPrBlock) void pr_push_apply(int32_t V, const float
   pr_finish(mydiff, partial, s_red);
 }
 
-static int pr_prepare(spm_graph* g) {
+static int pr_prepare(spm_graph* g, const CsrView& view) {
   if (g->pr) return SPM_OK;
   if (!g->roff) return set_error(SPM_ENOGRAPH, "PageRank needs the reverse CSR (rev_indexofNodes, srcList)");
   PrWork* w = new PrWork();
   g->pr = w;
+  w->view = view;
+  if (view.perm) SPM_TRY(dmalloc(g, (void**)&w->out, sizeof(float) * size_t(g->V), "pr result"));
   const int32_t V = g->V;
   for (int i = 0; i < 2; ++i) {
     SPM_TRY(dmalloc(g, (void**)&w->pr[i], sizeof(float) * V, "pr ranks"));
@@ -384,7 +388,7 @@ static int pr_prepare(spm_graph* g) {
   SPM_TRY(dmalloc(g, (void**)&w->st, sizeof(PrState), "pr state"));
   // Bin vertices by in-degree on the host (once per handle and schedule).
   std::vector<int32_t> roff(size_t(V) + 1);
-  SPM_CHECK_HIP(hipMemcpyAsync(roff.data(), g->roff, sizeof(int32_t) * (V + 1), hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(roff.data(), view.roff, sizeof(int32_t) * (V + 1), hipMemcpyDeviceToHost, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   const spm_schedule& s = g->sched;
   std::vector<int32_t> order(V);
@@ -420,8 +424,8 @@ static int pr_prepare(spm_graph* g) {
     w->sp_n = size_t(V) * size_t(w->S - 1);
     SPM_TRY(dmalloc(g, (void**)&w->sp, sizeof(int32_t) * w->sp_n, "pr segment splits"));
     const int grid = int(std::min<int64_t>((V + 255) / 256, 4096));
-    SPM_TRY(launch(g, "pr_split", pr_split, dim3(grid), dim3(256), 0, V, (const int32_t*)g->roff,
-                   (const int32_t*)g->radj, w->S, s.pr_seg_log2, w->sp));
+    SPM_TRY(launch(g, "pr_split", pr_split, dim3(grid), dim3(256), 0, V, view.roff, view.radj, w->S,
+                   s.pr_seg_log2, w->sp));
   }
   return SPM_OK;
 }
@@ -440,6 +444,7 @@ int pr_release(spm_graph* g) {
   dfree(g, w->partial, sizeof(float) * size_t(w->partial_n));
   dfree(g, w->sp, sizeof(int32_t) * w->sp_n);
   dfree(g, w->rows, sizeof(int4) * g->V);
+  dfree(g, w->out, sizeof(float) * g->V);
   dfree(g, w->st, sizeof(PrState));
   delete w;
   g->pr = nullptr;
@@ -455,8 +460,8 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
     const float* pin = w->pr[p];
     const float* cin = w->c[p];
     if (g->sched.pr_mode == 1) {
-      SPM_TRY(launch(g, "pr_push_scatter", pr_push_scatter, dim3(pgrid), block, 0, g->V, (const int32_t*)g->off,
-                     (const int32_t*)g->adj, cin, w->acc, (const PrState*)w->st));
+      SPM_TRY(launch(g, "pr_push_scatter", pr_push_scatter, dim3(pgrid), block, 0, g->V, w->view.off, w->view.adj,
+                     cin, w->acc, (const PrState*)w->st));
       SPM_TRY(launch(g, "pr_push_apply", pr_push_apply, dim3(pgrid), block, 0, g->V, (const float*)w->degf, w->acc,
                      pin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st, w->partial));
       SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial, pgrid, w->st));
@@ -466,8 +471,8 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
       const PrSeg sg{w->sp, w->acc, w->S, seg};
 #define SPM_PR_LAUNCH(G)                                                                          \
   case G:                                                                                         \
-    SPM_TRY(launch(g, "pr_iter", pr_iter<G>, grid, block, 0, (const int32_t*)g->roff,            \
-                   (const int32_t*)g->radj, (const int4*)w->rows, (const float*)w->degf,          \
+    SPM_TRY(launch(g, "pr_iter", pr_iter<G>, grid, block, 0, w->view.roff,                        \
+                   w->view.radj, (const int4*)w->rows, (const float*)w->degf,                     \
                    w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st,             \
                    w->partial, sg));                                                               \
     break;
@@ -490,14 +495,16 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
 int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out, float* h_out,
            int32_t* iter_out) {
   SPM_TRY(dyn_settle(g));
-  SPM_TRY(pr_prepare(g));
+  CsrView view;
+  SPM_TRY(csr_view(g, &view));
+  SPM_TRY(pr_prepare(g, view));
   PrWork* w = g->pr;
   const int32_t V = g->V;
   const float num_nodes = float(V);
   const float init = 1 / num_nodes;
   const float base = (1 - delta) / num_nodes;
   const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
-  SPM_TRY(launch(g, "pr_init", pr_init, dim3(igrid), dim3(256), 0, V, (const int32_t*)g->off, w->degf,
+  SPM_TRY(launch(g, "pr_init", pr_init, dim3(igrid), dim3(256), 0, V, w->view.off, w->degf,
                  w->pr[0], w->c[0], w->acc, w->st, init, int(maxIter), beta, delta, base));
   // Iterations per replay: even (ping-pong parity), up to 128.
   int R = std::max(2, std::min(128, (int(std::max(maxIter, 1)) + 1) & ~1));
@@ -526,7 +533,12 @@ int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out,
   }
   const int32_t iters = h[1];
   const float* res = w->pr[iters & 1];
-  if (d_out) SPM_CHECK_HIP(hipMemcpyAsync(d_out, res, sizeof(float) * V, hipMemcpyDeviceToDevice, g->stream));
+  if (w->view.perm) {  // ranks in original ids
+    SPM_TRY(gather_back(g, w->view, res, d_out ? d_out : w->out));
+    res = d_out ? d_out : w->out;
+  } else if (d_out) {
+    SPM_CHECK_HIP(hipMemcpyAsync(d_out, res, sizeof(float) * V, hipMemcpyDeviceToDevice, g->stream));
+  }
   if (h_out) SPM_CHECK_HIP(hipMemcpyAsync(h_out, res, sizeof(float) * V, hipMemcpyDeviceToHost, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   if (iter_out) *iter_out = iters;

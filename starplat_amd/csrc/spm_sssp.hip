@@ -66,6 +66,9 @@ struct SsspWork {
   uint8_t* active = nullptr;          // [2][V] dense-mode frontier flags
   uint2* tile_sum = nullptr;          // [tiles] (vertices, edges) per compaction tile
   int32_t* srcbuf = nullptr;          // multi-source seeds
+  int32_t* dist_out = nullptr;        // [V] distances in original ids (relabeled runs)
+  bool dist_out_ok = false;
+  CsrView view{};                     // the CSR the kernels run on
   size_t qcap = 0, ccap = 0, src_cap = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -425,10 +428,12 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_dense(
     atomicOr(ctr_out, 1ull);
 }
 
-static int sssp_alloc(spm_graph* g) {
+static int sssp_alloc(spm_graph* g, const CsrView& view) {
   if (g->sssp) return SPM_OK;
   SsspWork* w = new SsspWork();
   g->sssp = w;
+  w->view = view;
+  if (view.perm) SPM_TRY(dmalloc(g, (void**)&w->dist_out, sizeof(int32_t) * size_t(g->V), "sssp result"));
   const int32_t V = g->V;
   w->qcap = size_t(V) + 1;
   w->ccap = size_t(g->E) / kChunk + 3;
@@ -466,6 +471,7 @@ int sssp_release(spm_graph* g) {
   dfree(g, w->active, 2 * size_t(g->V));
   dfree(g, w->tile_sum, sizeof(uint2) * ((size_t(g->V) + kTileV - 1) / kTileV));
   dfree(g, w->srcbuf, sizeof(int32_t) * w->src_cap);
+  dfree(g, w->dist_out, sizeof(int32_t) * g->V);
   delete w;
   g->sssp = nullptr;
   return SPM_OK;
@@ -485,18 +491,18 @@ static int sssp_enqueue_rounds(spm_graph* g, int R) {
       uint8_t* cur = w->active + size_t(r & 1) * g->V;
       uint8_t* nxt = w->active + size_t((r + 1) & 1) * g->V;
       SPM_TRY(launch(g, "sssp_dense", sssp_dense, dim3(grid), dim3(kSsspBlock), 0, g->V,
-                     (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)g->w, w->dist,
+                     w->view.off, w->view.adj, w->view.w, w->dist,
                      cur, nxt, w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3));
     } else {
       SPM_TRY(launch(g, "sssp_relax", sssp_relax, dim3(grid), dim3(kSsspBlock), 0,
-                     (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)g->w, w->dist,
+                     w->view.off, w->view.adj, w->view.w, w->dist,
                      w->stamp, w->q[i], w->q[o], w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3,
                      (const uint32_t*)w->round_base, uint32_t(r)));
       const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
-      SPM_TRY(launch(g, "sssp_count", sssp_count, dim3(tiles), dim3(kSsspBlock), 0, g->V, (const int32_t*)g->off,
+      SPM_TRY(launch(g, "sssp_count", sssp_count, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
                      (const uint32_t*)w->stamp, (const unsigned long long*)(w->ctr + i),
                      (const uint32_t*)w->round_base, uint32_t(r), w->tile_sum));
-      SPM_TRY(launch(g, "sssp_emit", sssp_emit, dim3(tiles), dim3(kSsspBlock), 0, g->V, (const int32_t*)g->off,
+      SPM_TRY(launch(g, "sssp_emit", sssp_emit, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
                      (const uint32_t*)w->stamp, (const unsigned long long*)(w->ctr + i),
                      (const uint32_t*)w->round_base, uint32_t(r), (const uint2*)w->tile_sum, w->q[o],
                      w->ctr + o));
@@ -525,7 +531,18 @@ static int sssp_capture(spm_graph* g) {
 }
 
 // srcs: n distinct sources in host memory (n == 1 passes the value directly).
-int32_t* sssp_dist(spm_graph* g) { return g->sssp ? g->sssp->dist : nullptr; }
+// Distances of the last run in original ids (gathered back on first request
+// after a relabeled run; the gather is ordered on g's stream).
+int32_t* sssp_dist(spm_graph* g) {
+  SsspWork* w = g->sssp;
+  if (!w) return nullptr;
+  if (!w->view.perm) return w->dist;
+  if (!w->dist_out_ok) {
+    if (gather_back(g, w->view, w->dist, w->dist_out) != SPM_OK) return nullptr;
+    w->dist_out_ok = true;
+  }
+  return w->dist_out;
+}
 
 int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32_t* rounds_out) {
   SPM_TRY(dyn_settle(g));
@@ -533,8 +550,17 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   if (n < 1 || !srcs) return set_error(SPM_EINVAL, "SSSP needs at least one source");
   for (int32_t i = 0; i < n; ++i)
     if (srcs[i] < 0 || srcs[i] >= g->V) return set_error(SPM_EINVAL, "SSSP source out of range");
-  SPM_TRY(sssp_alloc(g));
+  CsrView view;
+  SPM_TRY(csr_view(g, &view));
+  SPM_TRY(sssp_alloc(g, view));
   SsspWork* w = g->sssp;
+  w->dist_out_ok = false;
+  std::vector<int32_t> mapped;
+  if (view.h_perm) {  // sources in the kernels' ids
+    mapped.resize(size_t(n));
+    for (int32_t i = 0; i < n; ++i) mapped[i] = view.h_perm[srcs[i]];
+    srcs = mapped.data();
+  }
   const int R = g->sched.sssp_rounds;
   const int dense = g->sched.sssp_mode == 1;
   if (w->host_base > 0xF0000000u) {  // stamp wrap-around: start over
@@ -556,7 +582,7 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   uint8_t* act = dense ? w->active : (uint8_t*)nullptr;
   SPM_TRY(launch(g, "sssp_init", sssp_init, dim3(igrid), dim3(256), 0, g->V, w->dist, act));
   SPM_TRY(launch(g, "sssp_seed", sssp_seed, dim3(1), dim3(kSsspBlock), 0, n > 1 ? (const int32_t*)w->srcbuf : nullptr,
-                 n, srcs[0], (const int32_t*)g->off, w->dist, w->q[0], w->ctr, act));
+                 n, srcs[0], w->view.off, w->dist, w->q[0], w->ctr, act));
   const bool use_graph = !g->prof.on;
   if (use_graph) SPM_TRY(sssp_capture(g));
   unsigned long long* h = reinterpret_cast<unsigned long long*>(g->h_pinned);
@@ -572,7 +598,7 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
     if (++replays > max_replays) return set_error(SPM_EDIVERGE, "SSSP did not converge within V rounds (negative weights?)");
   }
   SPM_CHECK_HIP(hipMemcpyAsync(h, w->ctr + 3, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
-  if (d_out) SPM_CHECK_HIP(hipMemcpyAsync(d_out, w->dist, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice, g->stream));
+  if (d_out) SPM_TRY(gather_back(g, w->view, w->dist, d_out));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   if (rounds_out) *rounds_out = int32_t(h[0]);
   SPM_TRY(prof_flush(g));
@@ -603,7 +629,9 @@ int spm_compute_sssp(spm_graph* g, int32_t* dist, const int32_t* weight, int32_t
   SPM_TRY(dyn_settle(g));
   if (weight) SPM_TRY(spm_graph_set_weights(g, weight));
   SPM_TRY(sssp_run(g, &src, 1, nullptr, nullptr));
-  SPM_CHECK_HIP(hipMemcpyAsync(dist, g->sssp->dist, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost, g->stream));
+  const int32_t* res = sssp_dist(g);
+  if (!res) return set_error(SPM_EHIP, "spm_compute_sssp: result gather failed");
+  SPM_CHECK_HIP(hipMemcpyAsync(dist, res, sizeof(int32_t) * g->V, hipMemcpyDeviceToHost, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   return SPM_OK;
 }

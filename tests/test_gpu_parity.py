@@ -176,6 +176,68 @@ def test_pr_schedules(oracle, t1, t2, t3, grp, seg, mode):
     assert it == 30 and rel_linf(out, ref) <= PR_TOL
 
 
+# ------------------------------------------------------------------ relabeled layout
+def test_relabel_sssp_bit_identical(oracle):
+    """The degree-ordered copy changes vertex ids inside the kernels only:
+    distances come back bit-identical in original ids, for both frontier
+    modes, several sources and a multi-source seed."""
+    g = graph.rmat(14, 16, seed=5, weights=(1, 100))
+    dev = g.device()
+    for mode in (0, 1):
+        dev.set_schedule(relabel=0, sssp_mode=mode)
+        base = [dev.sssp(s) for s in (0, 7, g.V - 1)]
+        dev.set_schedule(relabel=1, sssp_mode=mode)
+        for s, b in zip((0, 7, g.V - 1), base):
+            np.testing.assert_array_equal(dev.sssp(s), b)
+    np.testing.assert_array_equal(base[1], oracle.compute_sssp(g, g.edgeLen, 7))
+    from starplat_amd import hip
+    d = hip.DeviceArray(g.V, np.int32)
+    srcs = np.array([3, 99, 1000], np.int32)
+    dev.set_schedule(relabel=1, sssp_mode=0)
+    dev.sssp_multi_device(srcs, d)
+    multi = d.to_host()
+    d.free()
+    np.testing.assert_array_equal(multi, np.min([oracle.compute_sssp(g, g.edgeLen, int(s)) for s in srcs], axis=0))
+    # new weights reach the relabeled copy
+    w2 = (g.edgeLen * 3 + 1).astype(np.int32)
+    dist = np.empty(g.V, np.int32)
+    Compute_SSSP(g, dist, w2, 0)
+    np.testing.assert_array_equal(dist, oracle.compute_sssp(g, w2, 0))
+
+
+def test_relabel_pr_and_updates(oracle):
+    g = graph.rmat(14, 16, seed=6)
+    dev = g.device()
+    dev.set_schedule(relabel=1)
+    ref, it_ref = oracle.compute_pr(g, 1e-10, 0.85, 100)
+    out, it = dev.pr(1e-10, 0.85, 100)
+    assert exits_agree(oracle, g, it, it_ref), (it, it_ref)
+    check_pr_values(oracle, g, out, it, ref, it_ref)
+    # push mode on the relabeled copy
+    dev.set_schedule(relabel=1, pr_mode=1)
+    out2, it2 = dev.pr(1e-10, 0.85, 30)
+    ref2, _ = oracle.compute_pr(g, 1e-10, 0.85, 30)
+    assert it2 == 30 and rel_linf(out2, ref2) <= PR_TOL
+    # edge updates drop the copy; the next run sees the updated graph
+    dev.set_schedule(relabel=1, pr_mode=0)
+    src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
+    dev.delete_edges(src_of[::5], g.edgeList[::5])
+    off, adj, _, roff, radj = dev.export()
+
+    class _U:
+        V, indexofNodes, edgeList, rev_indexofNodes, srcList = g.V, off, adj, roff, radj
+    ref3, it3_ref = oracle.compute_pr(_U, 1e-10, 0.85, 20)
+    out3, it3 = dev.pr(1e-10, 0.85, 20)
+    assert it3 == 20 and rel_linf(out3, ref3) <= PR_TOL
+
+
+def test_relabel_default_heuristic():
+    small = graph.rmat(12, 16, seed=1)
+    assert small.device().default_schedule().relabel == 0
+    er = graph.erdos_renyi(1 << 20, 1 << 22, seed=1)
+    assert er.device().default_schedule().relabel == 0  # no degree skew
+
+
 # ------------------------------------------------------------------ TC
 @pytest.mark.parametrize("scale,seed", [(8, 1), (12, 2), (16, 3)])
 @pytest.mark.parametrize("mode", [0, 1])

@@ -43,7 +43,9 @@
 namespace spm {
 
 constexpr int kSsspBlock = 256;
-constexpr int kChunk = 2048;
+// 1024-edge chunks keep the LDS stage at 12 KB, so 8 wavefronts fit per SIMD
+// (2048-edge chunks: 24.6 KB, 6 per SIMD; measured at RMAT-22 1.66 vs 1.73 ms).
+constexpr int kChunk = 1024;
 constexpr int kPer = kChunk / kSsspBlock;
 constexpr int kSsspWaves = kSsspBlock / kWave;
 // A round whose frontier has fewer edges pushes through a counter (at most
@@ -477,8 +479,10 @@ int sssp_release(spm_graph* g) {
   return SPM_OK;
 }
 
+// Default: 6 workgroups per CU (measured at RMAT-22: 1536 workgroups 1.635 ms,
+// 1024 1.672 ms, 2048 1.658 ms, 768 1.758 ms).
 static int sssp_grid(const spm_graph* g) {
-  return g->sched.sssp_grid > 0 ? g->sched.sssp_grid : g->num_cus * 4;
+  return g->sched.sssp_grid > 0 ? g->sched.sssp_grid : g->num_cus * 6;
 }
 
 // Enqueues R rounds (R a multiple of 6: queues rotate by 3, dense flags by 2) followed by the round-base advance.

@@ -14,7 +14,7 @@
 // queue position that chunk starts at. A workgroup then stages its chunk's
 // vertices (at most kChunk+1) in LDS and every lane locates its edge by a
 // binary search in LDS (lanes take consecutive edges, so loads stay
-// coalesced). Hubs of 10^5 edges are spread over ~50 workgroups, and every
+// coalesced). Hubs of 10^5 edges are spread over ~100 workgroups, and every
 // relaxation lane does useful work. Frontier distances are read with
 // device-scope loads (a stale one could lose an improvement); the per-edge
 // test against the neighbour's distance is a cached pre-filter in front of

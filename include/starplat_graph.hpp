@@ -76,6 +76,12 @@ class graph {
   int num_edges() { return updated_ ? int(adj_.size()) : h_ ? int(h_->E) : 0; }
   int* getEdgeLen() { return updated_ ? w_.data() : h_ ? h_->edgeLen : nullptr; }
 
+  // Public arrays := the HBM mirror's current CSR (after updates applied on
+  // the device, e.g. by spm_dyn_sssp).
+  void sync_from_device() {
+    if (dev_) refresh();
+  }
+
   // graph::updateCSRDel(batchUpdate, k, size) (graph.hpp:869-930): the
   // deletions among batchUpdate[k, k+size). Every copy of an edge goes.
   void updateCSRDel(std::vector<update>& batchUpdate, int k, int size) { apply(batchUpdate, k, size, 'd'); }
@@ -154,6 +160,32 @@ class graph {
   bool updated_ = false;
   std::vector<int32_t> off_, adj_, w_, roff_, radj_;
 };
+
+// update.hpp:18-72: one "<type> <src> <dst>" per line; every line is pushed
+// twice, once reversed, as the reference does; additions weigh 1.
+inline std::vector<update> parseUpdateFile(const char* updateFile) {
+  std::vector<update> out;
+  FILE* f = std::fopen(updateFile, "r");
+  if (!f) return out;
+  char line[512];
+  while (std::fgets(line, sizeof line, f)) {
+    char t = 0;
+    int s = 0, d = 0;
+    if (std::sscanf(line, " %c %d %d", &t, &s, &d) != 3 || (t != 'a' && t != 'd')) continue;
+    update u;
+    u.type = t;
+    u.source = s;
+    u.destination = d;
+    u.weight = 1;
+    update r = u;
+    r.source = d;
+    r.destination = s;
+    out.push_back(u);
+    out.push_back(r);
+  }
+  std::fclose(f);
+  return out;
+}
 
 // Status check used by generated code: a failing library call is fatal, as a
 // failing CUDA call is in the reference's generated code (libcuda.cuh:215-222).

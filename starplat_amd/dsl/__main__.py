@@ -2,6 +2,7 @@
 (src/parser/lrparser.y:600-700):
 
     python -m starplat_amd.dsl -s -f <dsl file> -b mi355x [-o <graphcode dir>]
+    python -m starplat_amd.dsl -d -f <dynamic dsl file> -b mi355x [-o <graphcode dir>]
 
 writes <graphcode dir>/generated_mi355x/<dsl base name>.{h,cc}; the default
 graphcode dir is ../graphcode, as the reference writes
@@ -38,14 +39,11 @@ def main(argv=None):
     if b != "mi355x":
         print("Specified backend target is not implemented in the current version!", file=sys.stderr)
         return 255
-    if "-d" in o:
-        print("dynamic graphs are not supported by the mi355x backend", file=sys.stderr)
-        return 255
-    if "-s" not in o:
+    if "-s" not in o and "-d" not in o:
         print("Type of graph(static/dynamic) not specified!", file=sys.stderr)
         return 255
     try:
-        h, cc, plans = compile_file(f, o.get("-o", "../graphcode"))
+        h, cc, plans = compile_file(f, o.get("-o", "../graphcode"), dynamic="-d" in o)
     except FileNotFoundError:
         print("file doesn't exists!", file=sys.stderr)
         return 255

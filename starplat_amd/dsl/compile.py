@@ -5,21 +5,21 @@ import os
 
 from .emit import BACKEND, emit
 from .frontend import parse
-from .lower import lower
+from .lower import lower_program
 
 
-def compile_source(src: str, base: str, src_path: str = ""):
-    plans = [lower(f) for f in parse(src)]
+def compile_source(src: str, base: str, src_path: str = "", dynamic: bool = False):
+    plans = lower_program(parse(src), dynamic)
     h, cc = emit(plans, base, src_path or base)
     return h, cc, plans
 
 
-def compile_file(path: str, graphcode_dir: str):
+def compile_file(path: str, graphcode_dir: str, dynamic: bool = False):
     with open(path) as fh:
         src = fh.read()
     base = os.path.basename(path)
     base = os.path.splitext(base)[0] if base.endswith(".sp") else base
-    h, cc, plans = compile_source(src, base, path)
+    h, cc, plans = compile_source(src, base, path, dynamic)
     out = os.path.join(graphcode_dir, f"generated_{BACKEND}")
     os.makedirs(out, exist_ok=True)
     hp, cp = os.path.join(out, base + ".h"), os.path.join(out, base + ".cc")

@@ -28,6 +28,8 @@ def cpp_type(t):
         return "graph&"
     if t.base == "SetN":
         return "std::set<int>&"
+    if t.base == "updates":
+        return "std::vector<update>"
     raise ValueError(f"type {t} has no C++ mapping on this backend")
 
 
@@ -62,6 +64,25 @@ def body(plan: Plan) -> list:
         out.append(f"  uint64_t {r['count']} = 0;")
         out.append(f"  spm_require(spm_compute_tc({g}.device(), &{r['count']}), \"{plan.func.name}\");")
         out.append(f"  return ({r['count_type']}){r['count']};")
+    elif k == "dyn_static":
+        out.append(f"  // Static Bellman-Ford with parents -> spm_dynsssp_init / spm_dynsssp_result")
+        out.append(f"  spm_require(spm_graph_set_weights({g}.device(), {r['weight']}), \"{plan.func.name}\");")
+        out.append(f"  spm_require(spm_dynsssp_init({g}.device(), {r['src']}), \"{plan.func.name}\");")
+        out.append(f"  spm_require(spm_dynsssp_result({g}.device(), {r['dist']}, {r['parent']}), \"{plan.func.name}\");")
+    elif k == "dynsssp":
+        out.append(f"  // {r['static']} + Batch(OnDelete, updateCSRDel, Decremental, OnAdd, updateCSRAdd, Incremental)")
+        out.append(f"  //   -> spm_dyn_sssp (device edge updates; cut-off, pull and push fixed points)")
+        out.append(f"  spm_require(spm_graph_set_weights({g}.device(), {r['weight']}), \"{plan.func.name}\");")
+        out.append(f"  std::vector<char> type_;")
+        out.append(f"  std::vector<int32_t> src_, dst_;")
+        out.append(f"  for (const update& u : {r['updates']}) {{")
+        out.append(f"    type_.push_back(u.type);")
+        out.append(f"    src_.push_back(u.source);")
+        out.append(f"    dst_.push_back(u.destination);")
+        out.append(f"  }}")
+        out.append(f"  spm_require(spm_dyn_sssp({g}.device(), {r['src']}, type_.data(), src_.data(), dst_.data(), "
+                   f"(int32_t)src_.size(), {r['batchSize']}, {r['dist']}, {r['parent']}), \"{plan.func.name}\");")
+        out.append(f"  {g}.sync_from_device();  // the graph now holds the updated edges, as after updateCSRAdd")
     elif k == "bc":
         out.append(f"  // iterateInBFS / iterateInReverse -> spm_compute_bc (64-source batched kernels)")
         out.append(f"  std::vector<int32_t> sources_({r['sourceSet']}.begin(), {r['sourceSet']}.end());")

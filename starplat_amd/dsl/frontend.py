@@ -29,7 +29,8 @@ KEYWORDS = {
     "function", "Graph", "int", "long", "float", "double", "bool", "node", "edge",
     "propNode", "propEdge", "SetN", "SetE", "forall", "for", "in", "if", "else", "do",
     "while", "fixedPoint", "until", "iterateInBFS", "iterateInReverse", "from", "return",
-    "True", "False", "INF", "Min", "Max", "Incremental", "Decremental", "Dynamic",
+    "True", "False", "INF", "Min", "Max", "Incremental", "Decremental", "Dynamic", "Static",
+    "Batch", "OnDelete", "OnAdd", "updates",
 }
 _TOKEN = re.compile(r"""
     (?P<ws>[ \t\r]+) | (?P<nl>\n) | (?P<comment>//[^\n]*|/\*.*?\*/)
@@ -218,11 +219,27 @@ class Return:
 
 
 @dataclass
+class Batch:           # Batch(updates : batchSize) { ... }
+    updates: str
+    size: Any
+    body: Block
+
+
+@dataclass
+class OnUpdate:        # OnDelete / OnAdd (var in source): { ... }
+    kind: str          # 'delete' | 'add'
+    var: str
+    source: Any
+    body: Block
+
+
+@dataclass
 class Function:
     name: str
     params: list
     body: Block
     line: int = 0
+    kind: str = "function"   # or Static / Incremental / Decremental / Dynamic
 
     def param(self, name):
         for p in self.params:
@@ -271,10 +288,15 @@ class Parser:
 
     def function(self):
         t = self.peek()
-        if self.at("Incremental") or self.at("Decremental") or self.at("Dynamic"):
-            raise DSLError("dynamic-graph functions are not part of the static backend", t.line)
-        self.eat("function")
-        name = self.ident()
+        kind = "function"
+        if self.at("Incremental") or self.at("Decremental"):
+            kind = name = self.eat().text   # these headers carry no name
+        elif self.at("Static") or self.at("Dynamic"):
+            kind = self.eat().text
+            name = self.ident()
+        else:
+            self.eat("function")
+            name = self.ident()
         self.eat("(")
         params = []
         while not self.at(")"):
@@ -283,11 +305,11 @@ class Parser:
             if not self.at(")"):
                 self.eat(",")
         self.eat(")")
-        return Function(name, params, self.block(), t.line)
+        return Function(name, params, self.block(), t.line, kind)
 
     def is_type_start(self):
         t = self.peek()
-        return t.kind == "kw" and (t.text in PRIMS or t.text in ("propNode", "propEdge", "SetN", "SetE"))
+        return t.kind == "kw" and (t.text in PRIMS or t.text in ("propNode", "propEdge", "SetN", "SetE", "updates"))
 
     def type_(self):
         t = self.eat()
@@ -298,7 +320,7 @@ class Parser:
             inner = self.type_()
             self.eat(">")
             return Type(t.text, inner)
-        if t.text in ("SetN", "SetE"):
+        if t.text in ("SetN", "SetE", "updates"):
             self.eat("<")
             g = self.ident()
             self.eat(">")
@@ -383,6 +405,23 @@ class Parser:
                 self.eat(")")
                 node.rev_body = self.block()
             return node
+        if self.at("Batch"):
+            self.eat("Batch")
+            self.eat("(")
+            upd = self.ident()
+            self.eat(":")
+            size = self.expr()
+            self.eat(")")
+            return Batch(upd, size, self.block())
+        if self.at("OnDelete") or self.at("OnAdd"):
+            kind = "delete" if self.eat().text == "OnDelete" else "add"
+            self.eat("(")
+            var = self.ident()
+            self.eat("in")
+            src = self.expr()
+            self.eat(")")
+            self.eat(":")
+            return OnUpdate(kind, var, src, self.block())
         if self.at("if"):
             self.eat("if")
             self.eat("(")
@@ -547,6 +586,10 @@ class Parser:
             self.eat()
             args, _ = self.args()
             return MinMax(t.text, args)
+        if (self.at("Incremental") or self.at("Decremental")) and self.at("(", 1):
+            self.eat()
+            args, named = self.args()
+            return Call(None, t.text, args, named)
         if self.at("("):
             self.eat("(")
             e = self.expr()

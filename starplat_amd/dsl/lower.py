@@ -11,6 +11,9 @@ implements it:
   fixedPoint + Min reduction list over neighbors           -> spm_compute_sssp
   triple nested forall with is_an_edge and a scalar +=     -> spm_compute_tc
   iterateInBFS sigma sum + iterateInReverse dependency     -> spm_compute_bc
+  Dynamic SSSP: Static Bellman-Ford with parents, Batch of
+    OnDelete / updateCSRDel / Decremental / OnAdd / updateCSRAdd /
+    Incremental                                             -> spm_dyn_sssp
 
 The checks are structural and strict: a function whose arithmetic differs
 from the recognised idiom (another reduction, another update formula, a
@@ -21,9 +24,9 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
-from .frontend import (Assign, BinOp, Block, Bool, Call, Decl, DoWhile, DSLError, ExprStmt, FixedPoint,
-                       ForAll, Function, If, Inf, IterBFS, MinMax, Name, Num, Prop, ReduceAssign, Return,
-                       UnOp, While)
+from .frontend import (Assign, Batch, BinOp, Block, Bool, Call, Decl, DoWhile, DSLError, ExprStmt, FixedPoint,
+                       ForAll, Function, If, Inf, IterBFS, MinMax, Name, Num, OnUpdate, Prop, ReduceAssign,
+                       Return, UnOp, While)
 
 
 @dataclass
@@ -401,3 +404,245 @@ def lower(f: Function) -> Plan:
             return plan
     raise DSLError(f"function {f.name}: no lowering on the mi355x backend; supported shapes are pull PageRank, "
                    "fixedPoint Bellman-Ford SSSP, forward triangle counting and Brandes BC", f.line)
+
+
+# ---------------------------------------------------------------- dynamic SSSP
+# graphcode/dynamicDSLCodes/dynamicBatchSSSP:1-94 is the shape recognised here.
+def _is_false(e):
+    return (isinstance(e, Bool) and not e.value) or same(e, Name("false"))
+
+
+def _is_true(e):
+    return (isinstance(e, Bool) and e.value) or same(e, Name("true"))
+
+
+def _inf2(e):
+    """INT_MAX/2, the reference's 'removed' distance."""
+    return isinstance(e, BinOp) and e.op == "/" and same(e.lhs, Name("INT_MAX")) and same(e.rhs, Num(2.0, "2"))
+
+
+def _minus_one(e):
+    return (isinstance(e, UnOp) and e.op == "-" and same(e.arg, Num(1.0, "1"))) or same(e, Num(-1.0, "-1"))
+
+
+def _assigns(block):
+    return {norm(s.target): s.expr for s in stmts(block) if isinstance(s, Assign) and s.op == "="}
+
+
+def _relax_with_parent(f, g, where):
+    """The fixedPoint push relaxation <t.dist, t.flag, t.parent> =
+    <Min(t.dist, v.dist + e.weight), True, v> over neighbors(v) of the vertices
+    whose flag is set. Returns (dist, parent, weight, flag)."""
+    fps = find_all(f.body, FixedPoint)
+    if len(fps) != 1:
+        raise DSLError(f"{where}: expected one fixedPoint")
+    fp = fps[0]
+    if not (isinstance(fp.cond, UnOp) and fp.cond.op == "!" and isinstance(fp.cond.arg, Name)):
+        raise DSLError(f"{where}: fixedPoint condition must be !<flag>")
+    mod = fp.cond.arg.id
+    outer = [s for s in stmts(fp.body) if isinstance(s, ForAll)]
+    if len(outer) != 1 or not is_graph_call(outer[0].source, g, "nodes", 0) or not is_filter_true(outer[0].filter, mod):
+        raise DSLError(f"{where}: expected forall v in {g}.nodes().filter({mod} == True)")
+    v = outer[0].var
+    inner = [s for s in stmts(outer[0].body) if isinstance(s, ForAll)]
+    if len(inner) != 1 or not is_graph_call(inner[0].source, g, "neighbors", 1) or \
+            not same(inner[0].source.args[0], Name(v)):
+        raise DSLError(f"{where}: expected forall over {g}.neighbors({v})")
+    t = inner[0].var
+    red = [s for s in stmts(inner[0].body) if isinstance(s, ReduceAssign)]
+    ev = [s.name for s in stmts(inner[0].body) if isinstance(s, Decl) and s.type.base == "edge"
+          and is_graph_call(s.init, g, "get_edge", 2) and same(s.init.args[0], Name(v)) and same(s.init.args[1], Name(t))]
+    if len(red) != 1 or len(ev) != 1:
+        raise DSLError(f"{where}: expected edge e = get_edge({v}, {t}) and one reduction list")
+    r, e = red[0], ev[0]
+    ok = (len(r.targets) == 3 and all(isinstance(x, Prop) and same(x.obj, Name(t)) for x in r.targets)
+          and isinstance(r.exprs[0], MinMax) and r.exprs[0].kind == "Min" and len(r.exprs[0].args) == 2
+          and _is_true(r.exprs[1]) and same(r.exprs[2], Name(v)))
+    if ok:
+        dist, flag, parent = (x.name for x in r.targets)
+        a0, a1 = r.exprs[0].args
+        ok = (same(a0, P(t, dist)) and isinstance(a1, BinOp) and a1.op == "+" and same(a1.lhs, P(v, dist))
+              and isinstance(a1.rhs, Prop) and same(a1.rhs.obj, Name(e)))
+    if not ok:
+        raise DSLError(f"{where}: reduction must be <{t}.dist, {t}.modified_nxt, {t}.parent> = "
+                       f"<Min({t}.dist, {v}.dist + e.weight), True, {v}>")
+    return dist, parent, a1.rhs.name, flag
+
+
+def _recognize_decremental(f, g, dist, parent, weight, mod):
+    loops = [s for s in stmts(f.body) if isinstance(s, While)]
+    if len(loops) != 2:
+        raise DSLError("Decremental: expected two while loops (cut off the subtrees, then pull from in-neighbours)")
+    # loop 1: vertices whose parent is cut off are cut off
+    fa = [s for s in stmts(loops[0].body) if isinstance(s, ForAll)]
+    ok = len(fa) == 1 and is_graph_call(fa[0].source, g, "nodes", 0)
+    if ok:
+        flt = fa[0].filter
+        ok = isinstance(flt, BinOp) and flt.op == "==" and same(flt.lhs, Name(mod)) and _is_false(flt.rhs)
+    if ok:
+        v = fa[0].var
+        pv = {s.name for s in stmts(fa[0].body) if isinstance(s, Decl) and same(s.init, P(v, parent))}
+        ifs = [s for s in stmts(fa[0].body) if isinstance(s, If)]
+        ok = len(ifs) == 1 and len(pv) == 1
+    if ok:
+        (p,) = pv
+        c = ifs[0].cond
+        ok = (isinstance(c, BinOp) and c.op == "&&" and isinstance(c.lhs, BinOp) and c.lhs.op == ">"
+              and same(c.lhs.lhs, Name(p)) and _minus_one(c.lhs.rhs) and same(c.rhs, P(p, mod)))
+        a = _assigns(ifs[0].then)
+        ok = ok and _inf2(a.get(norm(P(v, dist)))) and _is_true(a.get(norm(P(v, mod)), Num(0, "0"))) \
+            and _minus_one(a.get(norm(P(v, parent))))
+    if not ok:
+        raise DSLError(f"Decremental: first loop must cut off v when v.{parent} > -1 and its parent is {mod} "
+                       f"(v.{dist} = INT_MAX/2, v.{mod} = True, v.{parent} = -1)")
+    # loop 2: cut-off vertices pull from in-neighbours
+    fa = [s for s in stmts(loops[1].body) if isinstance(s, ForAll)]
+    ok = len(fa) == 1 and is_graph_call(fa[0].source, g, "nodes", 0) and is_filter_true(fa[0].filter, mod)
+    if ok:
+        v = fa[0].var
+        inner = [s for s in stmts(fa[0].body) if isinstance(s, ForAll)]
+        ok = len(inner) == 1 and is_graph_call(inner[0].source, g, "nodes_to", 1) and \
+            same(inner[0].source.args[0], Name(v))
+    if ok:
+        u = inner[0].var
+        ev = [s.name for s in stmts(inner[0].body) if isinstance(s, Decl) and s.type.base == "edge"
+              and is_graph_call(s.init, g, "get_edge", 2) and same(s.init.args[0], Name(u))
+              and same(s.init.args[1], Name(v))]
+        ifs = [s for s in stmts(inner[0].body) if isinstance(s, If)]
+        ok = len(ev) == 1 and len(ifs) == 1
+    if ok:
+        cand = BinOp("+", P(u, dist), P(ev[0], weight))
+        c = ifs[0].cond
+        ok = isinstance(c, BinOp) and c.op == ">" and same(c.lhs, P(v, dist)) and same(c.rhs, cand)
+        a = _assigns(ifs[0].then)
+        ok = ok and same(a.get(norm(P(v, dist)), Num(0, "0")), cand) and same(a.get(norm(P(v, parent)), Num(0, "0")),
+                                                                           Name(u))
+    if not ok:
+        raise DSLError(f"Decremental: second loop must set v.{dist} = u.{dist} + e.{weight}, v.{parent} = u "
+                       f"for in-neighbours u with v.{dist} > u.{dist} + e.{weight}")
+
+
+def _on_block(b, kind):
+    """(source var, destination var, the one If) of an OnDelete / OnAdd block."""
+    if not (isinstance(b.source, Call) and b.source.method == "currentBatch"):
+        raise DSLError(f"On{kind.capitalize()}: expected (u in <updates>.currentBatch())")
+    decl = {s.name: s.init for s in stmts(b.body) if isinstance(s, Decl)}
+    src = [k for k, e in decl.items() if same(e, P(b.var, "source"))]
+    dst = [k for k, e in decl.items() if same(e, P(b.var, "destination"))]
+    ifs = [s for s in stmts(b.body) if isinstance(s, If)]
+    if len(src) != 1 or len(dst) != 1 or len(ifs) != 1:
+        raise DSLError(f"On{kind.capitalize()}: expected source / destination locals and one if")
+    return src[0], dst[0], ifs[0]
+
+
+def recognize_dynsssp(funcs):
+    dyn = [f for f in funcs if f.kind == "Dynamic"]
+    if len(dyn) != 1:
+        raise DSLError("a dynamic program needs exactly one Dynamic function")
+    D = dyn[0]
+    g = graph_param(D)
+    upd = params_of(D, "updates")
+    if g is None or len(upd) != 1:
+        raise DSLError(f"{D.name}: expected a Graph and an updates<g> parameter", D.line)
+    upd = upd[0]
+    body = stmts(D.body)
+    calls = [s.expr for s in body if isinstance(s, ExprStmt) and isinstance(s.expr, Call) and s.expr.obj is None]
+    batches = [s for s in body if isinstance(s, Batch)]
+    byname = {f.name: f for f in funcs}
+    if len(calls) != 1 or calls[0].method not in byname or byname[calls[0].method].kind != "Static" or len(batches) != 1:
+        raise DSLError(f"{D.name}: expected a call of the Static function, then one Batch block", D.line)
+    S = byname[calls[0].method]
+    gS = graph_param(S)
+    dist, parent, weight, _ = _relax_with_parent(S, gS, S.name)
+    if not isinstance(attached_value(S, gS, dist), Inf) or not _minus_one(attached_value(S, gS, parent)):
+        raise DSLError(f"{S.name}: {dist} must be attached as INF and {parent} as -1")
+    srcS = [p.name for p in S.params if p.type.base in ("int", "node")]
+    if len(srcS) != 1 or not any(isinstance(s, Assign) and same(s.target, P(srcS[0], dist)) and
+                                 same(s.expr, Num(0.0, "0")) for s in find_all(S.body, Assign)):
+        raise DSLError(f"{S.name}: missing <src>.{dist} = 0")
+    # the Static call's arguments name the Dynamic function's roles
+    pos = {p.name: i for i, p in enumerate(S.params)}
+    arg = lambda n: calls[0].args[pos[n]].id if isinstance(calls[0].args[pos[n]], Name) else None  # noqa: E731
+    roles = {"g": g, "dist": arg(dist), "parent": arg(parent), "weight": arg(weight), "src": arg(srcS[0]),
+             "updates": upd, "static": S.name}
+    if None in roles.values() or len(calls[0].args) != len(S.params):
+        raise DSLError(f"{D.name}: the Static call must pass the Dynamic function's parameters", D.line)
+    bt = batches[0]
+    if bt.updates != upd or not isinstance(bt.size, Name) or D.param(bt.size.id) is None:
+        raise DSLError(f"{D.name}: expected Batch({upd} : <batch size parameter>)")
+    roles["batchSize"] = bt.size.id
+    seq = []
+    for s in stmts(bt.body):
+        if isinstance(s, OnUpdate):
+            seq.append(("on" + s.kind, s))
+        elif isinstance(s, ExprStmt) and isinstance(s.expr, Call):
+            c = s.expr
+            if is_graph_call(c, g, "updateCSRDel", 1) or is_graph_call(c, g, "updateCSRAdd", 1):
+                if not same(c.args[0], Name(upd)):
+                    raise DSLError(f"{D.name}: {c.method} must take {upd}")
+                seq.append((c.method, c))
+            elif c.obj is None and c.method in ("Incremental", "Decremental"):
+                seq.append((c.method, c))
+    order = [k for k, _ in seq]
+    want = ["ondelete", "updateCSRDel", "Decremental", "onadd", "updateCSRAdd", "Incremental"]
+    if order != want:
+        raise DSLError(f"{D.name}: Batch must run {', '.join(want)} in this order (found {', '.join(order)})")
+    steps = dict(seq)
+    flags = [d.name for d in stmts(bt.body) if isinstance(d, Decl) and d.type.base == "propNode"
+             and str(d.type.inner) == "bool"]
+    # OnDelete: a deleted parent edge cuts the destination off
+    s_, d_, iff = _on_block(steps["ondelete"], "delete")
+    dec, inc = steps["Decremental"], steps["Incremental"]
+    if len(dec.args) != 5 or len(inc.args) != 5:
+        raise DSLError(f"{D.name}: Decremental / Incremental take (g, dist, parent, weight, flags)")
+    mod = dec.args[4].id if isinstance(dec.args[4], Name) else None
+    madd = inc.args[4].id if isinstance(inc.args[4], Name) else None
+    if mod not in flags or madd not in flags or mod == madd:
+        raise DSLError(f"{D.name}: Decremental and Incremental must take the batch's two bool node properties")
+    a = _assigns(iff.then)
+    if not (isinstance(iff.cond, BinOp) and iff.cond.op == "==" and same(iff.cond.lhs, P(d_, roles["parent"]))
+            and same(iff.cond.rhs, Name(s_)) and _inf2(a.get(norm(P(d_, roles["dist"]))))
+            and _is_true(a.get(norm(P(d_, mod)), Num(0, "0"))) and _minus_one(a.get(norm(P(d_, roles["parent"]))))):
+        raise DSLError("OnDelete: expected if (dest.parent == src) { dest.dist = INT_MAX/2; dest.modified = True; "
+                       "dest.parent = -1; }")
+    # OnAdd: an added edge (weight 1) that shortens the destination marks both ends
+    s_, d_, iff = _on_block(steps["onadd"], "add")
+    a = _assigns(iff.then)
+    if not (isinstance(iff.cond, BinOp) and iff.cond.op == ">" and same(iff.cond.lhs, P(d_, roles["dist"]))
+            and same(iff.cond.rhs, BinOp("+", P(s_, roles["dist"]), Num(1.0, "1")))
+            and _is_true(a.get(norm(P(d_, madd)), Num(0, "0"))) and _is_true(a.get(norm(P(s_, madd)), Num(0, "0")))):
+        raise DSLError("OnAdd: expected if (dest.dist > src.dist + 1) { dest.modified_add = True; "
+                       "src.modified_add = True; }")
+    for c, want_flag in ((dec, mod), (inc, madd)):
+        if not (same(c.args[0], Name(g)) and same(c.args[1], Name(roles["dist"])) and
+                same(c.args[2], Name(roles["parent"])) and same(c.args[3], Name(roles["weight"]))):
+            raise DSLError(f"{D.name}: {c.method} must take ({g}, {roles['dist']}, {roles['parent']}, "
+                           f"{roles['weight']}, {want_flag})")
+    # the Incremental and Decremental bodies
+    fi = [f for f in funcs if f.kind == "Incremental"]
+    fd = [f for f in funcs if f.kind == "Decremental"]
+    if len(fi) != 1 or len(fd) != 1:
+        raise DSLError("a dynamic SSSP program needs one Incremental and one Decremental function")
+    gi = graph_param(fi[0])
+    di, pi, wi, _ = _relax_with_parent(fi[0], gi, "Incremental")
+    if (di, pi, wi) != tuple(p.name for p in fi[0].params[1:4]):
+        raise DSLError("Incremental: the relaxation must use its dist, parent and weight parameters")
+    pd = [p.name for p in fd[0].params]
+    if len(pd) != 5:
+        raise DSLError("Decremental: expected (g, dist, parent, weight, modified)")
+    _recognize_decremental(fd[0], pd[0], pd[1], pd[2], pd[3], pd[4])
+    plan_static = Plan("dyn_static", S, {"g": gS, "dist": dist, "parent": parent, "weight": weight, "src": srcS[0]})
+    plan_dyn = Plan("dynsssp", D, roles)
+    plan_dyn.notes.append("added edges weigh 1 (update.hpp); distances >= INT_MAX/2 are infinite; parents are the "
+                          "smallest-id shortest predecessors")
+    return [plan_static, plan_dyn]
+
+
+def lower_program(funcs, dynamic=False):
+    """Plans for a static (-s) or dynamic (-d) program."""
+    if not dynamic:
+        for f in funcs:
+            if f.kind != "function":
+                raise DSLError(f"{f.kind} functions belong to dynamic programs (use -d)", f.line)
+        return [lower(f) for f in funcs]
+    return recognize_dynsssp(funcs)

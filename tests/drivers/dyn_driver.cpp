@@ -10,33 +10,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <fstream>
-#include <sstream>
 #include <string>
 #include <vector>
 
 #include "starplat_graph.hpp"
-
-// update.hpp:18-72: one "<type> <src> <dst>" per line, pushed twice (once
-// reversed), additions weighing 1.
-static std::vector<update> parse_updates(const char* path) {
-  std::vector<update> out;
-  std::ifstream in(path);
-  std::string line;
-  while (std::getline(in, line)) {
-    std::stringstream ss(line);
-    update u;
-    if (!(ss >> u.type >> u.source >> u.destination)) continue;
-    if (u.type != 'a' && u.type != 'd') continue;
-    u.weight = 1;
-    update r = u;
-    r.source = u.destination;
-    r.destination = u.source;
-    out.push_back(u);
-    out.push_back(r);
-  }
-  return out;
-}
 
 int main(int argc, char** argv) {
   if (argc < 6) {
@@ -45,7 +22,7 @@ int main(int argc, char** argv) {
   }
   graph g(argv[2]);
   g.parseGraph();
-  std::vector<update> updates = parse_updates(argv[3]);
+  std::vector<update> updates = parseUpdateFile(argv[3]);
   const int batch = std::atoi(argv[4]);
   FILE* out = std::fopen(argv[5], "wb");
   if (!out || batch < 1) return 3;

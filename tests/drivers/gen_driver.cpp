@@ -8,6 +8,9 @@
 //   gen_pr   <graph.txt> <out.bin> <beta> <delta> <maxIter>
 //   gen_tc   <graph.txt> <out.bin>
 //   gen_bc   <graph.txt> <out.bin> <src> [<src> ...]
+//   gen_dynsssp <graph.txt> <out.bin> <updates.txt> <batch> <src>
+//     (-DRUN_DYNSSSP -DDYN_FUNC=<the Dynamic function>): dist, parent, then
+//     the updated graph's edge count)
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -48,8 +51,21 @@ int main(int argc, char** argv) {
   std::vector<float> BC(V);
   Compute_BC(g, BC.data(), sourceSet);
   std::fwrite(BC.data(), sizeof(float), V, out);
+#elif defined(RUN_DYNSSSP)
+  if (argc < 6) return 2;
+  std::vector<int> dist(V), parent(V);
+  std::vector<int> weight(g.num_edges());
+  int k = 0;
+  for (int v = 0; v < V; v++)
+    for (int e = g.indexofNodes[v]; e < g.indexofNodes[v + 1]; e++) weight[k++] = g.getEdge(v, g.edgeList[e]).weight;
+  std::vector<update> updateBatch = parseUpdateFile(argv[3]);
+  DYN_FUNC(g, dist.data(), parent.data(), weight.data(), updateBatch, std::atoi(argv[4]), std::atoi(argv[5]));
+  std::fwrite(dist.data(), sizeof(int), V, out);
+  std::fwrite(parent.data(), sizeof(int), V, out);
+  const int E = g.num_edges();
+  std::fwrite(&E, sizeof E, 1, out);
 #else
-#error "define one of RUN_SSSP, RUN_PR, RUN_TC, RUN_BC"
+#error "define one of RUN_SSSP, RUN_PR, RUN_TC, RUN_BC, RUN_DYNSSSP"
 #endif
   std::fclose(out);
   return 0;

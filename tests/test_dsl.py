@@ -135,3 +135,96 @@ def test_generated_code_end_to_end(tmp_path, oracle):
     bref, _, _ = oracle.compute_bc(u, [0, 5, 17])
     got = np.fromfile(tmp_path / "b.bin", np.float32)
     assert np.max(np.abs(got - bref)) <= 1e-5 * np.max(bref)
+
+
+# ------------------------------------------------------------------ dynamic programs (-d)
+REF_DYN = "/root/reference/graphcode/dynamicDSLCodes"
+
+
+def test_dynamic_program_lowers():
+    from starplat_amd.dsl.lower import lower_program
+    plans = lower_program(parse(src("dyn_sssp.sp")), dynamic=True)
+    assert [p.kind for p in plans] == ["dyn_static", "dynsssp"]
+    r = plans[1].roles
+    assert (r["dist"], r["parent"], r["weight"], r["updates"], r["batchSize"], r["src"]) == \
+        ("d", "pred", "w", "stream", "perBatch", "root")
+    h, cc, _ = compile_source(src("dyn_sssp.sp"), "dyn_sssp", dynamic=True)
+    # the OMP backend's type mapping: updates<g> -> std::vector<update> (dsl_cpp_generator.cpp:2987-2988)
+    assert "void maintainDistances(graph& g, int* d, int* pred, int* w, std::vector<update> stream, int perBatch, " \
+           "int root);" in h
+    assert "spm_dyn_sssp(g.device(), root," in cc and "g.sync_from_device();" in cc
+    # a dynamic program is refused without -d, a static one with -d
+    with pytest.raises(DSLError):
+        lower_program(parse(src("dyn_sssp.sp")), dynamic=False)
+    with pytest.raises(DSLError):
+        lower_program(parse(src("sssp.sp")), dynamic=True)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DYN), reason="reference tree not present")
+def test_reference_dynamic_sssp_lowers():
+    from starplat_amd.dsl.lower import lower_program
+    h, _, plans = compile_source(open(os.path.join(REF_DYN, "dynamicBatchSSSP")).read(), "dynamicBatchSSSP",
+                                 dynamic=True)
+    assert [p.func.name for p in plans] == ["staticSSSP", "DynSSSP"]
+    assert "void DynSSSP(graph& g, int* dist, int* parent, int* weight, std::vector<update> updateBatch, " \
+           "int batchSize, int src);" in h
+
+
+DYN_MUTATIONS = [
+    ("if (b.d > a.d + 1)", "if (b.d > a.d + 2)"),                 # OnAdd weight
+    ("if (b.pred == a)", "if (b.pred == b)"),                       # OnDelete test
+    ("if (up > -1 && up.cut)", "if (up > -1)"),                     # cut-off needs the parent's flag
+    ("x.d = y.d + yx.w;", "x.d = y.d;"),                            # pull update
+    ("<y.d, y.touched_next, y.pred> = <Min(y.d, x.d + xy.w), True, x>;",
+     "<y.d, y.touched_next, y.pred> = <Max(y.d, x.d + xy.w), True, x>;"),
+    ("    g.updateCSRDel(stream);\n", ""),                          # batch step missing
+    ("    Decremental(g, d, pred, w, cut);\n    OnAdd", "    OnAdd"),
+    ("b.grown = True;\n        a.grown = True;", "b.grown = True;"),  # only one end marked
+    ("x.pred = -1;\n        stable", "stable"),                     # cut-off keeps its parent
+]
+
+
+@pytest.mark.parametrize("old,new", DYN_MUTATIONS)
+def test_mutated_dynamic_programs_are_rejected(old, new):
+    from starplat_amd.dsl.lower import lower_program
+    text = src("dyn_sssp.sp")
+    assert old in text
+    with pytest.raises(DSLError):
+        lower_program(parse(text.replace(old, new, 1)), dynamic=True)
+
+
+def test_cli_dynamic(tmp_path):
+    r = subprocess.run([sys.executable, "-m", "starplat_amd.dsl", "-d", "-f", os.path.join(PROG, "dyn_sssp.sp"),
+                        "-b", "mi355x", "-o", str(tmp_path)], cwd=ROOT, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "generated_mi355x" / "dyn_sssp.cc").exists()
+    r = subprocess.run([sys.executable, "-m", "starplat_amd.dsl", "-s", "-f", os.path.join(PROG, "dyn_sssp.sp"),
+                        "-b", "mi355x", "-o", str(tmp_path)], cwd=ROOT, capture_output=True, text=True)
+    assert r.returncode != 0
+
+
+@pytest.mark.gpu
+def test_generated_dynamic_code_end_to_end(tmp_path, oracle):
+    from starplat_amd.updates import as_arrays, parseUpdateFile, write_update_file
+    from test_oracle_dyn import batch_ops
+    assert os.access(os.path.join(BIN, "gen_dynsssp"), os.X_OK), "run build() first"
+    g = graph.rmat(11, 8, seed=14, weights=(1, 40), undirected=True)
+    p = tmp_path / "g.txt"
+    g.write_text(p)
+    g = graph(str(p)).parseGraph()
+    rng = np.random.default_rng(8)
+    src_of = np.repeat(np.arange(g.V), np.diff(g.indexofNodes))
+    e = rng.integers(0, g.E, 400)
+    types = np.where(rng.random(400) < 0.5, ord("d"), ord("a")).astype(np.uint8)
+    s = np.where(types == ord("d"), src_of[e], rng.integers(0, g.V, 400))
+    t = np.where(types == ord("d"), g.edgeList[e], rng.integers(0, g.V, 400))
+    up = tmp_path / "u.txt"
+    write_update_file(str(up), types.tolist(), s, t)
+    ty, us, ut, _ = as_arrays(parseUpdateFile(str(up)))
+    run_driver("dynsssp", p, tmp_path / "o.bin", up, 64, 5)
+    raw = np.fromfile(tmp_path / "o.bin", np.int32)
+    dref, pref = oracle.dyn_sssp(g, g.edgeLen, 5, ty, us, ut, 64)
+    np.testing.assert_array_equal(raw[:g.V], dref)
+    np.testing.assert_array_equal(raw[g.V:2 * g.V], pref)
+    off, _, _ = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, batch_ops(ty, us, ut, 64))
+    assert raw[2 * g.V] == off[-1]

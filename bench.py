@@ -281,6 +281,9 @@ def bench_dyn(args, O):
     t_fold = (time.perf_counter() - t0) * 1e3
     d_out = hip.DeviceArray(g.V, np.int32)
     t0 = time.perf_counter()
+    dev.sssp_device(0, d_out)  # rebuilds the layout the fold dropped (relabeled copy, queues)
+    t_static_first = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
     dev.sssp_device(0, d_out)
     t_static = (time.perf_counter() - t0) * 1e3
     ref = d_out.to_host()
@@ -293,12 +296,15 @@ def bench_dyn(args, O):
                        "additions of weight 1",
            "V": g.V, "E": g.E, "batch": args.dyn_batch, "batches_timed": len(times),
            "ms_per_batch": round(ms, 4), "updates_per_s": round(args.dyn_batch / (ms * 1e-3)),
-           "init_ms": round(t_init, 3), "fold_ms": round(t_fold, 3), "static_sssp_ms": round(t_static, 3),
+           "init_ms": round(t_init, 3), "fold_ms": round(t_fold, 3),
+           "static_sssp_first_ms": round(t_static_first, 3), "static_sssp_ms": round(t_static, 3),
            "speedup_vs_fold_plus_static": round((t_fold + t_static) / ms, 2),
            "dist_equals_static_recompute": ok,
            "kernel_ms_one_batch": {k: round(v[0], 4) for k, v in kt.items() if v[1] > 0},
            "roofline": None,
-           "note": "per-batch time includes the host's staging copies and one flag read-back per fixed-point round"}
+           "note": "per-batch time includes the host's staging copy and one flag read-back per 8 fixed-point "
+                   "iterations; the speedup compares with folding the updates plus a steady-state static SSSP "
+                   "(static_sssp_first_ms also rebuilds the relabeled layout the fold drops)"}
     if not args.no_cpu_baseline:
         tm = {}
         b = batches[1]

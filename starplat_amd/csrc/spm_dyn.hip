@@ -68,7 +68,7 @@ struct DiffRows {
 struct DiffStore {
   DiffRows r;
   size_t cap = 0;   // entries allocated
-  int64_t n = 0;    // entries in use (off[V])
+  int64_t n = 0;    // bound on the entries in use (off[V] <= n; not read back)
 };
 
 struct DynWork {
@@ -84,6 +84,9 @@ struct DynWork {
   int64_t removed = 0, rremoved = 0;  // CSR slots removed since the last fold
   int32_t* ubuf = nullptr;      // staged batch: src, dst, weight
   size_t ucap = 0;
+  int32_t* h_stage = nullptr;   // pinned host copy of the staged batch
+  hipEvent_t stage_done = nullptr;  // its copies to ubuf have completed
+  bool stage_pending = false;
   int32_t* cnt = nullptr;       // [V+1] per-vertex counts / lengths
   int32_t* cursor = nullptr;    // [V]
   int32_t* live = nullptr;      // [V]
@@ -336,9 +339,31 @@ __global__ void dsp_on_delete(int32_t n, const int32_t* __restrict__ us, const i
   }
 }
 
+// Fixed-point loops run as groups of launches with one host check per group.
+// Iteration k reads f[k % 3] (did iteration k-1 change anything?), records its
+// own changes in f[(k+1) % 3] and clears f[(k+2) % 3] for iteration k+1; an
+// iteration whose predecessor changed nothing returns at once, so launches
+// past convergence cost only their launch.
+struct IterFlags {
+  const unsigned long long* in;
+  unsigned long long* out;
+  unsigned long long* clear;
+};
+
+__device__ __forceinline__ bool iter_begin(const IterFlags& f) {
+  const bool go = __hip_atomic_load(f.in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *f.clear = 0ull;
+  return go;
+}
+
+__global__ void dyn_flags_init(unsigned long long* f) {
+  if (threadIdx.x < 3) f[threadIdx.x] = threadIdx.x == 0 ? 1ull : 0ull;
+}
+
 // Decremental, first loop: a vertex whose parent was cut off is cut off too.
 __global__ void dsp_invalidate(int32_t V, unsigned long long* __restrict__ dp, uint8_t* __restrict__ mod,
-                               unsigned long long* __restrict__ changed) {
+                               IterFlags fl) {
+  if (!iter_begin(fl)) return;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   bool any = false;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
@@ -350,7 +375,7 @@ __global__ void dsp_invalidate(int32_t V, unsigned long long* __restrict__ dp, u
       any = true;
     }
   }
-  if (__any(any) && __lane_id() == 0) atomicOr(changed, 1ull);
+  if (__any(any) && __lane_id() == 0) atomicOr(fl.out, 1ull);
 }
 
 // Decremental, second loop: each cut-off vertex takes the best finite
@@ -360,7 +385,8 @@ __global__ __launch_bounds__(kDynBlock) void dsp_pull(const int32_t* __restrict_
                                                       const int32_t* __restrict__ roff, const int32_t* __restrict__ radj,
                                                       const uint32_t* __restrict__ rdead, const int32_t* __restrict__ rw,
                                                       DiffRows in, unsigned long long* __restrict__ dp,
-                                                      unsigned long long* __restrict__ changed) {
+                                                      IterFlags fl) {
+  if (!iter_begin(fl)) return;
   const int lane = __lane_id();
   const int64_t n = *nsel;
   const int64_t nw = int64_t(gridDim.x) * kDynWaves;
@@ -391,7 +417,7 @@ __global__ __launch_bounds__(kDynBlock) void dsp_pull(const int32_t* __restrict_
       any = true;
     }
   }
-  if (__any(any) && lane == 0) atomicOr(changed, 1ull);
+  if (__any(any) && lane == 0) atomicOr(fl.out, 1ull);
 }
 
 // OnAdd: an added edge (s, t, w) that shortens t marks both ends.
@@ -415,7 +441,8 @@ __global__ __launch_bounds__(kDynBlock) void dsp_push(int32_t V, const int32_t* 
                                                       const int32_t* __restrict__ adj, const int32_t* __restrict__ w,
                                                       const uint32_t* __restrict__ dead, DiffRows out,
                                                       unsigned long long* __restrict__ dp, uint8_t* __restrict__ cur,
-                                                      uint8_t* __restrict__ nxt, unsigned long long* __restrict__ found_any) {
+                                                      uint8_t* __restrict__ nxt, IterFlags fl) {
+  if (!iter_begin(fl)) return;
   __shared__ uint32_t s_pre[kDynWaves][kWave];
   __shared__ int32_t s_a[kDynWaves][kWave];
   __shared__ int32_t s_da[kDynWaves][kWave];
@@ -478,9 +505,8 @@ __global__ __launch_bounds__(kDynBlock) void dsp_push(int32_t V, const int32_t* 
     }
     __builtin_amdgcn_wave_barrier();
   }
-  if (__any(found) && lane == 0 &&
-      __hip_atomic_load(found_any, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull)
-    atomicOr(found_any, 1ull);
+  if (__any(found) && lane == 0 && __hip_atomic_load(fl.out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull)
+    atomicOr(fl.out, 1ull);
 }
 
 __global__ void dsp_unpack(int32_t V, const unsigned long long* __restrict__ dp, int32_t* __restrict__ dist,
@@ -609,6 +635,9 @@ int dyn_release(spm_graph* g) {
     diff_free(g, w->in[i]);
   }
   dfree(g, w->ubuf, sizeof(int32_t) * 3 * w->ucap);
+  if (w->stage_pending) (void)hipEventSynchronize(w->stage_done);
+  if (w->h_stage) (void)hipHostFree(w->h_stage);
+  if (w->stage_done) (void)hipEventDestroy(w->stage_done);
   dfree(g, w->cnt, sizeof(int32_t) * (V + 1));
   dfree(g, w->cursor, sizeof(int32_t) * V);
   dfree(g, w->live, sizeof(int32_t) * V);
@@ -633,20 +662,33 @@ static int stage(spm_graph* g, const int32_t* s, const int32_t* t, const int32_t
   for (int32_t i = 0; i < n; ++i)
     if (s[i] < 0 || s[i] >= g->V || t[i] < 0 || t[i] >= g->V)
       return set_error(SPM_EINVAL, "edge update names a vertex outside 0..num_nodes()-1");
+  // the pinned buffer may still feed the previous batch's copies
+  if (w->stage_pending) {
+    SPM_CHECK_HIP(hipEventSynchronize(w->stage_done));
+    w->stage_pending = false;
+  }
+  if (!w->stage_done) SPM_CHECK_HIP(hipEventCreateWithFlags(&w->stage_done, hipEventDisableTiming));
   if (size_t(n) > w->ucap) {
     dfree(g, w->ubuf, sizeof(int32_t) * 3 * w->ucap);
+    if (w->h_stage) (void)hipHostFree(w->h_stage);
     w->ubuf = nullptr;
+    w->h_stage = nullptr;
     w->ucap = 0;
     const size_t cap = std::max<size_t>(size_t(n), 4096);
     SPM_TRY(dmalloc(g, (void**)&w->ubuf, sizeof(int32_t) * 3 * cap, "update batch"));
+    SPM_CHECK_HIP(hipHostMalloc((void**)&w->h_stage, sizeof(int32_t) * 3 * cap, hipHostMallocDefault));
     w->ucap = cap;
   }
+  std::memcpy(w->h_stage, s, sizeof(int32_t) * n);
+  std::memcpy(w->h_stage + w->ucap, t, sizeof(int32_t) * n);
+  if (wt) std::memcpy(w->h_stage + 2 * w->ucap, wt, sizeof(int32_t) * n);
   *ds = w->ubuf;
   *dt = w->ubuf + w->ucap;
   *dw = wt ? w->ubuf + 2 * w->ucap : nullptr;
-  SPM_CHECK_HIP(hipMemcpyAsync(*ds, s, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
-  SPM_CHECK_HIP(hipMemcpyAsync(*dt, t, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
-  if (wt) SPM_CHECK_HIP(hipMemcpyAsync(*dw, wt, sizeof(int32_t) * n, hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(w->ubuf, w->h_stage, sizeof(int32_t) * (wt ? 3 : 2) * w->ucap, hipMemcpyHostToDevice,
+                               g->stream));
+  SPM_CHECK_HIP(hipEventRecord(w->stage_done, g->stream));
+  w->stage_pending = true;
   return SPM_OK;
 }
 
@@ -660,11 +702,10 @@ static int delete_staged(spm_graph* g, const int32_t* ds, const int32_t* dt, int
   if (g->roff)
     SPM_TRY(launch(g, "dyn_delete", dyn_delete, dim3(grid), dim3(kDynBlock), 0, n, dt, ds, (const int32_t*)g->roff,
                    (const int32_t*)g->radj, w->rdead, w->rdead_row, w->in[w->cur].r, w->ctr + 2));
-  unsigned long long* h = reinterpret_cast<unsigned long long*>(g->h_pinned);
-  SPM_CHECK_HIP(hipMemcpyAsync(h, w->ctr, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, g->stream));
-  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
-  w->removed += int64_t(h[0]);
-  w->rremoved += int64_t(h[2]);
+  // counted as pending without reading the device counters back: an upper
+  // bound is enough to know a fold is due
+  w->removed += n;
+  if (g->roff) w->rremoved += n;
   return SPM_OK;
 }
 
@@ -682,15 +723,15 @@ static int add_one_side(spm_graph* g, DiffStore* st, const int32_t* key, const i
   SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, w->cnt, dst.r.off, V + 1, g->stream));
   SPM_TRY(tmp_reserve(g, need));
   SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(w->tmp, need, w->cnt, dst.r.off, V + 1, g->stream));
-  int32_t total = 0;
-  SPM_CHECK_HIP(hipMemcpyAsync(&total, dst.r.off + V, sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
-  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
-  SPM_TRY(diff_reserve(g, dst, size_t(total)));
+  // live old entries + adds <= old entries + adds: sized without reading the
+  // scan's total back
+  const int64_t bound = src.n + n;
+  SPM_TRY(diff_reserve(g, dst, size_t(bound)));
   SPM_CHECK_HIP(hipMemsetAsync(dst.r.dead, 0, sizeof(uint32_t) * (dst.cap / 32 + 1), g->stream));
   SPM_TRY(launch(g, "dyn_diffcopy", dyn_diffcopy, dim3(grid_for(g, V)), dim3(kDynBlock), 0, V, src.r, dst.r, w->cursor));
   SPM_TRY(launch(g, "dyn_place", dyn_place, dim3((n + kDynBlock - 1) / kDynBlock), dim3(kDynBlock), 0, n, key, val, wt,
                  dst.r, w->cursor));
-  dst.n = total;
+  dst.n = bound;
   return SPM_OK;
 }
 
@@ -702,7 +743,6 @@ static int add_staged(spm_graph* g, const int32_t* ds, const int32_t* dt, const 
   SPM_TRY(add_one_side(g, w->out, ds, dt, dw, n));
   if (g->roff) SPM_TRY(add_one_side(g, w->in, dt, ds, dw, n));
   w->cur = 1 - w->cur;
-  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   return SPM_OK;
 }
 
@@ -901,6 +941,23 @@ static int dsp_init(spm_graph* g, int32_t src) {
   return prof_flush(g);
 }
 
+// Runs iteration(k, flags) in groups of kIterGroup launches, reading one flag
+// back per group, until an iteration changes nothing.
+constexpr int kIterGroup = 8;
+template <typename Iter>
+static int fixed_point(spm_graph* g, const char* what, Iter iteration) {
+  unsigned long long* f = g->dyn->ctr + 4;  // three rotating flags
+  SPM_TRY(launch(g, "dyn_flags_init", dyn_flags_init, dim3(1), dim3(64), 0, f));
+  for (int64_t base = 0;; base += kIterGroup) {
+    for (int64_t k = base; k < base + kIterGroup; ++k)
+      SPM_TRY(iteration(k, IterFlags{f + k % 3, f + (k + 1) % 3, f + (k + 2) % 3}));
+    bool changed = false;
+    SPM_TRY(dsp_read_flag(g, f + (base + kIterGroup) % 3, &changed));
+    if (!changed) return SPM_OK;
+    if (base > int64_t(g->V)) return set_error(SPM_EDIVERGE, std::string("dynamic SSSP: ") + what + " did not converge");
+  }
+}
+
 // One batch of the Dynamic function: OnDelete, updateCSRDel, Decremental,
 // OnAdd, updateCSRAdd, Incremental (added edges weigh 1, update.hpp:56-60).
 static int dsp_batch(spm_graph* g, const char* type, const int32_t* us, const int32_t* ut, int32_t n) {
@@ -924,29 +981,19 @@ static int dsp_batch(spm_graph* g, const char* type, const int32_t* us, const in
     SPM_TRY(launch(g, "dsp_on_delete", dsp_on_delete, dim3((nd + kDynBlock - 1) / kDynBlock), dim3(kDynBlock), 0, nd,
                    (const int32_t*)s, (const int32_t*)t, w->dp, w->mod));
     SPM_TRY(delete_staged(g, s, t, nd));
-    for (int64_t it = 0;; ++it) {
-      SPM_CHECK_HIP(hipMemsetAsync(w->ctr + 4, 0, sizeof(unsigned long long), g->stream));
-      SPM_TRY(launch(g, "dsp_invalidate", dsp_invalidate, dim3(vgrid), dim3(kDynBlock), 0, V, w->dp, w->mod, w->ctr + 4));
-      bool changed = false;
-      SPM_TRY(dsp_read_flag(g, w->ctr + 4, &changed));
-      if (!changed) break;
-      if (it > V) return set_error(SPM_EDIVERGE, "dynamic SSSP: parent pointers form a cycle");
-    }
+    SPM_TRY(fixed_point(g, "the cut-off loop", [&](int64_t, IterFlags fl) {
+      return launch(g, "dsp_invalidate", dsp_invalidate, dim3(vgrid), dim3(kDynBlock), 0, V, w->dp, w->mod, fl);
+    }));
     size_t need = 0;
     hipcub::CountingInputIterator<int32_t> ids(0);
     SPM_CHECK_HIP(hipcub::DeviceSelect::Flagged(nullptr, need, ids, w->mod, w->list, w->nsel, V, g->stream));
     SPM_TRY(tmp_reserve(g, need));
     SPM_CHECK_HIP(hipcub::DeviceSelect::Flagged(w->tmp, need, ids, w->mod, w->list, w->nsel, V, g->stream));
-    for (int64_t it = 0;; ++it) {
-      SPM_CHECK_HIP(hipMemsetAsync(w->ctr + 4, 0, sizeof(unsigned long long), g->stream));
-      SPM_TRY(launch(g, "dsp_pull", dsp_pull, dim3(wgrid), dim3(kDynBlock), 0, (const int32_t*)w->list,
-                     (const int32_t*)w->nsel, (const int32_t*)g->roff, (const int32_t*)g->radj,
-                     (const uint32_t*)w->rdead, (const int32_t*)w->rw, w->in[w->cur].r, w->dp, w->ctr + 4));
-      bool changed = false;
-      SPM_TRY(dsp_read_flag(g, w->ctr + 4, &changed));
-      if (!changed) break;
-      if (it > V) return set_error(SPM_EDIVERGE, "dynamic SSSP: Decremental did not converge");
-    }
+    SPM_TRY(fixed_point(g, "Decremental", [&](int64_t, IterFlags fl) {
+      return launch(g, "dsp_pull", dsp_pull, dim3(wgrid), dim3(kDynBlock), 0, (const int32_t*)w->list,
+                    (const int32_t*)w->nsel, (const int32_t*)g->roff, (const int32_t*)g->radj,
+                    (const uint32_t*)w->rdead, (const int32_t*)w->rw, w->in[w->cur].r, w->dp, fl);
+    }));
   }
   // OnAdd + updateCSRAdd + Incremental
   if (!as.empty()) {
@@ -959,18 +1006,11 @@ static int dsp_batch(spm_graph* g, const char* type, const int32_t* us, const in
                    (const int32_t*)s, (const int32_t*)t, (const int32_t*)nullptr, (const unsigned long long*)w->dp,
                    w->fl[0]));
     SPM_TRY(add_staged(g, s, t, nullptr, na));
-    int c = 0;
-    for (int64_t it = 0;; ++it) {
-      SPM_CHECK_HIP(hipMemsetAsync(w->ctr + 5, 0, sizeof(unsigned long long), g->stream));
-      SPM_TRY(launch(g, "dsp_push", dsp_push, dim3(wgrid), dim3(kDynBlock), 0, V, (const int32_t*)g->off,
-                     (const int32_t*)g->adj, (const int32_t*)g->w, (const uint32_t*)w->dead, w->out[w->cur].r, w->dp,
-                     w->fl[c], w->fl[1 - c], w->ctr + 5));
-      bool found = false;
-      SPM_TRY(dsp_read_flag(g, w->ctr + 5, &found));
-      if (!found) break;
-      c = 1 - c;
-      if (it > V) return set_error(SPM_EDIVERGE, "dynamic SSSP: Incremental did not converge");
-    }
+    SPM_TRY(fixed_point(g, "Incremental", [&](int64_t k, IterFlags fl) {
+      return launch(g, "dsp_push", dsp_push, dim3(wgrid), dim3(kDynBlock), 0, V, (const int32_t*)g->off,
+                    (const int32_t*)g->adj, (const int32_t*)g->w, (const uint32_t*)w->dead, w->out[w->cur].r, w->dp,
+                    w->fl[k & 1], w->fl[(k + 1) & 1], fl);
+    }));
   }
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   return prof_flush(g);

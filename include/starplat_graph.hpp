@@ -46,6 +46,7 @@ class graph {
   int32_t* rev_indexofNodes = nullptr;
   int32_t* edgeList = nullptr;
   int32_t* srcList = nullptr;
+  int32_t* edgeMap = nullptr;  // reverse-CSR slot -> CSR position (graph.hpp:57)
 
   explicit graph(const char* file) : filePath_(file ? file : "") {}
   graph(const graph&) = delete;
@@ -70,7 +71,11 @@ class graph {
     rev_indexofNodes = h->rev_indexofNodes;
     edgeList = h->edgeList;
     srcList = h->srcList;
+    build_edge_map(h->V, h->E);
   }
+
+  int getOutDegree(int v) { return indexofNodes[v + 1] - indexofNodes[v]; }  // graph.hpp:316-320
+  int getInDegree(int v) { return rev_indexofNodes[v + 1] - rev_indexofNodes[v]; }  // :322-326
 
   int num_nodes() { return h_ ? h_->V : 1; }
   int num_edges() { return updated_ ? int(adj_.size()) : h_ ? int(h_->E) : 0; }
@@ -152,13 +157,20 @@ class graph {
     rev_indexofNodes = roff_.data();
     srcList = radj_.data();
     updated_ = true;
+    build_edge_map(h_->V, E);
+  }
+  void build_edge_map(int32_t V, int64_t E) {
+    emap_.assign(size_t(E > 0 ? E : 1), 0);
+    if (spmh_edge_map(V, indexofNodes, edgeList, rev_indexofNodes, srcList, emap_.data()) != 0)
+      throw std::runtime_error("graph: the CSR and the reverse CSR disagree");
+    edgeMap = emap_.data();
   }
 
   std::string filePath_;
   spmh_graph* h_ = nullptr;
   spm_graph* dev_ = nullptr;
   bool updated_ = false;
-  std::vector<int32_t> off_, adj_, w_, roff_, radj_;
+  std::vector<int32_t> off_, adj_, w_, roff_, radj_, emap_;
 };
 
 // update.hpp:18-72: one "<type> <src> <dst>" per line; every line is pushed

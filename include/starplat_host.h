@@ -85,6 +85,13 @@ void spmh_free(spmh_graph* g);
  * ascending (std::set<int> iteration order, as Compute_BC's sourceSet). */
 int spmh_sources(int32_t V, int32_t n, uint64_t seed, int32_t* out);
 
+/* graph.hpp's edgeMap (:57, built at :835-846): for every reverse-CSR slot p,
+ * the CSR position of the same edge, written to edgeMap[E]. Rows must be
+ * ascending; the k-th copy of an edge in the reverse row maps to its k-th
+ * copy in the CSR row. Returns 0, or -1 if the two CSRs disagree. */
+int spmh_edge_map(int32_t V, const int32_t* indexofNodes, const int32_t* edgeList,
+                  const int32_t* rev_indexofNodes, const int32_t* srcList, int32_t* edgeMap);
+
 /* The 16 graph features of the schedule selector, written to out[16]:
  *  0 log2 V            1 log2(E+1)          2 mean out-degree
  *  3 log2(1+var deg)   4 log2(1+max outdeg) 5 log2(1+max indeg)

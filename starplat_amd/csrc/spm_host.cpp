@@ -410,6 +410,27 @@ int spmh_sources(int32_t V, int32_t n, uint64_t seed, int32_t* out) {
   return 0;
 }
 
+int spmh_edge_map(int32_t V, const int32_t* off, const int32_t* adj, const int32_t* roff, const int32_t* radj,
+                  int32_t* emap) {
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(| : bad)
+  for (int32_t u = 0; u < V; ++u) {
+    for (int32_t j = off[u]; j < off[u + 1]; ++j) {
+      const int32_t v = adj[j];
+      int32_t k = 0;  // copies of (u, v) before position j
+      while (j - 1 - k >= off[u] && adj[j - 1 - k] == v) ++k;
+      const int32_t* lo = std::lower_bound(radj + roff[v], radj + roff[v + 1], u);
+      const int64_t p = (lo - radj) + k;
+      if (p >= roff[v + 1] || radj[p] != u) {
+        bad = 1;
+        continue;
+      }
+      emap[p] = j;
+    }
+  }
+  return bad ? -1 : 0;
+}
+
 int spmh_features(const spmh_graph* g, float job, float* out) {
   const int32_t V = g->V;
   const double E = double(g->E);

@@ -133,6 +133,28 @@ class graph:
     def getEdgeLen(self):
         return self.edgeLen
 
+    @property
+    def edgeMap(self):
+        """graph.hpp's edgeMap (:57, :835-846): CSR position of the edge at
+        every reverse-CSR slot."""
+        if getattr(self, "_edge_map", None) is None or self._edge_map.size != self.E:
+            m = np.empty(self.E, np.int32)
+            rc = N.host_lib().spmh_edge_map(self.V, N.ptr(self.indexofNodes, C.c_int32), N.ptr(self.edgeList, C.c_int32),
+                                            N.ptr(self.rev_indexofNodes, C.c_int32), N.ptr(self.srcList, C.c_int32),
+                                            N.ptr(m, C.c_int32))
+            if rc != 0:
+                raise RuntimeError("edgeMap: the CSR and the reverse CSR disagree")
+            self._edge_map = m
+        return self._edge_map
+
+    def getOutDegree(self, v):
+        """graph.hpp:316-320."""
+        return int(self.indexofNodes[v + 1] - self.indexofNodes[v])
+
+    def getInDegree(self, v):
+        """graph.hpp:322-326."""
+        return int(self.rev_indexofNodes[v + 1] - self.rev_indexofNodes[v])
+
     def count_outNbrs(self, v):
         return int(self.indexofNodes[v + 1] - self.indexofNodes[v])
 
@@ -205,6 +227,7 @@ class graph:
         self.edgeLen = w if w is not None else np.zeros(self.E, np.int32)
         if roff is not None:
             self.rev_indexofNodes, self.srcList = roff, radj
+        self._edge_map = None
         self._h = None
 
 

@@ -170,3 +170,26 @@ def test_binary_csr_round_trip_and_corruption(tmp_path):
     for bad in ("short", "flipped", "missing"):
         with pytest.raises(OSError):
             graph.load(tmp_path / bad)
+
+
+def test_edge_map_and_degrees():
+    """edgeMap (graph.hpp:57, built at :835-846): for each reverse-CSR slot
+    the CSR position of the same edge -- checked against the reference's
+    rule restated by brute force (match the source in the row's edge list),
+    and as a permutation that maps every slot to its own edge."""
+    rng = np.random.default_rng(3)
+    s = rng.integers(0, 40, 300).astype(np.int32)
+    d = rng.integers(0, 40, 300).astype(np.int32)
+    g = graph.from_edges(s, d)
+    pos = {}
+    for u in range(g.V):
+        for j in range(g.indexofNodes[u], g.indexofNodes[u + 1]):
+            pos[(u, int(g.edgeList[j]))] = j
+    m = g.edgeMap
+    for v in range(g.V):
+        for p in range(g.rev_indexofNodes[v], g.rev_indexofNodes[v + 1]):
+            assert m[p] == pos[(int(g.srcList[p]), v)]
+    assert np.array_equal(np.sort(m), np.arange(g.E))
+    for v in range(g.V):
+        assert g.getOutDegree(v) == g.count_outNbrs(v) == len(g.neighbors(v))
+        assert g.getInDegree(v) == len(g.nodes_to(v))

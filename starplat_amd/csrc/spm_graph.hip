@@ -206,7 +206,8 @@ int spm_graph_set_weights(spm_graph* g, const int32_t* weight) {
   if (!g->w) SPM_TRY(dmalloc(g, (void**)&g->w, eb, "edgeLen"));
   if (g->E) SPM_CHECK_HIP(hipMemcpyAsync(g->w, weight, eb, hipMemcpyHostToDevice, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
-  return relabel_weights(g);
+  SPM_TRY(relabel_weights(g));
+  return sssp_weights_changed(g);
 }
 
 void* spm_graph_stream(spm_graph* g) { return g ? (void*)g->stream : nullptr; }

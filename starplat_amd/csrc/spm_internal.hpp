@@ -139,6 +139,8 @@ int csr_from_pairs(spm_graph* g, int64_t n, unsigned long long* k0, unsigned lon
                    int32_t* off, int32_t* adj, int32_t* w, int32_t* roff, int32_t* radj, int32_t* rw);
 // Distances of the last SSSP run on g (device memory, V entries).
 int32_t* sssp_dist(spm_graph* g);
+// The weights SSSP runs on changed: refreshes SSSP's derived edge layout.
+int sssp_weights_changed(spm_graph* g);
 
 // ---- wavefront primitives ---------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

@@ -69,6 +69,9 @@ struct SsspWork {
   uint2* tile_sum = nullptr;          // [tiles] (vertices, edges) per compaction tile
   int32_t* srcbuf = nullptr;          // multi-source seeds
   int32_t* dist_out = nullptr;        // [V] distances in original ids (relabeled runs)
+  uint32_t* pk = nullptr;             // [E] (neighbour << wb | weight) when both fit 32 bits
+  int wb = 0;                         // weight bits of pk (0: not packed)
+  unsigned long long* wstat = nullptr;  // [2] weight range of the view (min, max)
   bool dist_out_ok = false;
   CsrView view{};                     // the CSR the kernels run on
   size_t qcap = 0, ccap = 0, src_cap = 0;
@@ -90,6 +93,31 @@ __global__ void sssp_init(int32_t V, int32_t* __restrict__ dist, uint8_t* __rest
 }
 
 __global__ void sssp_advance(uint32_t* round_base, uint32_t by) { *round_base += by; }
+
+// Weight range: st[0] = 1 if any weight is negative, st[1] = max weight.
+__global__ void sssp_wrange(int64_t E, const int32_t* __restrict__ w, unsigned long long* __restrict__ st) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  int32_t mx = 0;
+  bool neg = false;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += stride) {
+    mx = max(mx, w[i]);
+    neg |= w[i] < 0;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, kWave));
+  const bool any_neg = __any(neg);
+  if (__lane_id() == 0) {
+    atomicMax(&st[1], static_cast<unsigned long long>(mx));
+    if (any_neg) atomicOr(&st[0], 1ull);
+  }
+}
+
+__global__ void sssp_pack(int64_t E, const int32_t* __restrict__ adj, const int32_t* __restrict__ w, int wb,
+                          uint32_t* __restrict__ pk) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += stride)
+    pk[i] = (uint32_t(adj[i]) << wb) | uint32_t(w[i]);
+}
 
 // Block-wide exclusive scan of two counters (4 wavefronts of 64 lanes).
 __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
@@ -156,9 +184,13 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restric
   }
 }
 
+// PACKED: the frontier's edges are read as one 32-bit word (neighbour << wb |
+// weight) instead of two arrays -- half the streamed bytes per edge.
+template <bool PACKED>
 __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
     const int32_t* __restrict__ off, const int32_t* __restrict__ adj,
-    const int32_t* __restrict__ wgt, int32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
+    const int32_t* __restrict__ wgt, const uint32_t* __restrict__ pk, int wb,
+    int32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
     SsspQueue qin, SsspQueue qout, unsigned long long* __restrict__ ctr_in,
     unsigned long long* __restrict__ ctr_out, unsigned long long* __restrict__ ctr_clear,
     unsigned long long* __restrict__ rounds, const uint32_t* __restrict__ round_base, uint32_t rloc) {
@@ -219,8 +251,16 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
           else hi = mid - 1;
         }
         const int32_t e = s_base[lo] + int32_t(x - s_off[lo]);
-        const int32_t nb = adj[e];
-        const int32_t nd = s_dv[lo] + wgt[e];
+        int32_t nb, wt;
+        if (PACKED) {
+          const uint32_t word = pk[e];
+          nb = int32_t(word >> wb);
+          wt = int32_t(word & ((1u << wb) - 1u));
+        } else {
+          nb = adj[e];
+          wt = wgt[e];
+        }
+        const int32_t nd = s_dv[lo] + wt;
         // A cached (possibly stale) read is enough to skip hopeless updates:
         // distances only decrease, so a stale value is never below the true
         // one and atomicMin below stays the authority.
@@ -430,6 +470,42 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_dense(
     atomicOr(ctr_out, 1ull);
 }
 
+// Packs the view's edges as (neighbour << wb | weight) when the weights are
+// non-negative and wb + the bits of a vertex id fit 32 (RMAT-22 with weights
+// up to 100: 22 + 7); otherwise the relaxation reads the two arrays.
+static int sssp_pack_edges(spm_graph* g) {
+  SsspWork* w = g->sssp;
+  const int64_t E = g->E;
+  w->wb = 0;
+  if (!w->view.w || E == 0) return SPM_OK;
+  SPM_CHECK_HIP(hipMemsetAsync(w->wstat, 0, sizeof(unsigned long long) * 2, g->stream));
+  const int grid = int(std::min<int64_t>((E + 255) / 256, int64_t(g->num_cus) * 16));
+  SPM_TRY(launch(g, "sssp_wrange", sssp_wrange, dim3(grid), dim3(256), 0, E, w->view.w, w->wstat));
+  unsigned long long* h = reinterpret_cast<unsigned long long*>(g->h_pinned);
+  SPM_CHECK_HIP(hipMemcpyAsync(h, w->wstat, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  if (h[0]) return SPM_OK;  // negative weights: unpacked
+  int wb = 1, vb = 1;
+  while ((1ull << wb) <= h[1]) ++wb;
+  while ((int64_t(1) << vb) < int64_t(g->V)) ++vb;
+  if (wb + vb > 32) return SPM_OK;
+  if (!w->pk) SPM_TRY(dmalloc(g, (void**)&w->pk, sizeof(uint32_t) * size_t(E), "sssp packed edges"));
+  SPM_TRY(launch(g, "sssp_pack", sssp_pack, dim3(grid), dim3(256), 0, E, w->view.adj, w->view.w, wb, w->pk));
+  w->wb = wb;
+  return SPM_OK;
+}
+
+// The view's weights changed (set_weights): repack, or drop the workspace
+// (and its captured rounds) when the packing changes.
+int sssp_weights_changed(spm_graph* g) {
+  SsspWork* w = g->sssp;
+  if (!w) return SPM_OK;
+  const int old = w->wb;
+  SPM_TRY(sssp_pack_edges(g));
+  if (w->wb != old) return sssp_release(g);
+  return SPM_OK;
+}
+
 static int sssp_alloc(spm_graph* g, const CsrView& view) {
   if (g->sssp) return SPM_OK;
   SsspWork* w = new SsspWork();
@@ -450,6 +526,8 @@ static int sssp_alloc(spm_graph* g, const CsrView& view) {
   SPM_TRY(dmalloc(g, (void**)&w->dist, sizeof(int32_t) * V, "sssp dist"));
   SPM_TRY(dmalloc(g, (void**)&w->active, 2 * size_t(V), "sssp flags"));
   SPM_TRY(dmalloc(g, (void**)&w->tile_sum, sizeof(uint2) * ((size_t(V) + kTileV - 1) / kTileV), "sssp tiles"));
+  SPM_TRY(dmalloc(g, (void**)&w->wstat, sizeof(unsigned long long) * 2, "sssp weight range"));
+  SPM_TRY(sssp_pack_edges(g));
   SPM_CHECK_HIP(hipMemsetAsync(w->stamp, 0, sizeof(uint32_t) * V, g->stream));
   SPM_CHECK_HIP(hipMemcpyAsync(w->round_base, &w->host_base, sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
@@ -474,6 +552,8 @@ int sssp_release(spm_graph* g) {
   dfree(g, w->tile_sum, sizeof(uint2) * ((size_t(g->V) + kTileV - 1) / kTileV));
   dfree(g, w->srcbuf, sizeof(int32_t) * w->src_cap);
   dfree(g, w->dist_out, sizeof(int32_t) * g->V);
+  dfree(g, w->pk, sizeof(uint32_t) * size_t(g->E));
+  dfree(g, w->wstat, sizeof(unsigned long long) * 2);
   delete w;
   g->sssp = nullptr;
   return SPM_OK;
@@ -498,8 +578,8 @@ static int sssp_enqueue_rounds(spm_graph* g, int R) {
                      w->view.off, w->view.adj, w->view.w, w->dist,
                      cur, nxt, w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3));
     } else {
-      SPM_TRY(launch(g, "sssp_relax", sssp_relax, dim3(grid), dim3(kSsspBlock), 0,
-                     w->view.off, w->view.adj, w->view.w, w->dist,
+      SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(grid), dim3(kSsspBlock), 0,
+                     w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
                      w->stamp, w->q[i], w->q[o], w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3,
                      (const uint32_t*)w->round_base, uint32_t(r)));
       const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);

@@ -205,6 +205,22 @@ def test_relabel_sssp_bit_identical(oracle):
     np.testing.assert_array_equal(dist, oracle.compute_sssp(g, w2, 0))
 
 
+def test_sssp_packed_and_unpacked_edges(oracle):
+    """Edges are read as one (neighbour << wb | weight) word when both fit 32
+    bits; new weights that do not fit switch back to the two arrays (and
+    weights that fit again repack), with unchanged results."""
+    g = graph.rmat(12, 16, seed=9, weights=(1, 100))
+    dev = g.device()
+    for w in (g.edgeLen,                                           # 7 weight bits: packed
+              (g.edgeLen.astype(np.int64) * 1000003 % (1 << 28) + 1).astype(np.int32),  # 29 bits: unpacked
+              (g.edgeLen * 2).astype(np.int32)):                   # packed again
+        for relabel in (0, 1):
+            dev.set_schedule(relabel=relabel)
+            dist = np.empty(g.V, np.int32)
+            Compute_SSSP(g, dist, w, 3)
+            np.testing.assert_array_equal(dist, oracle.compute_sssp(g, w, 3))
+
+
 def test_relabel_pr_and_updates(oracle):
     g = graph.rmat(14, 16, seed=6)
     dev = g.device()

@@ -202,7 +202,8 @@ __global__ __launch_bounds__(kDynBlock) void dyn_emit(int32_t V, const int32_t* 
                                                       unsigned long long* __restrict__ keys, int32_t* __restrict__ vals) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kDynWaves;
-  for (int64_t v = int64_t(blockIdx.x) * kDynWaves + threadIdx.x / kWave; v < V; v += nw) {
+  for (int64_t x_ = int64_t(blockIdx.x) * kDynWaves + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t v = spread_vertex(x_, V);
     int64_t pos = noff[v];
     const unsigned long long hi = static_cast<unsigned long long>(v) << 32;
     for (int part = 0; part < 2; ++part) {
@@ -273,7 +274,8 @@ __global__ __launch_bounds__(kDynBlock) void dyn_build_rw(int32_t V, const int32
                                                           const int32_t* __restrict__ radj, int32_t* __restrict__ rw) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kDynWaves;
-  for (int64_t u = int64_t(blockIdx.x) * kDynWaves + threadIdx.x / kWave; u < V; u += nw) {
+  for (int64_t x_ = int64_t(blockIdx.x) * kDynWaves + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t u = spread_vertex(x_, V);
     const int32_t a = off[u], b = off[u + 1];
     for (int32_t j = a + lane; j < b; j += kWave) {
       const int32_t v = adj[j];
@@ -302,7 +304,8 @@ __global__ __launch_bounds__(kDynBlock) void dsp_parent(int32_t V, int32_t src, 
                                                         unsigned long long* __restrict__ dp) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kDynWaves;
-  for (int64_t v = int64_t(blockIdx.x) * kDynWaves + threadIdx.x / kWave; v < V; v += nw) {
+  for (int64_t x_ = int64_t(blockIdx.x) * kDynWaves + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t v = spread_vertex(x_, V);
     const int32_t dv = dist[v];
     unsigned long long best = kNoParent;
     if (v != src && dv < kInf2) {

@@ -93,7 +93,7 @@ void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
   s->sssp_mode = 0;
   s->sssp_rounds = 12;
   s->sssp_grid = 0;
-  s->tc_mode = 0;
+  s->tc_mode = 1;
   s->pr_mode = 0;
   // One sweep: measured at RMAT-22, 2 MB source segments double the
   // iteration time (each pass repeats the per-vertex load chain).

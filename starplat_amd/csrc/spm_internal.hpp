@@ -143,6 +143,27 @@ int32_t* sssp_dist(spm_graph* g);
 int sssp_weights_changed(spm_graph* g);
 
 // ---- wavefront primitives ---------------------------------------------------
+
+// Kernels with one wavefront per vertex visit x = wave, wave + nw, ... and
+// handle vertex spread_vertex(x, V): a fixed bijection of [0, V) (two
+// xorshift-multiply rounds on the bit width of V, which are bijections of
+// [0, 2^k), cycle-walked back into [0, V)). With the identity order,
+// wavefront k gets every vertex congruent to k mod nw, and the heavy
+// vertices of RMAT graphs -- ids with many zero bits -- all land on the same
+// few wavefronts (15-70x the mean load at RMAT-21/22).
+__host__ __device__ __forceinline__ int64_t spread_vertex(int64_t x, int32_t V) {
+  const uint32_t k = V > 1 ? 32u - uint32_t(__builtin_clz(uint32_t(V - 1))) : 0u;
+  const uint32_t mask = k >= 32 ? ~0u : (1u << k) - 1u;
+  const uint32_t h = (k + 1) / 2;
+  uint32_t y = uint32_t(x);
+  do {
+    y ^= y >> h;
+    y = (y * 0x9E3779B1u) & mask;
+    y ^= y >> h;
+    y = (y * 0x85EBCA6Bu) & mask;
+  } while (y >= uint32_t(V));
+  return int64_t(y);
+}
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 template <typename T>

@@ -54,16 +54,21 @@ __global__ void lay_unpack(int64_t n, const unsigned long long* __restrict__ key
 }
 
 // Row offsets of sorted keys (row in the high half): off[x] = first i whose
-// row is >= x, off[V] = n.
+// row is >= x, for x in [0, V] (off[V] = n), one binary search per row. (Writing
+// the offsets from the row boundaries instead leaves a run of empty rows to
+// one thread: the ~2M rows without out-edges that a degree-ordered relabel
+// puts last took 70 ms that way at RMAT-22.)
 __global__ void lay_segoff(int64_t n, const unsigned long long* __restrict__ keys, int32_t V,
                            int32_t* __restrict__ off) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t r = int64_t(keys[i] >> 32);
-    const int64_t p = i == 0 ? -1 : int64_t(keys[i - 1] >> 32);
-    for (int64_t x = p + 1; x <= r; ++x) off[x] = int32_t(i);
-    if (i == n - 1)
-      for (int64_t x = r + 1; x <= V; ++x) off[x] = int32_t(n);
+  for (int64_t x = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; x <= V; x += stride) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (int64_t(keys[mid] >> 32) < x) lo = mid + 1;
+      else hi = mid;
+    }
+    off[x] = int32_t(lo);
   }
 }
 
@@ -113,7 +118,7 @@ int csr_from_pairs(spm_graph* g, int64_t n, unsigned long long* k0, unsigned lon
                 (const unsigned long long*)kb.Current(), (const int32_t*)vb.Current(), adj, w,
                 roff ? rk : (unsigned long long*)nullptr, roff ? rv : (int32_t*)nullptr);
   if (rc == SPM_OK)
-    rc = launch(g, "lay_segoff", lay_segoff, dim3(lay_grid(g, n)), dim3(kLayBlock), 0, n,
+    rc = launch(g, "lay_segoff", lay_segoff, dim3(lay_grid(g, int64_t(V) + 1)), dim3(kLayBlock), 0, n,
                 (const unsigned long long*)kb.Current(), V, off);
   if (rc == SPM_OK && roff) {
     hipcub::DoubleBuffer<unsigned long long> rkb(rk, kb.Current());
@@ -124,7 +129,7 @@ int csr_from_pairs(spm_graph* g, int64_t n, unsigned long long* k0, unsigned lon
                   (const unsigned long long*)rkb.Current(), (const int32_t*)rvb.Current(), radj, rw,
                   (unsigned long long*)nullptr, (int32_t*)nullptr);
     if (rc == SPM_OK)
-      rc = launch(g, "lay_segoff", lay_segoff, dim3(lay_grid(g, n)), dim3(kLayBlock), 0, n,
+      rc = launch(g, "lay_segoff", lay_segoff, dim3(lay_grid(g, int64_t(V) + 1)), dim3(kLayBlock), 0, n,
                   (const unsigned long long*)rkb.Current(), V, roff);
   }
   if (rc == SPM_OK) {
@@ -159,7 +164,8 @@ __global__ __launch_bounds__(kLayBlock) void lay_relabel_pairs(int32_t V, const 
                                                                int32_t* __restrict__ vals) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kLayWaves;
-  for (int64_t u = int64_t(blockIdx.x) * kLayWaves + threadIdx.x / kWave; u < V; u += nw) {
+  for (int64_t x_ = int64_t(blockIdx.x) * kLayWaves + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t u = spread_vertex(x_, V);
     const unsigned long long hi = static_cast<unsigned long long>(uint32_t(perm[u])) << 32;
     for (int32_t j = off[u] + lane; j < off[u + 1]; j += kWave) {
       keys[j] = hi | uint32_t(perm[adj[j]]);

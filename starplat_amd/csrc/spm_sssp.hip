@@ -263,20 +263,23 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
         const int32_t nd = s_dv[lo] + wt;
         // A cached (possibly stale) read is enough to skip hopeless updates:
         // distances only decrease, so a stale value is never below the true
-        // one and atomicMin below stays the authority.
+        // one and atomicMin below stays the authority. Every lane that passes
+        // the test marks the neighbour, without waiting for the atomic's old
+        // value: the value read is at most the distance at the start of the
+        // round, so if this lane's atomicMin does not lower the distance,
+        // another lane lowered it during this round and marks it too. The
+        // marked set is exactly the set of vertices lowered in this round.
         if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-          const int32_t old = atomicMin(&dist[nb], nd);
-          if (nd < old) {
-            if (!sparse) {
-              stamp[nb] = sv;  // idempotent mark, compacted later
-            } else if (atomicExch(&stamp[nb], sv) != sv) {
-              const int32_t deg = off[nb + 1] - off[nb];
-              if (deg > 0) {
-                pv[k] = nb;
-                pd[k] = deg;
-                ++pc;
-                sd += uint32_t(deg);
-              }
+          atomicMin(&dist[nb], nd);  // no return value: the wave does not wait
+          if (!sparse) {
+            stamp[nb] = sv;  // idempotent mark, compacted later
+          } else if (atomicExch(&stamp[nb], sv) != sv) {
+            const int32_t deg = off[nb + 1] - off[nb];
+            if (deg > 0) {
+              pv[k] = nb;
+              pd[k] = deg;
+              ++pc;
+              sd += uint32_t(deg);
             }
           }
         }
@@ -455,11 +458,9 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_dense(
       const int32_t nd = s_dv[wid][lo] + wgt[e];
       // cached pre-filter; atomicMin decides (see sssp_relax)
       if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-        const int32_t old = atomicMin(&dist[nb], nd);
-        if (nd < old) {
-          nxt[nb] = 1;
-          found = 1;
-        }
+        atomicMin(&dist[nb], nd);  // marks without the old value (see sssp_relax)
+        nxt[nb] = 1;
+        found = 1;
       }
     }
     __builtin_amdgcn_wave_barrier();

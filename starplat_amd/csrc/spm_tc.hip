@@ -16,10 +16,13 @@
 //  * any other graph: the DSL's own sum, item = CSR edge (v,u) with u < v,
 //    count |nbrs(u) intersect {w in nbrs(v) : w > v}|.
 // Both reduce to "size of the intersection of two ascending rows" per item.
-// A wavefront takes 64 items, flattens the elements of each item's shorter
-// row across its lanes (wave prefix sum, LDS lookup) and binary-searches each
-// element in the longer row; hits are counted with ballot + popcount. The
-// integer total is exact, so results are bit-identical to the reference.
+// Symmetric graphs are counted per oriented source vertex with an LDS hash
+// set of its row (tc_count_hash, the default). The search form
+// (tc_count, schedule tc_mode 0, and the general sum) has a wavefront take 64
+// items, flatten the elements of each item's shorter row across its lanes
+// (wave prefix sum, LDS lookup) and binary-search each element in the longer
+// row; hits are counted with ballot + popcount. The integer total is exact,
+// so results are bit-identical to the reference.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -43,7 +46,10 @@ struct TcWork {
   int64_t m = 0;
   int32_t* maxd = nullptr;    // [1] longest oriented row (device)
   int32_t max_dplus = 0;
-  unsigned long long* acc = nullptr;  // [2] total, flags
+  unsigned long long* acc = nullptr;  // [3] total, flags, item counter
+  int2* items = nullptr;      // [nitems] hashed-count work items (u, first neighbour)
+  uint32_t nitems = 0;
+  int items_built = 0;
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
 };
@@ -80,7 +86,8 @@ __global__ __launch_bounds__(kTcBlock) void tc_check(int32_t V, const int32_t* _
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kTcWaves;
   unsigned long long f = 0;
-  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw) {
+  for (int64_t x_ = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t u = spread_vertex(x_, V);
     const int32_t b = off[u], e = off[u + 1];
     if (kRev && (roff[u] != b || roff[u + 1] != e)) f |= 4;
     for (int32_t i = b + lane; i < e; i += kWave) {
@@ -107,7 +114,8 @@ __global__ __launch_bounds__(kTcBlock) void tc_orient_count(int32_t V, const int
                                                              int32_t* __restrict__ dplus) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kTcWaves;
-  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw) {
+  for (int64_t x_ = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t u = spread_vertex(x_, V);
     const int32_t b = off[u], e = off[u + 1], du = e - b;
     int32_t c = 0;
     for (int32_t i = b + lane; i < e; i += kWave) {
@@ -128,7 +136,8 @@ __global__ __launch_bounds__(kTcBlock) void tc_orient_fill(int32_t V, const int3
                                                             int32_t* __restrict__ psrc) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kTcWaves;
-  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw) {
+  for (int64_t x_ = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t u = spread_vertex(x_, V);
     const int32_t b = off[u], e = off[u + 1], du = e - b;
     int32_t out = poff[u];
     for (int32_t i0 = b; i0 < e; i0 += kWave) {
@@ -153,8 +162,10 @@ __global__ __launch_bounds__(kTcBlock) void tc_orient_fill(int32_t V, const int3
 __global__ void tc_expand_src(int32_t V, const int32_t* __restrict__ off, int32_t* __restrict__ csrc) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kTcWaves;
-  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; u < V; u += nw)
+  for (int64_t x_ = int64_t(blockIdx.x) * kTcWaves + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t u = spread_vertex(x_, V);
     for (int32_t i = off[u] + lane; i < off[u + 1]; i += kWave) csrc[i] = int32_t(u);
+  }
 }
 
 // Intersection counting. ORIENTED: item e is oriented edge (src[e], adj[e])
@@ -230,82 +241,135 @@ __global__ __launch_bounds__(kTcBlock) void tc_count(int64_t nitems, const int32
   if (lane == 0 && cnt) atomicAdd(total, cnt);
 }
 
-// Oriented rows, hashed per source vertex: a wavefront owns vertex u, builds
-// an open-addressing hash set of N+(u) in its slice of LDS (size a power of
-// two >= 2 d+(u), linear probing, keys are distinct vertex ids), then streams
-// the rows N+(v) of every v in N+(u) -- flattened across the 64 lanes with a
-// wave prefix sum -- and probes each element: every hit is one triangle
-// u < v < w in the orientation. Rows of the (degree, id) orientation are at
-// most sqrt(2m) long, so N+(u) always fits kTcHash slots.
-constexpr int kTcHash = 2048;
-__global__ __launch_bounds__(kTcBlock) void tc_count_hash(int32_t V, const int32_t* __restrict__ poff,
+// Oriented rows, hashed per source vertex. Work items are (u, c): the
+// oriented neighbours N+(u)[c, c+64) of a vertex u with at least two (a
+// triangle needs two). A wavefront takes items in batches from a counter and
+// keeps a hash set of N+(u) in its slice of LDS, rebuilt only when the item's
+// vertex changes (items of one vertex are adjacent). It then walks the rows
+// N+(v) of the item's v's one at a time, all 64 lanes striding over a row
+// (coalesced loads, kTcUnroll in flight per lane), and probes each element:
+// every hit is one triangle u < v < w in the orientation. At RMAT-21, 99% of
+// the probed elements lie in rows of 64 or more, so striding rows keeps the
+// lanes busy without locating each element's row.
+// The set is bucketed: 4-slot buckets (one 16-byte LDS read), at least d+(u)
+// buckets (load <= 1/4 while 4 d+(u) <= kTcHash slots, <= 1/2 at the size
+// limit), keys placed in the first free slot of the first non-full bucket
+// from the hashed one. A bucket with a free slot ends a search, so a miss
+// almost always costs one read.
+// The host launches this kernel only when the longest oriented row fits the
+// set (2 d+ <= kTcHash; the orientation bounds rows by sqrt(2m)), otherwise
+// it runs the search; a row that does not fit is counted in `overflow`. Taking
+// items from a counter matters: with a static vertex-to-wavefront map, RMAT
+// ids with many zero low bits (the heavy vertices) land on the same
+// wavefronts (24x the mean load at RMAT-21).
+constexpr int kTcHash = 2048;  // slots per wavefront (int32 keys, -1 empty)
+constexpr int kTcItem = kWave;  // oriented neighbours per work item (one per lane)
+constexpr uint32_t kTcGrab = 2;  // items per counter bump (small: the last items set the tail)
+constexpr int kTcUnroll = 8;     // row elements per lane per step
+
+__global__ void tc_item_count(int32_t V, const int32_t* __restrict__ poff, int32_t* __restrict__ nitem) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; u < V; u += stride) {
+    const int32_t du = poff[u + 1] - poff[u];
+    nitem[u] = du >= 2 ? (du + kTcItem - 1) / kTcItem : 0;
+  }
+}
+
+__global__ void tc_item_fill(int32_t V, const int32_t* __restrict__ ioff, int2* __restrict__ items) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t u = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; u < V; u += stride)
+    for (int32_t k = ioff[u]; k < ioff[u + 1]; ++k) items[k] = make_int2(int32_t(u), (k - ioff[u]) * kTcItem);
+}
+
+__global__ __launch_bounds__(kTcBlock) void tc_count_hash(uint32_t nitems, const int2* __restrict__ items,
+                                                          const int32_t* __restrict__ poff,
                                                           const int32_t* __restrict__ padj,
                                                           unsigned long long* __restrict__ total,
-                                                          unsigned long long* __restrict__ overflow) {
-  __shared__ int32_t s_tab[kTcWaves][kTcHash];
-  __shared__ uint32_t s_pre[kTcWaves][kWave];
-  __shared__ int32_t s_row[kTcWaves][kWave];
+                                                          unsigned long long* __restrict__ overflow,
+                                                          uint32_t* __restrict__ next) {
+  __shared__ __align__(16) int32_t s_tab[kTcWaves][kTcHash];
   const int lane = __lane_id(), wid = threadIdx.x / kWave;
   int32_t* tab = s_tab[wid];
-  unsigned long long cnt = 0;
-  const int64_t nw = int64_t(gridDim.x) * kTcWaves;
-  for (int64_t u = int64_t(blockIdx.x) * kTcWaves + wid; u < V; u += nw) {
-    const int32_t ub = poff[u], du = poff[u + 1] - ub;
-    if (du < 2) continue;  // a triangle needs two oriented neighbours of u
-    if (2 * du > kTcHash) {  // cannot happen for the (degree, id) orientation; counted, not skipped silently
-      if (lane == 0) atomicAdd(overflow, 1ull);
-      continue;
-    }
-    int lg = 6;
-    while ((1 << lg) < 2 * du) ++lg;
-    const int size = 1 << lg;
-    const uint32_t mask = uint32_t(size - 1);
-    // Fibonacci hashing: the top lg bits of x * 2^32/phi.
-    const int shift = 32 - lg;
-    for (int i = lane; i < size; i += kWave) tab[i] = -1;
-    __builtin_amdgcn_wave_barrier();
-    for (int i = lane; i < du; i += kWave) {
-      const int32_t x = padj[ub + i];
-      uint32_t h = (uint32_t(x) * 2654435769u) >> shift;
-      while (atomicCAS(&tab[h], -1, x) != -1) h = (h + 1) & mask;
-    }
-    __builtin_amdgcn_wave_barrier();
-    // probe rows N+(v), v in N+(u), 64 v's at a time
-    for (int32_t i0 = 0; i0 < du; i0 += kWave) {
-      const int32_t i = i0 + lane;
-      uint32_t len = 0;
-      int32_t rb = 0;
+  const int4* bkt = reinterpret_cast<const int4*>(tab);
+  unsigned long long cnt = 0;  // per lane
+  int32_t cur = -1, ub = 0, du = 0, shift = 0;
+  uint32_t bmask = 0;
+  bool fits = false;
+  for (;;) {
+    uint32_t b = 0;
+    if (lane == 0) b = atomicAdd(next, kTcGrab);
+    b = __shfl(b, 0, kWave);
+    if (b >= nitems) break;
+    const uint32_t e = min(b + kTcGrab, nitems);
+    for (uint32_t it = b; it < e; ++it) {
+      const int2 item = items[it];
+      if (item.x != cur) {
+        cur = item.x;
+        ub = poff[cur];
+        du = poff[cur + 1] - ub;
+        fits = 2 * du <= kTcHash;
+        if (!fits) {  // excluded by the host's check of the longest row
+          if (lane == 0) atomicAdd(overflow, 1ull);
+          continue;
+        }
+        int lg = 4;  // log2 of the bucket count
+        while ((1 << lg) < du && (4 << lg) < kTcHash) ++lg;
+        bmask = (1u << lg) - 1u;
+        shift = 32 - lg;  // Fibonacci hashing: the top lg bits of x * 2^32/phi
+        __builtin_amdgcn_wave_barrier();  // the previous vertex's probes are done
+        for (int i = lane; i < (4 << lg); i += kWave) tab[i] = -1;
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < du; i += kWave) {
+          const int32_t x = padj[ub + i];
+          uint32_t h = (uint32_t(x) * 2654435769u) >> shift;
+          for (;;) {
+            int s = 0;
+            while (s < 4 && atomicCAS(&tab[4 * h + s], -1, x) != -1) ++s;
+            if (s < 4) break;
+            h = (h + 1) & bmask;
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (!fits) continue;
+      const int32_t i = item.y + lane;
+      int32_t rb = 0, len = 0;
       if (i < du) {
         const int32_t v = padj[ub + i];
         rb = poff[v];
-        len = uint32_t(poff[v + 1] - rb);
+        len = poff[v + 1] - rb;
       }
-      const uint32_t incl = wave_incl_scan(len);
-      const uint32_t tot = __shfl(incl, kWave - 1, kWave);
-      s_pre[wid][lane] = incl - len;
-      s_row[wid][lane] = rb;
-      __builtin_amdgcn_wave_barrier();
-      for (uint32_t t0 = 0; t0 < tot; t0 += kWave) {
-        const uint32_t t = t0 + lane;
-        bool hit = false;
-        if (t < tot) {
-          int lo = 0, hi = kWave - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_pre[wid][mid] <= t) lo = mid;
-            else hi = mid - 1;
+      const int nv = min(kTcItem, du - item.y);
+      for (int k = 0; k < nv; ++k) {
+        const int32_t r0 = __builtin_amdgcn_readlane(rb, k);
+        const int32_t n = __builtin_amdgcn_readlane(len, k);
+        for (int32_t j0 = 0; j0 < n; j0 += kTcUnroll * kWave) {
+          int32_t wq[kTcUnroll];
+#pragma unroll
+          for (int q = 0; q < kTcUnroll; ++q) {
+            const int32_t j = j0 + q * kWave + lane;
+            wq[q] = j < n ? padj[r0 + j] : -1;
           }
-          const int32_t w = padj[s_row[wid][lo] + int32_t(t - s_pre[wid][lo])];
-          uint32_t h = (uint32_t(w) * 2654435769u) >> shift;
-          int32_t k;
-          while ((k = tab[h]) != -1 && k != w) h = (h + 1) & mask;
-          hit = k == w;
+#pragma unroll
+          for (int q = 0; q < kTcUnroll; ++q) {
+            const int32_t w = wq[q];
+            if (w < 0) continue;
+            uint32_t h = (uint32_t(w) * 2654435769u) >> shift;
+            for (;;) {
+              const int4 c = bkt[h];
+              if (c.x == w || c.y == w || c.z == w || c.w == w) {
+                ++cnt;
+                break;
+              }
+              if (min(min(c.x, c.y), min(c.z, c.w)) < 0) break;  // a free slot: w is absent
+              h = (h + 1) & bmask;
+            }
+          }
         }
-        cnt += __popcll(__ballot(hit));
       }
-      __builtin_amdgcn_wave_barrier();
     }
   }
+  cnt = wave_sum(cnt);
   if (lane == 0 && cnt) atomicAdd(total, cnt);
 }
 
@@ -314,7 +378,7 @@ static int tc_prepare(spm_graph* g) {
   TcWork* w = g->tc;
   const int32_t V = g->V;
   const int grid = g->num_cus * 8;
-  if (!w->acc) SPM_TRY(dmalloc(g, (void**)&w->acc, 2 * sizeof(unsigned long long), "tc counters"));
+  if (!w->acc) SPM_TRY(dmalloc(g, (void**)&w->acc, 3 * sizeof(unsigned long long), "tc counters"));
   if (!w->maxd) SPM_TRY(dmalloc(g, (void**)&w->maxd, sizeof(int32_t), "tc max row"));
   if (!w->checked) {
     SPM_CHECK_HIP(hipMemsetAsync(w->acc, 0, 2 * sizeof(unsigned long long), g->stream));
@@ -382,6 +446,46 @@ static int tc_prepare(spm_graph* g) {
   return SPM_OK;
 }
 
+// Work items of the hashed count, built once per orientation.
+static int tc_build_items(spm_graph* g) {
+  TcWork* w = g->tc;
+  if (w->items_built) return SPM_OK;
+  const int32_t V = g->V;
+  int32_t* ioff = nullptr;
+  SPM_TRY(dmalloc(g, (void**)&ioff, sizeof(int32_t) * (size_t(V) + 1), "tc item offsets"));
+  const int igrid = int(std::min<int64_t>((int64_t(V) + 255) / 256, int64_t(g->num_cus) * 16));
+  int rc = launch(g, "tc_item_count", tc_item_count, dim3(igrid), dim3(256), 0, V, (const int32_t*)w->dplus, ioff);
+  size_t need = 0;
+  if (rc == SPM_OK && hipMemsetAsync(ioff + V, 0, sizeof(int32_t), g->stream) != hipSuccess)
+    rc = set_error(SPM_EHIP, "tc items: memset");
+  if (rc == SPM_OK && hipcub::DeviceScan::ExclusiveSum(nullptr, need, ioff, ioff, V + 1, g->stream) != hipSuccess)
+    rc = set_error(SPM_EHIP, "tc items: scan size");
+  if (rc == SPM_OK && need > w->tmp_bytes) {
+    dfree(g, w->tmp, w->tmp_bytes);
+    w->tmp = nullptr;
+    w->tmp_bytes = 0;
+    rc = dmalloc(g, &w->tmp, need, "tc scan scratch");
+    if (rc == SPM_OK) w->tmp_bytes = need;
+  }
+  if (rc == SPM_OK && hipcub::DeviceScan::ExclusiveSum(w->tmp, need, ioff, ioff, V + 1, g->stream) != hipSuccess)
+    rc = set_error(SPM_EHIP, "tc items: scan");
+  int32_t n = 0;
+  if (rc == SPM_OK && (hipMemcpyAsync(&n, ioff + V, sizeof n, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                       hipStreamSynchronize(g->stream) != hipSuccess))
+    rc = set_error(SPM_EHIP, "tc items: count read-back");
+  if (rc == SPM_OK && n > 0) {
+    rc = dmalloc(g, (void**)&w->items, sizeof(int2) * size_t(n), "tc work items");
+    if (rc == SPM_OK) {
+      w->nitems = uint32_t(n);
+      rc = launch(g, "tc_item_fill", tc_item_fill, dim3(igrid), dim3(256), 0, V, (const int32_t*)ioff, w->items);
+    }
+  }
+  if (rc == SPM_OK && hipStreamSynchronize(g->stream) != hipSuccess) rc = set_error(SPM_EHIP, "tc items: sync");
+  dfree(g, ioff, sizeof(int32_t) * (size_t(V) + 1));
+  if (rc == SPM_OK) w->items_built = 1;
+  return rc;
+}
+
 int tc_release(spm_graph* g) {
   TcWork* w = g->tc;
   if (!w) return SPM_OK;
@@ -389,7 +493,8 @@ int tc_release(spm_graph* g) {
   dfree(g, w->padj, sizeof(int32_t) * size_t(w->m));
   dfree(g, w->psrc, sizeof(int32_t) * size_t(w->m));
   dfree(g, w->csrc, sizeof(int32_t) * size_t(g->E));
-  dfree(g, w->acc, 2 * sizeof(unsigned long long));
+  dfree(g, w->acc, 3 * sizeof(unsigned long long));
+  dfree(g, w->items, sizeof(int2) * w->nitems);
   dfree(g, w->maxd, sizeof(int32_t));
   dfree(g, w->tmp, w->tmp_bytes);
   delete w;
@@ -403,14 +508,20 @@ int tc_run(spm_graph* g, uint64_t* out) {
   TcWork* w = g->tc;
   SPM_CHECK_HIP(hipMemsetAsync(w->acc, 0, sizeof(unsigned long long), g->stream));
   const int grid = g->num_cus * 8;
-  // tc_mode 1: hashed per-vertex count. Measured at RMAT-21 it is slower than
-  // the edge-centric search (it visits sum d+(v) = 11.3G row elements against
-  // sum min(d+(u), d+(v)) = 4.8G binary searches), so the search is the default.
+  // tc_mode 1 (default): hashed per-vertex count, 17.4 ms at RMAT-21. It
+  // visits sum d+(v) = 11.3G row elements, each one coalesced load and about
+  // one LDS read, against sum min(d+(u), d+(v)) = 4.8G binary searches of
+  // ~10 dependent loads each in mode 0 (41.6 ms). Mode 0 also serves rows
+  // too long for the LDS set and non-symmetric graphs.
   const bool hashed = w->symmetric && g->sched.tc_mode == 1 && 2 * w->max_dplus <= kTcHash;
   if (hashed) {
-    SPM_CHECK_HIP(hipMemsetAsync(w->acc + 1, 0, sizeof(unsigned long long), g->stream));
-    SPM_TRY(launch(g, "tc_count_hash", tc_count_hash, dim3(grid), dim3(kTcBlock), 0, g->V,
-                   (const int32_t*)w->dplus, (const int32_t*)w->padj, w->acc, w->acc + 1));
+    SPM_TRY(tc_build_items(g));
+    SPM_CHECK_HIP(hipMemsetAsync(w->acc + 1, 0, 2 * sizeof(unsigned long long), g->stream));
+    // one resident wave of workgroups: work is taken from the item counter
+    if (w->nitems)
+      SPM_TRY(launch(g, "tc_count_hash", tc_count_hash, dim3(g->num_cus * 4), dim3(kTcBlock), 0, w->nitems,
+                     (const int2*)w->items, (const int32_t*)w->dplus, (const int32_t*)w->padj, w->acc, w->acc + 1,
+                     reinterpret_cast<uint32_t*>(w->acc + 2)));
   } else if (w->symmetric) {
     if (w->m) SPM_TRY(launch(g, "tc_count", tc_count<true>, dim3(grid), dim3(kTcBlock), 0, w->m,
                             (const int32_t*)w->dplus, (const int32_t*)w->padj, (const int32_t*)w->psrc, w->acc));

@@ -329,7 +329,9 @@ def bench_schedule(args):
                           rmat_scales=tuple(args.sched_rmat), er_log2=tuple(args.sched_er), grid_sides=tuple(args.sched_grid))
     res["workload"] = ("mixed batch: RMAT (EF 4-32), Erdos-Renyi (avg degree 4-16) and 2-D grids, each with one SSSP "
                        "(weights U[1,100], src 0) or PageRank job; 8 schedule classes timed per job; MLP 16-64-64-8 "
-                       "trained on the GPU on the fastest class; makespan = sum of held-out job times")
+                       "trained on the GPU on the fastest class; makespan = sum of held-out job times; "
+                       "batched_held_out: the same held-out jobs run as disjoint unions, one batched launch "
+                       "sequence per job type (spm_pr_multi_device / spm_sssp_multi_device), default schedule")
     return res
 
 

@@ -77,6 +77,17 @@ int spm_compute_pr(spm_graph* g, float beta, float delta, int32_t maxIter,
 /* Same, result left in device memory d_pageRank (float[V]). */
 int spm_pr_device(spm_graph* g, float beta, float delta, int32_t maxIter,
                   float* d_pageRank, int32_t* iterCount);
+/* Many PageRank jobs in one launch sequence: g is the disjoint union of the
+ * jobs' graphs, job k owning vertices [bases[k], bases[k+1]) (bases[0] = 0,
+ * bases[n] = V, no empty job, no edge between jobs: checked, SPM_EINVAL).
+ * Each job runs Compute_PR on its own graph -- its own N_k in the 1/N_k
+ * start and (1-delta)/N_k term, its own diff and stopping test -- and stops
+ * independently; d_pageRank (device float[V]) receives every job's ranks and
+ * iterCounts (host int32[n], may be NULL) each job's iteration count. Runs on
+ * the original ids (no relabeled copy). No counterpart in the reference,
+ * which runs one Compute_PR per call. */
+int spm_pr_multi_device(spm_graph* g, const int32_t* bases, int32_t n, float beta, float delta,
+                        int32_t maxIter, float* d_pageRank, int32_t* iterCounts);
 
 /* ---- SSSP --------------------------------------------------------------
  * Replaces: Compute_SSSP(graph& g, int* dist, int* weight, int src) generated

@@ -1,12 +1,15 @@
-"""Batched execution of many SSSP jobs in one launch sequence.
+"""Batched execution of many SSSP or PageRank jobs in one launch sequence.
 
 Jobs on different graphs are packed into the disjoint union of their graphs
-(graph.union) with one source per component, and solved by one multi-source
-fixed point (spm_sssp_multi_device): components never exchange relaxations,
-so every job's distances are exactly those of its own Compute_SSSP. One
-launch sequence replaces one per job, which removes the per-job launch and
-host synchronisation cost that dominates small graphs. Multi-source BC is
-batched inside the library itself (64 sources per wavefront).
+(graph.union). SSSP jobs, one source per component, are solved by one
+multi-source fixed point (spm_sssp_multi_device): components never exchange
+relaxations, so every job's distances are exactly those of its own
+Compute_SSSP. PageRank jobs run as one sweep per iteration over every job
+still iterating (spm_pr_multi_device), each with its own vertex count,
+diff and stopping test. One launch sequence replaces one per job, which
+removes the per-job launch and host synchronisation cost that dominates
+small graphs. Multi-source BC is batched inside the library itself (64
+sources per wavefront).
 """
 from __future__ import annotations
 
@@ -32,6 +35,30 @@ class SsspBatch:
     def results(self):
         d = self.out.to_host()
         return [d[self.bases[i]:self.bases[i + 1]] for i in range(len(self.jobs))]
+
+    def close(self):
+        self.out.free()
+        self.union.release_device()
+
+
+class PrBatch:
+    """jobs: list of graphs, each a PageRank job with the same beta, delta and
+    maxIter. Holds the union graph in HBM."""
+
+    def __init__(self, graphs):
+        self.graphs = list(graphs)
+        self.union, self.bases = graph.union(self.graphs)
+        self.dev = self.union.device()
+        self.out = hip.DeviceArray(self.union.V, np.float32)
+        self.iters = None
+
+    def run(self, beta, delta, maxIter):
+        self.iters = self.dev.pr_multi_device(self.bases, beta, delta, maxIter, self.out)
+        return self.iters
+
+    def results(self):
+        r = self.out.to_host()
+        return [r[self.bases[i]:self.bases[i + 1]] for i in range(len(self.graphs))]
 
     def close(self):
         self.out.free()

@@ -38,6 +38,7 @@ struct DevBuf {
 
 struct SsspWork;
 struct PrWork;
+struct PrMulti;
 struct TcWork;
 struct BcWork;
 struct DynWork;
@@ -68,6 +69,7 @@ struct spm_graph {
   spm_schedule sched{};
   spm::SsspWork* sssp = nullptr;
   spm::PrWork* pr = nullptr;
+  spm::PrMulti* prm = nullptr;  // batched PageRank jobs (original ids)
   spm::TcWork* tc = nullptr;
   spm::BcWork* bc = nullptr;
   spm::DynWork* dyn = nullptr;  // edge updates since the CSR was last clean

@@ -49,6 +49,17 @@ struct PrBins {
   int32_t group;   // lanes per vertex in bin 1
 };
 
+// Batched jobs (spm_pr_multi_device): vertex v belongs to job job[v], each
+// job a contiguous range of the union graph's ids with no edge leaving it.
+// Jobs iterate with their own (1 - delta) / N_k and stopping test; a job that
+// has stopped keeps its ranks and is skipped by later iterations.
+struct PrJobs {
+  const int32_t* job;   // [V]
+  const int32_t* live;  // [n] 1 while the job iterates
+  const float* base;    // [n] (1 - delta) / N_k
+  float* dv;            // [V] |new - old| of the vertex's last iteration
+};
+
 struct PrWork {
   float* pr[2] = {nullptr, nullptr};
   float* c[2] = {nullptr, nullptr};
@@ -195,6 +206,31 @@ __device__ __forceinline__ void seg_emit(int32_t v, float sum, const PrSeg& sg, 
   pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
 }
 
+// A vertex's new rank: the single-job path through seg_emit; batched jobs use
+// the job's base term and leave |new - old| in dv for the per-job test.
+template <bool MULTI>
+__device__ __forceinline__ void pr_emit(int32_t v, float sum, const PrSeg& sg, const PrState& s,
+                                        const float* __restrict__ pr_in, float* __restrict__ pr_out,
+                                        float* __restrict__ c_out, const float* __restrict__ degf,
+                                        float& mydiff, const PrJobs& jb) {
+  if (!MULTI) {
+    seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
+    return;
+  }
+  const float nv = __fadd_rn(jb.base[jb.job[v]], __fmul_rn(s.delta, sum));
+  const float old = pr_in[v];
+  pr_out[v] = nv;
+  const float d = degf[v];
+  c_out[v] = d > 0.f ? nv / d : 0.f;
+  jb.dv[v] = nv - old >= 0.f ? nv - old : old - nv;
+}
+
+// Batched jobs: a vertex of a stopped job is skipped (no gather, no writes).
+template <bool MULTI>
+__device__ __forceinline__ bool pr_live(int32_t v, const PrJobs& jb) {
+  return !MULTI || jb.live[jb.job[v]] != 0;
+}
+
 // Sum of c[radj[e]] for e = ea+start, ea+start+stride, ... < eb, four rows
 // elements at a time: four independent index loads, then four independent
 // gathers, so a lane keeps several memory requests in flight instead of a
@@ -225,12 +261,13 @@ __device__ __forceinline__ void row_of(const int4* __restrict__ rows, int64_t i,
   if (sg.S > 1) seg_range(v, roff, sg, ea, eb);
 }
 
-template <int G>
+template <int G, bool MULTI>
 __global__ __launch_bounds__(kPrBlock) void pr_iter(
     const int32_t* __restrict__ roff, const int32_t* __restrict__ radj,
     const int4* __restrict__ rows, const float* __restrict__ degf, PrBins bins,
     const float* __restrict__ pr_in, const float* __restrict__ c_in, float* __restrict__ pr_out,
-    float* __restrict__ c_out, const PrState* __restrict__ st, float* __restrict__ partial, PrSeg sg) {
+    float* __restrict__ c_out, const PrState* __restrict__ st, float* __restrict__ partial, PrSeg sg,
+    PrJobs jb) {
   __shared__ float s_red[kPrBlock / kWave];
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const PrState s = *st;
@@ -244,7 +281,8 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     const int64_t i = int64_t(b) * kPrBlock + tid;
     if (i < bins.nb[0]) {
       row_of(rows, i, roff, sg, v, ea, eb);
-      seg_emit(v, gather_sum(radj, c_in, ea, eb, 0, 1), sg, s, pr_in, pr_out, c_out, degf, mydiff);
+      if (pr_live<MULTI>(v, jb))
+        pr_emit<MULTI>(v, gather_sum(radj, c_in, ea, eb, 0, 1), sg, s, pr_in, pr_out, c_out, degf, mydiff, jb);
     }
   } else if (b < bins.blk[2]) {
     // bin 1: G lanes per vertex
@@ -254,10 +292,11 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     v = -1;
     if (i < bins.nb[1]) {
       row_of(rows, bins.nb[0] + i, roff, sg, v, ea, eb);
-      sum = gather_sum(radj, c_in, ea, eb, g, G);
+      if (!pr_live<MULTI>(v, jb)) v = -1;
+      else sum = gather_sum(radj, c_in, ea, eb, g, G);
     }
     sum = group_sum<G>(sum);
-    if (v >= 0 && g == 0) seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
+    if (v >= 0 && g == 0) pr_emit<MULTI>(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff, jb);
   } else if (b < bins.blk[3]) {
     // bin 2: one wavefront per vertex
     const int64_t i = int64_t(b - bins.blk[2]) * (kPrBlock / kWave) + wid;
@@ -265,25 +304,131 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     v = -1;
     if (i < bins.nb[2]) {
       row_of(rows, bins.nb[0] + bins.nb[1] + i, roff, sg, v, ea, eb);
-      sum = gather_sum(radj, c_in, ea, eb, lane, kWave);
+      if (!pr_live<MULTI>(v, jb)) v = -1;
+      else sum = gather_sum(radj, c_in, ea, eb, lane, kWave);
     }
     sum = wave_sum(sum);
-    if (v >= 0 && lane == 0) seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
+    if (v >= 0 && lane == 0) pr_emit<MULTI>(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff, jb);
   } else {
     // bin 3: one workgroup per vertex
     row_of(rows, bins.nb[0] + bins.nb[1] + bins.nb[2] + (b - bins.blk[3]), roff, sg, v, ea, eb);
-    float sum = gather_sum(radj, c_in, ea, eb, tid, kPrBlock);
-    sum = wave_sum(sum);
-    if (lane == 0) s_red[wid] = sum;
-    __syncthreads();
-    if (tid == 0) {
-      float t = 0.f;
-      for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
-      seg_emit(v, t, sg, s, pr_in, pr_out, c_out, degf, mydiff);
+    if (pr_live<MULTI>(v, jb)) {  // uniform across the workgroup
+      float sum = gather_sum(radj, c_in, ea, eb, tid, kPrBlock);
+      sum = wave_sum(sum);
+      if (lane == 0) s_red[wid] = sum;
+      __syncthreads();
+      if (tid == 0) {
+        float t = 0.f;
+        for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
+        pr_emit<MULTI>(v, t, sg, s, pr_in, pr_out, c_out, degf, mydiff, jb);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
-  if (sg.seg == sg.S - 1) pr_finish(mydiff, partial, s_red);
+  if (!MULTI && sg.seg == sg.S - 1) pr_finish(mydiff, partial, s_red);
+}
+
+// ---- batched jobs -------------------------------------------------------------
+
+// job[v] = k for v in [bases[k], bases[k+1]) (binary search over n+1 bases).
+__global__ void pr_job_ids(int32_t V, const int32_t* __restrict__ bases, int32_t n, int32_t* __restrict__ job) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
+    int32_t lo = 0, hi = n - 1;  // last k with bases[k] <= v
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (bases[mid] <= v) lo = mid;
+      else hi = mid - 1;
+    }
+    job[v] = lo;
+  }
+}
+
+// flag |= 1 if some edge u -> x joins two jobs (one wavefront per row).
+__global__ __launch_bounds__(kPrBlock) void pr_job_edges(int32_t V, const int32_t* __restrict__ off,
+                                                         const int32_t* __restrict__ adj,
+                                                         const int32_t* __restrict__ job,
+                                                         unsigned int* __restrict__ flag) {
+  const int lane = __lane_id();
+  const int64_t nw = int64_t(gridDim.x) * (kPrBlock / kWave);
+  bool cross = false;
+  for (int64_t x_ = int64_t(blockIdx.x) * (kPrBlock / kWave) + threadIdx.x / kWave; x_ < V; x_ += nw) {
+    const int64_t u = spread_vertex(x_, V);
+    const int32_t k = job[u];
+    for (int32_t e = off[u] + lane; e < off[u + 1]; e += kWave) cross |= job[adj[e]] != k;
+  }
+  if (__any(cross) && lane == 0) atomicOr(flag, 1u);
+}
+
+__global__ void pr_init_multi(int32_t V, const int32_t* __restrict__ off, const int32_t* __restrict__ job,
+                              const float* __restrict__ init, float* __restrict__ degf, float* __restrict__ pr,
+                              float* __restrict__ c, PrState* st, int maxIter, float beta, float delta) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->done = 0;
+    st->iter = 0;
+    st->maxIter = maxIter;
+    st->beta = beta;
+    st->delta = delta;
+    st->base = 0.f;  // per job
+  }
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
+    const float d = float(off[v + 1] - off[v]);
+    const float x = init[job[v]];
+    degf[v] = d;
+    pr[v] = x;
+    c[v] = d > 0.f ? x / d : 0.f;
+  }
+}
+
+// One workgroup per job: the loop test of job k on the sum of dv over its
+// vertices, in a fixed order (so a batched job stops where the same sum
+// tells it to, run after run).
+__global__ __launch_bounds__(kPrBlock) void pr_job_check(const int32_t* __restrict__ bases, const float* __restrict__ dv,
+                                                         int32_t* __restrict__ live, int32_t* __restrict__ iters,
+                                                         const PrState* __restrict__ st) {
+  __shared__ float s_red[kPrBlock / kWave];
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const int k = blockIdx.x;
+  if (!live[k]) return;
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  float t = 0.f;
+  for (int32_t v = bases[k] + tid; v < bases[k + 1]; v += kPrBlock) t += dv[v];
+  t = wave_sum(t);
+  if (lane == 0) s_red[wid] = t;
+  __syncthreads();
+  if (tid == 0) {
+    float d = 0.f;
+    for (int w = 0; w < kPrBlock / kWave; ++w) d += s_red[w];
+    const int it = iters[k] + 1;
+    iters[k] = it;
+    if (!(d > st->beta && it < st->maxIter)) live[k] = 0;
+  }
+}
+
+// One workgroup: the run is done when no job is live.
+__global__ __launch_bounds__(kPrBlock) void pr_jobs_done(const int32_t* __restrict__ live, int32_t n,
+                                                         PrState* __restrict__ st) {
+  __shared__ int s_any;
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  bool any = false;
+  for (int32_t k = threadIdx.x; k < n; k += kPrBlock) any |= live[k] != 0;
+  if (__any(any) && __lane_id() == 0) s_any = 1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st->iter += 1;
+    if (!s_any) st->done = 1;
+  }
+}
+
+// out[v] = the rank buffer of job[v]'s last iteration.
+__global__ void pr_job_select(int32_t V, const int32_t* __restrict__ job, const int32_t* __restrict__ iters,
+                              const float* __restrict__ pr0, const float* __restrict__ pr1, float* __restrict__ out) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride)
+    out[v] = (iters[job[v]] & 1) ? pr1[v] : pr0[v];
 }
 
 // Split points of every in-row at the segment boundaries (one lane per vertex,
@@ -370,11 +515,9 @@ __global__ __launch_bounds__(kPrBlock) void pr_push_apply(int32_t V, const float
   pr_finish(mydiff, partial, s_red);
 }
 
-static int pr_prepare(spm_graph* g, const CsrView& view) {
-  if (g->pr) return SPM_OK;
-  if (!g->roff) return set_error(SPM_ENOGRAPH, "PageRank needs the reverse CSR (rev_indexofNodes, srcList)");
-  PrWork* w = new PrWork();
-  g->pr = w;
+// Ranks, contributions and in-degree bins over `view`. Batched jobs (multi)
+// use plain pull sweeps: no push accumulators, no source segments.
+static int pr_build(spm_graph* g, PrWork* w, const CsrView& view, bool multi) {
   w->view = view;
   if (view.perm) SPM_TRY(dmalloc(g, (void**)&w->out, sizeof(float) * size_t(g->V), "pr result"));
   const int32_t V = g->V;
@@ -383,7 +526,7 @@ static int pr_prepare(spm_graph* g, const CsrView& view) {
     SPM_TRY(dmalloc(g, (void**)&w->c[i], sizeof(float) * V, "pr contributions"));
   }
   SPM_TRY(dmalloc(g, (void**)&w->degf, sizeof(float) * V, "pr outdeg"));
-  SPM_TRY(dmalloc(g, (void**)&w->acc, sizeof(float) * V, "pr push accumulators"));
+  if (!multi) SPM_TRY(dmalloc(g, (void**)&w->acc, sizeof(float) * V, "pr push accumulators"));
   SPM_TRY(dmalloc(g, (void**)&w->rows, sizeof(int4) * V, "pr bins"));
   SPM_TRY(dmalloc(g, (void**)&w->st, sizeof(PrState), "pr state"));
   // Bin vertices by in-degree on the host (once per handle and schedule).
@@ -415,10 +558,11 @@ static int pr_prepare(spm_graph* g, const CsrView& view) {
   b.blk[2] = b.blk[1] + (cnt[1] + kPrBlock / s.pr_group - 1) / (kPrBlock / s.pr_group);
   b.blk[3] = b.blk[2] + (cnt[2] + kPrBlock / kWave - 1) / (kPrBlock / kWave);
   b.blk[4] = b.blk[3] + cnt[3];
+  w->S = 1;
+  if (multi) return SPM_OK;
   w->partial_n = std::max(b.blk[4], g->num_cus * 8) + 1;
   SPM_TRY(dmalloc(g, (void**)&w->partial, sizeof(float) * size_t(w->partial_n), "pr partial diffs"));
   // Source segments for cache blocking.
-  w->S = 1;
   if (s.pr_seg_log2 > 0) w->S = int((int64_t(V) + (int64_t(1) << s.pr_seg_log2) - 1) >> s.pr_seg_log2);
   if (w->S > 1) {
     w->sp_n = size_t(V) * size_t(w->S - 1);
@@ -430,9 +574,14 @@ static int pr_prepare(spm_graph* g, const CsrView& view) {
   return SPM_OK;
 }
 
-int pr_release(spm_graph* g) {
-  PrWork* w = g->pr;
-  if (!w) return SPM_OK;
+static int pr_prepare(spm_graph* g, const CsrView& view) {
+  if (g->pr) return SPM_OK;
+  if (!g->roff) return set_error(SPM_ENOGRAPH, "PageRank needs the reverse CSR (rev_indexofNodes, srcList)");
+  g->pr = new PrWork();
+  return pr_build(g, g->pr, view, false);
+}
+
+static void pr_free_work(spm_graph* g, PrWork* w) {
   if (w->exec) (void)hipGraphExecDestroy(w->exec);
   if (w->graph) (void)hipGraphDestroy(w->graph);
   for (int i = 0; i < 2; ++i) {
@@ -446,6 +595,47 @@ int pr_release(spm_graph* g) {
   dfree(g, w->rows, sizeof(int4) * g->V);
   dfree(g, w->out, sizeof(float) * g->V);
   dfree(g, w->st, sizeof(PrState));
+}
+
+// Workspace of batched jobs (spm_pr_multi_device), over the original ids.
+struct PrMulti {
+  PrWork w;
+  int32_t* job = nullptr;        // [V]
+  float* dv = nullptr;           // [V]
+  int32_t* bases = nullptr;      // [cap + 1]
+  int32_t* live = nullptr;       // [cap]
+  int32_t* iters = nullptr;      // [cap]
+  float* base = nullptr;         // [cap] (1 - delta) / N_k
+  float* init = nullptr;         // [cap] 1 / N_k
+  unsigned int* flag = nullptr;  // [1] cross-job edge found
+  int32_t cap = 0;
+  int32_t n_graph = 0;           // job count the captured iterations were built for
+};
+
+static void pr_multi_free_jobs(spm_graph* g, PrMulti* m) {
+  dfree(g, m->bases, sizeof(int32_t) * (size_t(m->cap) + 1));
+  dfree(g, m->live, sizeof(int32_t) * size_t(m->cap));
+  dfree(g, m->iters, sizeof(int32_t) * size_t(m->cap));
+  dfree(g, m->base, sizeof(float) * size_t(m->cap));
+  dfree(g, m->init, sizeof(float) * size_t(m->cap));
+  m->bases = m->live = m->iters = nullptr;
+  m->base = m->init = nullptr;
+  m->cap = 0;
+}
+
+int pr_release(spm_graph* g) {
+  if (PrMulti* m = g->prm) {
+    pr_free_work(g, &m->w);
+    pr_multi_free_jobs(g, m);
+    dfree(g, m->job, sizeof(int32_t) * g->V);
+    dfree(g, m->dv, sizeof(float) * g->V);
+    dfree(g, m->flag, sizeof(unsigned int));
+    delete m;
+    g->prm = nullptr;
+  }
+  PrWork* w = g->pr;
+  if (!w) return SPM_OK;
+  pr_free_work(g, w);
   delete w;
   g->pr = nullptr;
   return SPM_OK;
@@ -471,10 +661,10 @@ static int pr_enqueue_iters(spm_graph* g, int R) {
       const PrSeg sg{w->sp, w->acc, w->S, seg};
 #define SPM_PR_LAUNCH(G)                                                                          \
   case G:                                                                                         \
-    SPM_TRY(launch(g, "pr_iter", pr_iter<G>, grid, block, 0, w->view.roff,                        \
+    SPM_TRY(launch(g, "pr_iter", pr_iter<G, false>, grid, block, 0, w->view.roff,                 \
                    w->view.radj, (const int4*)w->rows, (const float*)w->degf,                     \
                    w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st,             \
-                   w->partial, sg));                                                               \
+                   w->partial, sg, PrJobs{}));                                                     \
     break;
       switch (w->bins.group) {
         SPM_PR_LAUNCH(2)
@@ -546,6 +736,140 @@ int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out,
   return SPM_OK;
 }
 
+// R iterations of batched jobs: the sweep over live jobs, the per-job loop
+// tests, then the all-stopped test.
+static int pr_multi_enqueue(spm_graph* g, int R, int32_t n) {
+  PrMulti* m = g->prm;
+  PrWork* w = &m->w;
+  const dim3 grid(std::max(w->bins.blk[4], 1)), block(kPrBlock);
+  const PrJobs jb{m->job, m->live, m->base, m->dv};
+  const PrSeg sg{nullptr, nullptr, 1, 0};
+  for (int r = 0; r < R; ++r) {
+    const int p = r & 1;
+#define SPM_PR_MULTI_LAUNCH(G)                                                                          \
+  case G:                                                                                               \
+    SPM_TRY(launch(g, "pr_iter_multi", pr_iter<G, true>, grid, block, 0, w->view.roff, w->view.radj,    \
+                   (const int4*)w->rows, (const float*)w->degf, w->bins, (const float*)w->pr[p],         \
+                   (const float*)w->c[p], w->pr[1 - p], w->c[1 - p], (const PrState*)w->st,              \
+                   (float*)nullptr, sg, jb));                                                            \
+    break;
+    switch (w->bins.group) {
+      SPM_PR_MULTI_LAUNCH(2)
+      SPM_PR_MULTI_LAUNCH(4)
+      SPM_PR_MULTI_LAUNCH(8)
+      SPM_PR_MULTI_LAUNCH(16)
+      SPM_PR_MULTI_LAUNCH(32)
+      default: return set_error(SPM_EINVAL, "pr: bad lane group");
+    }
+#undef SPM_PR_MULTI_LAUNCH
+    SPM_TRY(launch(g, "pr_job_check", pr_job_check, dim3(n), block, 0, (const int32_t*)m->bases,
+                   (const float*)m->dv, m->live, m->iters, (const PrState*)w->st));
+    SPM_TRY(launch(g, "pr_jobs_done", pr_jobs_done, dim3(1), block, 0, (const int32_t*)m->live, n, w->st));
+  }
+  return SPM_OK;
+}
+
+int pr_multi_run(spm_graph* g, const int32_t* bases, int32_t n, float beta, float delta, int32_t maxIter,
+                 float* d_out, int32_t* iters_out) {
+  SPM_TRY(dyn_settle(g));
+  if (!g->roff) return set_error(SPM_ENOGRAPH, "PageRank needs the reverse CSR (rev_indexofNodes, srcList)");
+  if (!bases || n < 1 || !d_out) return set_error(SPM_EINVAL, "spm_pr_multi_device: NULL argument or no jobs");
+  if (bases[0] != 0 || bases[n] != g->V)
+    return set_error(SPM_EINVAL, "spm_pr_multi_device: job bases must start at 0 and end at V");
+  for (int32_t k = 0; k < n; ++k)
+    if (bases[k + 1] <= bases[k])
+      return set_error(SPM_EINVAL, "spm_pr_multi_device: job bases must be strictly ascending (no empty job)");
+  const int32_t V = g->V;
+  if (!g->prm) {
+    PrMulti* m = new PrMulti();
+    g->prm = m;
+    CsrView view;
+    view.off = g->off;
+    view.adj = g->adj;
+    view.w = g->w;
+    view.roff = g->roff;
+    view.radj = g->radj;
+    view.perm = nullptr;
+    view.h_perm = nullptr;
+    SPM_TRY(pr_build(g, &m->w, view, true));
+    SPM_TRY(dmalloc(g, (void**)&m->job, sizeof(int32_t) * size_t(V), "pr job ids"));
+    SPM_TRY(dmalloc(g, (void**)&m->dv, sizeof(float) * size_t(V), "pr vertex diffs"));
+    SPM_TRY(dmalloc(g, (void**)&m->flag, sizeof(unsigned int), "pr job check"));
+  }
+  PrMulti* m = g->prm;
+  PrWork* w = &m->w;
+  if (n > m->cap) {  // the captured iterations hold the old buffers
+    if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
+    if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
+    pr_multi_free_jobs(g, m);
+    const size_t c = size_t(n);
+    SPM_TRY(dmalloc(g, (void**)&m->bases, sizeof(int32_t) * (c + 1), "pr job bases"));
+    SPM_TRY(dmalloc(g, (void**)&m->live, sizeof(int32_t) * c, "pr job flags"));
+    SPM_TRY(dmalloc(g, (void**)&m->iters, sizeof(int32_t) * c, "pr job iterations"));
+    SPM_TRY(dmalloc(g, (void**)&m->base, sizeof(float) * c, "pr job base terms"));
+    SPM_TRY(dmalloc(g, (void**)&m->init, sizeof(float) * c, "pr job initial ranks"));
+    m->cap = n;
+  }
+  // Per-job constants as the single-job run computes them (host float arithmetic).
+  const size_t nj = static_cast<size_t>(n);
+  std::vector<float> hbase(nj), hinit(nj);
+  std::vector<int32_t> ones(nj, 1), zeros(nj, 0);
+  for (int32_t k = 0; k < n; ++k) {
+    const float num_nodes = float(bases[k + 1] - bases[k]);
+    hinit[k] = 1 / num_nodes;
+    hbase[k] = (1 - delta) / num_nodes;
+  }
+  SPM_CHECK_HIP(hipMemcpyAsync(m->bases, bases, sizeof(int32_t) * (size_t(n) + 1), hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(m->live, ones.data(), sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(m->iters, zeros.data(), sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(m->base, hbase.data(), sizeof(float) * size_t(n), hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(m->init, hinit.data(), sizeof(float) * size_t(n), hipMemcpyHostToDevice, g->stream));
+  const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
+  SPM_TRY(launch(g, "pr_job_ids", pr_job_ids, dim3(igrid), dim3(256), 0, V, (const int32_t*)m->bases, n, m->job));
+  SPM_CHECK_HIP(hipMemsetAsync(m->flag, 0, sizeof(unsigned int), g->stream));
+  SPM_TRY(launch(g, "pr_job_edges", pr_job_edges, dim3(g->num_cus * 8), dim3(kPrBlock), 0, V, (const int32_t*)g->off,
+                 (const int32_t*)g->adj, (const int32_t*)m->job, m->flag));
+  unsigned int cross = 0;
+  SPM_CHECK_HIP(hipMemcpyAsync(&cross, m->flag, sizeof cross, hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  if (cross) return set_error(SPM_EINVAL, "spm_pr_multi_device: an edge joins two jobs");
+  SPM_TRY(launch(g, "pr_init_multi", pr_init_multi, dim3(igrid), dim3(256), 0, V, (const int32_t*)g->off,
+                 (const int32_t*)m->job, (const float*)m->init, w->degf, w->pr[0], w->c[0], w->st, int(maxIter),
+                 beta, delta));
+  int R = std::max(2, std::min(128, (int(std::max(maxIter, 1)) + 1) & ~1));
+  const bool use_graph = !g->prof.on;
+  if (use_graph && (!w->exec || w->R != R || m->n_graph != n)) {
+    if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
+    if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
+    SPM_CHECK_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = pr_multi_enqueue(g, R, n);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc != SPM_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return set_error(SPM_EHIP, std::string("pr capture: ") + hipGetErrorString(e));
+    w->graph = graph;
+    SPM_CHECK_HIP(hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0));
+    w->R = R;
+    m->n_graph = n;
+  }
+  int32_t* h = g->h_pinned;
+  for (int64_t rep = 0;; ++rep) {
+    if (use_graph) SPM_CHECK_HIP(hipGraphLaunch(w->exec, g->stream));
+    else SPM_TRY(pr_multi_enqueue(g, R, n));
+    SPM_CHECK_HIP(hipMemcpyAsync(h, &w->st->done, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    if (h[0]) break;
+    if (rep > int64_t(maxIter) / R + 2) return set_error(SPM_EHIP, "pr: loop did not terminate");
+  }
+  SPM_TRY(launch(g, "pr_job_select", pr_job_select, dim3(igrid), dim3(256), 0, V, (const int32_t*)m->job,
+                 (const int32_t*)m->iters, (const float*)w->pr[0], (const float*)w->pr[1], d_out));
+  if (iters_out)
+    SPM_CHECK_HIP(hipMemcpyAsync(iters_out, m->iters, sizeof(int32_t) * size_t(n), hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  SPM_TRY(prof_flush(g));
+  return SPM_OK;
+}
+
 }  // namespace spm
 
 using namespace spm;
@@ -556,6 +880,12 @@ int spm_pr_device(spm_graph* g, float beta, float delta, int32_t maxIter, float*
                   int32_t* iterCount) {
   if (!g || !d_pageRank) return set_error(SPM_EINVAL, "spm_pr_device: NULL argument");
   return pr_run(g, beta, delta, maxIter, d_pageRank, nullptr, iterCount);
+}
+
+int spm_pr_multi_device(spm_graph* g, const int32_t* bases, int32_t n, float beta, float delta, int32_t maxIter,
+                        float* d_pageRank, int32_t* iterCounts) {
+  if (!g) return set_error(SPM_EINVAL, "spm_pr_multi_device: NULL handle");
+  return pr_multi_run(g, bases, n, beta, delta, maxIter, d_pageRank, iterCounts);
 }
 
 int spm_compute_pr(spm_graph* g, float beta, float delta, int32_t maxIter, float* pageRank,

@@ -215,6 +215,16 @@ class DeviceGraph:
         N.check("spm_pr_device", N.device_lib().spm_pr_device(self.h, beta, delta, maxIter, d_out.ptr, C.byref(it)))
         return it.value
 
+    def pr_multi_device(self, bases, beta, delta, maxIter, d_out):
+        """Batched PageRank jobs on a disjoint union (job k = vertices
+        [bases[k], bases[k+1])); returns each job's iteration count."""
+        b = np.ascontiguousarray(bases, np.int32)
+        n = len(b) - 1
+        it = np.zeros(max(n, 1), np.int32)
+        N.check("spm_pr_multi_device", N.device_lib().spm_pr_multi_device(
+            self.h, N.ptr(b, C.c_int32), n, beta, delta, maxIter, d_out.ptr, N.ptr(it, C.c_int32)))
+        return it[:n]
+
     def sssp_device(self, src, d_out):
         r = C.c_int32(0)
         N.check("spm_sssp_device", N.device_lib().spm_sssp_device(self.h, src, d_out.ptr, C.byref(r)))

@@ -32,13 +32,16 @@ def main(argv=None):
     e.add_argument("--held-out", type=int, default=400)
     e.add_argument("--epochs", type=int, default=400)
     e.add_argument("--seed", type=int, default=5)
+    e.add_argument("--batched", action="store_true",
+                   help="also time the held-out jobs in batches (rows must hold jobs 0..n-1)")
     a = ap.parse_args(argv)
     log = lambda *m: print("[sched]", *m, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "sweep":
         mixed_batch.sweep(a.seed, a.start, a.stop, a.reps, log, rows_path=a.rows)
         return 0
     feats, times, _ = mixed_batch.load_rows(a.rows)
-    res = mixed_batch.run(held_out=a.held_out, seed=a.seed, epochs=a.epochs, rows=(feats, times))
+    res = mixed_batch.run(held_out=a.held_out, seed=a.seed, epochs=a.epochs, rows=(feats, times), batched=a.batched,
+                          log=log)
     print(json.dumps(res))
     return 0
 

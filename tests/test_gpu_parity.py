@@ -404,3 +404,49 @@ def test_sssp_multi_source_and_batch(oracle):
     g.device().sssp_multi_device(src, out)
     ref = np.min(np.stack([oracle.compute_sssp(g, g.edgeLen, int(s)) for s in set(src.tolist())]), axis=0)
     np.testing.assert_array_equal(out.to_host(), ref)
+
+
+def test_pr_batch_matches_single_jobs(oracle):
+    """Uneven PageRank jobs in one launch sequence: every job's ranks are those
+    of its own run (bit-identical: vertices are binned by in-degree with the
+    same lane mapping, none of these is large enough to be relabeled) and
+    within the tolerance of the restatement; jobs stop independently."""
+    from starplat_amd.batch import PrBatch
+
+    gs = [graph.rmat(s, 8, seed=s) for s in (6, 9, 12)]
+    gs += [graph.grid2d(30, 40), graph.erdos_renyi(3000, 9000, seed=2, undirected=False), graph.rmat(10, 16, seed=4)]
+    b = PrBatch(gs)
+    for beta, maxIter in ((1e-10, 100), (1e-4, 37), (1e-10, 7)):
+        iters = b.run(beta, 0.85, maxIter)
+        for g, got, it in zip(gs, b.results(), iters):
+            single, it_single = g.device().pr(beta, 0.85, maxIter)
+            assert it == it_single
+            np.testing.assert_array_equal(got, single)
+            if beta == 1e-10:  # away from the stopping threshold, so both runs stop together
+                ref, _ = oracle.compute_pr(g, beta, 0.85, int(it))
+                assert rel_linf(got, ref) <= PR_TOL
+        if beta == 1e-4:
+            assert len(set(iters.tolist())) > 1  # the jobs stopped at different iterations
+        if maxIter == 7:
+            assert set(iters.tolist()) == {7}
+    b.close()
+
+
+def test_pr_batch_rejects_bad_jobs():
+    from starplat_amd import hip
+
+    g = graph.rmat(8, 8, seed=1)
+    dev = g.device()
+    out = hip.DeviceArray(g.V, np.float32)
+    with pytest.raises(Exception):  # an edge joins the two halves
+        dev.pr_multi_device([0, g.V // 2, g.V], 1e-10, 0.85, 10, out)
+    with pytest.raises(Exception):  # empty job
+        dev.pr_multi_device([0, 0, g.V], 1e-10, 0.85, 10, out)
+    with pytest.raises(Exception):  # does not cover [0, V)
+        dev.pr_multi_device([0, g.V - 1], 1e-10, 0.85, 10, out)
+    # one job covering the graph is the ordinary run
+    it = dev.pr_multi_device([0, g.V], 1e-10, 0.85, 10, out)
+    single, it_single = dev.pr(1e-10, 0.85, 10)
+    assert it[0] == it_single
+    np.testing.assert_array_equal(out.to_host(), single)
+    out.free()

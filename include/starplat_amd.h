@@ -170,6 +170,24 @@ int spm_dynsssp_device(spm_graph* g, int32_t* d_dist, int32_t* d_parent);
 int spm_dyn_sssp(spm_graph* g, int32_t src, const char* type, const int32_t* usrc, const int32_t* udst, int32_t n,
                  int32_t batchSize, int32_t* dist, int32_t* parent);
 
+/* ---- dynamic triangle counting -------------------------------------------
+ * Replaces: the code generated for graphcode/dynamicDSLCodes/dynamicBatchTCV2
+ * -- DynTC (:137-187): triangleCount = staticTC(g) (:1-21); per batch
+ * OnDelete marks the deleted edges, Decremental (:81-135) counts on the graph
+ * before the deletions, updateCSRDel, updateCSRAdd, OnAdd marks the added
+ * edges, Incremental (:23-78) counts on the updated graph; each phase moves
+ * the count by class1/2 + class2/4 + class3/6 as the DSL writes it. The
+ * count is kept in 64 bits (the DSL's int would wrap past 2^31); DynTC
+ * returns nothing in the DSL, spm_dyntc_count reads it. Updates change the
+ * handle's graph as spm_graph_delete_edges / spm_graph_add_edges do. */
+int spm_dyntc_init(spm_graph* g);
+/* One batch of n updates: type[i] is 'a' (add) or 'd' (delete). */
+int spm_dyntc_batch(spm_graph* g, const char* type, const int32_t* src, const int32_t* dst, int32_t n);
+int spm_dyntc_count(spm_graph* g, int64_t* triangleCount);
+/* The whole Dynamic function, updates in batches of batchSize. */
+int spm_dyn_tc(spm_graph* g, const char* type, const int32_t* usrc, const int32_t* udst, int32_t n,
+               int32_t batchSize, int64_t* triangleCount);
+
 /* ---- schedules -----------------------------------------------------------
  * The schedule changes how work maps to lanes, never the arithmetic. No
  * counterpart in the reference (its CUDA backend hard-codes 512 threads per

@@ -1,7 +1,8 @@
 // oracle.cpp -- TEST INFRASTRUCTURE ONLY. CPU restatement of the OpenMP code
 // that StarPlat's OMP backend (src/backends/backend_omp/dsl_cpp_generator.cpp)
-// emits for the four hot-path DSL programs and for the dynamic SSSP program
-// (dynamicDSLCodes/dynamicBatchSSSP), plus independent textbook routines that
+// emits for the four hot-path DSL programs and for the dynamic SSSP and TC
+// programs (dynamicDSLCodes/dynamicBatchSSSP, dynamicBatchTCV2), plus
+// independent textbook routines that
 // pin them. Only the test suite, the smoke check and the CPU
 // baseline leg of the benchmark load this library; the product path never
 // does.
@@ -35,6 +36,7 @@
 #include <cstring>
 #include <cmath>
 #include <queue>
+#include <unordered_set>
 #include <vector>
 
 namespace {
@@ -578,6 +580,80 @@ int oracle_dyn_sssp(int32_t V, const int32_t* off, const int32_t* adj, const int
     dist_out[v] = kd(dp[v]);
     parent_out[v] = uint32_t(dp[v]) == uint32_t(NOP) ? -1 : int32_t(uint32_t(dp[v]));
   }
+  return batches;
+}
+
+// Restatement of the Dynamic function DynTC of
+// graphcode/dynamicDSLCodes/dynamicBatchTCV2 (staticTC :1-21, Incremental
+// :23-78, Decremental :81-135, DynTC :137-187), sequential, over an edge
+// multiset in graph::getNeighbors order (CSR part, then additions). Per
+// batch of `batch` updates:
+//   OnDelete      every deletion (s,t) marks edge (s,t) modified_del
+//                 (graph::getEdge returns the first copy, so a pair is marked
+//                 or not as a whole; an unmarked pair reads false)
+//   Decremental   on the graph before the deletions: for every deletion
+//                 (v1,v2) with v1 != v2, every neighbour occurrence v3 of v1
+//                 with v3 != v1, v2 that is a neighbour of v2 (check_if_nbr)
+//                 adds 1 to count[1 + mod(v1,v3) + mod(v2,v3)];
+//                 tc -= count1/2 + count2/4 + count3/6 (integer divisions)
+//   updateCSRDel  every copy of each deleted edge goes
+//   updateCSRAdd  one copy of each added edge is appended
+//   OnAdd         every addition marks its edge modified_add
+//   Incremental   the same count over the additions on the updated graph;
+//                 tc += count1/2 + count2/4 + count3/6
+// tc starts at staticTC, the static triangle sum. counts_out[b] receives tc
+// after batch b (the DSL keeps it in an int; 64 bits here). plant 1: the
+// three-edge class divides by 3; plant 2: the mark of (v2,v3) is not read.
+// batch_seconds (may be null) receives the wall time of the batch loop.
+// Returns the number of batches.
+int32_t oracle_dyn_tc(int32_t V, const int32_t* off, const int32_t* adj, int32_t n, const char* type,
+                      const int32_t* us, const int32_t* ut, int32_t batch, int64_t* counts_out, int plant,
+                      double* batch_seconds) {
+  std::vector<std::vector<int32_t>> out(V);
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t e = off[v]; e < off[v + 1]; ++e) out[v].push_back(adj[e]);
+  int64_t tc = oracle_compute_tc(V, off, adj, 0, 0);
+  auto pair = [](int32_t a, int32_t b) { return (uint64_t(uint32_t(a)) << 32) | uint32_t(b); };
+  auto is_nbr = [&](int32_t a, int32_t b) {
+    for (int32_t x : out[a])
+      if (x == b) return true;
+    return false;
+  };
+  // the count of one phase over its updates, against the marked pairs
+  auto count = [&](const std::vector<std::pair<int32_t, int32_t>>& ups) {
+    std::unordered_set<uint64_t> marked;
+    for (auto [s, t] : ups)
+      if (is_nbr(s, t)) marked.insert(pair(s, t));
+    int64_t c[4] = {0, 0, 0, 0};
+    for (auto [v1, v2] : ups) {
+      if (v1 == v2) continue;
+      for (int32_t v3 : out[v1]) {
+        if (v3 == v1 || v3 == v2) continue;
+        if (!is_nbr(v2, v3)) continue;
+        int newEdge = 1;
+        if (marked.count(pair(v1, v3))) ++newEdge;
+        if (plant != 2 && marked.count(pair(v2, v3))) ++newEdge;
+        ++c[newEdge];
+      }
+    }
+    return c[1] / 2 + c[2] / 4 + c[3] / (plant == 1 ? 3 : 6);
+  };
+  int32_t batches = 0;
+  const double t_start = omp_get_wtime();
+  for (int32_t k = 0; k < n; k += batch, ++batches) {
+    const int32_t m = std::min(batch, n - k);
+    std::vector<std::pair<int32_t, int32_t>> dels, adds;
+    for (int32_t i = k; i < k + m; ++i) (type[i] == 'd' ? dels : adds).push_back({us[i], ut[i]});
+    tc -= count(dels);
+    for (auto [s, t] : dels) {
+      auto& o = out[s];
+      o.erase(std::remove(o.begin(), o.end(), t), o.end());
+    }
+    for (auto [s, t] : adds) out[s].push_back(t);
+    tc += count(adds);
+    if (counts_out) counts_out[batches] = tc;
+  }
+  if (batch_seconds) *batch_seconds = omp_get_wtime() - t_start;
   return batches;
 }
 

@@ -41,6 +41,9 @@ def lib():
         L.oracle_dyn_sssp.argtypes = [i32, P_i32, P_i32, P_i32, i32, i32, C.c_char_p, P_i32, P_i32, i32, P_i32,
                                       P_i32, C.c_int, P_f64]
         L.oracle_dyn_sssp.restype = C.c_int
+        L.oracle_dyn_tc.argtypes = [i32, P_i32, P_i32, i32, C.c_char_p, P_i32, P_i32, i32, C.POINTER(C.c_int64),
+                                    C.c_int, P_f64]
+        L.oracle_dyn_tc.restype = C.c_int32
         L.oracle_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -141,6 +144,25 @@ def dyn_sssp(g, weight, src, types, us, ut, batch, plant=0, timing=None):
     if rc < 0:
         raise RuntimeError("oracle_dyn_sssp: a loop did not converge")
     return dist, parent
+
+
+def dyn_tc(g, types, us, ut, batch, plant=0, timing=None):
+    """DynTC restated (oracle.cpp): the triangle count after every batch of
+    `batch` updates; types is a bytes string of 'a' / 'd'. With a dict as
+    `timing`, timing["batch_seconds"] receives the batch loop's wall time."""
+    us = np.ascontiguousarray(us, np.int32)
+    ut = np.ascontiguousarray(ut, np.int32)
+    t = bytes(types)
+    assert len(t) == len(us) == len(ut)
+    nb = (len(us) + batch - 1) // batch
+    counts = np.zeros(max(nb, 1), np.int64)
+    secs = f64(0.0)
+    got = lib().oracle_dyn_tc(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), len(us), t, _p(us, i32),
+                              _p(ut, i32), int(batch), _p(counts, C.c_int64), int(plant), C.byref(secs))
+    if timing is not None:
+        timing["batch_seconds"] = secs.value
+    assert got == nb
+    return counts[:nb]
 
 
 def apply_updates(V, off, adj, w, ops, plant=0):

@@ -98,6 +98,10 @@ DEVICE_API = [
     ("spm_dynsssp_result", C.c_int, [vp, P_i32, P_i32]),
     ("spm_dynsssp_device", C.c_int, [vp, vp, vp]),
     ("spm_dyn_sssp", C.c_int, [vp, i32, C.c_char_p, P_i32, P_i32, i32, i32, P_i32, P_i32]),
+    ("spm_dyntc_init", C.c_int, [vp]),
+    ("spm_dyntc_batch", C.c_int, [vp, C.c_char_p, P_i32, P_i32, i32]),
+    ("spm_dyntc_count", C.c_int, [vp, P_i64]),
+    ("spm_dyn_tc", C.c_int, [vp, C.c_char_p, P_i32, P_i32, i32, i32, P_i64]),
     ("spm_get_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
     ("spm_set_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
     ("spm_default_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),

@@ -75,3 +75,18 @@ def DynSSSP(g, dist, parent, weight, updateBatch, batchSize, src):
     dist[:] = dd
     parent[:] = pp
     g.refresh_from_device()
+
+
+def DynTC(g, updateBatch, batchSize):
+    """The Dynamic function of graphcode/dynamicDSLCodes/dynamicBatchTCV2:137-187:
+    staticTC, then per batch of batchSize updates OnDelete + Decremental on the
+    graph before the deletions, updateCSRDel, updateCSRAdd, OnAdd +
+    Incremental. updateBatch: a list of ``updates.update``. g's arrays then
+    describe the updated graph. The DSL function returns nothing; the
+    triangle count it keeps is returned here."""
+    from .updates import as_arrays
+    dev = g.device()
+    types, s, d, _ = as_arrays(list(updateBatch))
+    count = dev.dyn_tc(types, s, d, int(batchSize))
+    g.refresh_from_device()
+    return count

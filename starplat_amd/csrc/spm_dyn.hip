@@ -2,8 +2,9 @@
 // for gfx950: the replacement of graph::updateCSRDel / graph::updateCSRAdd
 // (graphcode/graph.hpp:869-975, over graph_ompv2.hpp:13-454) and of the code
 // StarPlat generates for the Dynamic / Decremental / Incremental functions of
-// graphcode/dynamicDSLCodes/dynamicBatchSSSP (batch loop, OnDelete, OnAdd:
-// backend_omp/dsl_dyn_cpp_generator.cpp:44-160, update calls :301-316).
+// graphcode/dynamicDSLCodes/dynamicBatchSSSP and dynamicBatchTCV2 (batch
+// loop, OnDelete, OnAdd: backend_omp/dsl_dyn_cpp_generator.cpp:44-160,
+// update calls :301-316).
 //
 // Layout. The handle's CSR and reverse CSR stay where they are and stay
 // sorted. A deletion sets a bit in a removed-slot bitset per direction (one
@@ -100,6 +101,13 @@ struct DynWork {
   int32_t* list = nullptr;           // [V]
   int32_t* nsel = nullptr;           // [1]
   int32_t src = -1;
+  // dynamic TC
+  unsigned long long* tset = nullptr;  // [tset_cap] the phase's (s << 32 | t) pairs, open addressing
+  size_t tset_cap = 0;
+  int32_t* tchunk = nullptr;           // [tchunk_cap] work-item prefix over the phase's updates
+  size_t tchunk_cap = 0;
+  unsigned long long* tcv = nullptr;   // [4] class counts 1..3 of the phase, triangle count
+  bool tc_ready = false;
 };
 
 static bool dyn_dirty(const spm_graph* g) {
@@ -290,6 +298,150 @@ __global__ __launch_bounds__(kDynBlock) void dyn_build_rw(int32_t V, const int32
       if (lo + k < roff[v + 1] && radj[lo + k] == int32_t(u)) rw[lo + k] = w[j];
     }
   }
+}
+
+// ---- dynamic TC ----------------------------------------------------------------
+// The Decremental / Incremental functions of dynamicBatchTCV2 (:23-135): for
+// every update (v1, v2) of the phase with v1 != v2, every neighbour
+// occurrence v3 of v1 (CSR row minus removed slots, then the live overflow
+// entries, as graph::getNeighbors lists them) with v3 != v1, v2 that is a
+// neighbour of v2 adds one to class 1 + [(v1,v3) in the phase] +
+// [(v2,v3) in the phase]; the count moves by class1/2 + class2/4 + class3/6.
+// (The DSL marks modified_del / modified_add through graph::getEdge, which
+// returns the first copy of a pair, so a pair is marked or not as a whole;
+// the counting loop only asks about pairs that are edges, so "marked" is
+// "in the phase's updates".) Work items are 256 neighbour positions of one
+// update, so a hub's row is spread over many wavefronts.
+constexpr int kDtcChunk = 256;
+constexpr unsigned long long kSetEmpty = ~0ull;
+
+__device__ __forceinline__ uint32_t dtc_hash(unsigned long long k) {
+  k ^= k >> 31;
+  k *= 0x9E3779B97F4A7C15ull;
+  return uint32_t(k >> 32);
+}
+
+__global__ void dtc_set_insert(int32_t n, const int32_t* __restrict__ us, const int32_t* __restrict__ ut,
+                               unsigned long long* __restrict__ set, uint32_t mask) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned long long key = (static_cast<unsigned long long>(uint32_t(us[i])) << 32) | uint32_t(ut[i]);
+  uint32_t h = dtc_hash(key) & mask;
+  for (;;) {
+    const unsigned long long prev = atomicCAS(&set[h], kSetEmpty, key);
+    if (prev == kSetEmpty || prev == key) return;
+    h = (h + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ bool dtc_set_has(const unsigned long long* __restrict__ set, uint32_t mask, int32_t a,
+                                            int32_t b) {
+  const unsigned long long key = (static_cast<unsigned long long>(uint32_t(a)) << 32) | uint32_t(b);
+  uint32_t h = dtc_hash(key) & mask;
+  for (;;) {
+    const unsigned long long k = set[h];
+    if (k == key) return true;
+    if (k == kSetEmpty) return false;
+    h = (h + 1) & mask;
+  }
+}
+
+// Work items per update: ceil(neighbour positions of v1 / kDtcChunk), none
+// for v1 == v2 (the DSL's filter).
+__global__ void dtc_chunks(int32_t n, const int32_t* __restrict__ us, const int32_t* __restrict__ ut,
+                           const int32_t* __restrict__ off, DiffRows d, int32_t* __restrict__ cnt) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) {
+    cnt[n] = 0;
+    return;
+  }
+  const int32_t v1 = us[i];
+  const int32_t len = v1 == ut[i] ? 0 : (off[v1 + 1] - off[v1]) + (d.off[v1 + 1] - d.off[v1]);
+  cnt[i] = (len + kDtcChunk - 1) / kDtcChunk;
+}
+
+// Is v3 a live neighbour of v2: binary search in the sorted CSR row (every
+// copy of a pair is removed together, so one live copy decides), then the
+// overflow row.
+__device__ __forceinline__ bool dtc_is_nbr(int32_t v2, int32_t v3, const int32_t* __restrict__ off,
+                                           const int32_t* __restrict__ adj, const uint32_t* __restrict__ dead,
+                                           const DiffRows& d) {
+  int32_t lo = off[v2], hi = off[v2 + 1];
+  const int32_t end = hi;
+  while (lo < hi) {
+    const int32_t mid = lo + ((hi - lo) >> 1);
+    if (adj[mid] < v3) lo = mid + 1;
+    else hi = mid;
+  }
+  for (int32_t j = lo; j < end && adj[j] == v3; ++j)
+    if (!bit_of(dead, j)) return true;
+  for (int32_t j = d.off[v2]; j < d.off[v2 + 1]; ++j)
+    if (d.adj[j] == v3 && !bit_of(d.dead, j)) return true;
+  return false;
+}
+
+__global__ __launch_bounds__(kDynBlock) void dtc_count(int32_t n, const int32_t* __restrict__ us,
+                                                       const int32_t* __restrict__ ut,
+                                                       const int32_t* __restrict__ ioff,
+                                                       const int32_t* __restrict__ off,
+                                                       const int32_t* __restrict__ adj,
+                                                       const uint32_t* __restrict__ dead, DiffRows d,
+                                                       const unsigned long long* __restrict__ set, uint32_t mask,
+                                                       unsigned long long* __restrict__ cls) {
+  const int lane = __lane_id();
+  const int64_t nw = int64_t(gridDim.x) * kDynWaves;
+  const int64_t total = ioff[n];
+  uint32_t c1 = 0, c2 = 0, c3 = 0;
+  for (int64_t x = int64_t(blockIdx.x) * kDynWaves + threadIdx.x / kWave; x < total; x += nw) {
+    int32_t lo = 0, hi = n - 1;  // the update owning item x: last i with ioff[i] <= x
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (ioff[mid] <= x) lo = mid;
+      else hi = mid - 1;
+    }
+    const int32_t v1 = us[lo], v2 = ut[lo];
+    const int64_t base = (x - ioff[lo]) * int64_t(kDtcChunk);
+    const int32_t a = off[v1], la = off[v1 + 1] - a;
+    const int32_t da = d.off[v1], len = la + (d.off[v1 + 1] - da);
+#pragma unroll
+    for (int q = 0; q < kDtcChunk / kWave; ++q) {
+      const int64_t p = base + q * kWave + lane;
+      if (p >= len) continue;
+      int32_t v3;
+      bool live;
+      if (p < la) {
+        const int32_t j = a + int32_t(p);
+        live = !bit_of(dead, j);
+        v3 = adj[j];
+      } else {
+        const int32_t j = da + int32_t(p - la);
+        live = !bit_of(d.dead, j);
+        v3 = d.adj[j];
+      }
+      if (!live || v3 == v1 || v3 == v2) continue;
+      if (!dtc_is_nbr(v2, v3, off, adj, dead, d)) continue;
+      const int k = 1 + int(dtc_set_has(set, mask, v1, v3)) + int(dtc_set_has(set, mask, v2, v3));
+      c1 += k == 1;
+      c2 += k == 2;
+      c3 += k == 3;
+    }
+  }
+  const unsigned long long s1 = wave_sum(static_cast<unsigned long long>(c1));
+  const unsigned long long s2 = wave_sum(static_cast<unsigned long long>(c2));
+  const unsigned long long s3 = wave_sum(static_cast<unsigned long long>(c3));
+  if (lane == 0) {
+    if (s1) atomicAdd(&cls[0], s1);
+    if (s2) atomicAdd(&cls[1], s2);
+    if (s3) atomicAdd(&cls[2], s3);
+  }
+}
+
+// count += sign * (class1/2 + class2/4 + class3/6); clears the classes.
+__global__ void dtc_apply(unsigned long long* tcv, int sign) {
+  const long long delta = static_cast<long long>(tcv[0] / 2 + tcv[1] / 4 + tcv[2] / 6);
+  tcv[3] = static_cast<unsigned long long>(static_cast<long long>(tcv[3]) + sign * delta);
+  tcv[0] = tcv[1] = tcv[2] = 0ull;
 }
 
 // ---- dynamic SSSP ------------------------------------------------------------
@@ -652,6 +804,9 @@ int dyn_release(spm_graph* g) {
   dfree(g, w->fl[1], V);
   dfree(g, w->list, sizeof(int32_t) * V);
   dfree(g, w->nsel, sizeof(int32_t));
+  dfree(g, w->tset, sizeof(unsigned long long) * w->tset_cap);
+  dfree(g, w->tchunk, sizeof(int32_t) * w->tchunk_cap);
+  dfree(g, w->tcv, sizeof(unsigned long long) * 4);
   delete w;
   g->dyn = nullptr;
   return SPM_OK;
@@ -1019,6 +1174,88 @@ static int dsp_batch(spm_graph* g, const char* type, const int32_t* us, const in
   return prof_flush(g);
 }
 
+// ---- dynamic TC, host side ---------------------------------------------------
+
+// One counting phase (Decremental with sign -1, Incremental with +1) over the
+// staged updates (s, t), on the graph as it stands.
+static int dtc_phase(spm_graph* g, const int32_t* s, const int32_t* t, int32_t n, int sign) {
+  DynWork* w = g->dyn;
+  if (n == 0) return SPM_OK;
+  size_t cap = 64;
+  while (cap < 2 * size_t(n)) cap <<= 1;
+  if (cap > w->tset_cap) {
+    dfree(g, w->tset, sizeof(unsigned long long) * w->tset_cap);
+    w->tset = nullptr;
+    w->tset_cap = 0;
+    SPM_TRY(dmalloc(g, (void**)&w->tset, sizeof(unsigned long long) * cap, "dynamic TC pair set"));
+    w->tset_cap = cap;
+  }
+  if (size_t(n) + 1 > w->tchunk_cap) {
+    dfree(g, w->tchunk, sizeof(int32_t) * w->tchunk_cap);
+    w->tchunk = nullptr;
+    w->tchunk_cap = 0;
+    const size_t c = std::max<size_t>(size_t(n) + 1, 4097);
+    SPM_TRY(dmalloc(g, (void**)&w->tchunk, sizeof(int32_t) * c, "dynamic TC work items"));
+    w->tchunk_cap = c;
+  }
+  const uint32_t mask = uint32_t(cap - 1);
+  const DiffRows d = w->out[w->cur].r;
+  SPM_CHECK_HIP(hipMemsetAsync(w->tset, 0xFF, sizeof(unsigned long long) * cap, g->stream));
+  const int ugrid = int((int64_t(n) + kDynBlock) / kDynBlock);  // n + 1 threads for dtc_chunks
+  SPM_TRY(launch(g, "dtc_set_insert", dtc_set_insert, dim3(ugrid), dim3(kDynBlock), 0, n, s, t, w->tset, mask));
+  SPM_TRY(launch(g, "dtc_chunks", dtc_chunks, dim3(ugrid), dim3(kDynBlock), 0, n, s, t, (const int32_t*)g->off, d,
+                 w->tchunk));
+  size_t need = 0;
+  SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, w->tchunk, w->tchunk, n + 1, g->stream));
+  SPM_TRY(tmp_reserve(g, need));
+  SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(w->tmp, need, w->tchunk, w->tchunk, n + 1, g->stream));
+  SPM_TRY(launch(g, "dtc_count", dtc_count, dim3(g->num_cus * 8), dim3(kDynBlock), 0, n, s, t,
+                 (const int32_t*)w->tchunk, (const int32_t*)g->off, (const int32_t*)g->adj,
+                 (const uint32_t*)w->dead, d, (const unsigned long long*)w->tset, mask, w->tcv));
+  return launch(g, "dtc_apply", dtc_apply, dim3(1), dim3(1), 0, w->tcv, sign);
+}
+
+static int dtc_init(spm_graph* g) {
+  uint64_t count = 0;
+  SPM_TRY(tc_run(g, &count));  // staticTC, on the folded graph
+  SPM_TRY(dyn_alloc(g));
+  DynWork* w = g->dyn;
+  if (!w->tcv) SPM_TRY(dmalloc(g, (void**)&w->tcv, sizeof(unsigned long long) * 4, "dynamic TC counters"));
+  const unsigned long long h[4] = {0ull, 0ull, 0ull, static_cast<unsigned long long>(count)};
+  SPM_CHECK_HIP(hipMemcpyAsync(w->tcv, h, sizeof h, hipMemcpyHostToDevice, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  w->tc_ready = true;
+  return SPM_OK;
+}
+
+// One batch of DynTC (dynamicBatchTCV2:137-187): OnDelete + Decremental on the
+// graph before the deletions, updateCSRDel, updateCSRAdd, OnAdd + Incremental.
+static int dtc_batch(spm_graph* g, const char* type, const int32_t* us, const int32_t* ut, int32_t n) {
+  DynWork* w = g->dyn;
+  if (!w || !w->tc_ready) return set_error(SPM_EINVAL, "dynamic TC: call spm_dyntc_init first");
+  std::vector<int32_t> ds, dt, as, at;
+  for (int32_t i = 0; i < n; ++i) {
+    if (type[i] == 'd') { ds.push_back(us[i]); dt.push_back(ut[i]); }
+    else if (type[i] == 'a') { as.push_back(us[i]); at.push_back(ut[i]); }
+    else return set_error(SPM_EINVAL, "dynamic TC: update type must be 'a' or 'd'");
+  }
+  if (!ds.empty()) {
+    int32_t *s = nullptr, *t = nullptr, *x = nullptr;
+    const int32_t nd = int32_t(ds.size());
+    SPM_TRY(stage(g, ds.data(), dt.data(), nullptr, nd, &s, &t, &x));
+    SPM_TRY(dtc_phase(g, s, t, nd, -1));
+    SPM_TRY(delete_staged(g, s, t, nd));
+  }
+  if (!as.empty()) {
+    int32_t *s = nullptr, *t = nullptr, *x = nullptr;
+    const int32_t na = int32_t(as.size());
+    SPM_TRY(stage(g, as.data(), at.data(), nullptr, na, &s, &t, &x));
+    SPM_TRY(add_staged(g, s, t, nullptr, na));
+    SPM_TRY(dtc_phase(g, s, t, na, +1));
+  }
+  return prof_flush(g);
+}
+
 }  // namespace spm
 
 using namespace spm;
@@ -1107,6 +1344,38 @@ int spm_dynsssp_result(spm_graph* g, int32_t* dist, int32_t* parent) {
   }
   dfree(g, d, sizeof(int32_t) * 2 * size_t(g->V));
   return rc;
+}
+
+int spm_dyntc_init(spm_graph* g) {
+  if (!g) return set_error(SPM_EINVAL, "spm_dyntc_init: NULL handle");
+  return dtc_init(g);
+}
+
+int spm_dyntc_batch(spm_graph* g, const char* type, const int32_t* src, const int32_t* dst, int32_t n) {
+  if (!g || n < 0 || (n > 0 && (!type || !src || !dst))) return set_error(SPM_EINVAL, "spm_dyntc_batch: bad argument");
+  return dtc_batch(g, type, src, dst, n);
+}
+
+int spm_dyntc_count(spm_graph* g, int64_t* triangleCount) {
+  if (!g || !triangleCount) return set_error(SPM_EINVAL, "spm_dyntc_count: NULL argument");
+  DynWork* w = g->dyn;
+  if (!w || !w->tc_ready) return set_error(SPM_EINVAL, "spm_dyntc_count: call spm_dyntc_init first");
+  unsigned long long c = 0;
+  SPM_CHECK_HIP(hipMemcpyAsync(&c, w->tcv + 3, sizeof c, hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  *triangleCount = static_cast<int64_t>(c);
+  return SPM_OK;
+}
+
+int spm_dyn_tc(spm_graph* g, const char* type, const int32_t* usrc, const int32_t* udst, int32_t n, int32_t batchSize,
+               int64_t* triangleCount) {
+  if (!g || n < 0 || batchSize < 1 || !triangleCount) return set_error(SPM_EINVAL, "spm_dyn_tc: bad argument");
+  SPM_TRY(spm_dyntc_init(g));
+  for (int32_t k = 0; k < n; k += batchSize) {
+    const int32_t m = std::min(batchSize, n - k);
+    SPM_TRY(spm_dyntc_batch(g, type + k, usrc + k, udst + k, m));
+  }
+  return spm_dyntc_count(g, triangleCount);
 }
 
 int spm_dyn_sssp(spm_graph* g, int32_t src, const char* type, const int32_t* usrc, const int32_t* udst, int32_t n,

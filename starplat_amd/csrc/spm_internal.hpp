@@ -103,6 +103,8 @@ inline int launch(spm_graph* g, const char* name, Kernel k, dim3 grid, dim3 bloc
 }
 
 int pr_release(spm_graph* g);
+// Static triangle count (Compute_TC) of the handle's graph, updates folded first.
+int tc_run(spm_graph* g, uint64_t* out);
 int sssp_release(spm_graph* g);
 int tc_release(spm_graph* g);
 int bc_release(spm_graph* g);

@@ -143,6 +143,29 @@ class DeviceGraph:
             N.ptr(dist, C.c_int32), N.ptr(parent, C.c_int32)))
         return dist, parent
 
+    def dyntc_init(self):
+        N.check("spm_dyntc_init", N.device_lib().spm_dyntc_init(self.h))
+
+    def dyntc_batch(self, types, src, dst):
+        src, dst = self._pair(src, dst)
+        t = self._types(types, len(src))
+        N.check("spm_dyntc_batch", N.device_lib().spm_dyntc_batch(
+            self.h, t, N.ptr(src, C.c_int32), N.ptr(dst, C.c_int32), len(src)))
+
+    def dyntc_count(self):
+        c = C.c_int64(0)
+        N.check("spm_dyntc_count", N.device_lib().spm_dyntc_count(self.h, C.byref(c)))
+        return c.value
+
+    def dyn_tc(self, types, usrc, udst, batch_size):
+        """The whole Dynamic function (spm_dyn_tc): the final triangle count."""
+        usrc, udst = self._pair(usrc, udst)
+        t = self._types(types, len(usrc))
+        c = C.c_int64(0)
+        N.check("spm_dyn_tc", N.device_lib().spm_dyn_tc(
+            self.h, t, N.ptr(usrc, C.c_int32), N.ptr(udst, C.c_int32), len(usrc), int(batch_size), C.byref(c)))
+        return c.value
+
     # ---- schedules ----------------------------------------------------------
     def schedule(self):
         s = N.SpmSchedule()

@@ -249,3 +249,105 @@ def test_full_dyn_sssp_rmat22(oracle):
     np.testing.assert_array_equal(dist[reach], static[reach])
     assert np.all(dist[~reach] >= INF2)
     print(f"DynSSSP V={g.V}: reachable {int(reach.sum())}, parity with restatement and static recompute")
+
+
+# ---------------------------------------------------------------- DynTC
+@pytest.mark.parametrize("undirected", [True, False])
+@pytest.mark.parametrize("batch", [1, 64, 5000])
+def test_dyn_tc_matches_oracle(oracle, undirected, batch):
+    """Per-batch triangle counts equal the restatement's, for random updates
+    that include absent deletions, duplicate additions and self loops."""
+    g = graph.rmat(11, 8, seed=5, undirected=undirected)
+    types, us, ut = random_updates(g, 3000 if batch > 1 else 300, seed=batch, symmetric=undirected)
+    ref = oracle.dyn_tc(g, types, us, ut, batch)
+    dev = g.device()
+    dev.dyntc_init()
+    t = np.frombuffer(types, np.uint8)
+    for b, k in enumerate(range(0, len(us), batch)):
+        dev.dyntc_batch(bytes(t[k:k + batch]), us[k:k + batch], ut[k:k + batch])
+        assert dev.dyntc_count() == ref[b], b
+
+
+def test_dyn_tc_valid_batches_match_recount(oracle):
+    """Valid symmetric batches on a simple graph: after every batch the count
+    equals a recount of the graph the updates leave (on the GPU: the static
+    count of the folded graph), and the restatement's."""
+    from test_oracle_dyn import er_edges, triangles, undirected_csr, valid_batches
+    V = 2000
+    edges = er_edges(V, 24000, seed=1)
+    g0 = undirected_csr(V, edges)
+    src = np.repeat(np.arange(V, dtype=np.int32), np.diff(g0.indexofNodes))
+    g = graph.from_edges(src, g0.edgeList)
+    assert g.V == V
+    np.testing.assert_array_equal(g.edgeList, g0.edgeList)
+    types, us, ut, states = valid_batches(V, edges, 4, 600, seed=5)
+    ref = oracle.dyn_tc(g0, types, us, ut, 1200)
+    want = [triangles(V, st) for st in states]
+    assert ref.tolist() == want
+    dev = g.device()
+    dev.dyntc_init()
+    t = np.frombuffer(types, np.uint8)
+    for b, k in enumerate(range(0, len(us), 1200)):
+        dev.dyntc_batch(bytes(t[k:k + 1200]), us[k:k + 1200], ut[k:k + 1200])
+        assert dev.dyntc_count() == want[b]
+    assert dev.tc() == want[-1]  # the static count of the updated graph
+
+
+def test_dyn_tc_python_mirror(oracle, tmp_path):
+    """DynTC(g, parseUpdateFile(...), batchSize): the count, and g's arrays are
+    the updated graph (updates applied as the edge multiset the oracle keeps)."""
+    from starplat_amd import DynTC
+    g = graph.rmat(10, 8, seed=2, undirected=True)
+    off0, adj0 = g.indexofNodes.copy(), g.edgeList.copy()
+    types, us, ut = random_updates(g, 400, seed=3)
+    f = tmp_path / "u.txt"
+    write_update_file(f, types, us, ut)
+    ups = parseUpdateFile(str(f))
+    t = "".join(u.type for u in ups).encode()
+    s = np.array([u.source for u in ups], np.int32)
+    d = np.array([u.destination for u in ups], np.int32)
+    ref = oracle.dyn_tc(_G(g.V, off0, adj0, np.ones(len(adj0), np.int32)), t, s, d, 50)
+    assert DynTC(g, ups, 50) == ref[-1]
+    want = oracle.apply_updates(g.V, off0, adj0, np.ones(len(adj0), np.int32), batch_ops(t, s, d, 50))
+    np.testing.assert_array_equal(g.indexofNodes, want[0])
+
+
+@pytest.mark.slow
+def test_full_dyn_tc_rmat21():
+    """The TC configuration graph (undirected RMAT-21) with two batches of
+    65,536 directed updates (valid and symmetric): the incremental count equals
+    the static count of the updated graph."""
+    g = graph.rmat(21, 16, seed=3, undirected=True)
+    off, adj = g.indexofNodes, g.edgeList
+    rng = np.random.default_rng(9)
+    src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(off))
+
+    def is_edge(a, c):
+        row = adj[off[a]:off[a + 1]]
+        i = np.searchsorted(row, c)
+        return i < len(row) and row[i] == c
+
+    dev = g.device()
+    dev.dyntc_init()
+    assert dev.dyntc_count() == dev.tc()
+    touched = set()  # each pair is deleted or added at most once
+    for b in range(2):
+        ts, us, ut = [], [], []
+        for i in rng.choice(g.E, 16384 * 3, replace=False).tolist():
+            a, c = int(src_of[i]), int(adj[i])
+            key = (min(a, c), max(a, c))
+            if a == c or key in touched:
+                continue
+            touched.add(key)
+            ts.append("dd"), us.extend([a, c]), ut.extend([c, a])
+            if len(us) >= 32768:
+                break
+        while len(us) < 65536:
+            a, c = int(rng.integers(0, g.V)), int(rng.integers(0, g.V))
+            key = (min(a, c), max(a, c))
+            if a == c or key in touched or is_edge(a, c):
+                continue
+            touched.add(key)
+            ts.append("aa"), us.extend([a, c]), ut.extend([c, a])
+        dev.dyntc_batch("".join(ts).encode(), np.array(us, np.int32), np.array(ut, np.int32))
+    assert dev.dyntc_count() == dev.tc()

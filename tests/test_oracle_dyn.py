@@ -177,3 +177,129 @@ def test_dyn_sssp_plants_are_caught(oracle):
         dist, parent = oracle.dyn_sssp(g, g.edgeLen, 0, types, us, ut, 50, plant=plant)
         with pytest.raises(AssertionError):
             check_against_dijkstra(oracle, g, types, us, ut, 50, dist, parent, 0)
+
+
+# ---------------------------------------------------------------- DynTC
+def undirected_csr(V, edges):
+    """Symmetric CSR (sorted rows) of a simple undirected edge set."""
+    pairs = sorted({(a, b) for a, b in edges} | {(b, a) for a, b in edges})
+    off = np.zeros(V + 1, np.int32)
+    for a, _ in pairs:
+        off[a + 1] += 1
+    off = np.cumsum(off).astype(np.int32)
+    adj = np.array([b for _, b in pairs], np.int32)
+    return _G(V, off, adj, np.ones(len(adj), np.int32))
+
+
+def triangles(V, edges):
+    A = np.zeros((V, V), np.int64)
+    for a, b in edges:
+        A[a, b] = A[b, a] = 1
+    return int(np.trace(A @ A @ A)) // 6
+
+
+def valid_batches(V, edges, nbatch, per_batch, seed, whole_triangle=True):
+    """Batches of undirected updates, each pushed as two directed updates (as
+    parseUpdateFile does): deletions of current edges, additions of absent
+    pairs, no pair twice in a batch. The first batch deletes all three edges
+    of one triangle and two edges of another (when per_batch allows), so
+    every count class occurs. Every batch holds exactly per_batch undirected
+    updates. Returns (types, us, ut, edge set after each batch)."""
+    rng = np.random.default_rng(seed)
+    cur = set(edges)
+    types, us, ut, states = [], [], [], []
+
+    def push(k, a, b):
+        types.extend([k, k]), us.extend([a, b]), ut.extend([b, a])
+
+    for bi in range(nbatch):
+        dels, adds, used = [], [], set()
+        if bi == 0 and whole_triangle and per_batch >= 5:
+            adjs = {v: set() for v in range(V)}
+            for a, b in cur:
+                adjs[a].add(b), adjs[b].add(a)
+            tris = [(a, b, c) for a, b in sorted(cur) for c in sorted(adjs[a] & adjs[b]) if c > max(a, b)]
+            (a, b, c), (x, y, z) = tris[0], tris[-1]
+            for e in [(a, b), (a, c), (b, c), (x, y), (x, z)]:
+                e = (min(e), max(e))
+                if e not in used:
+                    dels.append(e), used.add(e)
+        pool = sorted(cur)
+        while len(dels) + len(adds) < per_batch:
+            if rng.random() < 0.5 and pool:
+                e = pool[int(rng.integers(0, len(pool)))]
+                if e in used:
+                    continue
+                dels.append(e)
+            else:
+                a, b = int(rng.integers(0, V)), int(rng.integers(0, V))
+                e = (min(a, b), max(a, b))
+                if a == b or e in cur or e in used:
+                    continue
+                adds.append(e)
+            used.add(e)
+        for a, b in dels:
+            push("d", a, b)
+        for a, b in adds:
+            push("a", a, b)
+        cur = (cur - set(dels)) | set(adds)
+        states.append(set(cur))
+    return "".join(types).encode(), np.array(us, np.int32), np.array(ut, np.int32), states
+
+
+def er_edges(V, m, seed):
+    rng = np.random.default_rng(seed)
+    out = set()
+    while len(out) < m:
+        a, b = int(rng.integers(0, V)), int(rng.integers(0, V))
+        if a != b:
+            out.add((min(a, b), max(a, b)))
+    return out
+
+
+@pytest.mark.parametrize("per_batch", [1, 4, 32, 400])
+def test_dyn_tc_restatement_is_exact(oracle, per_batch):
+    """Symmetric, valid batches on a simple graph: after every batch the
+    incremental count equals a recount (trace(A^3)/6) of the graph the updates
+    leave -- each removed or added triangle is seen from 2, 4 or 6 directed
+    updates according to how many of its edges the batch holds."""
+    V = 120
+    edges = er_edges(V, 900, seed=per_batch)
+    g = undirected_csr(V, edges)
+    assert oracle.compute_tc(g) == triangles(V, edges)
+    nb = max(2, 1200 // max(per_batch, 1) // 10)
+    types, us, ut, states = valid_batches(V, edges, nb, per_batch, seed=7)
+    counts = oracle.dyn_tc(g, types, us, ut, 2 * per_batch)
+    assert [int(c) for c in counts] == [triangles(V, st) for st in states]
+
+
+def test_dyn_tc_hand_case(oracle):
+    """K4 has four triangles. Deleting {0,1} removes 012 and 013; adding it
+    back restores them. Deleting the three edges of 012 in one batch removes
+    every triangle (each of the others uses one of those edges)."""
+    V = 4
+    edges = {(a, b) for a in range(4) for b in range(a + 1, 4)}
+    g = undirected_csr(V, edges)
+    assert oracle.compute_tc(g) == 4
+    t = b"dd" + b"aa"
+    us = np.array([0, 1, 0, 1], np.int32)
+    ut = np.array([1, 0, 1, 0], np.int32)
+    # batch 1 deletes {0,1} (triangles 012, 013 go), batch 2 adds it back
+    assert oracle.dyn_tc(g, t, us, ut, 2).tolist() == [2, 4]
+    # one batch deletes {0,1}, {0,2}, {1,2}
+    t = b"dddddd"
+    us = np.array([0, 1, 0, 2, 1, 2], np.int32)
+    ut = np.array([1, 0, 2, 0, 2, 1], np.int32)
+    assert oracle.dyn_tc(g, t, us, ut, 6).tolist() == [triangles(4, edges - {(0, 1), (0, 2), (1, 2)})]
+    assert triangles(4, edges - {(0, 1), (0, 2), (1, 2)}) == 0
+
+
+def test_dyn_tc_plants_are_caught(oracle):
+    V = 60
+    edges = er_edges(V, 500, seed=3)
+    g = undirected_csr(V, edges)
+    types, us, ut, states = valid_batches(V, edges, 3, 40, seed=11)
+    want = [triangles(V, st) for st in states]
+    assert oracle.dyn_tc(g, types, us, ut, 80).tolist() == want
+    for plant in (1, 2):
+        assert oracle.dyn_tc(g, types, us, ut, 80, plant=plant).tolist() != want

@@ -320,6 +320,94 @@ def bench_dyn(args, O):
     return res
 
 
+def dyn_tc_batch(g, src_of, size, rng, touched):
+    """size directed updates as parseUpdateFile pushes them (each undirected
+    update and its reverse): half deletions of existing edges, half additions
+    of absent pairs, no pair touched twice over the whole run (so the count
+    stays comparable with a recount)."""
+    off, adj = g.indexofNodes, g.edgeList
+    ts, us, ut = [], [], []
+    for i in rng.choice(g.E, size * 2, replace=False).tolist():
+        a, c = int(src_of[i]), int(adj[i])
+        key = (min(a, c), max(a, c))
+        if a == c or key in touched:
+            continue
+        touched.add(key)
+        ts.append("dd"), us.extend([a, c]), ut.extend([c, a])
+        if len(us) >= size // 2:
+            break
+    while len(us) < size:
+        a, c = int(rng.integers(0, g.V)), int(rng.integers(0, g.V))
+        key = (min(a, c), max(a, c))
+        row = adj[off[a]:off[a + 1]]
+        j = np.searchsorted(row, c)
+        if a == c or key in touched or (j < len(row) and row[j] == c):
+            continue
+        touched.add(key)
+        ts.append("aa"), us.extend([a, c]), ut.extend([c, a])
+    return "".join(ts).encode(), np.array(us, np.int32), np.array(ut, np.int32)
+
+
+def bench_dyn_tc(args, O):
+    log(f"dynamic TC on undirected RMAT-{args.tc_scale}: {args.dyn_batches} batches of {args.dyn_batch} updates")
+    g = graph.rmat(args.tc_scale, 16, seed=3, undirected=True)
+    src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
+    rng = np.random.default_rng(6)
+    touched = set()
+    batches = [dyn_tc_batch(g, src_of, args.dyn_batch, rng, touched) for _ in range(args.dyn_batches + 1)]
+    dev = g.device()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    dev.dyntc_init()  # staticTC (orientation, hashed count)
+    t_init = (time.perf_counter() - t0) * 1e3
+    dev.dyntc_batch(*batches[0])  # warm-up (first allocation of the overflow rows and the pair set)
+    times = []
+    for b in batches[1:-1]:
+        hip.synchronize()
+        t0 = time.perf_counter()
+        dev.dyntc_batch(*b)
+        hip.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    kt = kernel_time(dev, lambda: dev.dyntc_batch(*batches[-1]),
+                     ["dtc_set_insert", "dtc_chunks", "dtc_count", "dtc_apply", "dyn_delete", "dyn_count",
+                      "dyn_difflen", "dyn_diffcopy", "dyn_place"], reps=1)
+    count = dev.dyntc_count()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    dev.compact()
+    t_fold = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    recount = dev.tc()  # re-orients the folded graph
+    t_static_first = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    dev.tc()
+    t_static = (time.perf_counter() - t0) * 1e3
+    ms = statistics.median(times)
+    res = {"workload": f"DynTC (dynamicBatchTCV2) on undirected RMAT scale {args.tc_scale} EF16 seed 3; batches of "
+                       f"{args.dyn_batch} directed updates (each undirected update and its reverse), half deletions "
+                       "of existing edges, half additions of absent pairs",
+           "V": g.V, "E": g.E, "batch": args.dyn_batch, "batches_timed": len(times),
+           "ms_per_batch": round(ms, 4), "updates_per_s": round(args.dyn_batch / (ms * 1e-3)),
+           "init_ms": round(t_init, 3), "fold_ms": round(t_fold, 3), "static_tc_first_ms": round(t_static_first, 3),
+           "static_tc_ms": round(t_static, 3), "triangles": count,
+           "count_equals_static_recount": bool(count == recount),
+           "speedup_vs_fold_plus_static": round((t_fold + t_static) / ms, 2),
+           "kernel_ms_one_batch": {k: round(v[0], 4) for k, v in kt.items() if v[1] > 0},
+           "roofline": None}
+    if not args.no_cpu_baseline:
+        tm = {}
+        half, k = args.dyn_batch // 2, 1024  # the batch holds its deletions, then its additions
+        b = batches[1]
+        sl = np.r_[0:k, half:half + k]
+        O.dyn_tc(g, bytes(np.frombuffer(b[0], np.uint8)[sl]), b[1][sl], b[2][sl], 2 * k, timing=tm, initial=0)
+        res["cpu_baseline"] = {"value": round(2 * k / tm["batch_seconds"]), "unit": "updates/s", "cores": 1,
+                               "kind": "port", "sample": f"the first {k} deletions and first {k} additions of one "
+                               "batch as one batch (sequential DynTC restatement, batch loop timed, static count "
+                               "skipped)", "seconds": round(tm["batch_seconds"], 4)}
+    g.release_device()
+    return res
+
+
 # --------------------------------------------------------------------------- schedules
 def bench_schedule(args):
     from starplat_amd import mixed_batch
@@ -340,7 +428,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--algos", default="sssp,pr,pr_c0,tc,bc,dyn,sched")
+    ap.add_argument("--algos", default="sssp,pr,pr_c0,tc,bc,dyn,dyn_tc,sched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--sssp-scale", type=int, default=22)
@@ -394,6 +482,8 @@ def main():
         per["bc"] = bench_bc(args, O)
     if "dyn" in algos:
         per["dyn"] = bench_dyn(args, O)
+    if "dyn_tc" in algos:
+        per["dyn_tc"] = bench_dyn_tc(args, O)
     sched = bench_schedule(args) if "sched" in algos else None
     if head is None and per:
         first = next(iter(per.values()))

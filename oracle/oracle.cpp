@@ -601,18 +601,19 @@ int oracle_dyn_sssp(int32_t V, const int32_t* off, const int32_t* adj, const int
 //   OnAdd         every addition marks its edge modified_add
 //   Incremental   the same count over the additions on the updated graph;
 //                 tc += count1/2 + count2/4 + count3/6
-// tc starts at staticTC, the static triangle sum. counts_out[b] receives tc
+// tc starts at staticTC, the static triangle sum (or at `initial` when that
+// is >= 0, to time batches without the static count). counts_out[b] receives tc
 // after batch b (the DSL keeps it in an int; 64 bits here). plant 1: the
 // three-edge class divides by 3; plant 2: the mark of (v2,v3) is not read.
 // batch_seconds (may be null) receives the wall time of the batch loop.
 // Returns the number of batches.
 int32_t oracle_dyn_tc(int32_t V, const int32_t* off, const int32_t* adj, int32_t n, const char* type,
-                      const int32_t* us, const int32_t* ut, int32_t batch, int64_t* counts_out, int plant,
-                      double* batch_seconds) {
+                      const int32_t* us, const int32_t* ut, int32_t batch, int64_t initial, int64_t* counts_out,
+                      int plant, double* batch_seconds) {
   std::vector<std::vector<int32_t>> out(V);
   for (int32_t v = 0; v < V; ++v)
     for (int32_t e = off[v]; e < off[v + 1]; ++e) out[v].push_back(adj[e]);
-  int64_t tc = oracle_compute_tc(V, off, adj, 0, 0);
+  int64_t tc = initial >= 0 ? initial : oracle_compute_tc(V, off, adj, 0, 0);
   auto pair = [](int32_t a, int32_t b) { return (uint64_t(uint32_t(a)) << 32) | uint32_t(b); };
   auto is_nbr = [&](int32_t a, int32_t b) {
     for (int32_t x : out[a])

@@ -41,8 +41,8 @@ def lib():
         L.oracle_dyn_sssp.argtypes = [i32, P_i32, P_i32, P_i32, i32, i32, C.c_char_p, P_i32, P_i32, i32, P_i32,
                                       P_i32, C.c_int, P_f64]
         L.oracle_dyn_sssp.restype = C.c_int
-        L.oracle_dyn_tc.argtypes = [i32, P_i32, P_i32, i32, C.c_char_p, P_i32, P_i32, i32, C.POINTER(C.c_int64),
-                                    C.c_int, P_f64]
+        L.oracle_dyn_tc.argtypes = [i32, P_i32, P_i32, i32, C.c_char_p, P_i32, P_i32, i32, C.c_int64,
+                                    C.POINTER(C.c_int64), C.c_int, P_f64]
         L.oracle_dyn_tc.restype = C.c_int32
         L.oracle_num_threads.restype = C.c_int
         _lib = L
@@ -146,10 +146,11 @@ def dyn_sssp(g, weight, src, types, us, ut, batch, plant=0, timing=None):
     return dist, parent
 
 
-def dyn_tc(g, types, us, ut, batch, plant=0, timing=None):
+def dyn_tc(g, types, us, ut, batch, plant=0, timing=None, initial=None):
     """DynTC restated (oracle.cpp): the triangle count after every batch of
     `batch` updates; types is a bytes string of 'a' / 'd'. With a dict as
-    `timing`, timing["batch_seconds"] receives the batch loop's wall time."""
+    `timing`, timing["batch_seconds"] receives the batch loop's wall time.
+    `initial` replaces the static count the run starts from (for timing)."""
     us = np.ascontiguousarray(us, np.int32)
     ut = np.ascontiguousarray(ut, np.int32)
     t = bytes(types)
@@ -158,7 +159,8 @@ def dyn_tc(g, types, us, ut, batch, plant=0, timing=None):
     counts = np.zeros(max(nb, 1), np.int64)
     secs = f64(0.0)
     got = lib().oracle_dyn_tc(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), len(us), t, _p(us, i32),
-                              _p(ut, i32), int(batch), _p(counts, C.c_int64), int(plant), C.byref(secs))
+                              _p(ut, i32), int(batch), -1 if initial is None else int(initial),
+                              _p(counts, C.c_int64), int(plant), C.byref(secs))
     if timing is not None:
         timing["batch_seconds"] = secs.value
     assert got == nb

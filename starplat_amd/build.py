@@ -40,19 +40,20 @@ def build_generated(out_dir=BUILD):
                f"-Wl,-rpath,$ORIGIN/{rel}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}", "-o", exe]
         subprocess.run(cmd, check=True)
         exes[algo] = exe
-    # the dynamic SSSP program, generated with -d
-    h, cc, plans = compile_file(os.path.join(ROOT, "starplat_amd", "dsl", "programs", "dyn_sssp.sp"), gen_root,
-                                dynamic=True)
-    dyn_name = next(p.func.name for p in plans if p.kind == "dynsssp")
-    exe = os.path.join(bindir, "gen_dynsssp")
-    rel = os.path.relpath(os.path.join(ROOT, "starplat_amd"), bindir)
-    subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", os.path.dirname(h),
-                    "-include", h, "-DRUN_DYNSSSP", f"-DDYN_FUNC={dyn_name}",
-                    os.path.join(ROOT, "tests", "drivers", "gen_driver.cpp"), cc,
-                    "-L", os.path.join(ROOT, "starplat_amd"), "-lstarplat_amd", "-lspm_host",
-                    "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
-                    f"-Wl,-rpath,$ORIGIN/{rel}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}", "-o", exe], check=True)
-    exes["dynsssp"] = exe
+    # the dynamic SSSP and TC programs, generated with -d
+    for name, prog in (("dynsssp", "dyn_sssp.sp"), ("dyntc", "dyn_tc.sp")):
+        h, cc, plans = compile_file(os.path.join(ROOT, "starplat_amd", "dsl", "programs", prog), gen_root,
+                                    dynamic=True)
+        dyn_name = next(p.func.name for p in plans if p.kind == name)
+        exe = os.path.join(bindir, f"gen_{name}")
+        rel = os.path.relpath(os.path.join(ROOT, "starplat_amd"), bindir)
+        subprocess.run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I",
+                        os.path.dirname(h), "-include", h, f"-DRUN_{name.upper()}", f"-DDYN_FUNC={dyn_name}",
+                        os.path.join(ROOT, "tests", "drivers", "gen_driver.cpp"), cc,
+                        "-L", os.path.join(ROOT, "starplat_amd"), "-lstarplat_amd", "-lspm_host",
+                        "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+                        f"-Wl,-rpath,$ORIGIN/{rel}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}", "-o", exe], check=True)
+        exes[name] = exe
     # the dynamic-graph driver (graph::updateCSRDel/Add and DynSSSP through
     # starplat_graph.hpp)
     exe = os.path.join(bindir, "dyn_driver")

@@ -83,6 +83,20 @@ def body(plan: Plan) -> list:
         out.append(f"  spm_require(spm_dyn_sssp({g}.device(), {r['src']}, type_.data(), src_.data(), dst_.data(), "
                    f"(int32_t)src_.size(), {r['batchSize']}, {r['dist']}, {r['parent']}), \"{plan.func.name}\");")
         out.append(f"  {g}.sync_from_device();  // the graph now holds the updated edges, as after updateCSRAdd")
+    elif k == "dyntc":
+        out.append(f"  // {r['static']} + Batch(OnDelete, Decremental, updateCSRDel, updateCSRAdd, OnAdd, Incremental)")
+        out.append(f"  //   -> spm_dyn_tc (device edge updates; class counts over each phase's updates)")
+        out.append(f"  std::vector<char> type_;")
+        out.append(f"  std::vector<int32_t> src_, dst_;")
+        out.append(f"  for (const update& u : {r['updates']}) {{")
+        out.append(f"    type_.push_back(u.type);")
+        out.append(f"    src_.push_back(u.source);")
+        out.append(f"    dst_.push_back(u.destination);")
+        out.append(f"  }}")
+        out.append(f"  int64_t triangleCount_ = 0;")
+        out.append(f"  spm_require(spm_dyn_tc({g}.device(), type_.data(), src_.data(), dst_.data(), (int32_t)src_.size(), "
+                   f"{r['batchSize']}, &triangleCount_), \"{plan.func.name}\");")
+        out.append(f"  {g}.sync_from_device();  // the graph now holds the updated edges, as after updateCSRAdd")
     elif k == "bc":
         out.append(f"  // iterateInBFS / iterateInReverse -> spm_compute_bc (64-source batched kernels)")
         out.append(f"  std::vector<int32_t> sources_({r['sourceSet']}.begin(), {r['sourceSet']}.end());")

@@ -638,6 +638,212 @@ def recognize_dynsssp(funcs):
     return [plan_static, plan_dyn]
 
 
+# ---------------------------------------------------------------- dynamic TC
+# graphcode/dynamicDSLCodes/dynamicBatchTCV2:1-187 is the shape recognised here.
+def _conjuncts(e):
+    if isinstance(e, BinOp) and e.op == "&&":
+        return _conjuncts(e.lhs) + _conjuncts(e.rhs)
+    return [e]
+
+
+def _recognize_tc_phase(f, sign):
+    """Incremental (+) / Decremental (-) of dynamicBatchTCV2: the class count
+    over the phase's updates. Returns nothing; raises on any other shape."""
+    where = f.name
+    if len(f.params) != 4 or [p.type.base for p in f.params] != ["Graph", "int", "propEdge", "updates"]:
+        raise DSLError(f"{where}: expected (Graph g, int count, propEdge<bool> modified, updates<g> batch)", f.line)
+    g, cin, mod, batch = (p.name for p in f.params)
+    body = stmts(f.body)
+    decl = {s.name: s.init for s in body if isinstance(s, Decl)}
+    tcs = [k for k, e in decl.items() if same(e, Name(cin))]
+    loops = [s for s in body if isinstance(s, ForAll)]
+    if len(tcs) != 1 or len(loops) != 1 or not same(loops[0].source, Name(batch)):
+        raise DSLError(f"{where}: expected count = {cin} and one forall over {batch}", f.line)
+    tc = tcs[0]
+    up = loops[0]
+    ld = {s.name: s.init for s in stmts(up.body) if isinstance(s, Decl)}
+    v1 = [k for k, e in ld.items() if same(e, P(up.var, "source"))]
+    v2 = [k for k, e in ld.items() if same(e, P(up.var, "destination"))]
+    inner = [s for s in stmts(up.body) if isinstance(s, ForAll)]
+    if len(v1) != 1 or len(v2) != 1 or len(inner) != 1 or not is_graph_call(inner[0].source, g, "neighbors", 1) \
+            or not same(inner[0].source.args[0], Name(v1[0])):
+        raise DSLError(f"{where}: expected v1 = update.source, v2 = update.destination, forall v3 in "
+                       f"{g}.neighbors(v1)")
+    v1, v2, lp = v1[0], v2[0], inner[0]
+    v3 = lp.var
+    want = {norm(BinOp("!=", Name(v3), Name(v2))), norm(BinOp("!=", Name(v3), Name(v1))),
+            norm(BinOp("!=", Name(v1), Name(v2)))}
+    if lp.filter is None or {norm(c) for c in _conjuncts(lp.filter)} != want:
+        raise DSLError(f"{where}: the neighbour filter must be {v3} != {v2} && {v3} != {v1} && {v1} != {v2}")
+    ib = stmts(lp.body)
+    idecl = {s.name: s.init for s in ib if isinstance(s, Decl)}
+    e1 = [k for k, e in idecl.items() if is_graph_call(e, g, "get_edge", 2) and same(e.args[0], Name(v1))
+          and same(e.args[1], Name(v3))]
+    ne = [k for k, e in idecl.items() if same(e, Num(1.0, "1"))]
+    ist = [k for k, e in idecl.items() if _is_false(e)]
+    ifs = [s for s in ib if isinstance(s, If)]
+    if len(e1) != 1 or len(ne) != 1 or len(ist) != 1 or len(ifs) != 3:
+        raise DSLError(f"{where}: expected edge e1 = get_edge({v1}, {v3}), newEdge = 1, isTriangle = False and "
+                       "three ifs")
+    e1, ne, ist = e1[0], ne[0], ist[0]
+    bump = Assign(Name(ne), "=", BinOp("+", Name(ne), Num(1.0, "1")))
+
+    def is_bump(block):
+        st = stmts(block)
+        return len(st) == 1 and isinstance(st[0], Assign) and st[0].op == "=" and same(st[0].target, Name(ne)) \
+            and same(st[0].expr, bump.expr)
+
+    if not (same(ifs[0].cond, P(e1, mod)) and is_bump(ifs[0].then) and ifs[0].orelse is None):
+        raise DSLError(f"{where}: expected if ({e1}.{mod}) {ne} = {ne} + 1")
+    c = ifs[1]
+    inb = stmts(c.then) if c.cond is not None else []
+    e2 = [s.name for s in inb if isinstance(s, Decl) and is_graph_call(s.init, g, "get_edge", 2)
+          and same(s.init.args[0], Name(v2)) and same(s.init.args[1], Name(v3))]
+    sets = [s for s in inb if isinstance(s, Assign) and same(s.target, Name(ist)) and _is_true(s.expr)]
+    in_if = [s for s in inb if isinstance(s, If)]
+    if not (is_graph_call(c.cond, g, "is_an_edge", 2) and same(c.cond.args[0], Name(v2)) and
+            same(c.cond.args[1], Name(v3)) and len(e2) == 1 and len(sets) == 1 and len(in_if) == 1 and
+            same(in_if[0].cond, P(e2[0], mod)) and is_bump(in_if[0].then)):
+        raise DSLError(f"{where}: expected if ({g}.is_an_edge({v2}, {v3})) {{ edge e2 = get_edge({v2}, {v3}); "
+                       f"{ist} = True; if (e2.{mod}) {ne} = {ne} + 1; }}")
+    t = ifs[2]
+    # if (isTriangle) { if (ne == 1) c1 += 1; else if (ne == 2) c2 += 1; else c3 += 1; }
+    counts = []
+    node = stmts(t.then)[0] if same(t.cond, Name(ist)) and len(stmts(t.then)) == 1 else None
+    for k in (1, 2):
+        if not (isinstance(node, If) and isinstance(node.cond, BinOp) and node.cond.op == "=="
+                and same(node.cond.lhs, Name(ne)) and same(node.cond.rhs, Num(float(k), str(k)))):
+            node = None
+            break
+        st = stmts(node.then)
+        if len(st) != 1 or not isinstance(st[0], Assign) or st[0].op != "+=" or not same(st[0].expr, Num(1.0, "1")):
+            node = None
+            break
+        counts.append(st[0].target)
+        node = stmts(node.orelse)[0] if node.orelse is not None and len(stmts(node.orelse)) == 1 else None
+    if node is not None and isinstance(node, Assign) and node.op == "+=" and same(node.expr, Num(1.0, "1")):
+        counts.append(node.target)
+    if len(counts) != 3 or not all(isinstance(x, Name) for x in counts):
+        raise DSLError(f"{where}: expected if ({ist}) {{ if ({ne} == 1) count1 += 1; else if ({ne} == 2) "
+                       "count2 += 1; else count3 += 1; }")
+    c1, c2, c3 = (x.id for x in counts)
+    delta = BinOp("+", BinOp("+", BinOp("/", Name(c1), Num(2.0, "2")), BinOp("/", Name(c2), Num(4.0, "4"))),
+                  BinOp("/", Name(c3), Num(6.0, "6")))
+    fin = [s for s in body if isinstance(s, Assign) and same(s.target, Name(tc))]
+    ret = [s for s in body if isinstance(s, Return)]
+    if not (len(fin) == 1 and fin[0].op == "=" and same(fin[0].expr, BinOp(sign, Name(tc), delta))
+            and len(ret) == 1 and same(ret[0].expr, Name(tc))):
+        raise DSLError(f"{where}: expected {tc} = {tc} {sign} ({c1}/2 + {c2}/4 + {c3}/6); return {tc};")
+
+
+def _tc_mark_block(b, g, kind, flag):
+    """OnDelete / OnAdd of DynTC: for (nbr in g.neighbors(src)) { edge e =
+    g.get_edge(src, nbr); if (nbr == dest) e.<flag> = True; }"""
+    if not (isinstance(b.source, Call) and b.source.method == "currentBatch"):
+        raise DSLError(f"On{kind}: expected (u in <updates>.currentBatch())")
+    decl = {s.name: s.init for s in stmts(b.body) if isinstance(s, Decl)}
+    src = [k for k, e in decl.items() if same(e, P(b.var, "source"))]
+    dst = [k for k, e in decl.items() if same(e, P(b.var, "destination"))]
+    fl = [s for s in stmts(b.body) if isinstance(s, ForAll)]
+    ok = len(src) == 1 and len(dst) == 1 and len(fl) == 1 and is_graph_call(fl[0].source, g, "neighbors", 1) \
+        and same(fl[0].source.args[0], Name(src[0]))
+    if ok:
+        nbr = fl[0].var
+        st = stmts(fl[0].body)
+        ev = [s.name for s in st if isinstance(s, Decl) and is_graph_call(s.init, g, "get_edge", 2)
+              and same(s.init.args[0], Name(src[0])) and same(s.init.args[1], Name(nbr))]
+        ifs = [s for s in st if isinstance(s, If)]
+        ok = len(ev) == 1 and len(ifs) == 1 and same(ifs[0].cond, BinOp("==", Name(nbr), Name(dst[0])))
+        if ok:
+            a = _assigns(ifs[0].then)
+            ok = _is_true(a.get(norm(P(ev[0], flag)), Num(0, "0")))
+    if not ok:
+        raise DSLError(f"On{kind}: expected for (nbr in {g}.neighbors(src)) {{ edge e = {g}.get_edge(src, nbr); "
+                       f"if (nbr == dest) e.{flag} = True; }}")
+
+
+def recognize_dyntc(funcs):
+    dyn = [f for f in funcs if f.kind == "Dynamic"]
+    if len(dyn) != 1:
+        raise DSLError("a dynamic program needs exactly one Dynamic function")
+    D = dyn[0]
+    g = graph_param(D)
+    upd = params_of(D, "updates")
+    if g is None or len(upd) != 1:
+        raise DSLError(f"{D.name}: expected a Graph and an updates<g> parameter", D.line)
+    upd = upd[0]
+    byname = {f.name: f for f in funcs}
+    body = stmts(D.body)
+    init = [s for s in body if isinstance(s, Decl) and isinstance(s.init, Call) and s.init.obj is None
+            and s.init.method in byname and byname[s.init.method].kind == "Static"]
+    batches = [s for s in body if isinstance(s, Batch)]
+    if len(init) != 1 or len(batches) != 1:
+        raise DSLError(f"{D.name}: expected count = <Static function>(g), then one Batch block", D.line)
+    cnt = init[0].name
+    S = byname[init[0].init.method]
+    plan_static = recognize_tc(S)
+    if plan_static is None:
+        raise DSLError(f"{S.name}: the Static function must be the forward triangle sum", S.line)
+    bt = batches[0]
+    if bt.updates != upd or not isinstance(bt.size, Name) or D.param(bt.size.id) is None:
+        raise DSLError(f"{D.name}: expected Batch({upd} : <batch size parameter>)")
+    bdecl = {s.name: s.init for s in stmts(bt.body) if isinstance(s, Decl)}
+    cur = {k: e for k, e in bdecl.items() if isinstance(e, Call) and e.method == "currentBatch"
+           and same(e.obj, Name(upd)) and len(e.args) == 1}
+    dels = [k for k, e in cur.items() if same(e.args[0], Num(0.0, "0"))]
+    adds = [k for k, e in cur.items() if same(e.args[0], Num(1.0, "1"))]
+    if len(dels) != 1 or len(adds) != 1:
+        raise DSLError(f"{D.name}: expected deletions = {upd}.currentBatch(0), additions = {upd}.currentBatch(1)")
+    seq = []
+    for s in stmts(bt.body):
+        if isinstance(s, OnUpdate):
+            seq.append(("on" + s.kind, s))
+        elif isinstance(s, ExprStmt) and isinstance(s.expr, Call):
+            c = s.expr
+            if is_graph_call(c, g, "updateCSRDel", 1) or is_graph_call(c, g, "updateCSRAdd", 1):
+                if not same(c.args[0], Name(upd)):
+                    raise DSLError(f"{D.name}: {c.method} must take {upd}")
+                seq.append((c.method, c))
+        elif isinstance(s, Assign) and same(s.target, Name(cnt)) and isinstance(s.expr, Call) \
+                and s.expr.obj is None and s.expr.method in ("Incremental", "Decremental"):
+            seq.append((s.expr.method, s.expr))
+    order = [k for k, _ in seq]
+    want = ["ondelete", "Decremental", "updateCSRDel", "updateCSRAdd", "onadd", "Incremental"]
+    if order != want:
+        raise DSLError(f"{D.name}: Batch must run {', '.join(want)} in this order (found {', '.join(order)})")
+    steps = dict(seq)
+    dec, inc = steps["Decremental"], steps["Incremental"]
+    flags = [d.name for d in stmts(bt.body) if isinstance(d, Decl) and d.type.base == "propEdge"
+             and str(d.type.inner) == "bool"]
+    for c, batch in ((dec, dels[0]), (inc, adds[0])):
+        if not (len(c.args) == 4 and same(c.args[0], Name(g)) and same(c.args[1], Name(cnt))
+                and isinstance(c.args[2], Name) and c.args[2].id in flags and same(c.args[3], Name(batch))):
+            raise DSLError(f"{D.name}: {c.method} must take ({g}, {cnt}, <bool edge property>, {batch})")
+    if dec.args[2].id == inc.args[2].id:
+        raise DSLError(f"{D.name}: Decremental and Incremental must take the batch's two edge properties")
+    _tc_mark_block(steps["ondelete"], g, "Delete", dec.args[2].id)
+    _tc_mark_block(steps["onadd"], g, "Add", inc.args[2].id)
+    fi = [f for f in funcs if f.kind == "Incremental"]
+    fd = [f for f in funcs if f.kind == "Decremental"]
+    if len(fi) != 1 or len(fd) != 1:
+        raise DSLError("a dynamic TC program needs one Incremental and one Decremental function")
+    _recognize_tc_phase(fi[0], "+")
+    _recognize_tc_phase(fd[0], "-")
+    plan_dyn = Plan("dyntc", D, {"g": g, "updates": upd, "batchSize": bt.size.id, "static": S.name})
+    plan_dyn.notes.append("the triangle count is kept in 64 bits on the device (spm_dyntc_count reads it); "
+                          "the DSL function returns nothing")
+    return [plan_static, plan_dyn]
+
+
+def _is_dyn_tc(funcs):
+    """A dynamic program whose Static function is a triangle sum."""
+    st = [f for f in funcs if f.kind == "Static"]
+    try:
+        return len(st) == 1 and recognize_tc(st[0]) is not None
+    except DSLError:
+        return False
+
+
 def lower_program(funcs, dynamic=False):
     """Plans for a static (-s) or dynamic (-d) program."""
     if not dynamic:
@@ -645,4 +851,6 @@ def lower_program(funcs, dynamic=False):
             if f.kind != "function":
                 raise DSLError(f"{f.kind} functions belong to dynamic programs (use -d)", f.line)
         return [lower(f) for f in funcs]
+    if _is_dyn_tc(funcs):
+        return recognize_dyntc(funcs)
     return recognize_dynsssp(funcs)

@@ -11,6 +11,9 @@
 //   gen_dynsssp <graph.txt> <out.bin> <updates.txt> <batch> <src>
 //     (-DRUN_DYNSSSP -DDYN_FUNC=<the Dynamic function>): dist, parent, then
 //     the updated graph's edge count)
+//   gen_dyntc <graph.txt> <out.bin> <updates.txt> <batch>
+//     (-DRUN_DYNTC -DDYN_FUNC=<the Dynamic function>): the triangle count
+//     (int64), then the updated graph's edge count)
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -64,8 +67,18 @@ int main(int argc, char** argv) {
   std::fwrite(parent.data(), sizeof(int), V, out);
   const int E = g.num_edges();
   std::fwrite(&E, sizeof E, 1, out);
+#elif defined(RUN_DYNTC)
+  if (argc < 5) return 2;
+  std::vector<update> updateBatch = parseUpdateFile(argv[3]);
+  DYN_FUNC(g, updateBatch, std::atoi(argv[4]));
+  // the Dynamic function returns nothing; its count stays with the graph's device mirror
+  int64_t count = 0;
+  spm_require(spm_dyntc_count(g.device(), &count), "spm_dyntc_count");
+  std::fwrite(&count, sizeof count, 1, out);
+  const int E = g.num_edges();
+  std::fwrite(&E, sizeof E, 1, out);
 #else
-#error "define one of RUN_SSSP, RUN_PR, RUN_TC, RUN_BC, RUN_DYNSSSP"
+#error "define one of RUN_SSSP, RUN_PR, RUN_TC, RUN_BC, RUN_DYNSSSP, RUN_DYNTC"
 #endif
   std::fclose(out);
   return 0;

@@ -193,6 +193,50 @@ def test_mutated_dynamic_programs_are_rejected(old, new):
         lower_program(parse(text.replace(old, new, 1)), dynamic=True)
 
 
+def test_dynamic_tc_program_lowers():
+    from starplat_amd.dsl.lower import lower_program
+    plans = lower_program(parse(src("dyn_tc.sp")), dynamic=True)
+    assert [p.kind for p in plans] == ["tc", "dyntc"]
+    assert (plans[1].roles["updates"], plans[1].roles["batchSize"], plans[1].roles["static"]) == \
+        ("stream", "chunk", "countAll")
+    h, cc, _ = compile_source(src("dyn_tc.sp"), "dyn_tc", dynamic=True)
+    assert "long countAll(graph& g);" in h
+    assert "void keepCount(graph& g, std::vector<update> stream, int chunk);" in h
+    assert "spm_dyn_tc(g.device()," in cc and "g.sync_from_device();" in cc
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DYN), reason="reference tree not present")
+def test_reference_dynamic_tc_lowers():
+    h, _, plans = compile_source(open(os.path.join(REF_DYN, "dynamicBatchTCV2")).read(), "dynamicBatchTCV2",
+                                 dynamic=True)
+    assert [p.func.name for p in plans] == ["staticTC", "DynTC"]
+    assert "void DynTC(graph& g, std::vector<update> updateBatch, int batchSize);" in h
+    assert "long staticTC(graph& g);" in h
+
+
+DYN_TC_MUTATIONS = [
+    ("thrice / 6", "thrice / 3"),                                   # class-3 divisor
+    ("total = total - (", "total = total + ("),                     # Decremental's sign
+    ("filter(a != b && c != a && c != b)", "filter(c != a && c != b)"),  # the filter
+    ("        if (bc.tagged)", "        if (ac.tagged)"),          # second mark read
+    ("if (g.is_an_edge(b, c))", "if (g.is_an_edge(a, c))"),         # closing edge
+    ("          pr.gone = True;", "          pr.fresh = True;"),   # OnDelete marks the wrong property
+    ("    tri = Decremental(g, tri, gone, outgoing);\n    g.updateCSRDel(stream);",
+     "    g.updateCSRDel(stream);\n    tri = Decremental(g, tri, gone, outgoing);"),  # counts after deleting
+    ("stream.currentBatch(0)", "stream.currentBatch(1)"),           # no deletion batch
+    ("        } else if (mine == 2) {\n          twice += 1;", "        } else if (mine == 2) {\n          once += 1;"),
+]
+
+
+@pytest.mark.parametrize("old,new", DYN_TC_MUTATIONS)
+def test_mutated_dynamic_tc_programs_are_rejected(old, new):
+    from starplat_amd.dsl.lower import lower_program
+    text = src("dyn_tc.sp")
+    assert old in text
+    with pytest.raises(DSLError):
+        lower_program(parse(text.replace(old, new, 1)), dynamic=True)
+
+
 def test_cli_dynamic(tmp_path):
     r = subprocess.run([sys.executable, "-m", "starplat_amd.dsl", "-d", "-f", os.path.join(PROG, "dyn_sssp.sp"),
                         "-b", "mi355x", "-o", str(tmp_path)], cwd=ROOT, capture_output=True, text=True)
@@ -228,3 +272,33 @@ def test_generated_dynamic_code_end_to_end(tmp_path, oracle):
     np.testing.assert_array_equal(raw[g.V:2 * g.V], pref)
     off, _, _ = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, batch_ops(ty, us, ut, 64))
     assert raw[2 * g.V] == off[-1]
+
+
+@pytest.mark.gpu
+def test_generated_dynamic_tc_end_to_end(tmp_path, oracle):
+    """The -d code of dyn_tc.sp, linked with a main like the reference's
+    drivers, against the DynTC restatement (updates from an update file,
+    each line pushed twice)."""
+    from starplat_amd.updates import as_arrays, parseUpdateFile, write_update_file
+    from test_oracle_dyn import batch_ops
+    assert os.access(os.path.join(BIN, "gen_dyntc"), os.X_OK), "run build() first"
+    g = graph.rmat(11, 8, seed=21, undirected=True)
+    p = tmp_path / "g.txt"
+    g.write_text(p, weights=False)
+    g = graph(str(p)).parseGraph()
+    rng = np.random.default_rng(3)
+    src_of = np.repeat(np.arange(g.V), np.diff(g.indexofNodes))
+    e = rng.integers(0, g.E, 600)
+    types = np.where(rng.random(600) < 0.5, ord("d"), ord("a")).astype(np.uint8)
+    s = np.where(types == ord("d"), src_of[e], rng.integers(0, g.V, 600))
+    t = np.where(types == ord("d"), g.edgeList[e], rng.integers(0, g.V, 600))
+    up = tmp_path / "u.txt"
+    write_update_file(str(up), types.tolist(), s, t)
+    ty, us, ut, _ = as_arrays(parseUpdateFile(str(up)))
+    run_driver("dyntc", p, tmp_path / "o.bin", up, 100)
+    raw = np.fromfile(tmp_path / "o.bin", np.int64, count=1)
+    ref = oracle.dyn_tc(g, ty, us, ut, 100)
+    assert int(raw[0]) == int(ref[-1])
+    E = int(np.fromfile(tmp_path / "o.bin", np.int32)[2])
+    off, _, _ = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, batch_ops(ty, us, ut, 100))
+    assert E == off[-1]

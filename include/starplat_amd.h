@@ -188,6 +188,27 @@ int spm_dyntc_count(spm_graph* g, int64_t* triangleCount);
 int spm_dyn_tc(spm_graph* g, const char* type, const int32_t* usrc, const int32_t* udst, int32_t n,
                int32_t batchSize, int64_t* triangleCount);
 
+/* ---- dynamic PageRank ------------------------------------------------------
+ * Replaces: the code generated for graphcode/dynamicDSLCodes/dynamicBatchPR
+ * -- DynPR (:97-135): staticPR (:1-31), then per batch OnDelete flags the
+ * deletions' destinations, propagateNodeFlags (graph.hpp:444-462) flags every
+ * vertex with a live in-edge, updateCSRDel, Decremental (:65-95) sweeps the
+ * flagged vertices; the same with the additions (graph after the deletions),
+ * updateCSRAdd, Incremental (:32-63). As the dynamic OMP generator emits it:
+ * count_outNbrs is the out-degree in the CSR the run starts from
+ * (dsl_dyn_cpp_generator.cpp:279-288; the reference's updates never move
+ * indexofNodes), nodes_to the live in-edges, the diff signed (val - pr).
+ * Needs the reverse CSR. Ranks float[V]. */
+int spm_dynpr_init(spm_graph* g, float beta, float delta, int32_t maxIter);
+int spm_dynpr_batch(spm_graph* g, const char* type, const int32_t* src, const int32_t* dst, int32_t n);
+int spm_dynpr_result(spm_graph* g, float* pageRank);
+/* Sweeps run so far: staticPR's, then Decremental's and Incremental's per
+ * batch; *count receives how many there are, at most cap are copied. */
+int spm_dynpr_iterations(spm_graph* g, int32_t* iterCounts, int32_t cap, int32_t* count);
+/* The whole Dynamic function, updates in batches of batchSize. */
+int spm_dyn_pr(spm_graph* g, float beta, float delta, int32_t maxIter, float* pageRank, const char* type,
+               const int32_t* usrc, const int32_t* udst, int32_t n, int32_t batchSize);
+
 /* ---- schedules -----------------------------------------------------------
  * The schedule changes how work maps to lanes, never the arithmetic. No
  * counterpart in the reference (its CUDA backend hard-codes 512 threads per

@@ -1,7 +1,8 @@
 // oracle.cpp -- TEST INFRASTRUCTURE ONLY. CPU restatement of the OpenMP code
 // that StarPlat's OMP backend (src/backends/backend_omp/dsl_cpp_generator.cpp)
-// emits for the four hot-path DSL programs and for the dynamic SSSP and TC
-// programs (dynamicDSLCodes/dynamicBatchSSSP, dynamicBatchTCV2), plus
+// emits for the four hot-path DSL programs and for the dynamic SSSP, TC and
+// PR programs (dynamicDSLCodes/dynamicBatchSSSP, dynamicBatchTCV2,
+// dynamicBatchPR), plus
 // independent textbook routines that
 // pin them. Only the test suite, the smoke check and the CPU
 // baseline leg of the benchmark load this library; the product path never
@@ -655,6 +656,99 @@ int32_t oracle_dyn_tc(int32_t V, const int32_t* off, const int32_t* adj, int32_t
     if (counts_out) counts_out[batches] = tc;
   }
   if (batch_seconds) *batch_seconds = omp_get_wtime() - t_start;
+  return batches;
+}
+
+// Restatement of the Dynamic function DynPR of
+// graphcode/dynamicDSLCodes/dynamicBatchPR (staticPR :1-31, Incremental
+// :32-63, Decremental :65-95, DynPR :97-135) as the dynamic OMP generator
+// emits it, sequential, over an edge multiset. Emission facts followed:
+//   count_outNbrs(u) -> g.indexofNodes[u+1] - g.indexofNodes[u]
+//     (dsl_dyn_cpp_generator.cpp:279-288); updates never move indexofNodes
+//     (graph_ompv2.hpp:13-95 overwrite deleted slots with INT_MAX/2, later
+//     additions refill them or go to the diff rows), so this stays u's row
+//     length in the CSR the run starts from;
+//   nodes_to(v) -> graph::getInNeighbors (dsl_dyn_cpp_generator.cpp:569-577):
+//     live in-edges, CSR part then diff part;
+//   propagateNodeFlags (graph.hpp:444-462) flags the destination of every
+//     live edge, whatever its source's flag;
+//   diff += val - pr[v], signed as the DSL writes it; each do-while stops
+//     when !(diff > beta && iterCount < maxIter).
+// staticPR: ranks 1/N, then sweeps over every vertex. Per batch: modified =
+// destinations of the deletions, propagated on the graph before them;
+// updateCSRDel; Decremental sweeps the modified vertices; modified_add =
+// destinations of the additions, propagated on the graph after the
+// deletions; updateCSRAdd; Incremental sweeps those. Both phases run in every
+// batch. Sums are in row order, sequential. iters_out receives staticPR's
+// iteration count, then Decremental's and Incremental's per batch.
+// plant 1: count_outNbrs is the live out-degree; plant 2: propagateNodeFlags
+// is skipped (only the destinations of the batch are swept).
+int32_t oracle_dyn_pr(int32_t V, const int32_t* off, const int32_t* adj, float beta, float delta, int32_t maxIter,
+                      int32_t n, const char* type, const int32_t* us, const int32_t* ut, int32_t batch,
+                      float* pageRank, int32_t* iters_out, int plant, double* batch_seconds) {
+  std::vector<std::vector<int32_t>> out(V), in(V);
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t e = off[v]; e < off[v + 1]; ++e) {
+      out[v].push_back(adj[e]);
+      in[adj[e]].push_back(v);
+    }
+  auto outdeg = [&](int32_t u) { return plant == 1 ? int32_t(out[u].size()) : off[u + 1] - off[u]; };
+  std::vector<float> pr(V), nxt(V, 0.f);
+  const float num_nodes = float(V);
+  for (int32_t v = 0; v < V; ++v) pr[v] = 1 / num_nodes;
+  // one do-while of sweeps over the flagged vertices (all when flag is null)
+  auto sweeps = [&](const std::vector<char>* flag) {
+    int32_t iterCount = 0;
+    float diff;
+    do {
+      diff = 0.0f;
+      for (int32_t v = 0; v < V; ++v) {
+        if (flag && !(*flag)[v]) continue;
+        float sum = 0.0f;
+        for (int32_t nbr : in[v]) sum = sum + pr[nbr] / outdeg(nbr);
+        const float val = (1 - delta) / num_nodes + delta * sum;
+        diff += val - pr[v];
+        nxt[v] = val;
+      }
+      for (int32_t v = 0; v < V; ++v)
+        if (!flag || (*flag)[v]) pr[v] = nxt[v];
+      iterCount++;
+    } while ((diff > beta) && (iterCount < maxIter));
+    return iterCount;
+  };
+  auto propagate = [&](std::vector<char>& flag) {
+    if (plant == 2) return;
+    for (int32_t v = 0; v < V; ++v)
+      for (int32_t t : out[v]) flag[t] = 1;
+  };
+  int32_t k_it = 0;
+  iters_out[k_it++] = sweeps(nullptr);
+  int32_t batches = 0;
+  const double t_start = omp_get_wtime();
+  for (int32_t k = 0; k < n; k += batch, ++batches) {
+    const int32_t m = std::min(batch, n - k);
+    std::vector<std::pair<int32_t, int32_t>> dels, adds;
+    for (int32_t i = k; i < k + m; ++i) (type[i] == 'd' ? dels : adds).push_back({us[i], ut[i]});
+    std::vector<char> mod(V, 0), madd(V, 0);
+    for (auto [s, t] : dels) mod[t] = 1;
+    propagate(mod);
+    for (auto [s, t] : dels) {
+      auto& o = out[s];
+      o.erase(std::remove(o.begin(), o.end(), t), o.end());
+      auto& r = in[t];
+      r.erase(std::remove(r.begin(), r.end(), s), r.end());
+    }
+    iters_out[k_it++] = sweeps(&mod);
+    for (auto [s, t] : adds) madd[t] = 1;
+    propagate(madd);
+    for (auto [s, t] : adds) {
+      out[s].push_back(t);
+      in[t].push_back(s);
+    }
+    iters_out[k_it++] = sweeps(&madd);
+  }
+  if (batch_seconds) *batch_seconds = omp_get_wtime() - t_start;
+  std::copy(pr.begin(), pr.end(), pageRank);
   return batches;
 }
 

@@ -44,6 +44,9 @@ def lib():
         L.oracle_dyn_tc.argtypes = [i32, P_i32, P_i32, i32, C.c_char_p, P_i32, P_i32, i32, C.c_int64,
                                     C.POINTER(C.c_int64), C.c_int, P_f64]
         L.oracle_dyn_tc.restype = C.c_int32
+        L.oracle_dyn_pr.argtypes = [i32, P_i32, P_i32, C.c_float, C.c_float, i32, i32, C.c_char_p, P_i32, P_i32, i32,
+                                    C.POINTER(C.c_float), P_i32, C.c_int, P_f64]
+        L.oracle_dyn_pr.restype = C.c_int32
         L.oracle_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -165,6 +168,26 @@ def dyn_tc(g, types, us, ut, batch, plant=0, timing=None, initial=None):
         timing["batch_seconds"] = secs.value
     assert got == nb
     return counts[:nb]
+
+
+def dyn_pr(g, beta, delta, maxIter, types, us, ut, batch, plant=0, timing=None):
+    """DynPR restated (oracle.cpp): (ranks after the last batch, iteration
+    counts: staticPR's, then Decremental's and Incremental's per batch)."""
+    us = np.ascontiguousarray(us, np.int32)
+    ut = np.ascontiguousarray(ut, np.int32)
+    t = bytes(types)
+    assert len(t) == len(us) == len(ut)
+    nb = (len(us) + batch - 1) // batch
+    pr = np.empty(g.V, np.float32)
+    its = np.zeros(1 + 2 * nb, np.int32)
+    secs = f64(0.0)
+    got = lib().oracle_dyn_pr(g.V, _p(g.indexofNodes, i32), _p(g.edgeList, i32), float(beta), float(delta),
+                              int(maxIter), len(us), t, _p(us, i32), _p(ut, i32), int(batch),
+                              pr.ctypes.data_as(C.POINTER(C.c_float)), _p(its, i32), int(plant), C.byref(secs))
+    if timing is not None:
+        timing["batch_seconds"] = secs.value
+    assert got == nb
+    return pr, its
 
 
 def apply_updates(V, off, adj, w, ops, plant=0):

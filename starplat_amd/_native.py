@@ -102,6 +102,11 @@ DEVICE_API = [
     ("spm_dyntc_batch", C.c_int, [vp, C.c_char_p, P_i32, P_i32, i32]),
     ("spm_dyntc_count", C.c_int, [vp, P_i64]),
     ("spm_dyn_tc", C.c_int, [vp, C.c_char_p, P_i32, P_i32, i32, i32, P_i64]),
+    ("spm_dynpr_init", C.c_int, [vp, f32, f32, i32]),
+    ("spm_dynpr_batch", C.c_int, [vp, C.c_char_p, P_i32, P_i32, i32]),
+    ("spm_dynpr_result", C.c_int, [vp, P_f32]),
+    ("spm_dynpr_iterations", C.c_int, [vp, P_i32, i32, P_i32]),
+    ("spm_dyn_pr", C.c_int, [vp, f32, f32, i32, P_f32, C.c_char_p, P_i32, P_i32, i32, i32]),
     ("spm_get_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
     ("spm_set_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),
     ("spm_default_schedule", C.c_int, [vp, C.POINTER(SpmSchedule)]),

@@ -90,3 +90,19 @@ def DynTC(g, updateBatch, batchSize):
     count = dev.dyn_tc(types, s, d, int(batchSize))
     g.refresh_from_device()
     return count
+
+
+def DynPR(g, beta, delta, maxIter, pageRank, updateBatch, batchSize):
+    """The Dynamic function of graphcode/dynamicDSLCodes/dynamicBatchPR:97-135
+    as the dynamic OMP generator emits it: staticPR, then per batch of
+    batchSize updates the flagged sweeps Decremental (after updateCSRDel) and
+    Incremental (after updateCSRAdd); count_outNbrs is the out-degree of the
+    graph the run starts from and the diff is signed. pageRank is filled in
+    place; g's arrays then describe the updated graph."""
+    from .updates import as_arrays
+    V = g.num_nodes()
+    _out(pageRank, np.float32, V, "pageRank")
+    dev = g.device()
+    types, s, d, _ = as_arrays(list(updateBatch))
+    pageRank[:] = dev.dyn_pr(float(beta), float(delta), int(maxIter), types, s, d, int(batchSize))
+    g.refresh_from_device()

@@ -2,9 +2,9 @@
 // for gfx950: the replacement of graph::updateCSRDel / graph::updateCSRAdd
 // (graphcode/graph.hpp:869-975, over graph_ompv2.hpp:13-454) and of the code
 // StarPlat generates for the Dynamic / Decremental / Incremental functions of
-// graphcode/dynamicDSLCodes/dynamicBatchSSSP and dynamicBatchTCV2 (batch
-// loop, OnDelete, OnAdd: backend_omp/dsl_dyn_cpp_generator.cpp:44-160,
-// update calls :301-316).
+// graphcode/dynamicDSLCodes/dynamicBatchSSSP, dynamicBatchTCV2 and
+// dynamicBatchPR (batch loop, OnDelete, OnAdd:
+// backend_omp/dsl_dyn_cpp_generator.cpp:44-160, update calls :301-316).
 //
 // Layout. The handle's CSR and reverse CSR stay where they are and stay
 // sorted. A deletion sets a bit in a removed-slot bitset per direction (one
@@ -108,6 +108,18 @@ struct DynWork {
   size_t tchunk_cap = 0;
   unsigned long long* tcv = nullptr;   // [4] class counts 1..3 of the phase, triangle count
   bool tc_ready = false;
+  // dynamic PageRank
+  float* pr = nullptr;                 // [V] ranks
+  float* pr_nxt = nullptr;             // [V]
+  float* pr_c = nullptr;               // [V] rank / base out-degree
+  float* pr_deg = nullptr;             // [V] out-degree in the CSR the run started from
+  uint8_t* pr_flag = nullptr;          // [V] the phase's modified flags
+  float* pr_part = nullptr;            // [pr_part_n] per-workgroup signed diffs
+  int pr_part_n = 0;
+  float pr_beta = 0.f, pr_delta = 0.f;
+  int32_t pr_maxIter = 0;
+  std::vector<int32_t> pr_iters;       // sweeps of staticPR, then Decremental / Incremental per batch
+  bool pr_ready = false;
 };
 
 static bool dyn_dirty(const spm_graph* g) {
@@ -442,6 +454,108 @@ __global__ void dtc_apply(unsigned long long* tcv, int sign) {
   const long long delta = static_cast<long long>(tcv[0] / 2 + tcv[1] / 4 + tcv[2] / 6);
   tcv[3] = static_cast<unsigned long long>(static_cast<long long>(tcv[3]) + sign * delta);
   tcv[0] = tcv[1] = tcv[2] = 0ull;
+}
+
+// ---- dynamic PageRank -----------------------------------------------------------
+// dynamicBatchPR as the dynamic OMP generator emits it: count_outNbrs is the
+// row length of the CSR the run starts from (dsl_dyn_cpp_generator.cpp:279-288;
+// updates never move indexofNodes), nodes_to iterates the live in-edges (CSR
+// part, then overflow part), propagateNodeFlags flags the destination of every
+// live edge (graph.hpp:444-462), and the diff is signed (val - pr[v]).
+
+// c[u] = pr[u] / base out-degree (IEEE division, as the emitted expression).
+__global__ void dpr_contrib(int32_t V, const float* __restrict__ pr, const float* __restrict__ deg,
+                            float* __restrict__ c) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) c[v] = pr[v] / deg[v];
+}
+
+// flag[v] |= v has a live in-edge (propagateNodeFlags): its reverse row minus
+// removed slots, or a live entry of its reverse overflow row.
+__global__ void dpr_propagate(int32_t V, const int32_t* __restrict__ roff, const int32_t* __restrict__ rdead_row,
+                              DiffRows in, uint8_t* __restrict__ flag) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
+    bool live = roff[v + 1] - roff[v] - rdead_row[v] > 0;
+    for (int32_t j = in.off[v]; !live && j < in.off[v + 1]; ++j) live = !bit_of(in.dead, j);
+    if (live) flag[v] = 1;
+  }
+}
+
+__global__ void dpr_flag_dests(int32_t n, const int32_t* __restrict__ ut, uint8_t* __restrict__ flag) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) flag[ut[i]] = 1;
+}
+
+// One sweep over the flagged vertices (all when flag is null), one wavefront
+// per vertex: sum of c over live in-edges, val = (1 - delta)/N + delta*sum
+// without contraction, the signed diff val - pr[v] summed per workgroup in a
+// fixed order into part[blockIdx.x].
+__global__ __launch_bounds__(kDynBlock) void dpr_sweep(int32_t V, const int32_t* __restrict__ roff,
+                                                       const int32_t* __restrict__ radj,
+                                                       const uint32_t* __restrict__ rdead, DiffRows in,
+                                                       const float* __restrict__ c, const float* __restrict__ pr,
+                                                       const uint8_t* __restrict__ flag, float* __restrict__ nxt,
+                                                       float base, float delta, float* __restrict__ part) {
+  __shared__ float s_red[kDynWaves];
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  const int64_t nw = int64_t(gridDim.x) * kDynWaves;
+  float mydiff = 0.f;
+  for (int64_t x_ = int64_t(blockIdx.x) * kDynWaves + wid; x_ < V; x_ += nw) {
+    const int64_t v = spread_vertex(x_, V);
+    if (flag && !flag[v]) continue;
+    float sum = 0.f;
+    for (int32_t j = roff[v] + lane; j < roff[v + 1]; j += kWave)
+      if (!bit_of(rdead, j)) sum += c[radj[j]];
+    for (int32_t j = in.off[v] + lane; j < in.off[v + 1]; j += kWave)
+      if (!bit_of(in.dead, j)) sum += c[in.adj[j]];
+    sum = wave_sum(sum);
+    if (lane == 0) {
+      const float val = __fadd_rn(base, __fmul_rn(delta, sum));
+      mydiff += val - pr[v];
+      nxt[v] = val;
+    }
+  }
+  if (lane == 0) s_red[wid] = mydiff;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int w = 0; w < kDynWaves; ++w) t += s_red[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+// staticPR's start: every rank 1/N; the base out-degrees of the run.
+__global__ void dpr_init(int32_t V, const int32_t* __restrict__ off, float init, float* __restrict__ pr,
+                         float* __restrict__ deg) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
+    pr[v] = init;
+    deg[v] = float(off[v + 1] - off[v]);
+  }
+}
+
+__global__ void dpr_apply(int32_t V, const uint8_t* __restrict__ flag, const float* __restrict__ nxt,
+                          float* __restrict__ pr) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride)
+    if (!flag || flag[v]) pr[v] = nxt[v];
+}
+
+// The signed diff of one sweep: the per-workgroup parts summed in order (one
+// workgroup), written to out[0].
+__global__ __launch_bounds__(kDynBlock) void dpr_diff(const float* __restrict__ part, int n, float* __restrict__ out) {
+  __shared__ float s_red[kDynWaves];
+  float t = 0.f;
+  for (int i = threadIdx.x; i < n; i += kDynBlock) t += part[i];
+  t = wave_sum(t);
+  if (__lane_id() == 0) s_red[threadIdx.x / kWave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float d = 0.f;
+    for (int w = 0; w < kDynWaves; ++w) d += s_red[w];
+    out[0] = d;
+  }
 }
 
 // ---- dynamic SSSP ------------------------------------------------------------
@@ -807,6 +921,12 @@ int dyn_release(spm_graph* g) {
   dfree(g, w->tset, sizeof(unsigned long long) * w->tset_cap);
   dfree(g, w->tchunk, sizeof(int32_t) * w->tchunk_cap);
   dfree(g, w->tcv, sizeof(unsigned long long) * 4);
+  dfree(g, w->pr, sizeof(float) * V);
+  dfree(g, w->pr_nxt, sizeof(float) * V);
+  dfree(g, w->pr_c, sizeof(float) * V);
+  dfree(g, w->pr_deg, sizeof(float) * V);
+  dfree(g, w->pr_flag, V);
+  dfree(g, w->pr_part, sizeof(float) * (size_t(w->pr_part_n) + 1));
   delete w;
   g->dyn = nullptr;
   return SPM_OK;
@@ -1256,6 +1376,97 @@ static int dtc_batch(spm_graph* g, const char* type, const int32_t* us, const in
   return prof_flush(g);
 }
 
+// ---- dynamic PageRank, host side ----------------------------------------------
+
+// One do-while of sweeps (staticPR with flag null, Decremental or
+// Incremental): stops when !(diff > beta && iterCount < maxIter).
+static int dpr_sweeps(spm_graph* g, const uint8_t* flag) {
+  DynWork* w = g->dyn;
+  const int32_t V = g->V;
+  const float base = (1 - w->pr_delta) / float(V);  // (1 - delta) / g.num_nodes()
+  const int vgrid = grid_for(g, V);
+  const int sgrid = w->pr_part_n;
+  float* h = reinterpret_cast<float*>(g->h_pinned);
+  int32_t it = 0;
+  for (;;) {
+    SPM_TRY(launch(g, "dpr_contrib", dpr_contrib, dim3(vgrid), dim3(kDynBlock), 0, V, (const float*)w->pr,
+                   (const float*)w->pr_deg, w->pr_c));
+    SPM_TRY(launch(g, "dpr_sweep", dpr_sweep, dim3(sgrid), dim3(kDynBlock), 0, V, (const int32_t*)g->roff,
+                   (const int32_t*)g->radj, (const uint32_t*)w->rdead, w->in[w->cur].r, (const float*)w->pr_c,
+                   (const float*)w->pr, flag, w->pr_nxt, base, w->pr_delta, w->pr_part));
+    SPM_TRY(launch(g, "dpr_apply", dpr_apply, dim3(vgrid), dim3(kDynBlock), 0, V, flag, (const float*)w->pr_nxt,
+                   w->pr));
+    SPM_TRY(launch(g, "dpr_diff", dpr_diff, dim3(1), dim3(kDynBlock), 0, (const float*)w->pr_part, sgrid,
+                   w->pr_part + sgrid));
+    SPM_CHECK_HIP(hipMemcpyAsync(h, w->pr_part + sgrid, sizeof(float), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    ++it;
+    if (!(h[0] > w->pr_beta && it < w->pr_maxIter)) break;
+  }
+  w->pr_iters.push_back(it);
+  return SPM_OK;
+}
+
+static int dpr_init_run(spm_graph* g, float beta, float delta, int32_t maxIter) {
+  if (!g->roff) return set_error(SPM_ENOGRAPH, "dynamic PageRank needs the reverse CSR (rev_indexofNodes, srcList)");
+  SPM_TRY(dyn_settle(g));  // the run starts from a clean CSR: its row lengths are count_outNbrs
+  SPM_TRY(dyn_alloc(g));
+  DynWork* w = g->dyn;
+  const size_t V = size_t(g->V);
+  if (!w->pr) {
+    SPM_TRY(dmalloc(g, (void**)&w->pr, sizeof(float) * V, "dynamic PR ranks"));
+    SPM_TRY(dmalloc(g, (void**)&w->pr_nxt, sizeof(float) * V, "dynamic PR next ranks"));
+    SPM_TRY(dmalloc(g, (void**)&w->pr_c, sizeof(float) * V, "dynamic PR contributions"));
+    SPM_TRY(dmalloc(g, (void**)&w->pr_deg, sizeof(float) * V, "dynamic PR base degrees"));
+    SPM_TRY(dmalloc(g, (void**)&w->pr_flag, V, "dynamic PR flags"));
+    w->pr_part_n = g->num_cus * 8;
+    SPM_TRY(dmalloc(g, (void**)&w->pr_part, sizeof(float) * (size_t(w->pr_part_n) + 1), "dynamic PR diffs"));
+  }
+  w->pr_beta = beta;
+  w->pr_delta = delta;
+  w->pr_maxIter = maxIter;
+  w->pr_iters.clear();
+  const float num_nodes = float(g->V);
+  SPM_TRY(launch(g, "dpr_init", dpr_init, dim3(grid_for(g, g->V)), dim3(kDynBlock), 0, g->V, (const int32_t*)g->off,
+                 1 / num_nodes, w->pr, w->pr_deg));
+  SPM_TRY(dpr_sweeps(g, nullptr));  // staticPR
+  w->pr_ready = true;
+  return prof_flush(g);
+}
+
+// One batch of DynPR (dynamicBatchPR:97-135): OnDelete + propagateNodeFlags
+// on the graph before the deletions, updateCSRDel, Decremental; OnAdd +
+// propagateNodeFlags on the graph after them, updateCSRAdd, Incremental.
+static int dpr_batch(spm_graph* g, const char* type, const int32_t* us, const int32_t* ut, int32_t n) {
+  DynWork* w = g->dyn;
+  if (!w || !w->pr_ready) return set_error(SPM_EINVAL, "dynamic PageRank: call spm_dynpr_init first");
+  std::vector<int32_t> ds, dt, as, at;
+  for (int32_t i = 0; i < n; ++i) {
+    if (type[i] == 'd') { ds.push_back(us[i]); dt.push_back(ut[i]); }
+    else if (type[i] == 'a') { as.push_back(us[i]); at.push_back(ut[i]); }
+    else return set_error(SPM_EINVAL, "dynamic PageRank: update type must be 'a' or 'd'");
+  }
+  const int32_t V = g->V;
+  const int vgrid = grid_for(g, V);
+  for (int phase = 0; phase < 2; ++phase) {
+    const std::vector<int32_t>& s = phase == 0 ? ds : as;
+    const std::vector<int32_t>& t = phase == 0 ? dt : at;
+    const int32_t m = int32_t(s.size());
+    SPM_CHECK_HIP(hipMemsetAsync(w->pr_flag, 0, size_t(V), g->stream));
+    int32_t *ps = nullptr, *pt = nullptr, *px = nullptr;
+    if (m > 0) {
+      SPM_TRY(stage(g, s.data(), t.data(), nullptr, m, &ps, &pt, &px));
+      SPM_TRY(launch(g, "dpr_flag_dests", dpr_flag_dests, dim3((m + kDynBlock - 1) / kDynBlock), dim3(kDynBlock), 0,
+                     m, (const int32_t*)pt, w->pr_flag));
+    }
+    SPM_TRY(launch(g, "dpr_propagate", dpr_propagate, dim3(vgrid), dim3(kDynBlock), 0, V, (const int32_t*)g->roff,
+                   (const int32_t*)w->rdead_row, w->in[w->cur].r, w->pr_flag));
+    if (m > 0) SPM_TRY(phase == 0 ? delete_staged(g, ps, pt, m) : add_staged(g, ps, pt, nullptr, m));
+    SPM_TRY(dpr_sweeps(g, w->pr_flag));
+  }
+  return prof_flush(g);
+}
+
 }  // namespace spm
 
 using namespace spm;
@@ -1376,6 +1587,45 @@ int spm_dyn_tc(spm_graph* g, const char* type, const int32_t* usrc, const int32_
     SPM_TRY(spm_dyntc_batch(g, type + k, usrc + k, udst + k, m));
   }
   return spm_dyntc_count(g, triangleCount);
+}
+
+int spm_dynpr_init(spm_graph* g, float beta, float delta, int32_t maxIter) {
+  if (!g) return set_error(SPM_EINVAL, "spm_dynpr_init: NULL handle");
+  return dpr_init_run(g, beta, delta, maxIter);
+}
+
+int spm_dynpr_batch(spm_graph* g, const char* type, const int32_t* src, const int32_t* dst, int32_t n) {
+  if (!g || n < 0 || (n > 0 && (!type || !src || !dst))) return set_error(SPM_EINVAL, "spm_dynpr_batch: bad argument");
+  return dpr_batch(g, type, src, dst, n);
+}
+
+int spm_dynpr_result(spm_graph* g, float* pageRank) {
+  if (!g || !pageRank) return set_error(SPM_EINVAL, "spm_dynpr_result: NULL argument");
+  DynWork* w = g->dyn;
+  if (!w || !w->pr_ready) return set_error(SPM_EINVAL, "spm_dynpr_result: call spm_dynpr_init first");
+  SPM_CHECK_HIP(hipMemcpyAsync(pageRank, w->pr, sizeof(float) * size_t(g->V), hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  return SPM_OK;
+}
+
+int spm_dynpr_iterations(spm_graph* g, int32_t* iterCounts, int32_t cap, int32_t* count) {
+  if (!g || !count || (cap > 0 && !iterCounts)) return set_error(SPM_EINVAL, "spm_dynpr_iterations: bad argument");
+  DynWork* w = g->dyn;
+  if (!w || !w->pr_ready) return set_error(SPM_EINVAL, "spm_dynpr_iterations: call spm_dynpr_init first");
+  *count = int32_t(w->pr_iters.size());
+  for (int32_t i = 0; i < cap && i < *count; ++i) iterCounts[i] = w->pr_iters[size_t(i)];
+  return SPM_OK;
+}
+
+int spm_dyn_pr(spm_graph* g, float beta, float delta, int32_t maxIter, float* pageRank, const char* type,
+               const int32_t* usrc, const int32_t* udst, int32_t n, int32_t batchSize) {
+  if (!g || n < 0 || batchSize < 1 || !pageRank) return set_error(SPM_EINVAL, "spm_dyn_pr: bad argument");
+  SPM_TRY(spm_dynpr_init(g, beta, delta, maxIter));
+  for (int32_t k = 0; k < n; k += batchSize) {
+    const int32_t m = std::min(batchSize, n - k);
+    SPM_TRY(spm_dynpr_batch(g, type + k, usrc + k, udst + k, m));
+  }
+  return spm_dynpr_result(g, pageRank);
 }
 
 int spm_dyn_sssp(spm_graph* g, int32_t src, const char* type, const int32_t* usrc, const int32_t* udst, int32_t n,

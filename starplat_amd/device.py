@@ -166,6 +166,38 @@ class DeviceGraph:
             self.h, t, N.ptr(usrc, C.c_int32), N.ptr(udst, C.c_int32), len(usrc), int(batch_size), C.byref(c)))
         return c.value
 
+    def dynpr_init(self, beta, delta, maxIter):
+        N.check("spm_dynpr_init", N.device_lib().spm_dynpr_init(self.h, beta, delta, int(maxIter)))
+
+    def dynpr_batch(self, types, src, dst):
+        src, dst = self._pair(src, dst)
+        t = self._types(types, len(src))
+        N.check("spm_dynpr_batch", N.device_lib().spm_dynpr_batch(
+            self.h, t, N.ptr(src, C.c_int32), N.ptr(dst, C.c_int32), len(src)))
+
+    def dynpr_result(self):
+        pr = np.empty(self.V, np.float32)
+        N.check("spm_dynpr_result", N.device_lib().spm_dynpr_result(self.h, N.ptr(pr, C.c_float)))
+        return pr
+
+    def dynpr_iterations(self):
+        n = C.c_int32(0)
+        N.check("spm_dynpr_iterations", N.device_lib().spm_dynpr_iterations(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.int32)
+        N.check("spm_dynpr_iterations", N.device_lib().spm_dynpr_iterations(
+            self.h, N.ptr(out, C.c_int32), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def dyn_pr(self, beta, delta, maxIter, types, usrc, udst, batch_size):
+        """The whole Dynamic function (spm_dyn_pr): the final ranks."""
+        usrc, udst = self._pair(usrc, udst)
+        t = self._types(types, len(usrc))
+        pr = np.empty(self.V, np.float32)
+        N.check("spm_dyn_pr", N.device_lib().spm_dyn_pr(
+            self.h, beta, delta, int(maxIter), N.ptr(pr, C.c_float), t, N.ptr(usrc, C.c_int32),
+            N.ptr(udst, C.c_int32), len(usrc), int(batch_size)))
+        return pr
+
     # ---- schedules ----------------------------------------------------------
     def schedule(self):
         s = N.SpmSchedule()

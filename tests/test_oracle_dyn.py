@@ -303,3 +303,106 @@ def test_dyn_tc_plants_are_caught(oracle):
     assert oracle.dyn_tc(g, types, us, ut, 80).tolist() == want
     for plant in (1, 2):
         assert oracle.dyn_tc(g, types, us, ut, 80, plant=plant).tolist() != want
+
+
+# ---------------------------------------------------------------- DynPR
+def _sweep(V, src, dst, deg0, pr, flag, delta):
+    """One Jacobi sweep of the DSL's update over the flagged vertices, in
+    float64: val = (1 - delta)/N + delta * sum_{u -> v} pr[u] / deg0[u]."""
+    contrib = np.zeros(V)
+    np.add.at(contrib, dst, pr[src] / deg0[src])
+    val = (1 - delta) / V + delta * contrib
+    out = pr.copy()
+    out[flag] = val[flag]
+    return out, float(np.sum(val[flag] - pr[flag]))
+
+
+def test_dyn_pr_static_closed_forms(oracle):
+    # with dangling vertices the signed diff of the first sweep is negative
+    # (mass leaves through them), so staticPR stops after one sweep from 1/N
+    g = graph.rmat(9, 8, seed=3)
+    deg0 = np.diff(g.indexofNodes).astype(np.float64)
+    assert (deg0 == 0).any()
+    empty = np.zeros(0, np.int32)
+    pr, its = oracle.dyn_pr(g, 1e-10, 0.85, 100, b"", empty, empty, 10)
+    src = np.repeat(np.arange(g.V), np.diff(g.indexofNodes))
+    want, d = _sweep(g.V, src, g.edgeList, np.where(deg0 > 0, deg0, 1), np.full(g.V, 1 / g.V), np.ones(g.V, bool), 0.85)
+    assert its.tolist() == [1] and d < 0
+    np.testing.assert_allclose(pr, want, rtol=1e-5)
+    # a directed cycle is a fixed point: every rank stays 1/N
+    n = 50
+    cyc = graph.from_edges(np.arange(n, dtype=np.int32), ((np.arange(n) + 1) % n).astype(np.int32))
+    pr, its = oracle.dyn_pr(cyc, 1e-10, 0.85, 100, b"", empty, empty, 10)
+    np.testing.assert_allclose(pr, 1 / n, rtol=1e-6)
+
+
+def _dyn_pr_numpy(O, g, types, us, ut, batch, delta, beta, maxIter, its):
+    """The per-batch steps recomputed from the updated CSR (apply_updates), in
+    float64, running as many sweeps per phase as `its` says and checking that
+    the signed-diff rule stops each phase exactly there. O: the oracle module
+    (its apply_updates keeps the edge multiset)."""
+    V = g.V
+    deg0 = np.diff(g.indexofNodes).astype(np.float64)
+    off, adj = g.indexofNodes, g.edgeList
+    w = np.ones(len(adj), np.int32)
+    k_it = [0]
+
+    def phase(pr, off, adj, flag, deg):
+        src = np.repeat(np.arange(V), np.diff(off))
+        c = int(its[k_it[0]])
+        k_it[0] += 1
+        # float64 cannot reproduce where a float32 run reaches its fixed point
+        # (diff 0), so the rule is checked up to a margin: a phase never stops
+        # while its diff is clearly positive, never continues once it is clearly
+        # negative or zero
+        for i in range(c):
+            pr, d = _sweep(V, src, adj, deg, pr, flag, delta)
+            if i < c - 1:
+                assert d > beta - 1e-7
+        assert d <= beta + 1e-7 or c == maxIter
+        return pr
+
+    pr = phase(np.full(V, 1 / V), off, adj, np.ones(V, bool), np.where(deg0 > 0, deg0, 1))
+    t = np.frombuffer(types, np.uint8)
+    for k in range(0, len(us), batch):
+        sl = slice(k, k + batch)
+        dm, am = t[sl] == ord("d"), t[sl] == ord("a")
+        flag = np.zeros(V, bool)
+        flag[ut[sl][dm]] = True
+        flag[adj] = True  # every destination of the graph before the deletions
+        off, adj, w = O.apply_updates(V, off, adj, w, [("d", us[sl][dm], ut[sl][dm])])
+        pr = phase(pr, off, adj, flag, deg0)
+        flag = np.zeros(V, bool)
+        flag[ut[sl][am]] = True
+        flag[adj] = True  # after the deletions, before the additions
+        off, adj, w = O.apply_updates(V, off, adj, w, [("a", us[sl][am], ut[sl][am], np.ones(int(am.sum()), np.int32))])
+        pr = phase(pr, off, adj, flag, deg0)
+    return pr
+
+
+def test_dyn_pr_batches_against_recomputation(oracle):
+    """Every vertex the graph keeps some out-edge of keeps a positive base
+    degree, so the recomputation never divides by zero; updates delete
+    existing edges and add edges between vertices that had out-edges."""
+    g = graph.rmat(9, 8, seed=5)
+    rng = np.random.default_rng(2)
+    src_of = np.repeat(np.arange(g.V), np.diff(g.indexofNodes))
+    has_out = np.flatnonzero(np.diff(g.indexofNodes) > 0)
+    types, us, ut = [], [], []
+    for _ in range(300):
+        if rng.random() < 0.5:
+            e = int(rng.integers(0, g.E))
+            types.append("d"), us.append(int(src_of[e])), ut.append(int(g.edgeList[e]))
+        else:
+            types.append("a"), us.append(int(rng.choice(has_out))), ut.append(int(rng.integers(0, g.V)))
+    types = "".join(types).encode()
+    us, ut = np.array(us, np.int32), np.array(ut, np.int32)
+    pr, its = oracle.dyn_pr(g, 1e-10, 0.85, 100, types, us, ut, 60)
+    # phases stop after one sweep when mass leaves (signed diff < 0) and run
+    # to maxIter when it grows (base degrees undercount added out-edges)
+    assert its[0] == 1 and len(set(its.tolist())) > 1
+    want = _dyn_pr_numpy(oracle, g, types, us, ut, 60, 0.85, 1e-10, 100, its)
+    np.testing.assert_allclose(pr, want, rtol=1e-4, atol=1e-9)
+    for plant in (1, 2):
+        bad, _ = oracle.dyn_pr(g, 1e-10, 0.85, 100, types, us, ut, 60, plant=plant)
+        assert not np.allclose(bad, want, rtol=1e-4, atol=1e-9)

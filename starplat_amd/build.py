@@ -40,8 +40,8 @@ def build_generated(out_dir=BUILD):
                f"-Wl,-rpath,$ORIGIN/{rel}", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}", "-o", exe]
         subprocess.run(cmd, check=True)
         exes[algo] = exe
-    # the dynamic SSSP and TC programs, generated with -d
-    for name, prog in (("dynsssp", "dyn_sssp.sp"), ("dyntc", "dyn_tc.sp")):
+    # the dynamic SSSP, TC and PR programs, generated with -d
+    for name, prog in (("dynsssp", "dyn_sssp.sp"), ("dyntc", "dyn_tc.sp"), ("dynpr", "dyn_pr.sp")):
         h, cc, plans = compile_file(os.path.join(ROOT, "starplat_amd", "dsl", "programs", prog), gen_root,
                                     dynamic=True)
         dyn_name = next(p.func.name for p in plans if p.kind == name)

@@ -83,6 +83,26 @@ def body(plan: Plan) -> list:
         out.append(f"  spm_require(spm_dyn_sssp({g}.device(), {r['src']}, type_.data(), src_.data(), dst_.data(), "
                    f"(int32_t)src_.size(), {r['batchSize']}, {r['dist']}, {r['parent']}), \"{plan.func.name}\");")
         out.append(f"  {g}.sync_from_device();  // the graph now holds the updated edges, as after updateCSRAdd")
+    elif k == "dynpr_static":
+        out.append(f"  // staticPR (signed diff, as the dynamic OMP backend emits it) -> spm_dynpr_init / spm_dynpr_result")
+        out.append(f"  spm_require(spm_dynpr_init({g}.device(), {r['beta']}, {r['delta']}, {r['maxIter']}), "
+                   f"\"{plan.func.name}\");")
+        out.append(f"  spm_require(spm_dynpr_result({g}.device(), {r['pageRank']}), \"{plan.func.name}\");")
+    elif k == "dynpr":
+        out.append(f"  // {r['static']} + Batch(OnDelete, propagateNodeFlags, updateCSRDel, Decremental, OnAdd, "
+                   "propagateNodeFlags, updateCSRAdd, Incremental)")
+        out.append(f"  //   -> spm_dyn_pr (device edge updates; flagged pull sweeps)")
+        out.append(f"  std::vector<char> type_;")
+        out.append(f"  std::vector<int32_t> src_, dst_;")
+        out.append(f"  for (const update& u : {r['updates']}) {{")
+        out.append(f"    type_.push_back(u.type);")
+        out.append(f"    src_.push_back(u.source);")
+        out.append(f"    dst_.push_back(u.destination);")
+        out.append(f"  }}")
+        out.append(f"  spm_require(spm_dyn_pr({g}.device(), {r['beta']}, {r['delta']}, {r['maxIter']}, {r['pageRank']}, "
+                   f"type_.data(), src_.data(), dst_.data(), (int32_t)src_.size(), {r['batchSize']}), "
+                   f"\"{plan.func.name}\");")
+        out.append(f"  {g}.sync_from_device();  // the graph now holds the updated edges, as after updateCSRAdd")
     elif k == "dyntc":
         out.append(f"  // {r['static']} + Batch(OnDelete, Decremental, updateCSRDel, updateCSRAdd, OnAdd, Incremental)")
         out.append(f"  //   -> spm_dyn_tc (device edge updates; class counts over each phase's updates)")

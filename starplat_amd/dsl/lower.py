@@ -835,6 +835,174 @@ def recognize_dyntc(funcs):
     return [plan_static, plan_dyn]
 
 
+# ---------------------------------------------------------------- dynamic PR
+# graphcode/dynamicDSLCodes/dynamicBatchPR:1-135 is the shape recognised here.
+def _pr_sweeps(f, g, rank, flag, where):
+    """The do-while of staticPR / Incremental / Decremental: over the vertices
+    (flagged by `flag`, or all), sum = sum + nbr.rank / g.count_outNbrs(nbr)
+    over nodes_to(v), val = (1 - delta) / N + delta * sum, the signed diff
+    diff += val - v.rank, v.<next> = val; then the copy (rank = next, or a
+    flagged forall v.rank = v.next), iterCount++, while (diff > beta &&
+    iterCount < maxIter). Returns nothing; raises on any other shape."""
+    beta, delta, maxIter = (p.name for p in f.params[1:4])
+    dws = [s for s in stmts(f.body) if isinstance(s, DoWhile)]
+    if len(dws) != 1:
+        raise DSLError(f"{where}: expected one do-while")
+    dw = dws[0]
+    body = stmts(dw.body)
+    fas = [s for s in body if isinstance(s, ForAll)]
+    sweep = fas[0] if fas else None
+    if sweep is None or not is_graph_call(sweep.source, g, "nodes", 0) or \
+            (flag is None and sweep.filter is not None) or (flag is not None and not is_filter_true(sweep.filter, flag)):
+        raise DSLError(f"{where}: expected forall v in {g}.nodes()" + (f".filter({flag} == True)" if flag else ""))
+    v = sweep.var
+    sb = stmts(sweep.body)
+    inner = [s for s in sb if isinstance(s, ForAll)]
+    decl = {s.name: s.init for s in sb if isinstance(s, Decl)}
+    acc = [k for k, e in decl.items() if same(e, Num(0.0, "0"))]
+    if len(inner) != 1 or not is_graph_call(inner[0].source, g, "nodes_to", 1) or len(acc) != 1:
+        raise DSLError(f"{where}: expected float sum = 0 and a loop over {g}.nodes_to({v})")
+    acc, nbr = acc[0], inner[0].var
+    term = BinOp("/", P(nbr, rank), Call(Name(g), "count_outNbrs", [Name(nbr)]))
+    ib = stmts(inner[0].body)
+    if len(ib) != 1 or not isinstance(ib[0], Assign) or not same(ib[0].target, Name(acc)) or \
+            not same(ib[0].expr, BinOp("+", Name(acc), term)):
+        raise DSLError(f"{where}: the pull sum must be {acc} = {acc} + {nbr}.{rank} / {g}.count_outNbrs({nbr})")
+    vals = [k for k, e in decl.items() if isinstance(e, BinOp) and e.op == "+"]
+    ok = len(vals) == 1
+    if ok:
+        e = decl[vals[0]]
+        tele, damp = e.lhs, e.rhs
+        ok = (isinstance(tele, BinOp) and tele.op == "/" and same(tele.lhs, BinOp("-", Num(1.0, "1"), Name(delta)))
+              and same(damp, BinOp("*", Name(delta), Name(acc))))
+        if ok:
+            n = tele.rhs
+            nn = [d.name for d in find_all(f.body, Decl) if is_graph_call(d.init, g, "num_nodes", 0)]
+            ok = is_graph_call(n, g, "num_nodes", 0) or (isinstance(n, Name) and n.id in nn)
+    if not ok:
+        raise DSLError(f"{where}: the update must be val = (1 - {delta}) / {g}.num_nodes() + {delta} * {acc}")
+    val = vals[0]
+    diffs = [s for s in sb if isinstance(s, Assign) and s.op == "+=" and same(s.expr, BinOp("-", Name(val), P(v, rank)))]
+    nexts = [s for s in sb if isinstance(s, Assign) and s.op == "=" and isinstance(s.target, Prop)
+             and same(s.target.obj, Name(v)) and same(s.expr, Name(val))]
+    if len(diffs) != 1 or not isinstance(diffs[0].target, Name) or len(nexts) != 1:
+        raise DSLError(f"{where}: expected diff += {val} - {v}.{rank} (signed) and {v}.<next> = {val}")
+    diff, nxt = diffs[0].target.id, nexts[0].target.name
+    if not any(isinstance(s, Assign) and same(s.target, Name(diff)) and same(s.expr, Num(0.0, "0")) for s in body):
+        raise DSLError(f"{where}: expected {diff} = 0 at the top of the loop")
+    if flag is None:
+        copied = any(isinstance(s, Assign) and same(s.target, Name(rank)) and same(s.expr, Name(nxt)) for s in body)
+    else:
+        copied = False
+        if len(fas) == 2 and is_graph_call(fas[1].source, g, "nodes", 0) and is_filter_true(fas[1].filter, flag):
+            cb = stmts(fas[1].body)
+            w = fas[1].var
+            copied = len(cb) == 1 and isinstance(cb[0], Assign) and same(cb[0].target, P(w, rank)) and \
+                same(cb[0].expr, P(w, nxt))
+    if not copied:
+        raise DSLError(f"{where}: missing the copy of {nxt} into {rank}" + (f" over the {flag} vertices" if flag else ""))
+    its = [s.target.id for s in body if isinstance(s, Assign) and s.op == "++" and isinstance(s.target, Name)]
+    cond = dw.cond
+    ok = len(its) == 1 and isinstance(cond, BinOp) and cond.op == "&&" and \
+        same(cond.lhs, BinOp(">", Name(diff), Name(beta))) and same(cond.rhs, BinOp("<", Name(its[0]), Name(maxIter)))
+    if not ok:
+        raise DSLError(f"{where}: expected iterCount++ and while (({diff} > {beta}) && (iterCount < {maxIter}))")
+
+
+def recognize_dynpr(funcs):
+    dyn = [f for f in funcs if f.kind == "Dynamic"]
+    if len(dyn) != 1:
+        raise DSLError("a dynamic program needs exactly one Dynamic function")
+    D = dyn[0]
+    g = graph_param(D)
+    upd = params_of(D, "updates")
+    if g is None or len(upd) != 1:
+        raise DSLError(f"{D.name}: expected a Graph and an updates<g> parameter", D.line)
+    upd = upd[0]
+    byname = {f.name: f for f in funcs}
+    body = stmts(D.body)
+    calls = [s.expr for s in body if isinstance(s, ExprStmt) and isinstance(s.expr, Call) and s.expr.obj is None]
+    batches = [s for s in body if isinstance(s, Batch)]
+    if len(calls) != 1 or calls[0].method not in byname or byname[calls[0].method].kind != "Static" or \
+            len(batches) != 1:
+        raise DSLError(f"{D.name}: expected a call of the Static function, then one Batch block", D.line)
+    S = byname[calls[0].method]
+    if [p.type.base for p in S.params] != ["Graph", "float", "float", "int", "propNode"]:
+        raise DSLError(f"{S.name}: expected (Graph g, float beta, float delta, int maxIter, propNode<float> rank)")
+    gS, rankS = S.params[0].name, S.params[4].name
+    if not (isinstance(attached_value(S, gS, rankS), BinOp) and attached_value(S, gS, rankS).op == "/"
+            and same(attached_value(S, gS, rankS).lhs, Num(1.0, "1"))):
+        raise DSLError(f"{S.name}: ranks must start at 1 / num_nodes")
+    _pr_sweeps(S, gS, rankS, None, S.name)
+    names = [p.name for p in D.params]
+    if len(calls[0].args) != 5 or not all(isinstance(a, Name) for a in calls[0].args) or \
+            [a.id for a in calls[0].args] != names[:5]:
+        raise DSLError(f"{D.name}: the Static call must pass ({', '.join(names[:5])})")
+    roles = {"g": g, "beta": names[1], "delta": names[2], "maxIter": names[3], "pageRank": names[4],
+             "updates": upd, "static": S.name}
+    bt = batches[0]
+    if bt.updates != upd or not isinstance(bt.size, Name) or D.param(bt.size.id) is None:
+        raise DSLError(f"{D.name}: expected Batch({upd} : <batch size parameter>)")
+    roles["batchSize"] = bt.size.id
+    seq = []
+    for st in stmts(bt.body):
+        if isinstance(st, OnUpdate):
+            seq.append(("on" + st.kind, st))
+        elif isinstance(st, ExprStmt) and isinstance(st.expr, Call):
+            c = st.expr
+            if is_graph_call(c, g, "updateCSRDel", 1) or is_graph_call(c, g, "updateCSRAdd", 1):
+                if not same(c.args[0], Name(upd)):
+                    raise DSLError(f"{D.name}: {c.method} must take {upd}")
+                seq.append((c.method, c))
+            elif is_graph_call(c, g, "propagateNodeFlags", 1):
+                seq.append(("propagate", c))
+            elif c.obj is None and c.method in ("Incremental", "Decremental"):
+                seq.append((c.method, c))
+    order = [k for k, _ in seq]
+    want = ["ondelete", "propagate", "updateCSRDel", "Decremental", "onadd", "propagate", "updateCSRAdd",
+            "Incremental"]
+    if order != want:
+        raise DSLError(f"{D.name}: Batch must run {', '.join(want)} in this order (found {', '.join(order)})")
+    flags = [d.name for d in stmts(bt.body) if isinstance(d, Decl) and d.type.base == "propNode"
+             and str(d.type.inner) == "bool"]
+    dec, inc = seq[3][1], seq[7][1]
+    for c, prop in ((dec, seq[1][1]), (inc, seq[5][1])):
+        if not (len(c.args) == 6 and [a.id if isinstance(a, Name) else None for a in c.args[:5]] == names[:5]
+                and isinstance(c.args[5], Name) and c.args[5].id in flags and same(prop.args[0], c.args[5])):
+            raise DSLError(f"{D.name}: {c.method} must take ({', '.join(names[:5])}, <the flags just propagated>)")
+    mod, madd = dec.args[5].id, inc.args[5].id
+    if mod == madd:
+        raise DSLError(f"{D.name}: Decremental and Incremental must take the batch's two flag properties")
+    for (kind, blk), flag in ((seq[0], mod), (seq[4], madd)):
+        ok = isinstance(blk.source, Call) and blk.source.method == "currentBatch"
+        decl = {st.name: st.init for st in stmts(blk.body) if isinstance(st, Decl)}
+        dst = [k for k, e in decl.items() if same(e, P(blk.var, "destination"))]
+        asg = [st for st in stmts(blk.body) if isinstance(st, Assign)]
+        ok = ok and len(dst) == 1 and len(asg) == 1 and same(asg[0].target, P(dst[0], flag)) and _is_true(asg[0].expr)
+        if not ok:
+            raise DSLError(f"On{kind[2:].capitalize()}: expected dest = u.destination; dest.{flag} = True")
+    for f in funcs:
+        if f.kind in ("Incremental", "Decremental"):
+            if [p.type.base for p in f.params] != ["Graph", "float", "float", "int", "propNode", "propNode"]:
+                raise DSLError(f"{f.name}: expected (Graph g, float beta, float delta, int maxIter, "
+                               "propNode<float> rank, propNode<bool> modified)", f.line)
+            _pr_sweeps(f, f.params[0].name, f.params[4].name, f.params[5].name, f.name)
+    if {f.kind for f in funcs} != {"Static", "Incremental", "Decremental", "Dynamic"}:
+        raise DSLError("a dynamic PageRank program needs Static, Incremental, Decremental and Dynamic functions")
+    plan_static = Plan("dynpr_static", S, {"g": gS, "beta": S.params[1].name, "delta": S.params[2].name,
+                                           "maxIter": S.params[3].name, "pageRank": rankS})
+    plan_dyn = Plan("dynpr", D, roles)
+    plan_dyn.notes.append("count_outNbrs is the out-degree of the graph the run starts from and the diff is "
+                          "signed, as the dynamic OMP backend emits them")
+    return [plan_static, plan_dyn]
+
+
+def _is_dyn_pr(funcs):
+    st = [f for f in funcs if f.kind == "Static"]
+    return len(st) == 1 and any(p.type.base == "propNode" and str(p.type.inner) == "float" for p in st[0].params) \
+        and bool(find_all(st[0].body, DoWhile))
+
+
 def _is_dyn_tc(funcs):
     """A dynamic program whose Static function is a triangle sum."""
     st = [f for f in funcs if f.kind == "Static"]
@@ -853,4 +1021,6 @@ def lower_program(funcs, dynamic=False):
         return [lower(f) for f in funcs]
     if _is_dyn_tc(funcs):
         return recognize_dyntc(funcs)
+    if _is_dyn_pr(funcs):
+        return recognize_dynpr(funcs)
     return recognize_dynsssp(funcs)

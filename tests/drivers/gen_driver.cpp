@@ -14,6 +14,9 @@
 //   gen_dyntc <graph.txt> <out.bin> <updates.txt> <batch>
 //     (-DRUN_DYNTC -DDYN_FUNC=<the Dynamic function>): the triangle count
 //     (int64), then the updated graph's edge count)
+//   gen_dynpr <graph.txt> <out.bin> <updates.txt> <batch> <beta> <delta> <maxIter>
+//     (-DRUN_DYNPR -DDYN_FUNC=<the Dynamic function>): ranks, then the
+//     updated graph's edge count)
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -77,8 +80,17 @@ int main(int argc, char** argv) {
   std::fwrite(&count, sizeof count, 1, out);
   const int E = g.num_edges();
   std::fwrite(&E, sizeof E, 1, out);
+#elif defined(RUN_DYNPR)
+  if (argc < 8) return 2;
+  std::vector<float> pr(V);
+  std::vector<update> updateBatch = parseUpdateFile(argv[3]);
+  DYN_FUNC(g, (float)std::atof(argv[5]), (float)std::atof(argv[6]), std::atoi(argv[7]), pr.data(), updateBatch,
+           std::atoi(argv[4]));
+  std::fwrite(pr.data(), sizeof(float), V, out);
+  const int E = g.num_edges();
+  std::fwrite(&E, sizeof E, 1, out);
 #else
-#error "define one of RUN_SSSP, RUN_PR, RUN_TC, RUN_BC, RUN_DYNSSSP, RUN_DYNTC"
+#error "define one of RUN_SSSP, RUN_PR, RUN_TC, RUN_BC, RUN_DYNSSSP, RUN_DYNTC, RUN_DYNPR"
 #endif
   std::fclose(out);
   return 0;

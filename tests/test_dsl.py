@@ -237,6 +237,52 @@ def test_mutated_dynamic_tc_programs_are_rejected(old, new):
         lower_program(parse(text.replace(old, new, 1)), dynamic=True)
 
 
+def test_dynamic_pr_program_lowers():
+    from starplat_amd.dsl.lower import lower_program
+    plans = lower_program(parse(src("dyn_pr.sp")), dynamic=True)
+    assert [p.kind for p in plans] == ["dynpr_static", "dynpr"]
+    r = plans[1].roles
+    assert (r["beta"], r["delta"], r["maxIter"], r["pageRank"], r["updates"], r["batchSize"]) == \
+        ("tol", "damp", "rounds", "rank", "stream", "chunk")
+    h, cc, _ = compile_source(src("dyn_pr.sp"), "dyn_pr", dynamic=True)
+    assert "void firstRanks(graph& g, float tol, float damp, int rounds, float* rank);" in h
+    assert "void refreshRanks(graph& g, float tol, float damp, int rounds, float* rank, std::vector<update> stream, " \
+           "int chunk);" in h
+    assert "spm_dyn_pr(g.device(), tol, damp, rounds, rank," in cc and "spm_dynpr_init(g.device()" in cc
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_DYN), reason="reference tree not present")
+def test_reference_dynamic_pr_lowers():
+    h, _, plans = compile_source(open(os.path.join(REF_DYN, "dynamicBatchPR")).read(), "dynamicBatchPR", dynamic=True)
+    assert [p.func.name for p in plans] == ["staticPR", "DynPR"]
+    assert "void DynPR(graph& g, float beta, float delta, int maxIter, float* pageRank, std::vector<update> " \
+           "updateBatch, int batchSize);" in h
+
+
+DYN_PR_MUTATIONS = [
+    ("      change += fresh - x.rank;\n      x.rank_next = fresh;\n    }\n    rank = rank_next;",
+     "      change += x.rank - fresh;\n      x.rank_next = fresh;\n    }\n    rank = rank_next;"),  # diff sign
+    ("acc = acc + y.rank / g.count_outNbrs(y);", "acc = acc + y.rank;"),                 # the pull term
+    ("float fresh = (1 - damp) / total + damp * acc;", "float fresh = (1 - damp) / total + acc;"),  # damping
+    ("    g.propagateNodeFlags(lost);\n", ""),                                            # no propagation
+    ("    Decremental(g, tol, damp, rounds, rank, lost);\n    OnAdd", "    OnAdd"),       # phase missing
+    ("      q.gained = True;", "      p.gained = True;"),                                # OnAdd flags the source
+    ("    forall (x in g.nodes().filter(hit == True)) {\n      x.rank = x.rank_next;\n    }\n", ""),  # no copy
+    ("  } while ((change > tol) && (done < rounds));\n}\n\nIncremental",
+     "  } while ((change > tol) || (done < rounds));\n}\n\nIncremental"),                 # loop test
+    ("    g.updateCSRDel(stream);\n    Decremental", "    Decremental"),                  # deletions never applied
+]
+
+
+@pytest.mark.parametrize("old,new", DYN_PR_MUTATIONS)
+def test_mutated_dynamic_pr_programs_are_rejected(old, new):
+    from starplat_amd.dsl.lower import lower_program
+    text = src("dyn_pr.sp")
+    assert old in text
+    with pytest.raises(DSLError):
+        lower_program(parse(text.replace(old, new, 1)), dynamic=True)
+
+
 def test_cli_dynamic(tmp_path):
     r = subprocess.run([sys.executable, "-m", "starplat_amd.dsl", "-d", "-f", os.path.join(PROG, "dyn_sssp.sp"),
                         "-b", "mi355x", "-o", str(tmp_path)], cwd=ROOT, capture_output=True, text=True)
@@ -302,3 +348,30 @@ def test_generated_dynamic_tc_end_to_end(tmp_path, oracle):
     E = int(np.fromfile(tmp_path / "o.bin", np.int32)[2])
     off, _, _ = oracle.apply_updates(g.V, g.indexofNodes, g.edgeList, g.edgeLen, batch_ops(ty, us, ut, 100))
     assert E == off[-1]
+
+
+@pytest.mark.gpu
+def test_generated_dynamic_pr_end_to_end(tmp_path, oracle):
+    """The -d code of dyn_pr.sp, linked with a main like the reference's
+    drivers, against the DynPR restatement."""
+    from starplat_amd.updates import as_arrays, parseUpdateFile, write_update_file
+    assert os.access(os.path.join(BIN, "gen_dynpr"), os.X_OK), "run build() first"
+    g = graph.rmat(10, 8, seed=23)
+    p = tmp_path / "g.txt"
+    g.write_text(p, weights=False)
+    g = graph(str(p)).parseGraph()
+    rng = np.random.default_rng(4)
+    src_of = np.repeat(np.arange(g.V), np.diff(g.indexofNodes))
+    has_out = np.flatnonzero(np.diff(g.indexofNodes) > 0)
+    e = rng.integers(0, g.E, 300)
+    types = np.where(rng.random(300) < 0.5, ord("d"), ord("a")).astype(np.uint8)
+    s = np.where(types == ord("d"), src_of[e], rng.choice(has_out, 300))
+    t = np.where(types == ord("d"), g.edgeList[e], rng.choice(has_out, 300))
+    up = tmp_path / "u.txt"
+    write_update_file(str(up), types.tolist(), s, t)
+    ty, us, ut, _ = as_arrays(parseUpdateFile(str(up)))
+    run_driver("dynpr", p, tmp_path / "o.bin", up, 150, 1e-10, 0.85, 100)
+    got = np.fromfile(tmp_path / "o.bin", np.float32, count=g.V)
+    ref, _ = oracle.dyn_pr(g, 1e-10, 0.85, 100, ty, us, ut, 150)
+    assert np.all(np.isfinite(ref))
+    assert float(np.max(np.abs(got - ref.astype(np.float64))) / np.max(np.abs(ref))) <= 1e-4

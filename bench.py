@@ -408,6 +408,53 @@ def bench_dyn_tc(args, O):
     return res
 
 
+def bench_dyn_pr(args, O):
+    log(f"dynamic PageRank on RMAT-{args.sssp_scale}: 3 batches of {args.dyn_batch} updates")
+    g = graph.rmat(args.sssp_scale, 16, seed=1)
+    src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
+    has_out = np.flatnonzero(np.diff(g.indexofNodes) > 0).astype(np.int32)
+    rng = np.random.default_rng(7)
+
+    def pr_batch(size):
+        half = size // 2
+        e = rng.integers(0, g.E, half)
+        types = b"d" * half + b"a" * (size - half)
+        us = np.concatenate([src_of[e], rng.choice(has_out, size - half)]).astype(np.int32)
+        ut = np.concatenate([g.edgeList[e], rng.choice(has_out, size - half)]).astype(np.int32)
+        return types, us, ut
+
+    batches = [pr_batch(args.dyn_batch) for _ in range(3)]
+    dev = g.device()
+    hip.synchronize()
+    t0 = time.perf_counter()
+    dev.dynpr_init(1e-10, 0.85, 100)
+    t_init = (time.perf_counter() - t0) * 1e3
+    times, sweeps = [], []
+    for b in batches:
+        n0 = len(dev.dynpr_iterations())
+        t0 = time.perf_counter()
+        dev.dynpr_batch(*b)
+        times.append((time.perf_counter() - t0) * 1e3)
+        sweeps.append(int(dev.dynpr_iterations()[n0:].sum()))
+    kt = kernel_time(dev, lambda: dev.dynpr_batch(*pr_batch(args.dyn_batch)),
+                     ["dpr_contrib", "dpr_sweep_light", "dpr_sweep_heavy", "dpr_apply", "dpr_diff", "dpr_propagate"],
+                     reps=1)
+    ksweep = kt["dpr_sweep_light"][0] + kt["dpr_sweep_heavy"][0] + kt["dpr_contrib"][0] + kt["dpr_apply"][0]
+    nsweep = kt["dpr_sweep_light"][1]
+    res = {"workload": f"DynPR (dynamicBatchPR, as emitted: starting out-degrees, signed diff) on directed RMAT "
+                       f"scale {args.sssp_scale} EF16 seed 1, beta 1e-10, delta 0.85, maxIter 100; batches of "
+                       f"{args.dyn_batch} updates, half deletions of existing edges, half additions between "
+                       "vertices with out-edges",
+           "V": g.V, "E": g.E, "batch": args.dyn_batch, "init_ms": round(t_init, 3),
+           "ms_per_batch": [round(t, 3) for t in times], "sweeps_per_batch": sweeps,
+           "ms_per_sweep": round(ksweep / max(nsweep, 1), 4),  # contributions, both sweep kernels, copy
+           "gteps_per_sweep": round(g.E / (ksweep / max(nsweep, 1) * 1e-3) / 1e9, 2),
+           "kernel_ms_one_batch": {k: round(v[0], 4) for k, v in kt.items() if v[1] > 0},
+           "roofline": None}
+    g.release_device()
+    return res
+
+
 # --------------------------------------------------------------------------- schedules
 def bench_schedule(args):
     from starplat_amd import mixed_batch
@@ -484,6 +531,8 @@ def main():
         per["dyn"] = bench_dyn(args, O)
     if "dyn_tc" in algos:
         per["dyn_tc"] = bench_dyn_tc(args, O)
+    if "dyn_pr" in algos:
+        per["dyn_pr"] = bench_dyn_pr(args, O)
     sched = bench_schedule(args) if "sched" in algos else None
     if head is None and per:
         first = next(iter(per.values()))

@@ -114,8 +114,11 @@ struct DynWork {
   float* pr_c = nullptr;               // [V] rank / base out-degree
   float* pr_deg = nullptr;             // [V] out-degree in the CSR the run started from
   uint8_t* pr_flag = nullptr;          // [V] the phase's modified flags
-  float* pr_part = nullptr;            // [pr_part_n] per-workgroup signed diffs
+  float* pr_part = nullptr;            // [pr_part_n + 1] per-workgroup signed diffs, then the sum
   int pr_part_n = 0;
+  int32_t* pr_heavy = nullptr;         // [pr_nheavy] vertices with long in-rows (starting CSR)
+  int32_t pr_nheavy = 0;
+  uint8_t* pr_isheavy = nullptr;       // [V] 1 for the listed vertices
   float pr_beta = 0.f, pr_delta = 0.f;
   int32_t pr_maxIter = 0;
   std::vector<int32_t> pr_iters;       // sweeps of staticPR, then Decremental / Incremental per batch
@@ -487,42 +490,112 @@ __global__ void dpr_flag_dests(int32_t n, const int32_t* __restrict__ ut, uint8_
   if (i < n) flag[ut[i]] = 1;
 }
 
-// One sweep over the flagged vertices (all when flag is null), one wavefront
-// per vertex: sum of c over live in-edges, val = (1 - delta)/N + delta*sum
-// without contraction, the signed diff val - pr[v] summed per workgroup in a
-// fixed order into part[blockIdx.x].
-__global__ __launch_bounds__(kDynBlock) void dpr_sweep(int32_t V, const int32_t* __restrict__ roff,
-                                                       const int32_t* __restrict__ radj,
-                                                       const uint32_t* __restrict__ rdead, DiffRows in,
-                                                       const float* __restrict__ c, const float* __restrict__ pr,
-                                                       const uint8_t* __restrict__ flag, float* __restrict__ nxt,
-                                                       float base, float delta, float* __restrict__ part) {
-  __shared__ float s_red[kDynWaves];
-  const int lane = __lane_id(), wid = threadIdx.x / kWave;
-  const int64_t nw = int64_t(gridDim.x) * kDynWaves;
-  float mydiff = 0.f;
-  for (int64_t x_ = int64_t(blockIdx.x) * kDynWaves + wid; x_ < V; x_ += nw) {
-    const int64_t v = spread_vertex(x_, V);
-    if (flag && !flag[v]) continue;
-    float sum = 0.f;
-    for (int32_t j = roff[v] + lane; j < roff[v + 1]; j += kWave)
-      if (!bit_of(rdead, j)) sum += c[radj[j]];
-    for (int32_t j = in.off[v] + lane; j < in.off[v + 1]; j += kWave)
-      if (!bit_of(in.dead, j)) sum += c[in.adj[j]];
-    sum = wave_sum(sum);
-    if (lane == 0) {
-      const float val = __fadd_rn(base, __fmul_rn(delta, sum));
-      mydiff += val - pr[v];
-      nxt[v] = val;
-    }
-  }
-  if (lane == 0) s_red[wid] = mydiff;
+// One sweep over the flagged vertices (all when flag is null): sum of c over
+// live in-edges, val = (1 - delta)/N + delta*sum without contraction, the
+// signed diff val - pr[v] summed per workgroup in a fixed order into part[].
+// Vertices whose in-row in the starting CSR holds at most kDprHeavy entries
+// take kDprGroup lanes each, in id order (dpr_sweep_light); the others, listed
+// and marked once at the start (so a later fold cannot move a vertex between
+// the two kernels), take a wavefront each (dpr_sweep_heavy).
+constexpr int kDprGroup = 8;
+constexpr int32_t kDprHeavy = 64;
+
+__device__ __forceinline__ float dpr_row_sum(int64_t v, int lane, int lanes, const int32_t* __restrict__ roff,
+                                             const int32_t* __restrict__ radj, const uint32_t* __restrict__ rdead,
+                                             const DiffRows& in, const float* __restrict__ c) {
+  float sum = 0.f;
+  for (int32_t j = roff[v] + lane; j < roff[v + 1]; j += lanes)
+    if (!bit_of(rdead, j)) sum += c[radj[j]];
+  for (int32_t j = in.off[v] + lane; j < in.off[v + 1]; j += lanes)
+    if (!bit_of(in.dead, j)) sum += c[in.adj[j]];
+  return sum;
+}
+
+__device__ __forceinline__ void dpr_part_store(float mydiff, float* __restrict__ part, float* s_red) {
+  mydiff = wave_sum(mydiff);
+  if (__lane_id() == 0) s_red[threadIdx.x / kWave] = mydiff;
   __syncthreads();
   if (threadIdx.x == 0) {
     float t = 0.f;
     for (int w = 0; w < kDynWaves; ++w) t += s_red[w];
     part[blockIdx.x] = t;
   }
+}
+
+__global__ __launch_bounds__(kDynBlock) void dpr_sweep_light(int32_t V, const uint8_t* __restrict__ is_heavy,
+                                                             const int32_t* __restrict__ roff,
+                                                             const int32_t* __restrict__ radj,
+                                                             const uint32_t* __restrict__ rdead, DiffRows in,
+                                                             const float* __restrict__ c, const float* __restrict__ pr,
+                                                             const uint8_t* __restrict__ flag, float* __restrict__ nxt,
+                                                             float base, float delta, float* __restrict__ part) {
+  __shared__ float s_red[kDynWaves];
+  const int g = threadIdx.x % kDprGroup;
+  const int64_t ng = int64_t(gridDim.x) * (kDynBlock / kDprGroup);
+  float mydiff = 0.f;
+  for (int64_t v = int64_t(blockIdx.x) * (kDynBlock / kDprGroup) + threadIdx.x / kDprGroup; v < V; v += ng) {
+    const bool mine = !is_heavy[v] && (!flag || flag[v]);  // uniform in the group
+    float sum = mine ? dpr_row_sum(v, g, kDprGroup, roff, radj, rdead, in, c) : 0.f;
+    sum = group_sum<kDprGroup>(sum);
+    if (mine && g == 0) {
+      const float val = __fadd_rn(base, __fmul_rn(delta, sum));
+      mydiff += val - pr[v];
+      nxt[v] = val;
+    }
+  }
+  dpr_part_store(mydiff, part, s_red);
+}
+
+// Live in-edge sum of v over the lanes of a workgroup, four independent
+// gathers in flight per lane.
+__device__ __forceinline__ float dpr_row_sum4(int32_t a, int32_t b, const int32_t* __restrict__ adj,
+                                              const uint32_t* __restrict__ dead, const float* __restrict__ c) {
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int32_t j = a + int32_t(threadIdx.x);
+  for (; j + 3 * kDynBlock < b; j += 4 * kDynBlock) {
+    const int32_t j1 = j + kDynBlock, j2 = j + 2 * kDynBlock, j3 = j + 3 * kDynBlock;
+    const int32_t u0 = adj[j], u1 = adj[j1], u2 = adj[j2], u3 = adj[j3];
+    if (!bit_of(dead, j)) s0 += c[u0];
+    if (!bit_of(dead, j1)) s1 += c[u1];
+    if (!bit_of(dead, j2)) s2 += c[u2];
+    if (!bit_of(dead, j3)) s3 += c[u3];
+  }
+  for (; j < b; j += kDynBlock)
+    if (!bit_of(dead, j)) s0 += c[adj[j]];
+  return (s0 + s1) + (s2 + s3);
+}
+
+// One workgroup per listed vertex (the list is ordered by in-row length,
+// longest first, so the largest hubs start first).
+__global__ __launch_bounds__(kDynBlock) void dpr_sweep_heavy(int32_t nh, const int32_t* __restrict__ heavy,
+                                                             const int32_t* __restrict__ roff,
+                                                             const int32_t* __restrict__ radj,
+                                                             const uint32_t* __restrict__ rdead, DiffRows in,
+                                                             const float* __restrict__ c, const float* __restrict__ pr,
+                                                             const uint8_t* __restrict__ flag, float* __restrict__ nxt,
+                                                             float base, float delta, float* __restrict__ part) {
+  __shared__ float s_red[kDynWaves];
+  __shared__ float s_sum[kDynWaves];
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  float mydiff = 0.f;
+  for (int32_t i = blockIdx.x; i < nh; i += gridDim.x) {
+    const int32_t v = heavy[i];
+    if (flag && !flag[v]) continue;  // uniform across the workgroup
+    float sum = dpr_row_sum4(roff[v], roff[v + 1], radj, rdead, c) +
+                dpr_row_sum4(in.off[v], in.off[v + 1], in.adj, in.dead, c);
+    sum = wave_sum(sum);
+    if (lane == 0) s_sum[wid] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float t = 0.f;
+      for (int w = 0; w < kDynWaves; ++w) t += s_sum[w];
+      const float val = __fadd_rn(base, __fmul_rn(delta, t));
+      mydiff += val - pr[v];
+      nxt[v] = val;
+    }
+    __syncthreads();  // s_sum reuse
+  }
+  dpr_part_store(mydiff, part, s_red);
 }
 
 // staticPR's start: every rank 1/N; the base out-degrees of the run.
@@ -927,6 +1000,8 @@ int dyn_release(spm_graph* g) {
   dfree(g, w->pr_deg, sizeof(float) * V);
   dfree(g, w->pr_flag, V);
   dfree(g, w->pr_part, sizeof(float) * (size_t(w->pr_part_n) + 1));
+  dfree(g, w->pr_heavy, sizeof(int32_t) * size_t(w->pr_nheavy));
+  dfree(g, w->pr_isheavy, V);
   delete w;
   g->dyn = nullptr;
   return SPM_OK;
@@ -1385,15 +1460,21 @@ static int dpr_sweeps(spm_graph* g, const uint8_t* flag) {
   const int32_t V = g->V;
   const float base = (1 - w->pr_delta) / float(V);  // (1 - delta) / g.num_nodes()
   const int vgrid = grid_for(g, V);
+  const int lgrid = g->num_cus * 8, hgrid = w->pr_part_n - lgrid;
   const int sgrid = w->pr_part_n;
   float* h = reinterpret_cast<float*>(g->h_pinned);
   int32_t it = 0;
   for (;;) {
     SPM_TRY(launch(g, "dpr_contrib", dpr_contrib, dim3(vgrid), dim3(kDynBlock), 0, V, (const float*)w->pr,
                    (const float*)w->pr_deg, w->pr_c));
-    SPM_TRY(launch(g, "dpr_sweep", dpr_sweep, dim3(sgrid), dim3(kDynBlock), 0, V, (const int32_t*)g->roff,
+    SPM_TRY(launch(g, "dpr_sweep_light", dpr_sweep_light, dim3(lgrid), dim3(kDynBlock), 0, V,
+                   (const uint8_t*)w->pr_isheavy, (const int32_t*)g->roff,
                    (const int32_t*)g->radj, (const uint32_t*)w->rdead, w->in[w->cur].r, (const float*)w->pr_c,
                    (const float*)w->pr, flag, w->pr_nxt, base, w->pr_delta, w->pr_part));
+    SPM_TRY(launch(g, "dpr_sweep_heavy", dpr_sweep_heavy, dim3(hgrid), dim3(kDynBlock), 0, w->pr_nheavy,
+                   (const int32_t*)w->pr_heavy, (const int32_t*)g->roff, (const int32_t*)g->radj,
+                   (const uint32_t*)w->rdead, w->in[w->cur].r, (const float*)w->pr_c, (const float*)w->pr, flag,
+                   w->pr_nxt, base, w->pr_delta, w->pr_part + lgrid));
     SPM_TRY(launch(g, "dpr_apply", dpr_apply, dim3(vgrid), dim3(kDynBlock), 0, V, flag, (const float*)w->pr_nxt,
                    w->pr));
     SPM_TRY(launch(g, "dpr_diff", dpr_diff, dim3(1), dim3(kDynBlock), 0, (const float*)w->pr_part, sgrid,
@@ -1419,8 +1500,33 @@ static int dpr_init_run(spm_graph* g, float beta, float delta, int32_t maxIter) 
     SPM_TRY(dmalloc(g, (void**)&w->pr_c, sizeof(float) * V, "dynamic PR contributions"));
     SPM_TRY(dmalloc(g, (void**)&w->pr_deg, sizeof(float) * V, "dynamic PR base degrees"));
     SPM_TRY(dmalloc(g, (void**)&w->pr_flag, V, "dynamic PR flags"));
-    w->pr_part_n = g->num_cus * 8;
+    w->pr_part_n = g->num_cus * 8 + g->num_cus * 8;  // light sweep workgroups, then heavy ones
     SPM_TRY(dmalloc(g, (void**)&w->pr_part, sizeof(float) * (size_t(w->pr_part_n) + 1), "dynamic PR diffs"));
+  }
+  {  // vertices whose in-row (starting CSR) is long take a wavefront each
+    std::vector<int32_t> roff(V + 1), heavy;
+    std::vector<uint8_t> mark(V, 0);
+    SPM_CHECK_HIP(hipMemcpyAsync(roff.data(), g->roff, sizeof(int32_t) * (V + 1), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    for (size_t v = 0; v < V; ++v)
+      if (roff[v + 1] - roff[v] > kDprHeavy) {
+        heavy.push_back(int32_t(v));
+        mark[v] = 1;
+      }
+    std::stable_sort(heavy.begin(), heavy.end(), [&](int32_t a, int32_t b) {
+      return roff[a + 1] - roff[a] > roff[b + 1] - roff[b];
+    });
+    if (!w->pr_isheavy) SPM_TRY(dmalloc(g, (void**)&w->pr_isheavy, V, "dynamic PR heavy marks"));
+    SPM_CHECK_HIP(hipMemcpyAsync(w->pr_isheavy, mark.data(), V, hipMemcpyHostToDevice, g->stream));
+    dfree(g, w->pr_heavy, sizeof(int32_t) * size_t(w->pr_nheavy));
+    w->pr_heavy = nullptr;
+    w->pr_nheavy = int32_t(heavy.size());
+    if (!heavy.empty()) {
+      SPM_TRY(dmalloc(g, (void**)&w->pr_heavy, sizeof(int32_t) * heavy.size(), "dynamic PR heavy rows"));
+      SPM_CHECK_HIP(hipMemcpyAsync(w->pr_heavy, heavy.data(), sizeof(int32_t) * heavy.size(), hipMemcpyHostToDevice,
+                                   g->stream));
+    }
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));  // the host vectors end with this block
   }
   w->pr_beta = beta;
   w->pr_delta = delta;

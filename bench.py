@@ -475,7 +475,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--algos", default="sssp,pr,pr_c0,tc,bc,dyn,dyn_tc,sched")
+    ap.add_argument("--algos", default="sssp,pr,pr_c0,tc,bc,dyn,dyn_tc,dyn_pr,sched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--sssp-scale", type=int, default=22)

@@ -28,6 +28,15 @@
 // Every restated function takes `plant`: 0 is the restatement; a non-zero
 // value plants one deliberate mistake so the tests can show that the pinning
 // checks catch it.
+//
+// Pinning status: the static four, DynSSSP and DynTC are pinned by golden
+// vectors, textbook routines, closed forms and recounts (tests/test_oracle*.py).
+// DynPR is partly pinned: its closed forms and an independent float64
+// recomputation check the restatement's arithmetic and bookkeeping, but the
+// emitted semantics it keeps (the starting out-degree as count_outNbrs, the
+// signed diff, propagateNodeFlags flagging every vertex with an in-edge) are
+// fixed by the cited generator and graph.hpp lines alone -- no run of the
+// reference confirms them.
 #include <omp.h>
 
 #include <algorithm>

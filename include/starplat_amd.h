@@ -218,7 +218,9 @@ typedef struct spm_schedule {
   int32_t pr_t2;          /* <= t2: pr_group lanes per vertex                */
   int32_t pr_t3;          /* <= t3: one wavefront per vertex; above: block   */
   int32_t pr_group;       /* lanes per vertex in the second bin (2..32)      */
-  int32_t sssp_mode;      /* 0 edge-balanced frontier queue, 1 dense sweep   */
+  int32_t sssp_mode;      /* 0 edge-balanced frontier queue with the first
+                             frontier in distance bands, 1 dense sweep,
+                             2 queue in vertex order throughout            */
   int32_t sssp_rounds;    /* rounds per captured graph (multiple of 6)       */
   int32_t sssp_grid;      /* workgroups per relaxation launch (0 = auto)     */
   int32_t tc_mode;        /* 1 per-vertex LDS hash (default), 0 edge search */

@@ -109,7 +109,7 @@ int validate_schedule(const spm_schedule* s) {
   if (s->pr_t1 < 0 || s->pr_t2 < s->pr_t1 || s->pr_t3 < s->pr_t2) return set_error(SPM_EINVAL, "schedule: PR thresholds must be non-decreasing");
   if (s->pr_group != 2 && s->pr_group != 4 && s->pr_group != 8 && s->pr_group != 16 && s->pr_group != 32)
     return set_error(SPM_EINVAL, "schedule: pr_group must be 2, 4, 8, 16 or 32");
-  if (s->sssp_mode < 0 || s->sssp_mode > 1) return set_error(SPM_EINVAL, "schedule: sssp_mode must be 0 or 1");
+  if (s->sssp_mode < 0 || s->sssp_mode > 2) return set_error(SPM_EINVAL, "schedule: sssp_mode must be 0, 1 or 2");
   if (s->sssp_rounds < 6 || s->sssp_rounds > 96 || s->sssp_rounds % 6) return set_error(SPM_EINVAL, "schedule: sssp_rounds must be a multiple of 6 in [6, 96]");
   if (s->sssp_grid < 0 || s->sssp_grid > 65536) return set_error(SPM_EINVAL, "schedule: sssp_grid out of range");
   if (s->tc_mode < 0 || s->tc_mode > 1) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0 (search) or 1 (hash)");

@@ -52,6 +52,10 @@ constexpr int kSsspWaves = kSsspBlock / kWave;
 // kSparseEdges / kChunk reservations); larger rounds are compacted by tiles.
 constexpr uint32_t kSparseEdges = 1u << 19;
 constexpr int kTileV = kSsspBlock * 16;  // vertices per compaction tile
+// Distance bands of the first frontier (see sssp_count_b) and the block size
+// of their one-workgroup scan.
+constexpr int kBands = 16;
+constexpr int kScanBlock = 1024;
 
 struct SsspQueue {
   int32_t* vtx;     // [V] frontier vertices (each pushed at most once a round)
@@ -71,7 +75,9 @@ struct SsspWork {
   int32_t* dist_out = nullptr;        // [V] distances in original ids (relabeled runs)
   uint32_t* pk = nullptr;             // [E] (neighbour << wb | weight) when both fit 32 bits
   int wb = 0;                         // weight bits of pk (0: not packed)
-  unsigned long long* wstat = nullptr;  // [2] weight range of the view (min, max)
+  unsigned long long* wstat = nullptr;  // [2] weight range of the view (any negative, max)
+  uint2* tile_b = nullptr;            // [kBands][tiles] (vertices, edges) per band and tile
+  int64_t maxw = -1;                  // largest weight when none is negative, else -1
   bool dist_out_ok = false;
   CsrView view{};                     // the CSR the kernels run on
   size_t qcap = 0, ccap = 0, src_cap = 0;
@@ -119,22 +125,24 @@ __global__ void sssp_pack(int64_t E, const int32_t* __restrict__ adj, const int3
     pk[i] = (uint32_t(adj[i]) << wb) | uint32_t(w[i]);
 }
 
-// Block-wide exclusive scan of two counters (4 wavefronts of 64 lanes).
+// Block-wide exclusive scan of two counters (NW wavefronts of 64 lanes;
+// s_tot holds 2 * NW words).
+template <int NW = kSsspWaves>
 __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
                                             uint32_t& ta, uint32_t& tb, uint32_t* s_tot) {
   const int lane = __lane_id(), wid = threadIdx.x / kWave;
   const uint32_t ia = wave_incl_scan(a), ib = wave_incl_scan(b);
   if (lane == kWave - 1) {
     s_tot[wid] = ia;
-    s_tot[kSsspWaves + wid] = ib;
+    s_tot[NW + wid] = ib;
   }
   __syncthreads();
   uint32_t pa = 0, pb = 0;
   ta = 0;
   tb = 0;
 #pragma unroll
-  for (int w = 0; w < kSsspWaves; ++w) {
-    const uint32_t xa = s_tot[w], xb = s_tot[kSsspWaves + w];
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t xa = s_tot[w], xb = s_tot[NW + w];
     if (w < wid) { pa += xa; pb += xb; }
     ta += xa;
     tb += xb;
@@ -145,11 +153,12 @@ __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea
 
 // Seeds the first frontier with n distinct sources (one workgroup): dist 0,
 // queue entries with their degree prefix, and the chunk index of every chunk
-// starting inside a source's edges. srcs == nullptr means the single source
-// `one`.
+// starting inside a source's edges, in queue `slot` (counter ctr[slot]; the
+// other counters, rounds included, start at zero). srcs == nullptr means the
+// single source `one`.
 __global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restrict__ srcs, int32_t n, int32_t one,
                                                         const int32_t* __restrict__ off, int32_t* __restrict__ dist,
-                                                        SsspQueue q0, unsigned long long* __restrict__ ctr,
+                                                        SsspQueue q0, unsigned long long* __restrict__ ctr, int slot,
                                                         uint8_t* __restrict__ active) {
   __shared__ uint32_t s_tot[2 * kSsspWaves];
   uint32_t cnt = 0, edges = 0;  // running totals (uniform across the block)
@@ -176,12 +185,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restric
     edges += td;
     __syncthreads();  // s_tot reuse
   }
-  if (threadIdx.x == 0) {
-    ctr[0] = (static_cast<unsigned long long>(cnt) << 32) | edges;
-    ctr[1] = 0;
-    ctr[2] = 0;
-    ctr[3] = 0;
-  }
+  if (threadIdx.x < 4)
+    ctr[threadIdx.x] = int(threadIdx.x) == slot ? (static_cast<unsigned long long>(cnt) << 32) | edges : 0ull;
 }
 
 // PACKED: the frontier's edges are read as one 32-bit word (neighbour << wb |
@@ -193,7 +198,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
     int32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
     SsspQueue qin, SsspQueue qout, unsigned long long* __restrict__ ctr_in,
     unsigned long long* __restrict__ ctr_out, unsigned long long* __restrict__ ctr_clear,
-    unsigned long long* __restrict__ rounds, const uint32_t* __restrict__ round_base, uint32_t rloc) {
+    unsigned long long* __restrict__ rounds, const uint32_t* __restrict__ round_base, uint32_t rloc,
+    uint32_t sparse_max) {
   __shared__ uint32_t s_off[kChunk + 1];
   __shared__ int32_t s_dv[kChunk + 1];
   __shared__ int32_t s_base[kChunk + 1];
@@ -210,9 +216,10 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
   const uint32_t sv = *round_base + rloc;
   const uint32_t nchunks = (T + kChunk - 1) / kChunk;
   const int tid = threadIdx.x;
-  // Small rounds push through the block-aggregated counter; large rounds
-  // only mark improved vertices and leave the queue to sssp_count/sssp_emit.
-  const bool sparse = T < kSparseEdges;
+  // Small rounds (below sparse_max edges) push through the block-aggregated
+  // counter; large rounds only mark improved vertices and leave the queue to
+  // sssp_count/sssp_emit (or sssp_count_b/sssp_emit_b).
+  const bool sparse = T < sparse_max;
 
   for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const int32_t p0 = qin.cfirst[chunk];
@@ -409,6 +416,115 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
     *ctr_out = (static_cast<unsigned long long>(bc + tc) << 32) | (be + te);
 }
 
+// First frontier in distance bands. The seeds' round marks its improved
+// vertices (all of them at distance seed + one edge, so within [0, maxw]);
+// instead of ascending vertex order they are queued by band
+// b = min(kBands - 1, dist / delta), tile after tile within a band. The
+// next round's chunks are dispatched roughly nearest first, so the cached
+// pre-filter of its later, farther relaxations sees more final distances
+// (RMAT-22, simulated with 1536 resident 1024-edge chunks: 12% fewer edges
+// scanned over the whole run). Three kernels: per tile and band counts, one
+// workgroup's exclusive scan over them in band-major order, and the emit.
+__device__ __forceinline__ void tile_bands(int32_t V, const int32_t* __restrict__ off,
+                                           const uint32_t* __restrict__ stamp, const int32_t* __restrict__ dist,
+                                           uint32_t sv, int64_t v0, int32_t delta, uint32_t (&deg)[16],
+                                           uint32_t (&band)[16]) {
+  tile_flags(V, off, stamp, sv, v0, deg);
+#pragma unroll
+  for (int k = 0; k < 16; ++k)
+    band[k] = deg[k] ? min(uint32_t(kBands - 1), uint32_t(dist[v0 + k] / delta)) : 0u;
+}
+
+__global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int32_t* __restrict__ off,
+                                                           const uint32_t* __restrict__ stamp,
+                                                           const int32_t* __restrict__ dist,
+                                                           const uint32_t* __restrict__ round_base, int32_t delta,
+                                                           int32_t tiles, uint2* __restrict__ tile_b) {
+  __shared__ uint32_t s_c[kBands], s_e[kBands];
+  if (threadIdx.x < kBands) {
+    s_c[threadIdx.x] = 0;
+    s_e[threadIdx.x] = 0;
+  }
+  __syncthreads();
+  uint32_t deg[16], band[16];
+  tile_bands(V, off, stamp, dist, *round_base, int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16, delta, deg,
+             band);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (!deg[k]) continue;
+    atomicAdd(&s_c[band[k]], 1u);
+    atomicAdd(&s_e[band[k]], deg[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBands) tile_b[int64_t(threadIdx.x) * tiles + blockIdx.x] = make_uint2(s_c[threadIdx.x], s_e[threadIdx.x]);
+}
+
+// One workgroup: exclusive prefix of the n = kBands * tiles (vertices, edges)
+// pairs in place, 16 consecutive pairs per lane; the total is the frontier
+// size. (A last-arriving counting workgroup could scan instead, but the
+// device-scope release each workgroup would need before arriving costs more
+// than this launch.)
+__global__ __launch_bounds__(kScanBlock) void sssp_scan_b(int32_t n, uint2* __restrict__ tile_b,
+                                                          unsigned long long* __restrict__ ctr_out) {
+  __shared__ uint32_t s_tot[2 * (kScanBlock / kWave)];
+  uint32_t rc = 0, re = 0;
+  for (int64_t base = 0; base < n; base += int64_t(kScanBlock) * 16) {
+    const int64_t i0 = base + int64_t(threadIdx.x) * 16;
+    uint2 x[16];
+    uint32_t c = 0, e = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      x[k] = i0 + k < n ? tile_b[i0 + k] : make_uint2(0u, 0u);
+      c += x[k].x;
+      e += x[k].y;
+    }
+    uint32_t ec, ee, tc, te;
+    block_scan2<kScanBlock / kWave>(c, e, ec, ee, tc, te, s_tot);
+    uint32_t pc = rc + ec, pe = re + ee;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (i0 + k < n) tile_b[i0 + k] = make_uint2(pc, pe);
+      pc += x[k].x;
+      pe += x[k].y;
+    }
+    rc += tc;
+    re += te;
+    __syncthreads();  // s_tot reuse
+  }
+  if (threadIdx.x == 0) *ctr_out = (static_cast<unsigned long long>(rc) << 32) | re;
+}
+
+// Positions within a (band, tile) come from one 64-bit LDS counter per band,
+// (entries << 32 | edges), which starts at the band's prefix for this tile:
+// entry positions and edge prefixes advance together, so the queue stays
+// consistent whatever order the lanes take.
+__global__ __launch_bounds__(kSsspBlock) void sssp_emit_b(int32_t V, const int32_t* __restrict__ off,
+                                                          const uint32_t* __restrict__ stamp,
+                                                          const int32_t* __restrict__ dist,
+                                                          const uint32_t* __restrict__ round_base, int32_t delta,
+                                                          int32_t tiles, const uint2* __restrict__ tile_b,
+                                                          SsspQueue qout) {
+  __shared__ unsigned long long s_cur[kBands];
+  if (threadIdx.x < kBands) {
+    const uint2 p = tile_b[int64_t(threadIdx.x) * tiles + blockIdx.x];
+    s_cur[threadIdx.x] = (static_cast<unsigned long long>(p.x) << 32) | p.y;
+  }
+  const int64_t v0 = int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
+  uint32_t deg[16], band[16];
+  tile_bands(V, off, stamp, dist, *round_base, v0, delta, deg, band);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t d = deg[k];
+    if (!d) continue;
+    const unsigned long long at = atomicAdd(&s_cur[band[k]], (1ull << 32) | d);
+    const uint32_t pos = uint32_t(at >> 32), eo = uint32_t(at);
+    qout.vtx[pos] = int32_t(v0 + k);
+    qout.eoff[pos] = eo;
+    for (uint32_t cc = (eo + kChunk - 1) / kChunk; cc <= (eo + d - 1) / kChunk; ++cc) qout.cfirst[cc] = int32_t(pos);
+  }
+}
+
 // Dense mode: the reference's full-V sweep of the modified flags, one lane per
 // vertex with the out-edges of a wavefront's 64 vertices flattened across the
 // lanes (wave prefix sum + LDS binary search), two flag arrays swapped per round.
@@ -478,6 +594,7 @@ static int sssp_pack_edges(spm_graph* g) {
   SsspWork* w = g->sssp;
   const int64_t E = g->E;
   w->wb = 0;
+  w->maxw = -1;
   if (!w->view.w || E == 0) return SPM_OK;
   SPM_CHECK_HIP(hipMemsetAsync(w->wstat, 0, sizeof(unsigned long long) * 2, g->stream));
   const int grid = int(std::min<int64_t>((E + 255) / 256, int64_t(g->num_cus) * 16));
@@ -486,6 +603,7 @@ static int sssp_pack_edges(spm_graph* g) {
   SPM_CHECK_HIP(hipMemcpyAsync(h, w->wstat, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   if (h[0]) return SPM_OK;  // negative weights: unpacked
+  w->maxw = int64_t(h[1]);
   int wb = 1, vb = 1;
   while ((1ull << wb) <= h[1]) ++wb;
   while ((int64_t(1) << vb) < int64_t(g->V)) ++vb;
@@ -528,6 +646,8 @@ static int sssp_alloc(spm_graph* g, const CsrView& view) {
   SPM_TRY(dmalloc(g, (void**)&w->active, 2 * size_t(V), "sssp flags"));
   SPM_TRY(dmalloc(g, (void**)&w->tile_sum, sizeof(uint2) * ((size_t(V) + kTileV - 1) / kTileV), "sssp tiles"));
   SPM_TRY(dmalloc(g, (void**)&w->wstat, sizeof(unsigned long long) * 2, "sssp weight range"));
+  SPM_TRY(dmalloc(g, (void**)&w->tile_b, sizeof(uint2) * kBands * ((size_t(V) + kTileV - 1) / kTileV),
+                  "sssp band tiles"));
   SPM_TRY(sssp_pack_edges(g));
   SPM_CHECK_HIP(hipMemsetAsync(w->stamp, 0, sizeof(uint32_t) * V, g->stream));
   SPM_CHECK_HIP(hipMemcpyAsync(w->round_base, &w->host_base, sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
@@ -555,6 +675,7 @@ int sssp_release(spm_graph* g) {
   dfree(g, w->dist_out, sizeof(int32_t) * g->V);
   dfree(g, w->pk, sizeof(uint32_t) * size_t(g->E));
   dfree(g, w->wstat, sizeof(unsigned long long) * 2);
+  dfree(g, w->tile_b, sizeof(uint2) * kBands * ((size_t(g->V) + kTileV - 1) / kTileV));
   delete w;
   g->sssp = nullptr;
   return SPM_OK;
@@ -582,7 +703,7 @@ static int sssp_enqueue_rounds(spm_graph* g, int R) {
       SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(grid), dim3(kSsspBlock), 0,
                      w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
                      w->stamp, w->q[i], w->q[o], w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3,
-                     (const uint32_t*)w->round_base, uint32_t(r)));
+                     (const uint32_t*)w->round_base, uint32_t(r), kSparseEdges));
       const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
       SPM_TRY(launch(g, "sssp_count", sssp_count, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
                      (const uint32_t*)w->stamp, (const unsigned long long*)(w->ctr + i),
@@ -666,8 +787,30 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   const int igrid = int(std::min<int64_t>((g->V + 255) / 256, 4096));
   uint8_t* act = dense ? w->active : (uint8_t*)nullptr;
   SPM_TRY(launch(g, "sssp_init", sssp_init, dim3(igrid), dim3(256), 0, g->V, w->dist, act));
+  // Queue mode with non-negative weights: the seeds' round runs here, outside
+  // the captured rounds, and queues its frontier in distance bands.
+  const bool banded = g->sched.sssp_mode == 0 && w->maxw >= 0;
+  const int slot = banded ? 2 : 0;
   SPM_TRY(launch(g, "sssp_seed", sssp_seed, dim3(1), dim3(kSsspBlock), 0, n > 1 ? (const int32_t*)w->srcbuf : nullptr,
-                 n, srcs[0], w->view.off, w->dist, w->q[0], w->ctr, act));
+                 n, srcs[0], w->view.off, w->dist, w->q[slot], w->ctr, slot, act));
+  if (banded) {
+    const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
+    const int32_t delta = int32_t(w->maxw / kBands + 1);
+    SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(sssp_grid(g)),
+                   dim3(kSsspBlock), 0, w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
+                   w->stamp, w->q[2], w->q[0], w->ctr + 2, w->ctr + 0, w->ctr + 1, w->ctr + 3,
+                   (const uint32_t*)w->round_base, 0u, 0u));
+    SPM_TRY(launch(g, "sssp_count_b", sssp_count_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
+                   (const uint32_t*)w->stamp, (const int32_t*)w->dist, (const uint32_t*)w->round_base, delta, tiles,
+                   w->tile_b));
+    SPM_TRY(launch(g, "sssp_scan_b", sssp_scan_b, dim3(1), dim3(kScanBlock), 0, int32_t(kBands * tiles), w->tile_b,
+                   w->ctr + 0));
+    SPM_TRY(launch(g, "sssp_emit_b", sssp_emit_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
+                   (const uint32_t*)w->stamp, (const int32_t*)w->dist, (const uint32_t*)w->round_base, delta, tiles,
+                   (const uint2*)w->tile_b, w->q[0]));
+    SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, 1u));
+    w->host_base += 1;
+  }
   const bool use_graph = !g->prof.on;
   if (use_graph) SPM_TRY(sssp_capture(g));
   unsigned long long* h = reinterpret_cast<unsigned long long*>(g->h_pinned);

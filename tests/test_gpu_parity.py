@@ -61,7 +61,7 @@ def make(kind, kw, weights=(1, 100)):
 
 
 @pytest.mark.parametrize("kind,kw", SSSP_CASES)
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_sssp_parity(oracle, kind, kw, mode):
     g = make(kind, kw)
     dev = g.device()

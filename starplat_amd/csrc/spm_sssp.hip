@@ -81,9 +81,14 @@ struct SsspWork {
   bool dist_out_ok = false;
   CsrView view{};                     // the CSR the kernels run on
   size_t qcap = 0, ccap = 0, src_cap = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int rounds_per_graph = 0;
+  // Captured rounds: [0] the schedule's R rounds, [1] the first replay's
+  // length fitted to the previous run (a multiple of 3 below R).
+  struct Captured {
+    int rounds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } cap[2];
+  int last_rounds = 0;  // non-empty captured rounds of the previous queue-mode run (0: none)
   uint32_t host_base = 1;
 };
 
@@ -658,8 +663,10 @@ static int sssp_alloc(spm_graph* g, const CsrView& view) {
 int sssp_release(spm_graph* g) {
   SsspWork* w = g->sssp;
   if (!w) return SPM_OK;
-  if (w->exec) (void)hipGraphExecDestroy(w->exec);
-  if (w->graph) (void)hipGraphDestroy(w->graph);
+  for (auto& c : w->cap) {
+    if (c.exec) (void)hipGraphExecDestroy(c.exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+  }
   for (int i = 0; i < 3; ++i) {
     dfree(g, w->q[i].vtx, sizeof(int32_t) * w->qcap);
     dfree(g, w->q[i].eoff, sizeof(uint32_t) * w->qcap);
@@ -687,7 +694,8 @@ static int sssp_grid(const spm_graph* g) {
   return g->sched.sssp_grid > 0 ? g->sched.sssp_grid : g->num_cus * 6;
 }
 
-// Enqueues R rounds (R a multiple of 6: queues rotate by 3, dense flags by 2) followed by the round-base advance.
+// Enqueues R rounds followed by the round-base advance. R is a multiple of 3
+// (queues rotate by 3), and of 6 in dense mode (flags alternate by 2).
 static int sssp_enqueue_rounds(spm_graph* g, int R) {
   SsspWork* w = g->sssp;
   const int grid = sssp_grid(g);
@@ -718,21 +726,24 @@ static int sssp_enqueue_rounds(spm_graph* g, int R) {
   return SPM_OK;
 }
 
-static int sssp_capture(spm_graph* g) {
-  SsspWork* w = g->sssp;
-  const int R = g->sched.sssp_rounds;
-  if (w->exec && w->rounds_per_graph == R) return SPM_OK;
-  if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
-  if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
-  SPM_CHECK_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-  int rc = sssp_enqueue_rounds(g, R);
-  hipGraph_t graph = nullptr;
-  hipError_t e = hipStreamEndCapture(g->stream, &graph);
-  if (rc != SPM_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
-  if (e != hipSuccess) return set_error(SPM_EHIP, std::string("sssp capture: ") + hipGetErrorString(e));
-  w->graph = graph;
-  SPM_CHECK_HIP(hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0));
-  w->rounds_per_graph = R;
+// The executable graph of R rounds in slot k, captured on first use.
+static int sssp_capture(spm_graph* g, int k, int R, hipGraphExec_t* out) {
+  SsspWork::Captured& c = g->sssp->cap[k];
+  if (!c.exec || c.rounds != R) {
+    if (c.exec) { (void)hipGraphExecDestroy(c.exec); c.exec = nullptr; }
+    if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
+    c.rounds = 0;
+    SPM_CHECK_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    int rc = sssp_enqueue_rounds(g, R);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(g->stream, &graph);
+    if (rc != SPM_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return set_error(SPM_EHIP, std::string("sssp capture: ") + hipGetErrorString(e));
+    c.graph = graph;
+    SPM_CHECK_HIP(hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0));
+    c.rounds = R;
+  }
+  *out = c.exec;
   return SPM_OK;
 }
 
@@ -811,15 +822,28 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
     SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, 1u));
     w->host_base += 1;
   }
+  // Rounds after the frontier empties still cost their launches (~13 us a
+  // round at RMAT-22), so in queue mode the first replay is as long as the
+  // previous run on this handle needed plus one round of slack (the count
+  // varies by a round between runs of the same source), rounded up to a
+  // multiple of 3, the queue rotation. A longer run simply continues with
+  // replays of R rounds.
+  int R1 = R;
+  if (!dense && w->last_rounds > 0) R1 = std::min(R, 3 * ((w->last_rounds + 3) / 3));
   const bool use_graph = !g->prof.on;
-  if (use_graph) SPM_TRY(sssp_capture(g));
   unsigned long long* h = reinterpret_cast<unsigned long long*>(g->h_pinned);
-  const int64_t max_replays = int64_t(g->V) / R + 2;
+  const int64_t max_replays = int64_t(g->V) / R + 3;
   int64_t replays = 0;
   for (;;) {
-    if (use_graph) SPM_CHECK_HIP(hipGraphLaunch(w->exec, g->stream));
-    else SPM_TRY(sssp_enqueue_rounds(g, R));
-    w->host_base += uint32_t(R);
+    const int Rk = replays == 0 ? R1 : R;
+    if (use_graph) {
+      hipGraphExec_t exec = nullptr;
+      SPM_TRY(sssp_capture(g, Rk == R ? 0 : 1, Rk, &exec));
+      SPM_CHECK_HIP(hipGraphLaunch(exec, g->stream));
+    } else {
+      SPM_TRY(sssp_enqueue_rounds(g, Rk));
+    }
+    w->host_base += uint32_t(Rk);
     SPM_CHECK_HIP(hipMemcpyAsync(h, w->ctr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
     SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
     if (h[0] == 0ull) break;
@@ -829,6 +853,7 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   if (d_out) SPM_TRY(gather_back(g, w->view, w->dist, d_out));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   if (rounds_out) *rounds_out = int32_t(h[0]);
+  if (!dense) w->last_rounds = int(std::max<int64_t>(1, int64_t(h[0]) - (banded ? 1 : 0)));
   SPM_TRY(prof_flush(g));
   return SPM_OK;
 }

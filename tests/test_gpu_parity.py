@@ -82,6 +82,20 @@ def test_sssp_schedules_bit_identical(oracle, rounds, grid):
     np.testing.assert_array_equal(dev.sssp(0), ref)  # replay of the captured graph
 
 
+def test_sssp_rounds_fitted_to_previous_run(oracle):
+    # The first replay's length follows the previous run on the handle: a
+    # deep run after a shallow one (and after one with no out-edges) must
+    # continue with further replays; a shallow run after a deep one stops early.
+    n = 400
+    src = np.concatenate([np.arange(n - 1), np.arange(0, n - 1, 7)]).astype(np.int32)
+    dst = np.concatenate([np.arange(1, n), np.arange(0, n - 1, 7) + 1]).astype(np.int32)
+    w = (np.arange(src.size) % 5 + 1).astype(np.int32)
+    g = graph.from_edges(src, dst, w)
+    dev = g.device()
+    for s in (0, n - 1, 0, n - 3, n // 2, 0, 0):
+        np.testing.assert_array_equal(dev.sssp(s), oracle.compute_sssp(g, g.edgeLen, s))
+
+
 def test_sssp_edge_cases(oracle):
     # single vertex, no edges
     g = graph.from_edges(np.array([0], np.int32), np.array([0], np.int32), np.array([5], np.int32))

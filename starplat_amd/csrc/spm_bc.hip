@@ -45,6 +45,8 @@ constexpr int kBatch = 64;
 constexpr uint64_t kPushRatio = 14;
 // Active neighbours whose sigma rows one wavefront loads per step.
 constexpr int kGather = 4;
+// In-edges one lane of the bottom-up scan reads per step.
+constexpr int kScanStep = 4;
 
 struct BcWork {
   double* sigma = nullptr;          // [V][64]
@@ -97,8 +99,20 @@ __global__ __launch_bounds__(kBcBlock) void bc_scan(
       const unsigned long long vis = visited[w];
       const unsigned long long unv = batchmask & ~vis;
       if (unv) {
-        // Stops once every source still missing at w has been found.
-        for (int32_t i = roff[w], e = roff[w + 1]; i < e && (nm & unv) != unv; ++i) nm |= cur[radj[i]];
+        // Four in-edges per step (their loads issued together); stops once
+        // every source still missing at w has been found. Edges read past
+        // that point only add bits that are set already or masked off below.
+        const int32_t e = roff[w + 1];
+        for (int32_t i = roff[w]; i < e && (nm & unv) != unv; i += kScanStep) {
+          int32_t u[kScanStep];
+#pragma unroll
+          for (int k = 0; k < kScanStep; ++k) u[k] = i + k < e ? radj[i + k] : -1;
+          unsigned long long c[kScanStep];
+#pragma unroll
+          for (int k = 0; k < kScanStep; ++k) c[k] = u[k] >= 0 ? cur[u[k]] : 0ull;
+#pragma unroll
+          for (int k = 0; k < kScanStep; ++k) nm |= c[k];
+        }
         nm &= unv;
         if (nm) visited[w] = vis | nm;
       }

@@ -52,10 +52,8 @@ constexpr int kSsspWaves = kSsspBlock / kWave;
 // kSparseEdges / kChunk reservations); larger rounds are compacted by tiles.
 constexpr uint32_t kSparseEdges = 1u << 19;
 constexpr int kTileV = kSsspBlock * 16;  // vertices per compaction tile
-// Distance bands of the first frontier (see sssp_count_b) and the block size
-// of their one-workgroup scan.
+// Distance bands of the first frontier (see sssp_count_b).
 constexpr int kBands = 16;
-constexpr int kScanBlock = 1024;
 
 struct SsspQueue {
   int32_t* vtx;     // [V] frontier vertices (each pushed at most once a round)
@@ -76,7 +74,8 @@ struct SsspWork {
   uint32_t* pk = nullptr;             // [E] (neighbour << wb | weight) when both fit 32 bits
   int wb = 0;                         // weight bits of pk (0: not packed)
   unsigned long long* wstat = nullptr;  // [2] weight range of the view (any negative, max)
-  uint2* tile_b = nullptr;            // [kBands][tiles] (vertices, edges) per band and tile
+  uint2* tile_b = nullptr;            // [kBands][tiles] (vertices, edges) before each tile within its band
+  unsigned long long* band_tot = nullptr;  // [kBands] (vertices << 32 | edges) per band
   int64_t maxw = -1;                  // largest weight when none is negative, else -1
   bool dist_out_ok = false;
   CsrView view{};                     // the CSR the kernels run on
@@ -85,10 +84,14 @@ struct SsspWork {
   // length fitted to the previous run (a multiple of 3 below R).
   struct Captured {
     int rounds = 0;
+    std::vector<uint8_t> comp;  // rounds with compaction kernels (empty: all)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } cap[2];
   int last_rounds = 0;  // non-empty captured rounds of the previous queue-mode run (0: none)
+  unsigned long long big_hist[2] = {~0ull, ~0ull};  // big rounds of the previous run (bit r)
+  uint32_t* run_start = nullptr;         // [1] round stamp of the first captured round
+  unsigned long long* big = nullptr;     // [2] big rounds of this run (bit r)
   uint32_t host_base = 1;
 };
 
@@ -130,24 +133,22 @@ __global__ void sssp_pack(int64_t E, const int32_t* __restrict__ adj, const int3
     pk[i] = (uint32_t(adj[i]) << wb) | uint32_t(w[i]);
 }
 
-// Block-wide exclusive scan of two counters (NW wavefronts of 64 lanes;
-// s_tot holds 2 * NW words).
-template <int NW = kSsspWaves>
+// Block-wide exclusive scan of two counters (4 wavefronts of 64 lanes).
 __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea, uint32_t& eb,
                                             uint32_t& ta, uint32_t& tb, uint32_t* s_tot) {
   const int lane = __lane_id(), wid = threadIdx.x / kWave;
   const uint32_t ia = wave_incl_scan(a), ib = wave_incl_scan(b);
   if (lane == kWave - 1) {
     s_tot[wid] = ia;
-    s_tot[NW + wid] = ib;
+    s_tot[kSsspWaves + wid] = ib;
   }
   __syncthreads();
   uint32_t pa = 0, pb = 0;
   ta = 0;
   tb = 0;
 #pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    const uint32_t xa = s_tot[w], xb = s_tot[NW + w];
+  for (int w = 0; w < kSsspWaves; ++w) {
+    const uint32_t xa = s_tot[w], xb = s_tot[kSsspWaves + w];
     if (w < wid) { pa += xa; pb += xb; }
     ta += xa;
     tb += xb;
@@ -159,12 +160,19 @@ __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea
 // Seeds the first frontier with n distinct sources (one workgroup): dist 0,
 // queue entries with their degree prefix, and the chunk index of every chunk
 // starting inside a source's edges, in queue `slot` (counter ctr[slot]; the
-// other counters, rounds included, start at zero). srcs == nullptr means the
-// single source `one`.
+// other counters, rounds included, start at zero; so do the band totals and
+// the big-round log, and run_start takes the first captured round's stamp).
+// srcs == nullptr means the single source `one`.
 __global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restrict__ srcs, int32_t n, int32_t one,
                                                         const int32_t* __restrict__ off, int32_t* __restrict__ dist,
                                                         SsspQueue q0, unsigned long long* __restrict__ ctr, int slot,
+                                                        unsigned long long* __restrict__ band_tot,
+                                                        uint32_t* __restrict__ run_start, uint32_t start_val,
+                                                        unsigned long long* __restrict__ big,
                                                         uint8_t* __restrict__ active) {
+  if (band_tot && threadIdx.x < kBands) band_tot[threadIdx.x] = 0ull;
+  if (threadIdx.x < 2) big[threadIdx.x] = 0ull;
+  if (threadIdx.x == 0) *run_start = start_val;
   __shared__ uint32_t s_tot[2 * kSsspWaves];
   uint32_t cnt = 0, edges = 0;  // running totals (uniform across the block)
   for (int32_t base = 0; base < n; base += kSsspBlock) {
@@ -204,7 +212,7 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
     SsspQueue qin, SsspQueue qout, unsigned long long* __restrict__ ctr_in,
     unsigned long long* __restrict__ ctr_out, unsigned long long* __restrict__ ctr_clear,
     unsigned long long* __restrict__ rounds, const uint32_t* __restrict__ round_base, uint32_t rloc,
-    uint32_t sparse_max) {
+    uint32_t sparse_max, const uint32_t* __restrict__ run_start, unsigned long long* __restrict__ big) {
   __shared__ uint32_t s_off[kChunk + 1];
   __shared__ int32_t s_dv[kChunk + 1];
   __shared__ int32_t s_base[kChunk + 1];
@@ -213,12 +221,16 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
 
   const unsigned long long c = *ctr_in;
   const uint32_t n = uint32_t(c >> 32), T = uint32_t(c);
+  const uint32_t sv = *round_base + rloc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *ctr_clear = 0ull;
     if (T) *rounds += 1;
+    // Rounds of this run (counted from the captured ones) whose frontier was
+    // large enough to want compaction kernels next time.
+    const uint32_t ar = sv - *run_start;
+    if (big && ar < 128u && T >= kSparseEdges / 4) atomicOr(&big[ar >> 6], 1ull << (ar & 63u));
   }
   if (T == 0) return;
-  const uint32_t sv = *round_base + rloc;
   const uint32_t nchunks = (T + kChunk - 1) / kChunk;
   const int tid = threadIdx.x;
   // Small rounds (below sparse_max edges) push through the block-aggregated
@@ -428,8 +440,9 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
 // next round's chunks are dispatched roughly nearest first, so the cached
 // pre-filter of its later, farther relaxations sees more final distances
 // (RMAT-22, simulated with 1536 resident 1024-edge chunks: 12% fewer edges
-// scanned over the whole run). Three kernels: per tile and band counts, one
-// workgroup's exclusive scan over them in band-major order, and the emit.
+// scanned over the whole run). Two kernels: per tile and band counts, each
+// tile reserving its place within every band it occupies with one atomic on
+// the band's total (tiles of a band may sit in any order), and the emit.
 __device__ __forceinline__ void tile_bands(int32_t V, const int32_t* __restrict__ off,
                                            const uint32_t* __restrict__ stamp, const int32_t* __restrict__ dist,
                                            uint32_t sv, int64_t v0, int32_t delta, uint32_t (&deg)[16],
@@ -444,7 +457,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int3
                                                            const uint32_t* __restrict__ stamp,
                                                            const int32_t* __restrict__ dist,
                                                            const uint32_t* __restrict__ round_base, int32_t delta,
-                                                           int32_t tiles, uint2* __restrict__ tile_b) {
+                                                           int32_t tiles, uint2* __restrict__ tile_b,
+                                                           unsigned long long* __restrict__ band_tot) {
   __shared__ uint32_t s_c[kBands], s_e[kBands];
   if (threadIdx.x < kBands) {
     s_c[threadIdx.x] = 0;
@@ -461,58 +475,34 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int3
     atomicAdd(&s_e[band[k]], deg[k]);
   }
   __syncthreads();
-  if (threadIdx.x < kBands) tile_b[int64_t(threadIdx.x) * tiles + blockIdx.x] = make_uint2(s_c[threadIdx.x], s_e[threadIdx.x]);
-}
-
-// One workgroup: exclusive prefix of the n = kBands * tiles (vertices, edges)
-// pairs in place, 16 consecutive pairs per lane; the total is the frontier
-// size. (A last-arriving counting workgroup could scan instead, but the
-// device-scope release each workgroup would need before arriving costs more
-// than this launch.)
-__global__ __launch_bounds__(kScanBlock) void sssp_scan_b(int32_t n, uint2* __restrict__ tile_b,
-                                                          unsigned long long* __restrict__ ctr_out) {
-  __shared__ uint32_t s_tot[2 * (kScanBlock / kWave)];
-  uint32_t rc = 0, re = 0;
-  for (int64_t base = 0; base < n; base += int64_t(kScanBlock) * 16) {
-    const int64_t i0 = base + int64_t(threadIdx.x) * 16;
-    uint2 x[16];
-    uint32_t c = 0, e = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      x[k] = i0 + k < n ? tile_b[i0 + k] : make_uint2(0u, 0u);
-      c += x[k].x;
-      e += x[k].y;
-    }
-    uint32_t ec, ee, tc, te;
-    block_scan2<kScanBlock / kWave>(c, e, ec, ee, tc, te, s_tot);
-    uint32_t pc = rc + ec, pe = re + ee;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      if (i0 + k < n) tile_b[i0 + k] = make_uint2(pc, pe);
-      pc += x[k].x;
-      pe += x[k].y;
-    }
-    rc += tc;
-    re += te;
-    __syncthreads();  // s_tot reuse
+  // The tile's place within each band (tiles of a band may sit in any order).
+  if (threadIdx.x < kBands && s_c[threadIdx.x]) {
+    const unsigned long long at =
+        atomicAdd(&band_tot[threadIdx.x], (static_cast<unsigned long long>(s_c[threadIdx.x]) << 32) | s_e[threadIdx.x]);
+    tile_b[int64_t(threadIdx.x) * tiles + blockIdx.x] = make_uint2(uint32_t(at >> 32), uint32_t(at));
   }
-  if (threadIdx.x == 0) *ctr_out = (static_cast<unsigned long long>(rc) << 32) | re;
 }
 
 // Positions within a (band, tile) come from one 64-bit LDS counter per band,
-// (entries << 32 | edges), which starts at the band's prefix for this tile:
-// entry positions and edge prefixes advance together, so the queue stays
-// consistent whatever order the lanes take.
+// (entries << 32 | edges), which starts at the band's start (the totals of the
+// bands before it: packed sums add both halves at once, edges staying below
+// 2^32) plus the tile's place within the band. Entry positions and edge
+// prefixes advance together, so the queue stays consistent whatever order
+// the lanes take. The first workgroup publishes the frontier size.
 __global__ __launch_bounds__(kSsspBlock) void sssp_emit_b(int32_t V, const int32_t* __restrict__ off,
                                                           const uint32_t* __restrict__ stamp,
                                                           const int32_t* __restrict__ dist,
                                                           const uint32_t* __restrict__ round_base, int32_t delta,
                                                           int32_t tiles, const uint2* __restrict__ tile_b,
-                                                          SsspQueue qout) {
+                                                          const unsigned long long* __restrict__ band_tot,
+                                                          SsspQueue qout, unsigned long long* __restrict__ ctr_out) {
   __shared__ unsigned long long s_cur[kBands];
   if (threadIdx.x < kBands) {
-    const uint2 p = tile_b[int64_t(threadIdx.x) * tiles + blockIdx.x];
-    s_cur[threadIdx.x] = (static_cast<unsigned long long>(p.x) << 32) | p.y;
+    unsigned long long start = 0;
+    for (int b = 0; b < int(threadIdx.x); ++b) start += band_tot[b];
+    const uint2 p = tile_b[int64_t(threadIdx.x) * tiles + blockIdx.x];  // read only if the band occurs here
+    s_cur[threadIdx.x] = start + ((static_cast<unsigned long long>(p.x) << 32) | p.y);
+    if (blockIdx.x == 0 && threadIdx.x == kBands - 1) *ctr_out = start + band_tot[kBands - 1];
   }
   const int64_t v0 = int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
   uint32_t deg[16], band[16];
@@ -653,6 +643,9 @@ static int sssp_alloc(spm_graph* g, const CsrView& view) {
   SPM_TRY(dmalloc(g, (void**)&w->wstat, sizeof(unsigned long long) * 2, "sssp weight range"));
   SPM_TRY(dmalloc(g, (void**)&w->tile_b, sizeof(uint2) * kBands * ((size_t(V) + kTileV - 1) / kTileV),
                   "sssp band tiles"));
+  SPM_TRY(dmalloc(g, (void**)&w->band_tot, sizeof(unsigned long long) * kBands, "sssp band totals"));
+  SPM_TRY(dmalloc(g, (void**)&w->run_start, sizeof(uint32_t), "sssp run start"));
+  SPM_TRY(dmalloc(g, (void**)&w->big, sizeof(unsigned long long) * 2, "sssp big rounds"));
   SPM_TRY(sssp_pack_edges(g));
   SPM_CHECK_HIP(hipMemsetAsync(w->stamp, 0, sizeof(uint32_t) * V, g->stream));
   SPM_CHECK_HIP(hipMemcpyAsync(w->round_base, &w->host_base, sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
@@ -683,6 +676,9 @@ int sssp_release(spm_graph* g) {
   dfree(g, w->pk, sizeof(uint32_t) * size_t(g->E));
   dfree(g, w->wstat, sizeof(unsigned long long) * 2);
   dfree(g, w->tile_b, sizeof(uint2) * kBands * ((size_t(g->V) + kTileV - 1) / kTileV));
+  dfree(g, w->band_tot, sizeof(unsigned long long) * kBands);
+  dfree(g, w->run_start, sizeof(uint32_t));
+  dfree(g, w->big, sizeof(unsigned long long) * 2);
   delete w;
   g->sssp = nullptr;
   return SPM_OK;
@@ -696,7 +692,10 @@ static int sssp_grid(const spm_graph* g) {
 
 // Enqueues R rounds followed by the round-base advance. R is a multiple of 3
 // (queues rotate by 3), and of 6 in dense mode (flags alternate by 2).
-static int sssp_enqueue_rounds(spm_graph* g, int R) {
+// comp[r] == 0 leaves round r without the compaction kernels: it pushes
+// through the counter whatever its size (correct for any frontier, slower for
+// a big one); comp == nullptr compacts every round.
+static int sssp_enqueue_rounds(spm_graph* g, int R, const uint8_t* comp) {
   SsspWork* w = g->sssp;
   const int grid = sssp_grid(g);
   for (int r = 0; r < R; ++r) {
@@ -711,7 +710,9 @@ static int sssp_enqueue_rounds(spm_graph* g, int R) {
       SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(grid), dim3(kSsspBlock), 0,
                      w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
                      w->stamp, w->q[i], w->q[o], w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3,
-                     (const uint32_t*)w->round_base, uint32_t(r), kSparseEdges));
+                     (const uint32_t*)w->round_base, uint32_t(r),
+                     comp && !comp[r] ? 0xFFFFFFFFu : kSparseEdges, (const uint32_t*)w->run_start, w->big));
+      if (comp && !comp[r]) continue;
       const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
       SPM_TRY(launch(g, "sssp_count", sssp_count, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
                      (const uint32_t*)w->stamp, (const unsigned long long*)(w->ctr + i),
@@ -726,15 +727,16 @@ static int sssp_enqueue_rounds(spm_graph* g, int R) {
   return SPM_OK;
 }
 
-// The executable graph of R rounds in slot k, captured on first use.
-static int sssp_capture(spm_graph* g, int k, int R, hipGraphExec_t* out) {
+// The executable graph of R rounds in slot k (compaction as comp says),
+// captured on first use.
+static int sssp_capture(spm_graph* g, int k, int R, const std::vector<uint8_t>& comp, hipGraphExec_t* out) {
   SsspWork::Captured& c = g->sssp->cap[k];
-  if (!c.exec || c.rounds != R) {
+  if (!c.exec || c.rounds != R || c.comp != comp) {
     if (c.exec) { (void)hipGraphExecDestroy(c.exec); c.exec = nullptr; }
     if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
     c.rounds = 0;
     SPM_CHECK_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-    int rc = sssp_enqueue_rounds(g, R);
+    int rc = sssp_enqueue_rounds(g, R, comp.empty() ? nullptr : comp.data());
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(g->stream, &graph);
     if (rc != SPM_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -742,6 +744,7 @@ static int sssp_capture(spm_graph* g, int k, int R, hipGraphExec_t* out) {
     c.graph = graph;
     SPM_CHECK_HIP(hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0));
     c.rounds = R;
+    c.comp = comp;
   }
   *out = c.exec;
   return SPM_OK;
@@ -803,22 +806,22 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   const bool banded = g->sched.sssp_mode == 0 && w->maxw >= 0;
   const int slot = banded ? 2 : 0;
   SPM_TRY(launch(g, "sssp_seed", sssp_seed, dim3(1), dim3(kSsspBlock), 0, n > 1 ? (const int32_t*)w->srcbuf : nullptr,
-                 n, srcs[0], w->view.off, w->dist, w->q[slot], w->ctr, slot, act));
+                 n, srcs[0], w->view.off, w->dist, w->q[slot], w->ctr, slot, banded ? w->band_tot : nullptr,
+                 w->run_start, w->host_base + (banded ? 1u : 0u), w->big, act));
   if (banded) {
     const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
     const int32_t delta = int32_t(w->maxw / kBands + 1);
     SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(sssp_grid(g)),
                    dim3(kSsspBlock), 0, w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
                    w->stamp, w->q[2], w->q[0], w->ctr + 2, w->ctr + 0, w->ctr + 1, w->ctr + 3,
-                   (const uint32_t*)w->round_base, 0u, 0u));
+                   (const uint32_t*)w->round_base, 0u, 0u, (const uint32_t*)w->run_start,
+                   (unsigned long long*)nullptr));
     SPM_TRY(launch(g, "sssp_count_b", sssp_count_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
                    (const uint32_t*)w->stamp, (const int32_t*)w->dist, (const uint32_t*)w->round_base, delta, tiles,
-                   w->tile_b));
-    SPM_TRY(launch(g, "sssp_scan_b", sssp_scan_b, dim3(1), dim3(kScanBlock), 0, int32_t(kBands * tiles), w->tile_b,
-                   w->ctr + 0));
+                   w->tile_b, w->band_tot));
     SPM_TRY(launch(g, "sssp_emit_b", sssp_emit_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
                    (const uint32_t*)w->stamp, (const int32_t*)w->dist, (const uint32_t*)w->round_base, delta, tiles,
-                   (const uint2*)w->tile_b, w->q[0]));
+                   (const uint2*)w->tile_b, (const unsigned long long*)w->band_tot, w->q[0], w->ctr + 0));
     SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, 1u));
     w->host_base += 1;
   }
@@ -828,20 +831,32 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   // varies by a round between runs of the same source), rounded up to a
   // multiple of 3, the queue rotation. A longer run simply continues with
   // replays of R rounds.
+  // The same history drops the compaction kernels from the first replay's
+  // rounds whose frontier stayed below a quarter of the compaction threshold
+  // (a small round pushes through the counter and skips them anyway; their
+  // launches cost ~9 us a round).
   int R1 = R;
-  if (!dense && w->last_rounds > 0) R1 = std::min(R, 3 * ((w->last_rounds + 3) / 3));
+  std::vector<uint8_t> comp1;
+  if (!dense && w->last_rounds > 0) {
+    R1 = std::min(R, 3 * ((w->last_rounds + 3) / 3));
+    comp1.resize(size_t(R1));
+    for (int r = 0; r < R1; ++r) comp1[r] = r >= 128 || ((w->big_hist[r >> 6] >> (r & 63)) & 1ull);
+  }
+  const std::vector<uint8_t> all;
   const bool use_graph = !g->prof.on;
   unsigned long long* h = reinterpret_cast<unsigned long long*>(g->h_pinned);
   const int64_t max_replays = int64_t(g->V) / R + 3;
   int64_t replays = 0;
   for (;;) {
-    const int Rk = replays == 0 ? R1 : R;
+    const bool first = replays == 0 && !comp1.empty();
+    const int Rk = first ? R1 : R;
+    const std::vector<uint8_t>& comp = first ? comp1 : all;
     if (use_graph) {
       hipGraphExec_t exec = nullptr;
-      SPM_TRY(sssp_capture(g, Rk == R ? 0 : 1, Rk, &exec));
+      SPM_TRY(sssp_capture(g, first ? 1 : 0, Rk, comp, &exec));
       SPM_CHECK_HIP(hipGraphLaunch(exec, g->stream));
     } else {
-      SPM_TRY(sssp_enqueue_rounds(g, Rk));
+      SPM_TRY(sssp_enqueue_rounds(g, Rk, comp.empty() ? nullptr : comp.data()));
     }
     w->host_base += uint32_t(Rk);
     SPM_CHECK_HIP(hipMemcpyAsync(h, w->ctr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
@@ -850,10 +865,15 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
     if (++replays > max_replays) return set_error(SPM_EDIVERGE, "SSSP did not converge within V rounds (negative weights?)");
   }
   SPM_CHECK_HIP(hipMemcpyAsync(h, w->ctr + 3, sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(h + 1, w->big, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
   if (d_out) SPM_TRY(gather_back(g, w->view, w->dist, d_out));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   if (rounds_out) *rounds_out = int32_t(h[0]);
-  if (!dense) w->last_rounds = int(std::max<int64_t>(1, int64_t(h[0]) - (banded ? 1 : 0)));
+  if (!dense) {
+    w->last_rounds = int(std::max<int64_t>(1, int64_t(h[0]) - (banded ? 1 : 0)));
+    w->big_hist[0] = h[1];
+    w->big_hist[1] = h[2];
+  }
   SPM_TRY(prof_flush(g));
   return SPM_OK;
 }

@@ -94,6 +94,16 @@ def test_sssp_rounds_fitted_to_previous_run(oracle):
     dev = g.device()
     for s in (0, n - 1, 0, n - 3, n // 2, 0, 0):
         np.testing.assert_array_equal(dev.sssp(s), oracle.compute_sssp(g, g.edgeLen, s))
+    # The same history decides which rounds get compaction kernels: after a
+    # run from a vertex without out-edges, the big frontiers from vertex 0 land
+    # in rounds captured without them and push through the counter.
+    g = graph.rmat(17, 16, seed=9, weights=(1, 100))
+    deg = np.diff(g.indexofNodes)
+    leaf = int(np.flatnonzero(deg == 0)[0])
+    dev = g.device()
+    ref0 = oracle.compute_sssp(g, g.edgeLen, 0)
+    for s in (leaf, 0, leaf, 0, 0):
+        np.testing.assert_array_equal(dev.sssp(s), ref0 if s == 0 else oracle.compute_sssp(g, g.edgeLen, s))
 
 
 def test_sssp_edge_cases(oracle):

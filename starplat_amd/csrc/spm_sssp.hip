@@ -31,6 +31,13 @@
 // input queue is empty exits at once. Per-vertex round stamps deduplicate
 // pushes without clearing flags between rounds.
 //
+// With non-negative weights the seeds' round runs before the captured ones
+// and queues its frontier in distance bands (nearest first), so the big next
+// round's chunks run roughly in distance order and its pre-filter discards
+// more. Each handle remembers how many rounds its last run needed and which
+// of them were big: the first replay of a run is fitted to that, without
+// trailing empty rounds and without compaction kernels in small rounds.
+//
 // Distances are exact shortest paths, so the result is bit-identical to any
 // correct schedule of the reference's fixed point.
 #include <algorithm>

@@ -60,7 +60,7 @@ constexpr int kSsspWaves = kSsspBlock / kWave;
 constexpr uint32_t kSparseEdges = 1u << 19;
 constexpr int kTileV = kSsspBlock * 16;  // vertices per compaction tile
 // Distance bands of the first frontier (see sssp_count_b).
-constexpr int kBands = 16;
+constexpr int kBands = 32;
 
 struct SsspQueue {
   int32_t* vtx;     // [V] frontier vertices (each pushed at most once a round)

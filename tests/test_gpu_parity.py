@@ -106,6 +106,42 @@ def test_sssp_rounds_fitted_to_previous_run(oracle):
         np.testing.assert_array_equal(dev.sssp(s), ref0 if s == 0 else oracle.compute_sssp(g, g.edgeLen, s))
 
 
+@pytest.mark.parametrize("mode", [0, 2])
+def test_sssp_negative_weights_and_many_sources(oracle, mode):
+    # Negative weights (no negative cycle: every edge goes up in id) leave the
+    # edges unpacked and the first frontier unbanded; the fixed point is the
+    # same Bellman-Ford distance.
+    rng = np.random.default_rng(11)
+    n, m = 3000, 24000
+    a = rng.integers(0, n, m)
+    b = rng.integers(0, n, m)
+    keep = a != b
+    s, d = np.minimum(a, b)[keep].astype(np.int32), np.maximum(a, b)[keep].astype(np.int32)
+    w = rng.integers(-20, 100, s.size).astype(np.int32)
+    g = graph.from_edges(s, d, w)
+    dev = g.device()
+    dev.set_schedule(sssp_mode=mode)
+    for src in (0, 17, 1500):
+        np.testing.assert_array_equal(dev.sssp(src), oracle.compute_sssp(g, g.edgeLen, src))
+    # Many seeds at once: the seeds' round is itself large, and its frontier
+    # is banded over the whole graph.
+    from starplat_amd import hip
+    g = graph.rmat(16, 16, seed=4, weights=(1, 100))
+    srcs = np.arange(0, g.V, 7, dtype=np.int32)
+    out = hip.DeviceArray(g.V, np.int32)
+    dev = g.device()
+    dev.set_schedule(sssp_mode=mode)
+    dev.sssp_multi_device(srcs, out)
+    # nearest-seed distance = distance from a virtual vertex joined to every
+    # seed by a zero-weight edge
+    V = g.V
+    ss = np.concatenate([np.repeat(np.arange(V, dtype=np.int32), np.diff(g.indexofNodes)), np.full(srcs.size, V, np.int32)])
+    dd = np.concatenate([g.edgeList, srcs])
+    ww = np.concatenate([g.edgeLen, np.zeros(srcs.size, np.int32)])
+    gv = graph.from_edges(ss.astype(np.int32), dd.astype(np.int32), ww.astype(np.int32))
+    np.testing.assert_array_equal(out.to_host(), oracle.compute_sssp(gv, gv.edgeLen, V)[:V])
+
+
 def test_sssp_edge_cases(oracle):
     # single vertex, no edges
     g = graph.from_edges(np.array([0], np.int32), np.array([0], np.int32), np.array([5], np.int32))

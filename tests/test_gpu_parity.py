@@ -394,6 +394,37 @@ def test_bc_hub_frontier(oracle):
     assert rel_linf(g.device().bc(srcs), ref) <= BC_TOL
 
 
+def test_bc_path_counts_beyond_fp32(oracle):
+    """Path counts beyond the FP32 mantissa stay exact (FP64 sigma): a chain
+    of 2x2 complete layers doubles them per layer, up to 2^38; a second
+    component keeps its counts small, and batches mix both."""
+    L = 40
+    s, d = [], []
+    for i in range(L - 1):
+        for a in (2 * i, 2 * i + 1):
+            for b in (2 * i + 2, 2 * i + 3):
+                s.append(a)
+                d.append(b)
+    base = 2 * L  # a small undirected ER component after the chain
+    rng = np.random.default_rng(3)
+    es = rng.integers(0, 500, 3000) + base
+    ed = rng.integers(0, 500, 3000) + base
+    k = es != ed
+    s = np.concatenate([np.array(s), es[k], ed[k]]).astype(np.int32)
+    d = np.concatenate([np.array(d), ed[k], es[k]]).astype(np.int32)
+    g = graph.from_edges(s, d)
+    for src in (0, 1, base + 3):
+        _, lev_ref, sig_ref = oracle.compute_bc(g, [src])
+        lev, sig = g.device().bc_paths(src)
+        np.testing.assert_array_equal(lev, lev_ref)
+        np.testing.assert_array_equal(sig, sig_ref)
+    assert sig_ref.max() < 2**24 and oracle.compute_bc(g, [0])[2].max() == 2.0 ** (L - 2)
+    for srcs in ([0, 1, base + 7], list(range(base, base + 70)), [0] + list(range(base, base + 100))):
+        srcs = np.array(srcs, np.int32)
+        ref, _, _ = oracle.compute_bc(g, srcs)
+        assert rel_linf(g.device().bc(srcs), ref) <= BC_TOL
+
+
 def test_bc_directed_rmat(oracle):
     g = graph.rmat(12, 16, seed=8)
     src = sources(g.V, 70, seed=1)

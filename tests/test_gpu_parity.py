@@ -333,6 +333,26 @@ def test_tc_long_oriented_rows_fall_back():
     assert Compute_TC(g) == n * (n - 1) * (n - 2) // 6
 
 
+def test_nearly_empty_graph_all_algorithms(oracle):
+    """Ragged input: 1,000 vertices, two edges between the last three, all
+    other vertices isolated (no out- or in-edges)."""
+    g = graph.from_edges(np.array([997, 998], np.int32), np.array([998, 999], np.int32), np.array([3, 4], np.int32))
+    assert g.V == 1000 and g.E == 2
+    dev = g.device()
+    for s in (0, 500, 997, 999):
+        np.testing.assert_array_equal(dev.sssp(s), oracle.compute_sssp(g, g.edgeLen, s))
+    assert dev.sssp(997)[999] == 7 and dev.sssp(0)[1] == INT_MAX
+    pr = np.empty(g.V, np.float32)
+    it = Compute_PR(g, 1e-10, 0.85, 100, pr)
+    ref, it_ref = oracle.compute_pr(g)
+    assert rel_linf(pr, ref) <= PR_TOL
+    assert Compute_TC(g) == 0
+    ref_bc, _, _ = oracle.compute_bc(g, [0, 997, 998])
+    bc = dev.bc(np.array([0, 997, 998], np.int32))
+    np.testing.assert_allclose(bc, ref_bc, rtol=0, atol=1e-6)
+    assert bc[998] == 1.0  # on the path 997 -> 998 -> 999
+
+
 def test_tc_closed_forms_and_directed(oracle):
     n = 30
     a, b = np.meshgrid(np.arange(n), np.arange(n))

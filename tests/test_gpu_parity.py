@@ -123,6 +123,13 @@ def test_sssp_negative_weights_and_many_sources(oracle, mode):
     dev.set_schedule(sssp_mode=mode)
     for src in (0, 17, 1500):
         np.testing.assert_array_equal(dev.sssp(src), oracle.compute_sssp(g, g.edgeLen, src))
+    # Non-negative weights too wide to pack with a 16-bit vertex id (21 + 16
+    # bits): unpacked edges, banded first frontier with bands of 2^15.
+    g = graph.rmat(16, 16, seed=6, weights=(1, 1 << 20))
+    dev = g.device()
+    dev.set_schedule(sssp_mode=mode)
+    for src in (0, 9):
+        np.testing.assert_array_equal(dev.sssp(src), oracle.compute_sssp(g, g.edgeLen, src))
     # Many seeds at once: the seeds' round is itself large, and its frontier
     # is banded over the whole graph.
     from starplat_amd import hip

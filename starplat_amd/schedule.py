@@ -29,7 +29,7 @@ JOB_SSSP, JOB_PR = 0, 1
 # Schedule classes: each sets the SSSP half and the PageRank half; a job uses
 # the half for its own algorithm.
 CLASSES = [
-    dict(sssp_mode=0, sssp_rounds=12, sssp_grid=0, pr_mode=0, pr_t1=4, pr_t2=48, pr_t3=2048, pr_group=8),
+    dict(),  # the library default as it is for the graph (spm_default_schedule)
     dict(sssp_mode=0, sssp_rounds=6, sssp_grid=256, pr_mode=0, pr_t1=0, pr_t2=16, pr_t3=1024, pr_group=4),
     dict(sssp_mode=0, sssp_rounds=24, sssp_grid=512, pr_mode=0, pr_t1=8, pr_t2=128, pr_t3=4096, pr_group=16),
     dict(sssp_mode=1, sssp_rounds=12, sssp_grid=0, pr_mode=0, pr_t1=2, pr_t2=32, pr_t3=512, pr_group=8),

@@ -89,10 +89,12 @@ __global__ __launch_bounds__(kBcBlock) void bc_scan(
     const unsigned long long* __restrict__ cur, unsigned long long* __restrict__ nxt,
     unsigned long long* __restrict__ visited, unsigned long long batchmask,
     int32_t* __restrict__ list, unsigned int* __restrict__ cnt) {
+  __shared__ unsigned int s_cnt[kBcWaves];
+  __shared__ unsigned int s_base;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  const int lane = __lane_id();
-  // Rounded so whole wavefronts take part in the ballot-aggregated append.
-  const int64_t vend = (int64_t(V) + kWave - 1) / kWave * kWave;
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  // Rounded so whole workgroups take part in the aggregated append.
+  const int64_t vend = (int64_t(V) + stride - 1) / stride * stride;
   for (int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; w < vend; w += stride) {
     unsigned long long nm = 0;
     if (w < V) {
@@ -118,14 +120,23 @@ __global__ __launch_bounds__(kBcBlock) void bc_scan(
       }
       nxt[w] = nm;
     }
+    // One list reservation per workgroup and step: a shared counter hit once
+    // per wavefront serialises (~65K same-address atomics a level at 4M
+    // vertices).
     const unsigned long long b = __ballot(nm != 0ull);
-    if (b) {
-      unsigned int base = 0;
-      const int leader = __ffsll((long long)b) - 1;
-      if (lane == leader) base = atomicAdd(cnt, unsigned(__popcll(b)));
-      base = __shfl(base, leader, kWave);
-      if (nm) list[base + __popcll(b & ((1ull << lane) - 1ull))] = int32_t(w);
+    if (lane == 0) s_cnt[wid] = unsigned(__popcll(b));
+    __syncthreads();
+    unsigned int before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < kBcWaves; ++k) {
+      const unsigned int x = s_cnt[k];
+      before += k < wid ? x : 0u;
+      all += x;
     }
+    if (threadIdx.x == 0) s_base = all ? atomicAdd(cnt, all) : 0u;
+    __syncthreads();
+    if (nm) list[s_base + before + __popcll(b & ((1ull << lane) - 1ull))] = int32_t(w);
+    __syncthreads();  // s_cnt and s_base are reused by the next step
   }
 }
 

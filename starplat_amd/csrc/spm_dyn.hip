@@ -134,32 +134,41 @@ static bool dyn_dirty(const spm_graph* g) {
 
 // One lane per update (s, t): removes every copy of t from s's row (binary
 // search in the sorted row, then the run of equal entries) and from s's
-// overflow row.
+// overflow row. The removal totals are summed per wavefront before the one
+// atomic on each shared counter.
 __global__ void dyn_delete(int32_t n, const int32_t* __restrict__ us, const int32_t* __restrict__ ut,
                            const int32_t* __restrict__ off, const int32_t* __restrict__ adj,
                            uint32_t* __restrict__ dead, int32_t* __restrict__ dead_row, DiffRows d,
                            unsigned long long* __restrict__ ctr) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int32_t s = us[i], t = ut[i];
-  int32_t lo = off[s], hi = off[s + 1];
-  const int32_t end = hi;
-  while (lo < hi) {
-    const int32_t mid = lo + ((hi - lo) >> 1);
-    if (adj[mid] < t) lo = mid + 1;
-    else hi = mid;
-  }
-  for (int32_t j = lo; j < end && adj[j] == t; ++j) {
-    const uint32_t b = 1u << (j & 31);
-    if (!(atomicOr(&dead[j >> 5], b) & b)) {
-      atomicAdd(&dead_row[s], 1);
-      atomicAdd(&ctr[0], 1ull);
+  unsigned long long rb = 0, ro = 0;
+  if (i < n) {
+    const int32_t s = us[i], t = ut[i];
+    int32_t lo = off[s], hi = off[s + 1];
+    const int32_t end = hi;
+    while (lo < hi) {
+      const int32_t mid = lo + ((hi - lo) >> 1);
+      if (adj[mid] < t) lo = mid + 1;
+      else hi = mid;
+    }
+    for (int32_t j = lo; j < end && adj[j] == t; ++j) {
+      const uint32_t b = 1u << (j & 31);
+      if (!(atomicOr(&dead[j >> 5], b) & b)) {
+        atomicAdd(&dead_row[s], 1);
+        ++rb;
+      }
+    }
+    for (int32_t j = d.off[s]; j < d.off[s + 1]; ++j) {
+      if (d.adj[j] != t) continue;
+      const uint32_t b = 1u << (j & 31);
+      if (!(atomicOr(&d.dead[j >> 5], b) & b)) ++ro;
     }
   }
-  for (int32_t j = d.off[s]; j < d.off[s + 1]; ++j) {
-    if (d.adj[j] != t) continue;
-    const uint32_t b = 1u << (j & 31);
-    if (!(atomicOr(&d.dead[j >> 5], b) & b)) atomicAdd(&ctr[1], 1ull);
+  rb = wave_sum(rb);
+  ro = wave_sum(ro);
+  if (__lane_id() == 0) {
+    if (rb) atomicAdd(&ctr[0], rb);
+    if (ro) atomicAdd(&ctr[1], ro);
   }
 }
 

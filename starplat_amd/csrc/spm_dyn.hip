@@ -505,19 +505,35 @@ __global__ void dpr_flag_dests(int32_t n, const int32_t* __restrict__ ut, uint8_
 // Vertices whose in-row in the starting CSR holds at most kDprHeavy entries
 // take kDprGroup lanes each, in id order (dpr_sweep_light); the others, listed
 // and marked once at the start (so a later fold cannot move a vertex between
-// the two kernels), take a wavefront each (dpr_sweep_heavy).
-constexpr int kDprGroup = 8;
-constexpr int32_t kDprHeavy = 64;
+// the two kernels), take a workgroup each (dpr_sweep_heavy).
+constexpr int kDprGroup = 4;
+constexpr int32_t kDprHeavy = 32;
+
+// Live entries of adj[a, b) taken by one lane of `lanes`, four at a time into
+// independent sums so the lane keeps four gathers in flight.
+__device__ __forceinline__ float dpr_part_sum(int32_t a, int32_t b, int lane, int lanes,
+                                              const int32_t* __restrict__ adj, const uint32_t* __restrict__ dead,
+                                              const float* __restrict__ c) {
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int32_t j = a + lane;
+  for (; j + 3 * lanes < b; j += 4 * lanes) {
+    const int32_t j1 = j + lanes, j2 = j + 2 * lanes, j3 = j + 3 * lanes;
+    const int32_t u0 = adj[j], u1 = adj[j1], u2 = adj[j2], u3 = adj[j3];
+    if (!bit_of(dead, j)) s0 += c[u0];
+    if (!bit_of(dead, j1)) s1 += c[u1];
+    if (!bit_of(dead, j2)) s2 += c[u2];
+    if (!bit_of(dead, j3)) s3 += c[u3];
+  }
+  for (; j < b; j += lanes)
+    if (!bit_of(dead, j)) s0 += c[adj[j]];
+  return (s0 + s1) + (s2 + s3);
+}
 
 __device__ __forceinline__ float dpr_row_sum(int64_t v, int lane, int lanes, const int32_t* __restrict__ roff,
                                              const int32_t* __restrict__ radj, const uint32_t* __restrict__ rdead,
                                              const DiffRows& in, const float* __restrict__ c) {
-  float sum = 0.f;
-  for (int32_t j = roff[v] + lane; j < roff[v + 1]; j += lanes)
-    if (!bit_of(rdead, j)) sum += c[radj[j]];
-  for (int32_t j = in.off[v] + lane; j < in.off[v + 1]; j += lanes)
-    if (!bit_of(in.dead, j)) sum += c[in.adj[j]];
-  return sum;
+  return dpr_part_sum(roff[v], roff[v + 1], lane, lanes, radj, rdead, c) +
+         dpr_part_sum(in.off[v], in.off[v + 1], lane, lanes, in.adj, in.dead, c);
 }
 
 __device__ __forceinline__ void dpr_part_store(float mydiff, float* __restrict__ part, float* s_red) {

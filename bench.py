@@ -484,7 +484,7 @@ def main():
     ap.add_argument("--bc-sources", type=int, default=256)
     ap.add_argument("--dyn-batch", type=int, default=65536)
     ap.add_argument("--dyn-batches", type=int, default=6)
-    ap.add_argument("--sched-graphs", type=int, default=150)
+    ap.add_argument("--sched-graphs", type=int, default=300)
     ap.add_argument("--sched-held-out", type=int, default=50)
     ap.add_argument("--sched-rmat", type=int, nargs=2, default=[10, 16])
     ap.add_argument("--sched-er", type=int, nargs=2, default=[10, 16])

@@ -19,7 +19,7 @@ import time
 import numpy as np
 
 from .graph import graph
-from .schedule import FIXED_CLASS, JOB_PR, JOB_SSSP, N_CLASSES, Job, Selector, features, time_job
+from .schedule import FIXED_CLASS, JOB_PR, JOB_SSSP, N_CLASSES, Job, Selector, features, tie_aware_labels, time_job
 
 
 def _rng(seed, i):
@@ -204,7 +204,7 @@ def run(n=2000, held_out=400, seed=5, reps=2, epochs=400, log=None, rows=None, b
     fixed = float(ho[:, FIXED_CLASS].sum())
     learned = float(ho[idx, pick].sum())
     best = float(ho.min(axis=1).sum())
-    train_acc = float(np.mean(sel.choose(feats[:ntr]) == np.argmin(times[:ntr], axis=1)))
+    train_acc = float(np.mean(sel.choose(feats[:ntr]) == tie_aware_labels(times[:ntr])))
     bat = None
     if batched:
         bat = batched_makespan(seed, range(ntr, n), reps=reps, log=log, **ranges)

@@ -41,6 +41,19 @@ CLASSES = [
 FIXED_CLASS = 0
 
 
+def tie_aware_labels(times, tol=0.03):
+    """Training label per job: among the classes within `tol` of its fastest
+    time (timing noise makes these a tie), the one that is near-fastest on
+    the most training jobs, ties broken by time. Hard argmin labels flip
+    between near-equal classes from one sweep to the next, which the network
+    then learns as noise."""
+    t = np.asarray(times, np.float64)
+    near = t <= t.min(axis=1, keepdims=True) * (1.0 + tol)
+    freq = near.sum(axis=0).astype(np.float64)
+    score = np.where(near, freq[None, :] - t / (t.max() + 1.0), -np.inf)
+    return np.argmax(score, axis=1).astype(np.int32)
+
+
 def features(g, job):
     out = np.zeros(N_FEATURES, np.float32)
     N.check("spmh_features", N.host_lib().spmh_features(g._native(), float(job), N.ptr(out, C.c_float)))
@@ -177,7 +190,7 @@ class Selector:
         feats = np.asarray(feats, np.float32)
         self.mu = feats.mean(0)
         self.sd = feats.std(0) + 1e-6
-        y = np.argmin(times, axis=1).astype(np.int32)
+        y = tie_aware_labels(times)
         if self.mlp is None:
             self.mlp = MLP(seed)
         return self.mlp.train((feats - self.mu) / self.sd, y, epochs=epochs, seed=seed)

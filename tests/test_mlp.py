@@ -126,3 +126,18 @@ def test_selector_beats_fixed_on_synthetic_costs():
     learned = times[600:][np.arange(200), pick].sum()
     fixed = times[600:, 0].sum()
     assert learned < 0.6 * fixed
+
+
+def test_tie_aware_labels():
+    from starplat_amd.schedule import tie_aware_labels
+
+    # class 0 is within 3% of the fastest on three jobs out of four, so it
+    # labels every job where it is near-fastest; job 2 has one clear winner
+    t = np.array([[1.0, 1.02, 2.0], [1.0, 3.0, 1.01], [2.0, 1.0, 1.5], [1.0, 1.0, 1.0]])
+    np.testing.assert_array_equal(tie_aware_labels(t), [0, 0, 1, 0])
+    # without ties it is the argmin
+    t = np.array([[3.0, 1.0, 2.0], [1.0, 2.0, 3.0], [2.0, 3.0, 1.0]])
+    np.testing.assert_array_equal(tie_aware_labels(t), np.argmin(t, axis=1))
+    # among equally frequent near-best classes the faster one wins
+    t = np.array([[1.0, 1.01]])
+    assert tie_aware_labels(t)[0] == 0

@@ -385,26 +385,6 @@ __global__ void dtc_chunks(int32_t n, const int32_t* __restrict__ us, const int3
   cnt[i] = (len + kDtcChunk - 1) / kDtcChunk;
 }
 
-// Is v3 a live neighbour of v2: binary search in the sorted CSR row (every
-// copy of a pair is removed together, so one live copy decides), then the
-// overflow row.
-__device__ __forceinline__ bool dtc_is_nbr(int32_t v2, int32_t v3, const int32_t* __restrict__ off,
-                                           const int32_t* __restrict__ adj, const uint32_t* __restrict__ dead,
-                                           const DiffRows& d) {
-  int32_t lo = off[v2], hi = off[v2 + 1];
-  const int32_t end = hi;
-  while (lo < hi) {
-    const int32_t mid = lo + ((hi - lo) >> 1);
-    if (adj[mid] < v3) lo = mid + 1;
-    else hi = mid;
-  }
-  for (int32_t j = lo; j < end && adj[j] == v3; ++j)
-    if (!bit_of(dead, j)) return true;
-  for (int32_t j = d.off[v2]; j < d.off[v2 + 1]; ++j)
-    if (d.adj[j] == v3 && !bit_of(d.dead, j)) return true;
-  return false;
-}
-
 __global__ __launch_bounds__(kDynBlock) void dtc_count(int32_t n, const int32_t* __restrict__ us,
                                                        const int32_t* __restrict__ ut,
                                                        const int32_t* __restrict__ ioff,
@@ -428,24 +408,59 @@ __global__ __launch_bounds__(kDynBlock) void dtc_count(int32_t n, const int32_t*
     const int64_t base = (x - ioff[lo]) * int64_t(kDtcChunk);
     const int32_t a = off[v1], la = off[v1 + 1] - a;
     const int32_t da = d.off[v1], len = la + (d.off[v1 + 1] - da);
+    // A lane's kDtcChunk / 64 candidates v3 are looked up in v2's sorted row
+    // together: the binary searches advance in lockstep, so their loads are in
+    // flight at once instead of one dependent chain after another.
+    constexpr int Q = kDtcChunk / kWave;
+    const int32_t b2 = off[v2], e2 = off[v2 + 1];
+    int32_t v3[Q], slo[Q], shi[Q];
 #pragma unroll
-    for (int q = 0; q < kDtcChunk / kWave; ++q) {
+    for (int q = 0; q < Q; ++q) {
       const int64_t p = base + q * kWave + lane;
+      v3[q] = -1;
+      slo[q] = shi[q] = b2;
       if (p >= len) continue;
-      int32_t v3;
+      int32_t x;
       bool live;
       if (p < la) {
         const int32_t j = a + int32_t(p);
         live = !bit_of(dead, j);
-        v3 = adj[j];
+        x = adj[j];
       } else {
         const int32_t j = da + int32_t(p - la);
         live = !bit_of(d.dead, j);
-        v3 = d.adj[j];
+        x = d.adj[j];
       }
-      if (!live || v3 == v1 || v3 == v2) continue;
-      if (!dtc_is_nbr(v2, v3, off, adj, dead, d)) continue;
-      const int k = 1 + int(dtc_set_has(set, mask, v1, v3)) + int(dtc_set_has(set, mask, v2, v3));
+      if (!live || x == v1 || x == v2) continue;
+      v3[q] = x;
+      shi[q] = e2;
+    }
+    for (bool more = true; more;) {  // lower bound of each v3 in v2's row
+      int32_t mid[Q], val[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        mid[q] = slo[q] + ((shi[q] - slo[q]) >> 1);
+        val[q] = slo[q] < shi[q] ? adj[mid[q]] : 0;
+      }
+      more = false;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        if (slo[q] >= shi[q]) continue;
+        if (val[q] < v3[q]) slo[q] = mid[q] + 1;
+        else shi[q] = mid[q];
+        more |= slo[q] < shi[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      if (v3[q] < 0) continue;
+      // every copy of a pair is removed together, so one live copy decides;
+      // then the overflow row
+      bool nb = false;
+      for (int32_t j = slo[q]; j < e2 && adj[j] == v3[q] && !nb; ++j) nb = !bit_of(dead, j);
+      for (int32_t j = d.off[v2]; j < d.off[v2 + 1] && !nb; ++j) nb = d.adj[j] == v3[q] && !bit_of(d.dead, j);
+      if (!nb) continue;
+      const int k = 1 + int(dtc_set_has(set, mask, v1, v3[q])) + int(dtc_set_has(set, mask, v2, v3[q]));
       c1 += k == 1;
       c2 += k == 2;
       c3 += k == 3;

@@ -70,7 +70,7 @@ __global__ void bc_init(int32_t V, unsigned long long* __restrict__ visited,
     mask0[v] = 0ull;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *cnt = unsigned(nb);
-  if (blockIdx.x == 0 && threadIdx.x < nb) list0[threadIdx.x] = src[threadIdx.x];
+  if (blockIdx.x == 0 && int(threadIdx.x) < nb) list0[threadIdx.x] = src[threadIdx.x];
 }
 
 __global__ void bc_seed(const int32_t* __restrict__ src, int nb, unsigned long long* __restrict__ visited,

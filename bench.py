@@ -479,7 +479,9 @@ def bench_schedule(args):
                        "(weights U[1,100], src 0) or PageRank job; 8 schedule classes timed per job; MLP 16-64-64-8 "
                        "trained on the GPU on the fastest class; makespan = sum of held-out job times; "
                        "batched_held_out: the same held-out jobs run as disjoint unions, one batched launch "
-                       "sequence per job type (spm_pr_multi_device / spm_sssp_multi_device), default schedule")
+                       "sequence per job type (spm_pr_multi_device / spm_sssp_multi_device), default schedule; "
+                       "hybrid_held_out: SSSP jobs and PageRank jobs below 2^17 vertices batched so, the other "
+                       "PageRank jobs run alone under the learned class")
     return res
 
 

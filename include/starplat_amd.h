@@ -50,7 +50,9 @@ int spm_version(void);
  * of the same arrays (backend_cuda/dsl_cpp_generator.cpp:4440-4500).
  * Arrays are copied to HBM once; rev_* may be NULL (PR and BC then fail with
  * SPM_ENOGRAPH); edgeLen may be NULL (SSSP then needs spm_graph_set_weights
- * or a weight argument). */
+ * or a weight argument). Offsets must be non-decreasing and every neighbour
+ * id in [0, num_nodes), else SPM_EINVAL (checked on the host before any
+ * device work). */
 int spm_graph_create(int32_t num_nodes, int64_t num_edges,
                      const int32_t* indexofNodes, const int32_t* edgeList,
                      const int32_t* edgeLen,

@@ -139,6 +139,20 @@ extern "C" {
 const char* spm_last_error(void) { return spm::g_last_error.c_str(); }
 int spm_version(void) { return 100; }
 
+// Offsets non-decreasing and neighbour ids in [0, V): the bounds every
+// kernel relies on. Min/max reductions so the scan vectorises.
+static bool csr_ok(int32_t V, int64_t E, const int32_t* off, const int32_t* adj) {
+  int32_t bad = 0;
+  for (int32_t v = 0; v < V; ++v) bad |= int32_t(off[v + 1] < off[v]);
+  if (bad) return false;
+  int32_t lo = 0, hi = 0;
+  for (int64_t e = 0; e < E; ++e) {
+    lo = std::min(lo, adj[e]);
+    hi = std::max(hi, adj[e]);
+  }
+  return lo >= 0 && (E == 0 || hi < V);
+}
+
 int spm_graph_create(int32_t V, int64_t E, const int32_t* off, const int32_t* adj,
                      const int32_t* w, const int32_t* roff, const int32_t* radj,
                      spm_graph** out) {
@@ -149,6 +163,11 @@ int spm_graph_create(int32_t V, int64_t E, const int32_t* off, const int32_t* ad
   if ((roff == nullptr) != (radj == nullptr)) return set_error(SPM_EINVAL, "spm_graph_create: rev_indexofNodes and srcList go together");
   if (off[0] != 0 || off[V] != E) return set_error(SPM_EINVAL, "spm_graph_create: indexofNodes[0] must be 0 and indexofNodes[V] == E");
   if (roff && (roff[0] != 0 || roff[V] != E)) return set_error(SPM_EINVAL, "spm_graph_create: rev_indexofNodes[0] must be 0 and rev_indexofNodes[V] == E");
+  // Every kernel indexes rows and vertex arrays by these values without
+  // bounds checks, so a malformed CSR is refused here rather than faulting
+  // the device later.
+  if (!csr_ok(V, E, off, adj)) return set_error(SPM_EINVAL, "spm_graph_create: indexofNodes must be non-decreasing and edgeList ids in [0, V)");
+  if (roff && !csr_ok(V, E, roff, radj)) return set_error(SPM_EINVAL, "spm_graph_create: rev_indexofNodes must be non-decreasing and srcList ids in [0, V)");
   int dev = 0;
   SPM_CHECK_HIP(hipGetDevice(&dev));
   spm_graph* g = new spm_graph();

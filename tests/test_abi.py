@@ -41,3 +41,39 @@ def test_version_and_error_string_without_gpu():
     lib = N.device_lib()
     assert lib.spm_version() >= 100
     assert isinstance(lib.spm_last_error(), bytes)
+
+
+def test_graph_create_rejects_malformed_csr_without_gpu():
+    """A CSR whose offsets decrease or whose neighbour ids leave [0, V) is
+    refused with SPM_EINVAL before any device call (kernels index by these
+    values unchecked)."""
+    import numpy as np
+
+    lib = N.device_lib()
+    P = ctypes.POINTER(ctypes.c_int32)
+
+    def create(off, adj, roff=None, radj=None):
+        off = np.asarray(off, np.int32)
+        adj = np.asarray(adj, np.int32)
+        h = ctypes.c_void_p()
+        args = [N.ptr(off, ctypes.c_int32), N.ptr(adj, ctypes.c_int32), None]
+        if roff is not None:
+            roff = np.asarray(roff, np.int32)
+            radj = np.asarray(radj, np.int32)
+            args += [N.ptr(roff, ctypes.c_int32), N.ptr(radj, ctypes.c_int32)]
+        else:
+            args += [ctypes.cast(None, P), ctypes.cast(None, P)]
+        return lib.spm_graph_create(len(off) - 1, len(adj), *args, ctypes.byref(h)), h
+
+    cases = [
+        ([0, 2, 1, 3], [1, 2, 0]),            # offsets decrease
+        ([0, 1, 2, 3], [1, 3, 0]),            # neighbour id == V
+        ([0, 1, 2, 3], [1, -1, 0]),           # negative neighbour id
+    ]
+    for off, adj in cases:
+        rc, h = create(off, adj)
+        assert rc == -1 and not h.value, (off, adj)
+        assert b"edgeList" in lib.spm_last_error()
+    # a valid forward CSR with a bad reverse CSR
+    rc, h = create([0, 1, 2, 3], [1, 2, 0], [0, 1, 2, 3], [2, 0, 7])
+    assert rc == -1 and not h.value and b"srcList" in lib.spm_last_error()

@@ -19,7 +19,6 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
-#include <parallel/algorithm>
 #include <string>
 #include <vector>
 
@@ -34,9 +33,9 @@ inline uint64_t mix64(uint64_t z) {
 }
 
 // Counter-based stream: value depends only on (seed, stream, counter).
-inline uint64_t rng(uint64_t seed, uint64_t stream, uint64_t counter) {
-  return mix64(mix64(seed ^ mix64(stream * 0xD1B54A32D192ED03ull)) + counter);
-}
+inline uint64_t rng_base(uint64_t seed, uint64_t stream) { return mix64(seed ^ mix64(stream * 0xD1B54A32D192ED03ull)); }
+inline uint64_t rng_at(uint64_t base, uint64_t counter) { return mix64(base + counter); }
+inline uint64_t rng(uint64_t seed, uint64_t stream, uint64_t counter) { return rng_at(rng_base(seed, stream), counter); }
 
 inline uint64_t key_of(uint32_t s, uint32_t d) { return (uint64_t(s) << 32) | d; }
 inline int32_t key_src(uint64_t k) { return int32_t(k >> 32); }
@@ -48,107 +47,252 @@ int32_t* alloc_i32(int64_t n) {
   return p;
 }
 
+// Exclusive prefix sum of n counts: out[i] = cnt[0] + ... + cnt[i-1], out[n]
+// the total (two passes over per-thread slices).
+int64_t scan_counts(const int32_t* cnt, int64_t n, int64_t* out) {
+  const int T = omp_get_max_threads();
+  std::vector<int64_t> part(size_t(T) + 1, 0);
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+    const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+    int64_t s = 0;
+    for (int64_t i = lo; i < hi; ++i) s += cnt[i];
+    part[size_t(t) + 1] = s;
+#pragma omp barrier
+#pragma omp single
+    for (int i = 0; i < nt; ++i) part[size_t(i) + 1] += part[size_t(i)];
+    int64_t acc = part[size_t(t)];
+    for (int64_t i = lo; i < hi; ++i) {
+      out[i] = acc;
+      acc += cnt[i];
+    }
+  }
+  int64_t total = 0;
+  for (int t = 0; t < T; ++t) total += part[size_t(t) + 1] - part[size_t(t)];
+  out[n] = total;
+  return total;
+}
+
+// Ascending sort of one adjacency row (rows are mostly short).
+inline void sort_row(int32_t* a, int64_t n) {
+  if (n > 32) {
+    std::sort(a, a + n);
+    return;
+  }
+  for (int64_t i = 1; i < n; ++i) {
+    const int32_t x = a[i];
+    int64_t j = i - 1;
+    while (j >= 0 && a[j] > x) {
+      a[j + 1] = a[j];
+      --j;
+    }
+    a[j + 1] = x;
+  }
+}
+
+spmh_graph* alloc_graph(int32_t V, int64_t E) {
+  spmh_graph* g = static_cast<spmh_graph*>(std::calloc(1, sizeof(spmh_graph)));
+  g->V = V;
+  g->E = E;
+  g->indexofNodes = alloc_i32(int64_t(V) + 1);
+  g->edgeList = alloc_i32(E);
+  g->edgeLen = alloc_i32(E);
+  g->rev_indexofNodes = alloc_i32(int64_t(V) + 1);
+  g->srcList = alloc_i32(E);
+  return g;
+}
+
+// Stable bucketing of n items (row(i) in [0, V), val(i)) into CSR rows:
+// out[start[r] .. start[r+1]) receives the values of row r in input order.
+// Two counting passes, no atomics: items are first partitioned by the high
+// bits of their row (per-thread histograms over contiguous input slices, so
+// each thread writes its items in input order), then each partition is
+// counted and placed by the full row id within its small row range.
+template <class Row, class Val>
+void stable_rows(int64_t n, int32_t V, Row row, Val val, int64_t* start, int32_t* out) {
+  int shift = 0;
+  while ((int64_t(V - 1) >> shift) >= 1024) ++shift;  // at most 1024 partitions
+  const int32_t B = int32_t((int64_t(V - 1) >> shift) + 1);
+  const int T = omp_get_max_threads();
+  std::vector<int64_t> hist(size_t(T) * size_t(B), 0);
+  std::vector<uint64_t> tmp(size_t(std::max<int64_t>(n, 1)));
+  std::vector<int64_t> pstart(size_t(B) + 1, 0);
+  int used = T;
+#pragma omp parallel num_threads(T)
+  {
+    const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+#pragma omp single
+    used = nt;
+    const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+    int64_t* h = hist.data() + size_t(t) * B;
+    for (int64_t i = lo; i < hi; ++i) {
+      const int32_t r = row(i);
+      if (r >= 0) ++h[r >> shift];
+    }
+#pragma omp barrier
+#pragma omp single
+    {
+      int64_t acc = 0;
+      for (int32_t b = 0; b < B; ++b) {
+        pstart[size_t(b)] = acc;
+        for (int u = 0; u < nt; ++u) {
+          const int64_t c = hist[size_t(u) * B + b];
+          hist[size_t(u) * B + b] = acc;
+          acc += c;
+        }
+      }
+      pstart[size_t(B)] = acc;
+    }
+    for (int64_t i = lo; i < hi; ++i) {
+      const int32_t r = row(i);
+      if (r >= 0) tmp[size_t(h[r >> shift]++)] = (uint64_t(uint32_t(r)) << 32) | uint32_t(val(i));
+    }
+  }
+  (void)used;
+  const int32_t span = int32_t(1) << shift;
+#pragma omp parallel
+  {
+    std::vector<int64_t> local(size_t(span) + 1);
+#pragma omp for schedule(dynamic, 1)
+    for (int32_t b = 0; b < B; ++b) {
+      const int32_t r0 = b << shift;
+      const int32_t r1 = int32_t(std::min<int64_t>(int64_t(r0) + span, V));
+      std::fill(local.begin(), local.end(), 0);
+      const int64_t a = pstart[size_t(b)], z = pstart[size_t(b) + 1];
+      for (int64_t i = a; i < z; ++i) ++local[size_t(int32_t(tmp[size_t(i)] >> 32) - r0) + 1];
+      for (int32_t r = r0; r < r1; ++r) {
+        local[size_t(r - r0) + 1] += local[size_t(r - r0)];
+        start[r] = a + local[size_t(r - r0)];
+      }
+      for (int64_t i = a; i < z; ++i) {
+        const uint64_t x = tmp[size_t(i)];
+        out[a + local[size_t(int32_t(x >> 32) - r0)]++] = int32_t(uint32_t(x));
+      }
+    }
+  }
+  start[V] = pstart[size_t(B)];
+}
+
+// Reverse CSR of g's forward CSR: the edges, visited in forward order (sources
+// ascending), are bucketed stably by destination, so every srcList row comes
+// out ascending (graph.hpp:810-832 sorts each row the same way).
+void build_reverse(spmh_graph* g) {
+  const int32_t V = g->V;
+  const int64_t E = g->E;
+  std::vector<int32_t> src_of(size_t(std::max<int64_t>(E, 1)));
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int32_t u = 0; u < V; ++u)
+    for (int32_t e = g->indexofNodes[u]; e < g->indexofNodes[u + 1]; ++e) src_of[size_t(e)] = u;
+  std::vector<int64_t> start(size_t(V) + 1);
+  stable_rows(
+      E, V, [&](int64_t e) { return g->edgeList[e]; }, [&](int64_t e) { return src_of[size_t(e)]; }, start.data(),
+      g->srcList);
+#pragma omp parallel for schedule(static)
+  for (int32_t v = 0; v <= V; ++v) g->rev_indexofNodes[v] = int32_t(start[size_t(v)]);
+}
+
 // Builds the forward and reverse CSR from keys sorted ascending and unique,
 // with weights aligned to the keys (or NULL for zero weights).
 spmh_graph* build_csr(const std::vector<uint64_t>& keys, const int32_t* w, int32_t V) {
   const int64_t E = (int64_t)keys.size();
-  spmh_graph* g = static_cast<spmh_graph*>(std::calloc(1, sizeof(spmh_graph)));
-  g->V = V;
-  g->E = E;
-  g->indexofNodes = alloc_i32(V + 1);
-  g->edgeList = alloc_i32(E);
-  g->edgeLen = alloc_i32(E);
-  g->rev_indexofNodes = alloc_i32(V + 1);
-  g->srcList = alloc_i32(E);
-
-  std::vector<std::atomic<int32_t>> outc(V), inc(V);
-#pragma omp parallel for schedule(static)
-  for (int32_t v = 0; v < V; ++v) { outc[v].store(0, std::memory_order_relaxed); inc[v].store(0, std::memory_order_relaxed); }
+  spmh_graph* g = alloc_graph(V, E);
+  std::vector<int32_t> cnt(size_t(V), 0);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < E; ++i) {
-    outc[key_src(keys[i])].fetch_add(1, std::memory_order_relaxed);
-    inc[key_dst(keys[i])].fetch_add(1, std::memory_order_relaxed);
+    __atomic_fetch_add(&cnt[size_t(key_src(keys[i]))], 1, __ATOMIC_RELAXED);
     g->edgeList[i] = key_dst(keys[i]);
     g->edgeLen[i] = w ? w[i] : 0;
   }
-  int64_t acc = 0, racc = 0;
-  for (int32_t v = 0; v < V; ++v) {
-    g->indexofNodes[v] = (int32_t)acc;
-    g->rev_indexofNodes[v] = (int32_t)racc;
-    acc += outc[v].load(std::memory_order_relaxed);
-    racc += inc[v].load(std::memory_order_relaxed);
-  }
-  g->indexofNodes[V] = (int32_t)acc;
-  g->rev_indexofNodes[V] = (int32_t)racc;
-
-  // Reverse CSR: scatter sources with per-row cursors, then sort each row
-  // ascending (graph.hpp:810-832 sorts each srcList row the same way).
-  std::vector<std::atomic<int32_t>>& cur = inc;
+  std::vector<int64_t> off(size_t(V) + 1);
+  scan_counts(cnt.data(), V, off.data());
 #pragma omp parallel for schedule(static)
-  for (int32_t v = 0; v < V; ++v) cur[v].store(g->rev_indexofNodes[v], std::memory_order_relaxed);
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < E; ++i) {
-    int32_t d = key_dst(keys[i]);
-    int32_t pos = cur[d].fetch_add(1, std::memory_order_relaxed);
-    g->srcList[pos] = key_src(keys[i]);
-  }
-#pragma omp parallel for schedule(dynamic, 1024)
-  for (int32_t v = 0; v < V; ++v)
-    std::sort(g->srcList + g->rev_indexofNodes[v], g->srcList + g->rev_indexofNodes[v + 1]);
+  for (int32_t v = 0; v <= V; ++v) g->indexofNodes[v] = int32_t(off[size_t(v)]);
+  build_reverse(g);
   return g;
 }
 
-int32_t max_id(const std::vector<uint64_t>& keys) {
-  int32_t m = 0;  // graph.hpp starts nodesTotal at 0 (graph.hpp:73)
+// CSR of the distinct pairs among n raw (src, dst) samples in any order, with
+// duplicates: self-loops dropped; for undirected graphs every kept pair also
+// stands for its mirror. num_nodes = largest id of a kept pair + 1 (graph.hpp
+// starts nodesTotal at 0, :73). Weights, when asked for, are hashed from the
+// pair (undirected: from the ordered pair, so both directions agree). The
+// samples are bucketed by source (stable_rows) and each row sorted and
+// deduplicated on its own, instead of sorting all samples globally.
+spmh_graph* finish(const uint64_t* keys, int64_t n, uint64_t seed, int undirected, int wmin, int wmax) {
+  int32_t m = 0;
 #pragma omp parallel for reduction(max : m) schedule(static)
-  for (int64_t i = 0; i < (int64_t)keys.size(); ++i) {
-    m = std::max(m, key_src(keys[i]));
-    m = std::max(m, key_dst(keys[i]));
+  for (int64_t i = 0; i < n; ++i) {
+    const int32_t s = key_src(keys[i]), d = key_dst(keys[i]);
+    if (s != d) m = std::max(m, std::max(s, d));
   }
-  return m;
-}
-
-// Sort, drop duplicates, optionally assign hashed weights, build the CSR.
-spmh_graph* finish(std::vector<uint64_t>& keys, uint64_t seed, int undirected, int wmin, int wmax) {
-  __gnu_parallel::sort(keys.begin(), keys.end());
-  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
-  const int32_t V = max_id(keys) + 1;
-  std::vector<int32_t> w;
-  if (wmax > 0) {
-    w.resize(keys.size());
-    const uint64_t span = uint64_t(wmax - wmin + 1);
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < (int64_t)keys.size(); ++i) {
-      uint32_t s = (uint32_t)key_src(keys[i]), d = (uint32_t)key_dst(keys[i]);
-      if (undirected && s > d) std::swap(s, d);
-      w[i] = wmin + int32_t(rng(seed, 0x57, key_of(s, d)) % span);
+  const int32_t V = m + 1;
+  // Item j < n is sample j as (src, dst); with undirected, item n + j its mirror.
+  const int64_t items = undirected ? 2 * n : n;
+  auto row = [&](int64_t j) {
+    const uint64_t k = keys[j < n ? j : j - n];
+    const int32_t s = key_src(k), d = key_dst(k);
+    return s == d ? -1 : (j < n ? s : d);
+  };
+  auto val = [&](int64_t j) {
+    const uint64_t k = keys[j < n ? j : j - n];
+    return j < n ? key_dst(k) : key_src(k);
+  };
+  std::vector<int64_t> start(size_t(V) + 1);
+  int64_t kept = 0;
+#pragma omp parallel for reduction(+ : kept) schedule(static)
+  for (int64_t i = 0; i < n; ++i) kept += key_src(keys[i]) != key_dst(keys[i]);
+  std::vector<int32_t> rows(size_t(std::max<int64_t>(undirected ? 2 * kept : kept, 1)));
+  stable_rows(items, V, row, val, start.data(), rows.data());
+  // Sort and deduplicate each row in place; cnt[v] becomes its distinct count.
+  std::vector<int32_t> cnt(static_cast<size_t>(V));
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int32_t v = 0; v < V; ++v) {
+    int32_t* a = rows.data() + start[size_t(v)];
+    const int64_t len = start[size_t(v) + 1] - start[size_t(v)];
+    sort_row(a, len);
+    cnt[size_t(v)] = len ? int32_t(std::unique(a, a + len) - a) : 0;
+  }
+  std::vector<int64_t> off(size_t(V) + 1);
+  const int64_t E = scan_counts(cnt.data(), V, off.data());
+  if (E > 0x7fffffff) {
+    std::fprintf(stderr, "spm_host: %lld edges exceed the int32 CSR\n", (long long)E);
+    std::abort();
+  }
+  spmh_graph* g = alloc_graph(V, E);
+  const uint64_t span = wmax > 0 ? uint64_t(wmax - wmin + 1) : 1;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int32_t v = 0; v < V; ++v) {
+    const int32_t* a = rows.data() + start[size_t(v)];
+    const int64_t o = off[size_t(v)];
+    g->indexofNodes[v] = int32_t(o);
+    for (int32_t j = 0; j < cnt[size_t(v)]; ++j) {
+      const int32_t d = a[j];
+      g->edgeList[o + j] = d;
+      uint32_t s0 = uint32_t(v), d0 = uint32_t(d);
+      if (undirected && s0 > d0) std::swap(s0, d0);
+      g->edgeLen[o + j] = wmax > 0 ? wmin + int32_t(rng(seed, 0x57, key_of(s0, d0)) % span) : 0;
     }
   }
-  return build_csr(keys, w.empty() ? nullptr : w.data(), V);
+  g->indexofNodes[V] = int32_t(E);
+  if (undirected) {  // symmetric: the in-rows are the out-rows
+    std::memcpy(g->rev_indexofNodes, g->indexofNodes, sizeof(int32_t) * (size_t(V) + 1));
+    std::memcpy(g->srcList, g->edgeList, sizeof(int32_t) * size_t(E));
+  } else {
+    build_reverse(g);
+  }
+  return g;
 }
 
-void drop_loops_and_symmetrise(std::vector<uint64_t>& keys, int undirected) {
-  // Self-loops removed; for undirected graphs each kept pair is mirrored.
-  const int64_t n = (int64_t)keys.size();
-  std::vector<uint8_t> keep(n);
-  int64_t cnt = 0;
-#pragma omp parallel for reduction(+ : cnt) schedule(static)
-  for (int64_t i = 0; i < n; ++i) {
-    keep[i] = key_src(keys[i]) != key_dst(keys[i]);
-    cnt += keep[i];
+// Uninitialised sample buffer (the sampling loop writes every element, so
+// its pages are first touched by the threads that fill them).
+struct Samples {
+  uint64_t* p;
+  explicit Samples(int64_t n) : p(static_cast<uint64_t*>(std::malloc(sizeof(uint64_t) * size_t(n > 0 ? n : 1)))) {
+    if (!p) { std::fprintf(stderr, "spm_host: out of memory (%lld samples)\n", (long long)n); std::abort(); }
   }
-  std::vector<uint64_t> out;
-  out.reserve(undirected ? 2 * cnt : cnt);
-  for (int64_t i = 0; i < n; ++i)
-    if (keep[i]) out.push_back(keys[i]);
-  if (undirected) {
-    const int64_t m = (int64_t)out.size();
-    out.resize(2 * m);
-#pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < m; ++i) out[m + i] = key_of((uint32_t)key_dst(out[i]), (uint32_t)key_src(out[i]));
-  }
-  keys.swap(out);
-}
+  ~Samples() { std::free(p); }
+};
 
 }  // namespace
 
@@ -162,42 +306,42 @@ spmh_graph* spmh_rmat(int scale, int edge_factor, double a, double b, double c,
   const uint64_t ta = uint64_t(a * 4294967296.0);
   const uint64_t tb = uint64_t((a + b) * 4294967296.0);
   const uint64_t tc = uint64_t((a + b + c) * 4294967296.0);
-  std::vector<uint64_t> keys(n);
+  Samples keys(n);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     uint32_t s = 0, d = 0;
+    const uint64_t base = rng_base(seed, uint64_t(i));
     for (int l = 0; l < scale; l += 2) {
-      const uint64_t h = rng(seed, uint64_t(i), uint64_t(l));
+      const uint64_t h = rng_at(base, uint64_t(l));
       for (int k = 0; k < 2 && l + k < scale; ++k) {
         const uint64_t r = k ? (h >> 32) : (h & 0xffffffffu);
-        uint32_t bs, bd;
-        if (r < ta) { bs = 0; bd = 0; }
-        else if (r < tb) { bs = 0; bd = 1; }
-        else if (r < tc) { bs = 1; bd = 0; }
-        else { bs = 1; bd = 1; }
+        // Quadrant a | b | c | d of [0, 2^32) without branches (their
+        // outcomes are random, so a branchy form mispredicts every level):
+        // bs = r >= tb; bd = (ta <= r < tb) or (r >= tc).
+        const uint32_t bs = uint32_t(r >= tb);
+        const uint32_t bd = uint32_t(r >= ta) ^ bs ^ uint32_t(r >= tc);
         s = (s << 1) | bs;
         d = (d << 1) | bd;
       }
     }
-    keys[i] = key_of(s, d);
+    keys.p[i] = key_of(s, d);
   }
-  drop_loops_and_symmetrise(keys, undirected);
-  return finish(keys, seed, undirected, wmin, wmax);
+  return finish(keys.p, n, seed, undirected, wmin, wmax);
 }
 
 spmh_graph* spmh_er(int64_t nv, int64_t m, uint64_t seed, int undirected, int wmin, int wmax) {
   if (nv < 1 || nv > 0x7fffffff || m < 0) return nullptr;
-  std::vector<uint64_t> keys(m);
+  Samples keys(m);
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < m; ++i) {
-    const uint64_t h1 = rng(seed, uint64_t(i), 1), h2 = rng(seed, uint64_t(i), 2);
+    const uint64_t base = rng_base(seed, uint64_t(i));
+    const uint64_t h1 = rng_at(base, 1), h2 = rng_at(base, 2);
     // Multiply-shift maps a 64-bit uniform onto [0, nv).
     const uint32_t s = uint32_t((unsigned __int128)h1 * (uint64_t)nv >> 64);
     const uint32_t d = uint32_t((unsigned __int128)h2 * (uint64_t)nv >> 64);
-    keys[i] = key_of(s, d);
+    keys.p[i] = key_of(s, d);
   }
-  drop_loops_and_symmetrise(keys, undirected);
-  return finish(keys, seed, undirected, wmin, wmax);
+  return finish(keys.p, m, seed, undirected, wmin, wmax);
 }
 
 spmh_graph* spmh_grid(int rows, int cols, uint64_t seed, int wmin, int wmax) {
@@ -210,9 +354,7 @@ spmh_graph* spmh_grid(int rows, int cols, uint64_t seed, int wmin, int wmax) {
       if (c + 1 < cols) keys.push_back(key_of(v, v + 1));
       if (r + 1 < rows) keys.push_back(key_of(v, v + cols));
     }
-  drop_loops_and_symmetrise(keys, 1);
-  spmh_graph* g = finish(keys, seed, 1, wmin, wmax);
-  return g;
+  return finish(keys.data(), int64_t(keys.size()), seed, 1, wmin, wmax);
 }
 
 spmh_graph* spmh_from_edges(int64_t n, const int32_t* src, const int32_t* dst, const int32_t* w) {
@@ -457,29 +599,59 @@ int spmh_features(const spmh_graph* g, float job, float* out) {
   }
   const double skew = var > 0 ? (s3 / V) / std::pow(var, 1.5) : 0.0;
   // Double-sweep BFS over out-edges: from the highest-degree vertex, then
-  // from the deepest vertex of that sweep.
+  // from the deepest vertex of that sweep (the smallest id of its last level,
+  // so the estimate does not depend on the thread schedule). Level-synchronous;
+  // levels with many vertices are expanded by all threads (CAS on the level).
   auto bfs = [&](int32_t root, int32_t* far, int64_t* reached, int64_t* widest) {
-    std::vector<int32_t> level(V, -1), q;
-    q.reserve(V);
-    level[root] = 0;
-    q.push_back(root);
+    std::vector<int32_t> level(size_t(V), -1), cur{root}, nxt;
+    level[size_t(root)] = 0;
     int32_t depth = 0;
-    std::vector<int64_t> width(1, 1);
-    for (size_t h = 0; h < q.size(); ++h) {
-      const int32_t v = q[h];
-      for (int32_t e = off[v]; e < off[v + 1]; ++e) {
-        const int32_t w = g->edgeList[e];
-        if (level[w] < 0) {
-          level[w] = level[v] + 1;
-          q.push_back(w);
-          if (level[w] > depth) { depth = level[w]; width.push_back(0); }
-          ++width[level[w]];
+    int64_t seen = 1, wide = 1;
+    int32_t last = root;
+    const int T = omp_get_max_threads();
+    std::vector<std::vector<int32_t>> loc(static_cast<size_t>(T));
+    for (;;) {
+      nxt.clear();
+      const int32_t nl = depth + 1;
+      if (cur.size() < 1024) {
+        for (const int32_t v : cur)
+          for (int32_t e = off[v]; e < off[v + 1]; ++e) {
+            const int32_t w = g->edgeList[e];
+            if (level[size_t(w)] < 0) {
+              level[size_t(w)] = nl;
+              nxt.push_back(w);
+            }
+          }
+      } else {
+#pragma omp parallel num_threads(T)
+        {
+          std::vector<int32_t>& L = loc[size_t(omp_get_thread_num())];
+          L.clear();
+#pragma omp for schedule(dynamic, 64)
+          for (int64_t i = 0; i < int64_t(cur.size()); ++i) {
+            const int32_t v = cur[size_t(i)];
+            for (int32_t e = off[v]; e < off[v + 1]; ++e) {
+              const int32_t w = g->edgeList[e];
+              int32_t expect = -1;
+              if (__atomic_load_n(&level[size_t(w)], __ATOMIC_RELAXED) < 0 &&
+                  __atomic_compare_exchange_n(&level[size_t(w)], &expect, nl, false, __ATOMIC_RELAXED,
+                                              __ATOMIC_RELAXED))
+                L.push_back(w);
+            }
+          }
         }
+        for (const auto& L : loc) nxt.insert(nxt.end(), L.begin(), L.end());
       }
+      if (nxt.empty()) break;
+      depth = nl;
+      seen += int64_t(nxt.size());
+      wide = std::max<int64_t>(wide, int64_t(nxt.size()));
+      last = *std::min_element(nxt.begin(), nxt.end());
+      cur.swap(nxt);
     }
-    *far = q.back();
-    *reached = int64_t(q.size());
-    *widest = *std::max_element(width.begin(), width.end());
+    *far = last;
+    *reached = seen;
+    *widest = wide;
     return depth;
   };
   int32_t root = 0;
@@ -491,13 +663,19 @@ int spmh_features(const spmh_graph* g, float job, float* out) {
   int32_t far2 = far;
   int32_t d2 = bfs(far, &far2, &r2, &w2);
   const int32_t diam = std::max(d1, d2);
+  // Reciprocated edges: (u, w) with (w, u) also present, i.e. w in both the
+  // out-row and the in-row of u; both rows are sorted, so one merge per vertex.
   int64_t recip = 0;
 #pragma omp parallel for reduction(+ : recip) schedule(dynamic, 1024)
-  for (int32_t u = 0; u < V; ++u)
+  for (int32_t u = 0; u < V; ++u) {
+    int32_t j = roff[u];
+    const int32_t jend = roff[u + 1];
     for (int32_t e = off[u]; e < off[u + 1]; ++e) {
       const int32_t w = g->edgeList[e];
-      recip += std::binary_search(g->edgeList + off[w], g->edgeList + off[w + 1], u) ? 1 : 0;
+      while (j < jend && g->srcList[j] < w) ++j;
+      recip += (j < jend && g->srcList[j] == w) ? 1 : 0;
     }
+  }
   out[0] = float(std::log2(double(V)));
   out[1] = float(std::log2(E + 1));
   out[2] = float(mean);

@@ -193,3 +193,96 @@ def test_edge_map_and_degrees():
     for v in range(g.V):
         assert g.getOutDegree(v) == g.count_outNbrs(v) == len(g.neighbors(v))
         assert g.getInDegree(v) == len(g.nodes_to(v))
+
+
+# ---- generators restated in numpy --------------------------------------------
+# The generators draw from a counter-based stream: value(seed, stream, counter)
+# = mix64(mix64(seed ^ mix64(stream * K)) + counter) with the splitmix64
+# finaliser. Restated here with wrapping uint64 arithmetic, the samples are
+# then reduced to a CSR by numpy's unique/lexsort, an algorithm independent
+# of the library's bucketing.
+M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _mix64(z):
+    z = z + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _rng(seed, stream, counter):
+    with np.errstate(over="ignore"):
+        base = _mix64(np.uint64(seed) ^ _mix64(np.asarray(stream, np.uint64) * np.uint64(0xD1B54A32D192ED03)))
+        return _mix64(base + np.asarray(counter, np.uint64))
+
+
+def _csr_from_samples(s, d, seed, undirected, wts):
+    keep = s != d
+    s, d = s[keep].astype(np.int64), d[keep].astype(np.int64)
+    if undirected:
+        s, d = np.concatenate([s, d]), np.concatenate([d, s])
+    V = int(max(s.max(initial=0), d.max(initial=0))) + 1
+    key = np.unique(s * V + d)
+    src, dst = key // V, key % V
+    off = np.searchsorted(src, np.arange(V + 1)).astype(np.int32)
+    w = np.zeros(len(key), np.int32)
+    if wts:
+        lo, hi = wts
+        a, b = (np.minimum(src, dst), np.maximum(src, dst)) if undirected else (src, dst)
+        h = _rng(seed, 0x57, (a.astype(np.uint64) << np.uint64(32)) | b.astype(np.uint64))
+        w = (lo + (h % np.uint64(hi - lo + 1))).astype(np.int32)
+    order = np.lexsort((src, dst))
+    roff = np.searchsorted(dst[order], np.arange(V + 1)).astype(np.int32)
+    return V, off, dst.astype(np.int32), w, roff, src[order].astype(np.int32)
+
+
+def _check_against(g, ref):
+    V, off, adj, w, roff, radj = ref
+    assert g.V == V
+    np.testing.assert_array_equal(g.indexofNodes, off)
+    np.testing.assert_array_equal(g.edgeList, adj)
+    np.testing.assert_array_equal(g.edgeLen, w)
+    np.testing.assert_array_equal(g.rev_indexofNodes, roff)
+    np.testing.assert_array_equal(g.srcList, radj)
+
+
+@pytest.mark.parametrize("scale,ef,undirected,wts", [(9, 16, False, (1, 100)), (10, 8, True, None), (7, 30, True, (1, 9))])
+def test_rmat_matches_numpy_restatement(scale, ef, undirected, wts):
+    a, b, c, seed = 0.57, 0.19, 0.19, 11
+    n = ef << scale
+    ta, tb, tc = (np.uint64(int(x * 4294967296.0)) for x in (a, a + b, a + b + c))
+    i = np.arange(n, dtype=np.uint64)
+    s = np.zeros(n, np.uint64)
+    d = np.zeros(n, np.uint64)
+    for lev in range(0, scale, 2):
+        h = _rng(seed, i, lev)
+        for k in range(2):
+            if lev + k >= scale:
+                break
+            r = (h >> np.uint64(32)) if k else (h & np.uint64(0xFFFFFFFF))
+            bs = (r >= tb).astype(np.uint64)
+            bd = (((r >= ta) & (r < tb)) | (r >= tc)).astype(np.uint64)
+            s = (s << np.uint64(1)) | bs
+            d = (d << np.uint64(1)) | bd
+    g = graph.rmat(scale, ef, a, b, c, seed=seed, undirected=undirected, weights=wts)
+    _check_against(g, _csr_from_samples(s.astype(np.int64), d.astype(np.int64), seed, undirected, wts))
+
+
+@pytest.mark.parametrize("undirected", [False, True])
+def test_er_matches_numpy_restatement(undirected):
+    nv, m, seed = 3000, 12000, 6
+    i = np.arange(m, dtype=np.uint64)
+    mul = lambda h: ((h >> np.uint64(32)) * np.uint64(nv) + (((h & np.uint64(0xFFFFFFFF)) * np.uint64(nv)) >> np.uint64(32))) >> np.uint64(32)  # noqa: E731
+    s, d = mul(_rng(seed, i, 1)), mul(_rng(seed, i, 2))
+    g = graph.erdos_renyi(nv, m, seed=seed, undirected=undirected, weights=(1, 100))
+    _check_against(g, _csr_from_samples(s.astype(np.int64), d.astype(np.int64), seed, undirected, (1, 100)))
+
+
+def test_grid_matches_numpy_restatement():
+    rows, cols = 13, 17
+    v = np.arange(rows * cols).reshape(rows, cols)
+    s = np.concatenate([v[:, :-1].ravel(), v[:-1, :].ravel()])
+    d = np.concatenate([v[:, 1:].ravel(), v[1:, :].ravel()])
+    g = graph.grid2d(rows, cols, seed=2, weights=(1, 50))
+    _check_against(g, _csr_from_samples(s, d, 2, True, (1, 50)))

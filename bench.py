@@ -5,13 +5,17 @@ scale-22 graph (edge factor 16, a,b,c = .57,.19,.19, int32 weights uniform in
 [1,100], source 0, seed 2), in GTEPS = m_reach / time, where m_reach is the
 number of out-edges of vertices reachable from the source (Graph500
 convention). One step = one Compute_SSSP on the graph already resident in HBM.
-PageRank, triangle counting and batched BC are measured the same way and
-reported under "per_algo" with their own rooflines.
+PageRank, triangle counting, batched BC, the dynamic programs and the
+mixed-batch schedule study (2,000 graphs, 400 held out) are measured in the
+same run and reported under "per_algo" / "schedule_selection".
 
 Roofline: algorithmic bytes of the dominant kernel's work divided by that
 kernel's device time, measured live with HIP events on the handle's stream.
 CPU baseline: the OpenMP restatement of StarPlat's generated code (oracle/),
-timed on this host's cores on a bounded sample.
+run after the GPU legs in a child process of this script whose environment
+sets OMP_PROC_BIND=close and OMP_PLACES=cores before OpenMP starts; the child
+regenerates the same seeded graphs, times the restatement and checks the GPU
+outputs handed over in files.
 """
 from __future__ import annotations
 
@@ -101,10 +105,51 @@ def host_cores():
         return platform.processor()
 
 
+# --------------------------------------------------------------------------- graphs
+# The seeded inputs of every leg; the CPU child rebuilds them from the same
+# arguments (the generators are deterministic).
+def graph_for(leg, args):
+    if leg in ("sssp", "dyn"):
+        return graph.rmat(args.sssp_scale, 16, seed=2, weights=(1, 100))
+    if leg in ("pr", "dyn_pr"):
+        return graph.rmat(22, 16, seed=1)
+    if leg == "pr_c0":
+        return graph.rmat(16, 16, seed=1)
+    if leg in ("tc", "dyn_tc"):
+        return graph.rmat(args.tc_scale, 16, seed=3, undirected=True)
+    if leg == "bc":
+        return graph.erdos_renyi(args.bc_n, args.bc_n * 4, seed=4, undirected=True)
+    raise KeyError(leg)
+
+
+class Stash:
+    """GPU outputs handed to the CPU child: one .npz of arrays and a JSON of
+    scalars per leg, in a temporary directory."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def put(self, leg, arrays=None, **meta):
+        if arrays:
+            np.savez(os.path.join(self.path, leg + ".npz"), **arrays)
+        with open(os.path.join(self.path, leg + ".json"), "w") as f:
+            json.dump(meta, f)
+
+    def legs(self):
+        return sorted(n[:-5] for n in os.listdir(self.path) if n.endswith(".json") and n != "cpu.json")
+
+    def get(self, leg):
+        with open(os.path.join(self.path, leg + ".json")) as f:
+            meta = json.load(f)
+        p = os.path.join(self.path, leg + ".npz")
+        arrays = dict(np.load(p)) if os.path.exists(p) else {}
+        return meta, arrays
+
+
 # --------------------------------------------------------------------------- SSSP
-def bench_sssp(args, O):
+def bench_sssp(args, stash):
     log("generating RMAT-22 (directed, weights U[1,100], seed 2)")
-    g = graph.rmat(args.sssp_scale, 16, seed=2, weights=(1, 100))
+    g = graph_for("sssp", args)
     r, m, rl = C.c_int64(), C.c_int64(), C.c_int64()
     model = N.host_lib().spmh_model_sssp(g._native(), N.ptr(g.edgeLen, C.c_int32), 0, C.byref(r), C.byref(m), C.byref(rl))
     dev = g.device()
@@ -123,31 +168,21 @@ def bench_sssp(args, O):
         "relaxations_sync_trace": rl.value, "ms_per_step": round(ms, 4),
         "first_run_ms": round(first_ms, 3), "relabel": int(dev.schedule().relabel),
         "gteps": m.value / (ms * 1e-3) / 1e9,
+        "gteps_first_call": m.value / (first_ms * 1e-3) / 1e9,
         "roofline": roofline(model, kms, launches, f"sssp-rmat{args.sssp_scale}", ["sssp_relax"]),
         "algorithmic_bytes": model,
     }
-    if not args.no_cpu_baseline:
-        ref = O.compute_sssp(g, g.edgeLen, 0)
-        got = d_out.to_host()
-        res["parity_bit_exact"] = bool(np.array_equal(got, ref))
-        times = []
-        for _ in range(args.cpu_reps):
-            t0 = time.perf_counter()
-            O.compute_sssp(g, g.edgeLen, 0)
-            times.append(time.perf_counter() - t0)
-        t = statistics.median(times)
-        res["cpu_baseline"] = {"value": round(m.value / t / 1e9, 4), "unit": "GTEPS", "cores": O.num_threads(),
-                               "kind": "port", "sample": f"full workload, median of {args.cpu_reps} runs after 1 warm-up",
-                               "seconds": round(t, 4)}
+    stash.put("sssp", {"dist": d_out.to_host()}, m_reach=m.value)
     d_out.free()
     g.release_device()
     return res
 
 
 # --------------------------------------------------------------------------- PR
-def bench_pr(args, O, scale, seed, label):
+def bench_pr(args, stash, leg, label):
+    scale = 22 if leg == "pr" else 16
     log(f"generating RMAT-{scale} for PageRank")
-    g = graph.rmat(scale, 16, seed=seed)
+    g = graph_for(leg, args)
     model = N.host_lib().spmh_model_pr_bytes(g._native())
     dev = g.device()
     d_out = hip.DeviceArray(g.V, np.float32)
@@ -158,43 +193,29 @@ def bench_pr(args, O, scale, seed, label):
     ms = timed_steps(lambda: dev.pr_device(1e-10, 0.85, 100, d_out), max(1, args.steps // 2), 1)
     kt = kernel_time(dev, lambda: dev.pr_device(1e-10, 0.85, 100, d_out), ["pr_iter"], reps=1)
     kms, launches = kt["pr_iter"]
-    res = {"workload": f"{label}: pull PageRank, directed RMAT scale {scale} EF16, beta 1e-10, delta 0.85, maxIter 100, seed {seed}",
+    res = {"workload": f"{label}: pull PageRank, directed RMAT scale {scale} EF16, beta 1e-10, delta 0.85, maxIter 100, seed 1",
            "V": g.V, "E": g.E, "iterations": it, "ms_per_run": round(ms, 4),
            "first_run_ms": round(first_ms, 3), "relabel": int(dev.schedule().relabel),
            "gteps": g.E * it / (ms * 1e-3) / 1e9,
+           "gteps_first_call": g.E * it / (first_ms * 1e-3) / 1e9,
            "roofline": roofline(model * it, kms, launches, f"pr-rmat{scale}", ["pr_iter"])}
-    if not args.no_cpu_baseline:
-        ref, it_ref = O.compute_pr(g, 1e-10, 0.85, 100)
-        got = d_out.to_host()
-        res["rel_linf_vs_restatement"] = float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
-        res["iterations_oracle"] = it_ref
-        # The tolerance is against FP64 at the same iteration count; the FP32
-        # restatement's own error is reported beside it (a sequential FP32 sum
-        # over a hub's in-edges is itself off by about 2e-5 at RMAT-22).
-        fp64, _ = O.pr_fp64(g, 1e-10, 0.85, it)
-        fp64_ref, _ = O.pr_fp64(g, 1e-10, 0.85, it_ref)
-        res["rel_linf_vs_fp64"] = float(np.max(np.abs(got - fp64)) / np.max(np.abs(fp64)))
-        res["restatement_rel_linf_vs_fp64"] = float(np.max(np.abs(ref - fp64_ref)) / np.max(np.abs(fp64_ref)))
-        res["tolerance_vs_fp64"] = 1e-5
-        k = 10
-        t0 = time.perf_counter()
-        O.compute_pr(g, 1e-10, 0.85, k)
-        t = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": round(g.E * k / t / 1e9, 4), "unit": "GTEPS", "cores": O.num_threads(),
-                               "kind": "port", "sample": f"first {k} iterations", "seconds": round(t, 4)}
+    stash.put(leg, {"ranks": d_out.to_host()}, iterations=it)
     d_out.free()
     g.release_device()
     return res
 
 
 # --------------------------------------------------------------------------- TC
-def bench_tc(args, O):
+def bench_tc(args, stash):
     log("generating undirected RMAT-21 for TC")
-    g = graph.rmat(args.tc_scale, 16, seed=3, undirected=True)
+    g = graph_for("tc", args)
     mu, wk = C.c_int64(), C.c_int64()
     model = N.host_lib().spmh_model_tc_bytes(g._native(), C.byref(mu), C.byref(wk))
     dev = g.device()
-    count = dev.tc()  # includes one-time orientation
+    hip.synchronize()
+    t0 = time.perf_counter()
+    count = dev.tc()  # includes the one-time symmetry test and orientation
+    first_ms = (time.perf_counter() - t0) * 1e3
     ms = timed_steps(lambda: dev.tc(), max(1, args.steps // 4), 1)
     # the hashed count is the default; the search runs when the schedule or the graph asks for it
     kt = kernel_time(dev, lambda: dev.tc(), ["tc_count_hash", "tc_count"], reps=1)
@@ -202,25 +223,19 @@ def bench_tc(args, O):
     kms, launches = kt[kname]
     res = {"workload": f"Triangle counting, undirected RMAT scale {args.tc_scale} EF16, symmetrised, sorted, seed 3",
            "V": g.V, "E": g.E, "m_und": mu.value, "triangles": count, "ms_per_run": round(ms, 4),
-           "tc_mode": int(dev.schedule().tc_mode), "kernel": kname,
+           "first_run_ms": round(first_ms, 3), "tc_mode": int(dev.schedule().tc_mode), "kernel": kname,
            "gteps": mu.value / (ms * 1e-3) / 1e9,
+           "gteps_first_call": mu.value / (first_ms * 1e-3) / 1e9,
            "roofline": roofline(model, kms, launches, f"tc-rmat{args.tc_scale}", [kname])}
-    if not args.no_cpu_baseline:
-        stride = 16
-        t0 = time.perf_counter()
-        O.compute_tc_strided(g, stride)
-        t = (time.perf_counter() - t0) * stride
-        res["cpu_baseline"] = {"value": round(mu.value / t / 1e9, 5), "unit": "GTEPS", "cores": O.num_threads(),
-                               "kind": "port", "sample": f"outer loop over every {stride}th vertex (1/{stride} of the v range), time x{stride}",
-                               "seconds_estimated": round(t, 3)}
+    stash.put("tc", triangles=count, m_und=mu.value)
     g.release_device()
     return res
 
 
 # --------------------------------------------------------------------------- BC
-def bench_bc(args, O):
+def bench_bc(args, stash):
     log("generating Erdos-Renyi 4M / avg degree 8 for BC")
-    g = graph.erdos_renyi(args.bc_n, args.bc_n * 4, seed=4, undirected=True)
+    g = graph_for("bc", args)
     src = sources(g.V, args.bc_sources, seed=4)
     et = C.c_int64()
     log("BC byte model (one BFS per source on the host)")
@@ -240,24 +255,13 @@ def bench_bc(args, O):
            "roofline": roofline(model, kms, launches, f"bc-er{args.bc_n}-{len(src)}", ks),
            "note": "model bytes count every source's traversal separately; the batched kernels share adjacency reads "
                    "across 64 sources, so achieved model GB/s can exceed the HBM peak"}
-    if not args.no_cpu_baseline:
-        k = 2
-        t0 = time.perf_counter()
-        ref = O.compute_bc(g, src[:k])[0]
-        t = (time.perf_counter() - t0) / k * len(src)
-        # the same sample on the device, against the FP64 restatement
-        dev.bc_device(src[:k], d_out)
-        got = d_out.to_host()
-        res["parity_sample"] = {"sources": k, "rel_linf_vs_fp64": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref))),
-                                "tolerance": 1e-5}
-        res["cpu_baseline"] = {"value": round(len(src) * g.E / t / 1e9, 5), "unit": "GTEPS", "cores": O.num_threads(),
-                               "kind": "port", "sample": f"first {k} sources, time scaled to {len(src)}",
-                               "seconds_estimated": round(t, 2)}
+    # the first full 64-source batch alone, for the CPU child's parity check and timing sample
+    k = min(64, len(src))
+    dev.bc_device(src[:k], d_out)
+    stash.put("bc", {"bc_first_batch": d_out.to_host()}, sources=len(src), batch=k)
     d_out.free()
     g.release_device()
     return res
-
-
 # --------------------------------------------------------------------------- dynamic SSSP
 def dyn_batch(g, src_of, size, rng):
     """One batch of `size` updates: half deletions of edges of the original
@@ -269,9 +273,9 @@ def dyn_batch(g, src_of, size, rng):
     return b"d" * nd + b"a" * (size - nd), us, ut
 
 
-def bench_dyn(args, O):
+def bench_dyn(args, stash):
     log(f"dynamic SSSP on RMAT-{args.sssp_scale}: {args.dyn_batches} batches of {args.dyn_batch} updates")
-    g = graph.rmat(args.sssp_scale, 16, seed=2, weights=(1, 100))
+    g = graph_for("dyn", args)
     src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
     rng = np.random.default_rng(5)
     batches = [dyn_batch(g, src_of, args.dyn_batch, rng) for _ in range(args.dyn_batches + 1)]
@@ -321,14 +325,7 @@ def bench_dyn(args, O):
            "note": "per-batch time includes the host's staging copy and one flag read-back per 8 fixed-point "
                    "iterations; the speedup compares with folding the updates plus a steady-state static SSSP "
                    "(static_sssp_first_ms also rebuilds the relabeled layout the fold drops)"}
-    if not args.no_cpu_baseline:
-        tm = {}
-        b = batches[1]
-        O.dyn_sssp(g, g.edgeLen, 0, b[0], b[1], b[2], args.dyn_batch, timing=tm)
-        res["cpu_baseline"] = {"value": round(args.dyn_batch / tm["batch_seconds"]), "unit": "updates/s", "cores": 1,
-                               "kind": "port", "sample": "one batch after the restatement's own static SSSP "
-                               "(sequential DynSSSP restatement, batch loop timed)",
-                               "seconds": round(tm["batch_seconds"], 4)}
+    stash.put("dyn")
     g.release_device()
     return res
 
@@ -361,9 +358,9 @@ def dyn_tc_batch(g, src_of, size, rng, touched):
     return "".join(ts).encode(), np.array(us, np.int32), np.array(ut, np.int32)
 
 
-def bench_dyn_tc(args, O):
+def bench_dyn_tc(args, stash):
     log(f"dynamic TC on undirected RMAT-{args.tc_scale}: {args.dyn_batches} batches of {args.dyn_batch} updates")
-    g = graph.rmat(args.tc_scale, 16, seed=3, undirected=True)
+    g = graph_for("dyn_tc", args)
     src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
     rng = np.random.default_rng(6)
     touched = set()
@@ -407,23 +404,14 @@ def bench_dyn_tc(args, O):
            "speedup_vs_fold_plus_static": round((t_fold + t_static) / ms, 2),
            "kernel_ms_one_batch": {k: round(v[0], 4) for k, v in kt.items() if v[1] > 0},
            "roofline": None}
-    if not args.no_cpu_baseline:
-        tm = {}
-        half, k = args.dyn_batch // 2, 1024  # the batch holds its deletions, then its additions
-        b = batches[1]
-        sl = np.r_[0:k, half:half + k]
-        O.dyn_tc(g, bytes(np.frombuffer(b[0], np.uint8)[sl]), b[1][sl], b[2][sl], 2 * k, timing=tm, initial=0)
-        res["cpu_baseline"] = {"value": round(2 * k / tm["batch_seconds"]), "unit": "updates/s", "cores": 1,
-                               "kind": "port", "sample": f"the first {k} deletions and first {k} additions of one "
-                               "batch as one batch (sequential DynTC restatement, batch loop timed, static count "
-                               "skipped)", "seconds": round(tm["batch_seconds"], 4)}
+    stash.put("dyn_tc")
     g.release_device()
     return res
 
 
-def bench_dyn_pr(args, O):
+def bench_dyn_pr(args):
     log(f"dynamic PageRank on RMAT-{args.sssp_scale}: 3 batches of {args.dyn_batch} updates")
-    g = graph.rmat(args.sssp_scale, 16, seed=1)
+    g = graph_for("dyn_pr", args)
     src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
     has_out = np.flatnonzero(np.diff(g.indexofNodes) > 0).astype(np.int32)
     rng = np.random.default_rng(7)
@@ -469,20 +457,203 @@ def bench_dyn_pr(args, O):
 
 
 # --------------------------------------------------------------------------- schedules
-def bench_schedule(args):
+def bench_schedule(args, stash):
     from starplat_amd import mixed_batch
 
-    log(f"mixed batch: {args.sched_graphs} graphs ({args.sched_held_out} held out), autotune + train")
-    res = mixed_batch.run(n=args.sched_graphs, held_out=args.sched_held_out, seed=5, reps=2, epochs=400, log=log,
-                          rmat_scales=tuple(args.sched_rmat), er_log2=tuple(args.sched_er), grid_sides=tuple(args.sched_grid))
-    res["workload"] = ("mixed batch: RMAT (EF 4-32), Erdos-Renyi (avg degree 4-16) and 2-D grids, each with one SSSP "
-                       "(weights U[1,100], src 0) or PageRank job; 8 schedule classes timed per job; MLP 16-64-64-8 "
-                       "trained on the GPU on the fastest class; makespan = sum of held-out job times; "
-                       "batched_held_out: the same held-out jobs run as disjoint unions, one batched launch "
-                       "sequence per job type (spm_pr_multi_device / spm_sssp_multi_device), default schedule; "
-                       "hybrid_held_out: SSSP jobs and PageRank jobs below 2^17 vertices batched so, the other "
-                       "PageRank jobs run alone under the learned class")
+    log(f"mixed batch: {args.sched_graphs} graphs ({args.sched_held_out} held out): sweep, train, held-out pass")
+    ranges = dict(rmat_scales=tuple(args.sched_rmat), er_log2=tuple(args.sched_er), grid_sides=tuple(args.sched_grid))
+    sample = {}
+    ntr = args.sched_graphs - args.sched_held_out
+
+    def keep(i, job, outs, picks):
+        # every k-th held-out job: its class-0 and learned-class outputs, for
+        # the CPU child's oracle check and timing sample
+        if (i - ntr) % args.sched_cpu_every == 0:
+            kl = picks["learned"]
+            sample[i] = {"kind": job.kind, "learned": kl, "out_fixed": outs[0][0], "it_fixed": outs[0][1],
+                         "out_learned": outs[kl][0], "it_learned": outs[kl][1]}
+
+    res = mixed_batch.run(n=args.sched_graphs, held_out=args.sched_held_out, seed=5, epochs=400, log=log,
+                          on_held_out=keep, batched=args.sched_batched, **ranges)
+    res["workload"] = ("mixed batch (BASELINE config 4): RMAT (scale %d-%d, EF 4-32), Erdos-Renyi (2^%d-2^%d vertices, "
+                       "avg degree 4-16) and 2-D grids (%d^2-%d^2), each with one SSSP (weights U[1,100], src 0) or "
+                       "PageRank job (beta 1e-10, delta 0.85, maxIter 100), seed 5; 8 schedule classes timed per job; "
+                       "MLP 16-64-64-8 trained on the GPU; makespan = sum of held-out job times"
+                       % (*ranges["rmat_scales"], *ranges["er_log2"], *ranges["grid_sides"]))
+    arrays, meta = {}, {"jobs": [], "ranges": {k: list(v) for k, v in ranges.items()}}
+    for i, d in sample.items():
+        arrays[f"fixed_{i}"] = d["out_fixed"]
+        arrays[f"learned_{i}"] = d["out_learned"]
+        meta["jobs"].append({"i": int(i), "kind": int(d["kind"]), "learned": int(d["learned"]),
+                             "it_fixed": int(d["it_fixed"]), "it_learned": int(d["it_learned"])})
+    meta["held_out"] = args.sched_held_out
+    stash.put("sched", arrays, **meta)
     return res
+
+
+# --------------------------------------------------------------------------- CPU leg
+def cpu_median(fn, reps, warm=None):
+    """(median seconds of `reps` timed calls after one warm-up call, last result)."""
+    (warm or fn)()
+    times, out = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), out
+
+
+def rel_linf(got, ref):
+    den = float(np.max(np.abs(ref)))
+    return float(np.max(np.abs(np.asarray(got, np.float64) - ref))) / den if den > 0 else 0.0
+
+
+def cpu_child(args):
+    """The CPU-baseline leg (runs in a child process; see main): the oracle
+    timed on this host's cores and checked against the GPU outputs in
+    args.cpu_child."""
+    nproc = len(os.sched_getaffinity(0))  # before OpenMP binds this thread
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # CPU baseline leg only
+
+    st = Stash(args.cpu_child)
+    threads = O.num_threads()
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
+    base = {"cores": threads, "kind": "port", "omp": env}
+    out = {"host": {"nproc": nproc, "cpu_count": os.cpu_count(), "lscpu": host_cores(), "omp_threads": threads, **env}}
+    reps = args.cpu_reps
+
+    def clog(*a):
+        print("[bench-cpu]", *a, file=sys.stderr, flush=True)
+
+    for leg in st.legs():
+        meta, arr = st.get(leg)
+        clog(leg)
+        r = {}
+        if leg == "sssp":
+            g = graph_for(leg, args)
+            t, ref = cpu_median(lambda: O.compute_sssp(g, g.edgeLen, 0), reps)
+            r["parity_bit_exact"] = bool(np.array_equal(arr["dist"], ref))
+            r["cpu_baseline"] = {**base, "value": round(meta["m_reach"] / t / 1e9, 4), "unit": "GTEPS",
+                                 "sample": f"full workload, median of {reps} runs after 1 warm-up", "seconds": round(t, 4)}
+        elif leg in ("pr", "pr_c0"):
+            g = graph_for(leg, args)
+            t, (ref, it_ref) = cpu_median(lambda: O.compute_pr(g, 1e-10, 0.85, 100), reps)
+            got, it = arr["ranks"], meta["iterations"]
+            fp64, _ = O.pr_fp64(g, 1e-10, 0.85, it)
+            fp64_ref, _ = O.pr_fp64(g, 1e-10, 0.85, it_ref)
+            # The tolerance is against FP64 at the same iteration count; the FP32
+            # restatement's own error is reported beside it (a sequential FP32 sum
+            # over a hub's in-edges is itself off by about 2e-5 at RMAT-22).
+            r.update({"rel_linf_vs_restatement": rel_linf(got, ref), "iterations_oracle": it_ref,
+                      "rel_linf_vs_fp64": rel_linf(got, fp64), "restatement_rel_linf_vs_fp64": rel_linf(ref, fp64_ref),
+                      "tolerance_vs_fp64": 1e-5})
+            r["cpu_baseline"] = {**base, "value": round(g.E * it_ref / t / 1e9, 4), "unit": "GTEPS",
+                                 "sample": f"full run to convergence ({it_ref} iterations), median of {reps} runs "
+                                           "after 1 warm-up", "seconds": round(t, 4)}
+        elif leg == "tc":
+            g = graph_for(leg, args)
+            t0 = time.perf_counter()
+            ref = O.compute_tc(g)
+            t = time.perf_counter() - t0
+            r["parity_bit_exact"] = bool(ref == meta["triangles"])
+            r["triangles_oracle"] = ref
+            r["cpu_baseline"] = {**base, "value": round(meta["m_und"] / t / 1e9, 6), "unit": "GTEPS",
+                                 "sample": "full workload (every outer vertex, binary-search is_an_edge), one run",
+                                 "seconds": round(t, 3)}
+        elif leg == "bc":
+            g = graph_for(leg, args)
+            src = sources(g.V, meta["sources"], seed=4)[:meta["batch"]]
+            t, (ref, _, _) = cpu_median(lambda: O.compute_bc(g, src), reps, warm=lambda: O.compute_bc(g, src[:8]))
+            t_all = t * meta["sources"] / len(src)
+            r["parity_batch"] = {"sources": len(src), "rel_linf_vs_fp64": rel_linf(arr["bc_first_batch"], ref),
+                                 "tolerance": 1e-5}
+            r["cpu_baseline"] = {**base, "value": round(meta["sources"] * g.E / t_all / 1e9, 5), "unit": "GTEPS",
+                                 "sample": f"the first {len(src)}-source batch (median of {reps} runs after a warm-up "
+                                           f"on 8 sources), time scaled to {meta['sources']} sources",
+                                 "seconds_estimated": round(t_all, 2)}
+        elif leg == "dyn":
+            g = graph_for(leg, args)
+            src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
+            rng = np.random.default_rng(5)
+            b = [dyn_batch(g, src_of, args.dyn_batch, rng) for _ in range(2)][1]
+            tm = {}
+            O.dyn_sssp(g, g.edgeLen, 0, b[0], b[1], b[2], args.dyn_batch, timing=tm)
+            r["cpu_baseline"] = {**base, "cores": 1, "value": round(args.dyn_batch / tm["batch_seconds"]),
+                                 "unit": "updates/s", "sample": "one batch after the restatement's own static SSSP "
+                                 "(sequential DynSSSP restatement, batch loop timed)",
+                                 "seconds": round(tm["batch_seconds"], 4)}
+        elif leg == "dyn_tc":
+            g = graph_for(leg, args)
+            src_of = np.repeat(np.arange(g.V, dtype=np.int32), np.diff(g.indexofNodes))
+            rng = np.random.default_rng(6)
+            touched = set()
+            b = [dyn_tc_batch(g, src_of, args.dyn_batch, rng, touched) for _ in range(2)][1]
+            tm = {}
+            half, k = args.dyn_batch // 2, 1024  # the batch holds its deletions, then its additions
+            sl = np.r_[0:k, half:half + k]
+            O.dyn_tc(g, bytes(np.frombuffer(b[0], np.uint8)[sl]), b[1][sl], b[2][sl], 2 * k, timing=tm, initial=0)
+            r["cpu_baseline"] = {**base, "cores": 1, "value": round(2 * k / tm["batch_seconds"]), "unit": "updates/s",
+                                 "sample": f"the first {k} deletions and first {k} additions of one batch as one "
+                                 "batch (sequential DynTC restatement, batch loop timed, static count skipped)",
+                                 "seconds": round(tm["batch_seconds"], 4)}
+        elif leg == "sched":
+            from starplat_amd import mixed_batch
+            from starplat_amd.schedule import JOB_SSSP
+            ranges = {k: tuple(v) for k, v in meta["ranges"].items()}
+            total, checked, bad, worst = 0.0, 0, [], 0.0
+            for j in meta["jobs"]:
+                job = mixed_batch.make_job(5, j["i"], **ranges)
+                g = job.g
+                t0 = time.perf_counter()
+                if job.kind == JOB_SSSP:
+                    ref = O.compute_sssp(g, g.edgeLen, 0)
+                else:
+                    ref, _ = O.compute_pr(g, 1e-10, 0.85, 100)
+                total += time.perf_counter() - t0
+                for which in ("fixed", "learned"):
+                    got = arr[f"{which}_{j['i']}"]
+                    checked += 1
+                    if job.kind == JOB_SSSP:
+                        ok = bool(np.array_equal(got, ref))
+                    else:
+                        fp64, _ = O.pr_fp64(g, 1e-10, 0.85, j[f"it_{which}"])
+                        e = rel_linf(got, fp64)
+                        worst = max(worst, e)
+                        ok = e <= 1e-5
+                    if not ok:
+                        bad.append({"job": j["i"], "name": job.name, "policy": which})
+            n = len(meta["jobs"])
+            r["oracle_check"] = {"jobs": n, "outputs_checked": checked, "mismatches": bad,
+                                 "pr_max_rel_linf_vs_fp64": worst, "tolerance_pr": 1e-5,
+                                 "rule": "SSSP bit-exact vs the restatement; PageRank vs the FP64 restatement at the "
+                                         "run's own iteration count"}
+            est = total * meta["held_out"] / max(n, 1)
+            r["cpu_baseline"] = {**base, "value": round(est * 1e3, 1), "unit": "ms (held-out makespan)",
+                                 "sample": f"every {meta['held_out'] // max(n, 1)}th held-out job ({n} jobs), each "
+                                           f"restated once back to back, time scaled to {meta['held_out']} jobs",
+                                 "seconds": round(total, 3)}
+        out[leg] = r
+    with open(os.path.join(args.cpu_child, "cpu.json"), "w") as f:
+        json.dump(out, f)
+
+
+def run_cpu_leg(args, stash_dir):
+    """Runs cpu_child in a child process with OpenMP bound to cores (the
+    binding has to be in the environment before the OpenMP runtime starts,
+    and must not bind this process's own threads). Returns its results."""
+    import subprocess
+
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-child", stash_dir, "--cpu-reps", str(args.cpu_reps),
+           "--sssp-scale", str(args.sssp_scale), "--tc-scale", str(args.tc_scale), "--bc-n", str(args.bc_n),
+           "--dyn-batch", str(args.dyn_batch)]
+    log("CPU baseline leg (child process, OMP_PROC_BIND=close OMP_PLACES=cores)")
+    rc = subprocess.call(cmd, env=env)
+    if rc != 0:
+        raise RuntimeError(f"CPU baseline child failed with status {rc}")
+    with open(os.path.join(stash_dir, "cpu.json")) as f:
+        return json.load(f)
 
 
 def main():
@@ -493,21 +664,28 @@ def main():
     ap.add_argument("--algos", default="sssp,pr,pr_c0,tc,bc,dyn,dyn_tc,dyn_pr,sched")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--sssp-scale", type=int, default=22)
     ap.add_argument("--tc-scale", type=int, default=21)
     ap.add_argument("--bc-n", type=int, default=4194304)
     ap.add_argument("--bc-sources", type=int, default=256)
     ap.add_argument("--dyn-batch", type=int, default=65536)
     ap.add_argument("--dyn-batches", type=int, default=6)
-    ap.add_argument("--sched-graphs", type=int, default=300)
-    ap.add_argument("--sched-held-out", type=int, default=50)
-    ap.add_argument("--sched-rmat", type=int, nargs=2, default=[10, 16])
-    ap.add_argument("--sched-er", type=int, nargs=2, default=[10, 16])
-    ap.add_argument("--sched-grid", type=int, nargs=2, default=[32, 256])
+    ap.add_argument("--sched-graphs", type=int, default=2000)
+    ap.add_argument("--sched-held-out", type=int, default=400)
+    ap.add_argument("--sched-rmat", type=int, nargs=2, default=[12, 20])
+    ap.add_argument("--sched-er", type=int, nargs=2, default=[12, 20])
+    ap.add_argument("--sched-grid", type=int, nargs=2, default=[64, 1024])
+    ap.add_argument("--sched-cpu-every", type=int, default=20,
+                    help="the CPU leg restates every k-th held-out job of the mixed batch")
+    ap.add_argument("--sched-batched", action="store_true", help="also time the held-out jobs as batched unions")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a known-size kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_child(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
@@ -524,31 +702,43 @@ def main():
         a.free()
         b.free()
         log(f"calibration copy of {args.copy_calibration_mib} MiB done")
-    O = None
-    if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O  # CPU baseline leg only
+    import tempfile
+
     algos = args.algos.split(",")
     per = {}
     head = None
-    if "sssp" in algos:
-        head = bench_sssp(args, O)
+    sched = None
+    with tempfile.TemporaryDirectory(prefix="spm_bench_") as tmp:
+        stash = Stash(tmp)
+        if "sssp" in algos:
+            per["sssp"] = bench_sssp(args, stash)
+        if "pr" in algos:
+            per["pr"] = bench_pr(args, stash, "pr", "PageRank at RMAT-22")
+        if "pr_c0" in algos:
+            per["pr_c0"] = bench_pr(args, stash, "pr_c0", "PageRank config C0")
+        if "tc" in algos:
+            per["tc"] = bench_tc(args, stash)
+        if "bc" in algos:
+            per["bc"] = bench_bc(args, stash)
+        if "dyn" in algos:
+            per["dyn"] = bench_dyn(args, stash)
+        if "dyn_tc" in algos:
+            per["dyn_tc"] = bench_dyn_tc(args, stash)
+        if "dyn_pr" in algos:
+            per["dyn_pr"] = bench_dyn_pr(args)
+        if "sched" in algos:
+            sched = bench_schedule(args, stash)
+        cpu = None if args.no_cpu_baseline else run_cpu_leg(args, tmp)
+    if cpu is not None:
+        for leg, r in cpu.items():
+            if leg == "host":
+                continue
+            dst = sched if leg == "sched" else per.get(leg)
+            if dst is not None:
+                dst.update(r)
+    if "sssp" in per:
+        head = per["sssp"]
         per["sssp"] = {k: v for k, v in head.items() if k not in ("roofline", "cpu_baseline")}
-    if "pr" in algos:
-        per["pr"] = bench_pr(args, O, 22, 1, "PageRank at RMAT-22")
-    if "pr_c0" in algos:
-        per["pr_c0"] = bench_pr(args, O, 16, 1, "PageRank config C0")
-    if "tc" in algos:
-        per["tc"] = bench_tc(args, O)
-    if "bc" in algos:
-        per["bc"] = bench_bc(args, O)
-    if "dyn" in algos:
-        per["dyn"] = bench_dyn(args, O)
-    if "dyn_tc" in algos:
-        per["dyn_tc"] = bench_dyn_tc(args, O)
-    if "dyn_pr" in algos:
-        per["dyn_pr"] = bench_dyn_pr(args, O)
-    sched = bench_schedule(args) if "sched" in algos else None
     if head is None and per:
         first = next(iter(per.values()))
         head = {"workload": first["workload"], "gteps": first.get("gteps", float("nan")),
@@ -566,7 +756,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": head["roofline"],
         "cpu_baseline": head.get("cpu_baseline"),
-        "host": host_cores(),
+        "host": cpu["host"] if cpu else host_cores(),
         "per_algo": per,
     }
     if sched is not None:

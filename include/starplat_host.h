@@ -103,6 +103,12 @@ int spmh_edge_map(int32_t V, const int32_t* indexofNodes, const int32_t* edgeLis
  *    edges whose reverse exists) */
 int spmh_features(const spmh_graph* g, float job, float* out);
 
+/* OpenMP threads used by this library's parallel regions started from the
+ * calling thread (n > 0; the other threads keep their own setting). A
+ * thread that builds graphs in the background while another thread times
+ * GPU work leaves cores to it this way. */
+void spmh_set_threads(int n);
+
 /* Algorithmic-byte models used for roofline accounting. */
 /* PR: bytes of one pull iteration = 8E + 20V + 4. */
 double spmh_model_pr_bytes(const spmh_graph* g);

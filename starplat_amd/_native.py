@@ -138,6 +138,7 @@ HOST_API = [
     ("spmh_sources", C.c_int, [i32, i32, u64, P_i32]),
     ("spmh_edge_map", C.c_int, [i32, P_i32, P_i32, P_i32, P_i32, P_i32]),
     ("spmh_features", C.c_int, [P_spmh, f32, P_f32]),
+    ("spmh_set_threads", None, [C.c_int]),
     ("spmh_model_pr_bytes", f64, [P_spmh]),
     ("spmh_model_sssp", f64, [P_spmh, P_i32, i32, P_i64, P_i64, P_i64]),
     ("spmh_model_tc_bytes", f64, [P_spmh, P_i64, P_i64]),

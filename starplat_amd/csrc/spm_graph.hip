@@ -139,19 +139,95 @@ extern "C" {
 const char* spm_last_error(void) { return spm::g_last_error.c_str(); }
 int spm_version(void) { return 100; }
 
-// Offsets non-decreasing and neighbour ids in [0, V): the bounds every
-// kernel relies on. Min/max reductions so the scan vectorises.
-static bool csr_ok(int32_t V, int64_t E, const int32_t* off, const int32_t* adj) {
+// The CSR contract every kernel relies on: offsets non-decreasing, each row
+// ascending (graph.hpp:722-734 and :810-832 sort them; edge updates and the
+// triangle count binary-search rows) and its neighbour ids in [0, V). With a
+// row ascending, its first and last elements bound the rest, so one pass of
+// neighbour comparisons checks both. Returns 0, or 1 for offsets, 2 for an
+// id out of range, 3 for a row out of order.
+static int csr_check(int32_t V, const int32_t* off, const int32_t* adj) {
   int32_t bad = 0;
   for (int32_t v = 0; v < V; ++v) bad |= int32_t(off[v + 1] < off[v]);
-  if (bad) return false;
-  int32_t lo = 0, hi = 0;
-  for (int64_t e = 0; e < E; ++e) {
-    lo = std::min(lo, adj[e]);
-    hi = std::max(hi, adj[e]);
+  if (bad) return 1;
+  int32_t lo = 0, hi = 0, down = 0;
+  for (int32_t v = 0; v < V; ++v) {
+    const int32_t a = off[v], b = off[v + 1];
+    if (a == b) continue;
+    lo = std::min(lo, adj[a]);
+    hi = std::max(hi, adj[b - 1]);
+    for (int32_t e = a + 1; e < b; ++e) down |= int32_t(adj[e] < adj[e - 1]);
   }
-  return lo >= 0 && (E == 0 || hi < V);
+  if (down) return 3;
+  return (lo < 0 || hi >= V) ? 2 : 0;
 }
+
+}  // extern "C"
+
+namespace spm {
+
+__device__ __forceinline__ unsigned long long pair_hash(uint32_t u, uint32_t v) {
+  // splitmix64 finaliser of the packed pair (Steele, Lea, Flood 2014)
+  unsigned long long z = (static_cast<unsigned long long>(u) << 32 | v) + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// sums[dir] += hash(u, v) over every edge u -> v of one direction: the
+// forward CSR (rows are sources) or the reverse CSR (rows are targets).
+// Equal sums mean equal edge multisets (up to a 2^-64 collision chance), i.e.
+// the reverse CSR is the forward one's transpose. One wavefront per row
+// (grid-stride), one 64-bit atomic per workgroup. Rows are already checked
+// (offsets and ids), so every access stays in bounds.
+constexpr int kCheckWg = 256;
+__global__ __launch_bounds__(kCheckWg) void csr_pair_hash(int32_t V, const int32_t* __restrict__ off,
+                                                          const int32_t* __restrict__ adj, int reverse,
+                                                          unsigned long long* __restrict__ sum) {
+  __shared__ unsigned long long s_part[kCheckWg / kWave];
+  const int lane = __lane_id(), wid = threadIdx.x / kWave;
+  const int64_t nw = int64_t(gridDim.x) * (kCheckWg / kWave);
+  unsigned long long acc = 0;
+  for (int64_t r = int64_t(blockIdx.x) * (kCheckWg / kWave) + wid; r < V; r += nw)
+    for (int32_t e = off[r] + lane; e < off[r + 1]; e += kWave) {
+      const uint32_t x = uint32_t(r), y = uint32_t(adj[e]);
+      acc += reverse ? pair_hash(y, x) : pair_hash(x, y);
+    }
+  acc = wave_sum(acc);
+  if (lane == 0) s_part[wid] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kCheckWg / kWave; ++w) t += s_part[w];
+    atomicAdd(sum, t);
+  }
+}
+
+// The reverse CSR given at creation is the transpose of the forward one.
+static int check_transpose(spm_graph* g) {
+  unsigned long long* d = nullptr;
+  SPM_TRY(dmalloc(g, (void**)&d, 2 * sizeof(unsigned long long), "transpose check"));
+  int rc = SPM_OK;
+  unsigned long long h[2] = {0, 0};
+  hipError_t e = hipMemsetAsync(d, 0, sizeof h, g->stream);
+  const int grid = int(std::min<int64_t>((int64_t(g->V) + 3) / 4, int64_t(g->num_cus) * 8));
+  if (e == hipSuccess) rc = launch(g, "csr_pair_hash", csr_pair_hash, dim3(grid), dim3(kCheckWg), 0, g->V,
+                                   (const int32_t*)g->off, (const int32_t*)g->adj, 0, d);
+  if (e == hipSuccess && rc == SPM_OK)
+    rc = launch(g, "csr_pair_hash", csr_pair_hash, dim3(grid), dim3(kCheckWg), 0, g->V, (const int32_t*)g->roff,
+                (const int32_t*)g->radj, 1, d + 1);
+  if (e == hipSuccess && rc == SPM_OK) e = hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, g->stream);
+  if (e == hipSuccess && rc == SPM_OK) e = hipStreamSynchronize(g->stream);
+  dfree(g, d, 2 * sizeof(unsigned long long));
+  if (e != hipSuccess) return set_error(SPM_EHIP, std::string("transpose check: ") + hipGetErrorString(e));
+  SPM_TRY(rc);
+  if (h[0] != h[1])
+    return set_error(SPM_EINVAL, "spm_graph_create: rev_indexofNodes/srcList is not the transpose of indexofNodes/edgeList");
+  return SPM_OK;
+}
+
+}  // namespace spm
+
+extern "C" {
 
 int spm_graph_create(int32_t V, int64_t E, const int32_t* off, const int32_t* adj,
                      const int32_t* w, const int32_t* roff, const int32_t* radj,
@@ -166,8 +242,12 @@ int spm_graph_create(int32_t V, int64_t E, const int32_t* off, const int32_t* ad
   // Every kernel indexes rows and vertex arrays by these values without
   // bounds checks, so a malformed CSR is refused here rather than faulting
   // the device later.
-  if (!csr_ok(V, E, off, adj)) return set_error(SPM_EINVAL, "spm_graph_create: indexofNodes must be non-decreasing and edgeList ids in [0, V)");
-  if (roff && !csr_ok(V, E, roff, radj)) return set_error(SPM_EINVAL, "spm_graph_create: rev_indexofNodes must be non-decreasing and srcList ids in [0, V)");
+  static const char* const what[] = {"", "offsets must be non-decreasing", "ids must lie in [0, V)",
+                                      "rows must be ascending"};
+  if (int c = csr_check(V, off, adj))
+    return set_error(SPM_EINVAL, std::string("spm_graph_create: indexofNodes/edgeList: ") + what[c]);
+  if (int c = roff ? csr_check(V, roff, radj) : 0)
+    return set_error(SPM_EINVAL, std::string("spm_graph_create: rev_indexofNodes/srcList: ") + what[c]);
   int dev = 0;
   SPM_CHECK_HIP(hipGetDevice(&dev));
   spm_graph* g = new spm_graph();
@@ -197,6 +277,7 @@ int spm_graph_create(int32_t V, int64_t E, const int32_t* off, const int32_t* ad
     if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
     if (e != hipSuccess) rc = set_error(SPM_EHIP, std::string("spm_graph_create upload: ") + hipGetErrorString(e));
   }
+  if (rc == SPM_OK && roff) rc = check_transpose(g);
   if (rc != SPM_OK) {
     spm_graph_destroy(g);
     return rc;

@@ -573,6 +573,10 @@ int spmh_edge_map(int32_t V, const int32_t* off, const int32_t* adj, const int32
   return bad ? -1 : 0;
 }
 
+void spmh_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+}
+
 int spmh_features(const spmh_graph* g, float job, float* out) {
   const int32_t V = g->V;
   const double E = double(g->E);

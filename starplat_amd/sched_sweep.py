@@ -1,13 +1,15 @@
-"""Command line for the mixed-batch schedule study at full size.
+"""Command line for the mixed-batch schedule study.
 
+    python -m starplat_amd.sched_sweep study --graphs 2000 --held-out 400
     python -m starplat_amd.sched_sweep sweep --start 0 --stop 500 --rows rows.jsonl
-    python -m starplat_amd.sched_sweep evaluate --rows rows_a.jsonl rows_b.jsonl --held-out 400
 
+`study` runs the whole study (mixed_batch.run): training sweep, selector
+training on the GPU, held-out sweep with output checks and the fresh timed
+pass of the fixed, learned and best-autotuned picks; it prints one JSON line.
 `sweep` times every schedule class on jobs start..stop-1 of the seeded set
 (RMAT scales 12-20 with edge factor 4-32, Erdos-Renyi 2^12-2^20 vertices with
 average degree 4-16, grids 64^2 to 1024^2) and appends one JSON line per job;
-slices can run in separate processes. `evaluate` trains the selector on the
-GPU on all but the last `held-out` jobs and prints the makespan comparison.
+slices can run in separate processes.
 """
 from __future__ import annotations
 
@@ -27,21 +29,18 @@ def main(argv=None):
     s.add_argument("--rows", required=True)
     s.add_argument("--seed", type=int, default=5)
     s.add_argument("--reps", type=int, default=2)
-    e = sub.add_parser("evaluate")
-    e.add_argument("--rows", nargs="+", required=True)
+    e = sub.add_parser("study")
+    e.add_argument("--graphs", type=int, default=2000)
     e.add_argument("--held-out", type=int, default=400)
     e.add_argument("--epochs", type=int, default=400)
     e.add_argument("--seed", type=int, default=5)
-    e.add_argument("--batched", action="store_true",
-                   help="also time the held-out jobs in batches (rows must hold jobs 0..n-1)")
+    e.add_argument("--batched", action="store_true", help="also time the held-out jobs in batches")
     a = ap.parse_args(argv)
     log = lambda *m: print("[sched]", *m, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "sweep":
         mixed_batch.sweep(a.seed, a.start, a.stop, a.reps, log, rows_path=a.rows)
         return 0
-    feats, times, _ = mixed_batch.load_rows(a.rows)
-    res = mixed_batch.run(held_out=a.held_out, seed=a.seed, epochs=a.epochs, rows=(feats, times), batched=a.batched,
-                          log=log)
+    res = mixed_batch.run(n=a.graphs, held_out=a.held_out, seed=a.seed, epochs=a.epochs, batched=a.batched, log=log)
     print(json.dumps(res))
     return 0
 

@@ -75,15 +75,16 @@ class Job:
     def __init__(self, g, kind, src=0, name=""):
         self.g, self.kind, self.src, self.name = g, kind, int(src), name
         self._out = None
+        self.iterations = None  # PageRank iterations / SSSP rounds of the last run
 
     def run(self):
         dev = self.g.device()
         if self._out is None:
             self._out = hip.DeviceArray(self.g.V, np.int32 if self.kind == JOB_SSSP else np.float32)
         if self.kind == JOB_SSSP:
-            dev.sssp_device(self.src, self._out)
+            self.iterations = dev.sssp_device(self.src, self._out)
         else:
-            dev.pr_device(1e-10, 0.85, 100, self._out)
+            self.iterations = dev.pr_device(1e-10, 0.85, 100, self._out)
 
     def result(self):
         return self._out.to_host()

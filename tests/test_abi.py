@@ -44,9 +44,10 @@ def test_version_and_error_string_without_gpu():
 
 
 def test_graph_create_rejects_malformed_csr_without_gpu():
-    """A CSR whose offsets decrease or whose neighbour ids leave [0, V) is
-    refused with SPM_EINVAL before any device call (kernels index by these
-    values unchecked)."""
+    """A CSR whose offsets decrease, whose neighbour ids leave [0, V) or whose
+    rows are not ascending is refused with SPM_EINVAL before any device call
+    (kernels index by these values unchecked, and edge updates and the
+    triangle count binary-search the rows)."""
     import numpy as np
 
     lib = N.device_lib()
@@ -69,11 +70,18 @@ def test_graph_create_rejects_malformed_csr_without_gpu():
         ([0, 2, 1, 3], [1, 2, 0]),            # offsets decrease
         ([0, 1, 2, 3], [1, 3, 0]),            # neighbour id == V
         ([0, 1, 2, 3], [1, -1, 0]),           # negative neighbour id
+        ([0, 2, 3, 3], [2, 1, 0]),            # row 0 descending
+        ([0, 3, 3, 3], [0, 2, 1]),            # row 0 out of order in its middle
+        ([0, 2, 2, 2], [3, 1]),               # out of range and out of order
     ]
     for off, adj in cases:
         rc, h = create(off, adj)
         assert rc == -1 and not h.value, (off, adj)
         assert b"edgeList" in lib.spm_last_error()
+    assert b"ascending" in (create([0, 2, 3, 3], [2, 1, 0])[0], lib.spm_last_error())[1]
+    # a reverse CSR with an unsorted row
+    rc, h = create([0, 1, 2, 3], [1, 2, 0], [0, 0, 3, 3], [2, 0, 1])
+    assert rc == -1 and not h.value and b"srcList" in lib.spm_last_error()
     # a valid forward CSR with a bad reverse CSR
     rc, h = create([0, 1, 2, 3], [1, 2, 0], [0, 1, 2, 3], [2, 0, 7])
     assert rc == -1 and not h.value and b"srcList" in lib.spm_last_error()

@@ -89,7 +89,7 @@ __global__ void pr_init(int32_t V, const int32_t* __restrict__ off, float* __res
     degf[v] = d;
     pr[v] = init;
     c[v] = d > 0.f ? init / d : 0.f;
-    acc[v] = 0.f;
+    if (acc) acc[v] = 0.f;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->done = 0;
@@ -515,6 +515,147 @@ __global__ __launch_bounds__(kPrBlock) void pr_push_apply(int32_t V, const float
   pr_finish(mydiff, partial, s_red);
 }
 
+// ---- in-degree bins on the device -------------------------------------------
+// Vertices are grouped by in-degree into the four bins of the lane mapping,
+// in id order within each bin (the order the kernels' per-workgroup diff sums
+// follow, so it is fixed). Three launches: per-chunk bin counts, one scan,
+// and a stable placement (ballot ranks inside each wavefront).
+struct PrThresh {
+  int32_t t1, t2, t3;
+};
+constexpr int kBinBlock = 256;
+constexpr int kBinChunk = 4 * kBinBlock;  // vertices per binning workgroup
+
+__device__ __forceinline__ int pr_bin_of(int32_t d, const PrThresh& t) {
+  return d <= t.t1 ? 0 : d <= t.t2 ? 1 : d <= t.t3 ? 2 : 3;
+}
+
+// cnt[b * nblk + blk] = vertices of bin b among chunk blk's vertices.
+__global__ __launch_bounds__(kBinBlock) void pr_bin_count(int32_t V, const int32_t* __restrict__ roff, PrThresh t,
+                                                          int32_t nblk, int32_t* __restrict__ cnt) {
+  __shared__ int32_t s_cnt[4];
+  if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+  __syncthreads();
+  int32_t mine[4] = {0, 0, 0, 0};
+  const int64_t base = int64_t(blockIdx.x) * kBinChunk;
+  for (int k = 0; k < kBinChunk / kBinBlock; ++k) {
+    const int64_t v = base + k * kBinBlock + threadIdx.x;
+    if (v < V) ++mine[pr_bin_of(roff[v + 1] - roff[v], t)];
+  }
+  for (int b = 0; b < 4; ++b) {
+    const int32_t w = wave_sum(mine[b]);
+    if (__lane_id() == 0 && w) atomicAdd(&s_cnt[b], w);
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) cnt[int64_t(threadIdx.x) * nblk + blockIdx.x] = s_cnt[threadIdx.x];
+}
+
+// One workgroup: cnt (bin-major) becomes the exclusive prefix over all
+// (bin, chunk) pairs, i.e. each chunk's first position within the bin order;
+// nb[b] receives the size of bin b.
+__global__ __launch_bounds__(1024) void pr_bin_scan(int32_t nblk, int32_t* __restrict__ cnt, int32_t* __restrict__ nb) {
+  __shared__ int32_t s_wave[1024 / kWave];
+  __shared__ int32_t s_carry;
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  if (tid == 0) s_carry = 0;
+  __syncthreads();
+  const int64_t n = int64_t(4) * nblk;
+  for (int64_t base = 0; base < n; base += 1024) {
+    const int64_t i = base + tid;
+    const int32_t x = i < n ? cnt[i] : 0;
+    const int32_t incl = wave_incl_scan(x);
+    if (lane == kWave - 1) s_wave[wid] = incl;
+    __syncthreads();
+    int32_t before = s_carry;
+    for (int w = 0; w < wid; ++w) before += s_wave[w];
+    if (i < n) cnt[i] = before + incl - x;
+    __syncthreads();
+    if (tid == 0) {
+      int32_t t = 0;
+      for (int w = 0; w < 1024 / kWave; ++w) t += s_wave[w];
+      s_carry += t;
+    }
+    __syncthreads();
+  }
+  // bin sizes: start of the next bin minus start of this one
+  if (tid < 4) {
+    const int32_t a = cnt[int64_t(tid) * nblk];
+    const int32_t b = tid < 3 ? cnt[int64_t(tid + 1) * nblk] : s_carry;
+    nb[tid] = b - a;
+  }
+}
+
+// rows[pos] = {v, first in-edge, end, 0} with pos = the chunk's start in v's
+// bin plus v's rank among the chunk's earlier vertices of the same bin.
+__global__ __launch_bounds__(kBinBlock) void pr_bin_place(int32_t V, const int32_t* __restrict__ roff, PrThresh t,
+                                                          int32_t nblk, const int32_t* __restrict__ start,
+                                                          int4* __restrict__ rows) {
+  __shared__ int32_t s_next[4];
+  __shared__ int32_t s_wave[4][kBinBlock / kWave];
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  if (tid < 4) s_next[tid] = start[int64_t(tid) * nblk + blockIdx.x];
+  __syncthreads();
+  const int64_t base = int64_t(blockIdx.x) * kBinChunk;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  for (int k = 0; k < kBinChunk / kBinBlock; ++k) {
+    const int64_t v = base + k * kBinBlock + tid;
+    int b = -1;
+    int32_t a = 0, z = 0;
+    if (v < V) {
+      a = roff[v];
+      z = roff[v + 1];
+      b = pr_bin_of(z - a, t);
+    }
+    int32_t rank = 0;
+    for (int bb = 0; bb < 4; ++bb) {
+      const uint64_t m = __ballot(b == bb);
+      if (b == bb) rank = __popcll(m & below);
+      if (lane == 0) s_wave[bb][wid] = __popcll(m);
+    }
+    __syncthreads();
+    if (b >= 0) {
+      int32_t pos = s_next[b] + rank;
+      for (int w = 0; w < wid; ++w) pos += s_wave[b][w];
+      rows[pos] = make_int4(int32_t(v), a, z, 0);
+    }
+    __syncthreads();
+    if (tid < 4) {
+      int32_t tot = 0;
+      for (int w = 0; w < kBinBlock / kWave; ++w) tot += s_wave[tid][w];
+      s_next[tid] += tot;
+    }
+    __syncthreads();
+  }
+}
+
+// Bins the in-rows of `roff` into `rows` (int4[V]) with the schedule's
+// thresholds; cnt is scratch of 4 * ceil(V / kBinChunk) + 4 ints. Reads the
+// four bin sizes back (one synchronisation) and fills the launch layout.
+static int pr_bin_device(spm_graph* g, const int32_t* roff, int4* rows, int32_t* cnt, int32_t* h_nb, PrBins* out) {
+  const spm_schedule& s = g->sched;
+  const int32_t V = g->V;
+  const int32_t nblk = int32_t((int64_t(V) + kBinChunk - 1) / kBinChunk);
+  const PrThresh t{s.pr_t1, s.pr_t2, s.pr_t3};
+  int32_t* nb = cnt + int64_t(4) * nblk;
+  SPM_TRY(launch(g, "pr_bin_count", pr_bin_count, dim3(nblk), dim3(kBinBlock), 0, V, roff, t, nblk, cnt));
+  SPM_TRY(launch(g, "pr_bin_scan", pr_bin_scan, dim3(1), dim3(1024), 0, nblk, cnt, nb));
+  SPM_TRY(launch(g, "pr_bin_place", pr_bin_place, dim3(nblk), dim3(kBinBlock), 0, V, roff, t, nblk,
+                 (const int32_t*)cnt, rows));
+  SPM_CHECK_HIP(hipMemcpyAsync(h_nb, nb, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  PrBins& b = *out;
+  for (int i = 0; i < 4; ++i) b.nb[i] = h_nb[i];
+  b.group = s.pr_group;
+  b.blk[0] = 0;
+  b.blk[1] = (b.nb[0] + kPrBlock - 1) / kPrBlock;
+  b.blk[2] = b.blk[1] + (b.nb[1] + kPrBlock / s.pr_group - 1) / (kPrBlock / s.pr_group);
+  b.blk[3] = b.blk[2] + (b.nb[2] + kPrBlock / kWave - 1) / (kPrBlock / kWave);
+  b.blk[4] = b.blk[3] + b.nb[3];
+  return SPM_OK;
+}
+
+static size_t pr_bin_scratch_ints(int32_t V) { return size_t(4) * size_t((int64_t(V) + kBinChunk - 1) / kBinChunk) + 4; }
+
 // Ranks, contributions and in-degree bins over `view`. Batched jobs (multi)
 // use plain pull sweeps: no push accumulators, no source segments.
 static int pr_build(spm_graph* g, PrWork* w, const CsrView& view, bool multi) {
@@ -529,35 +670,17 @@ static int pr_build(spm_graph* g, PrWork* w, const CsrView& view, bool multi) {
   if (!multi) SPM_TRY(dmalloc(g, (void**)&w->acc, sizeof(float) * V, "pr push accumulators"));
   SPM_TRY(dmalloc(g, (void**)&w->rows, sizeof(int4) * V, "pr bins"));
   SPM_TRY(dmalloc(g, (void**)&w->st, sizeof(PrState), "pr state"));
-  // Bin vertices by in-degree on the host (once per handle and schedule).
-  std::vector<int32_t> roff(size_t(V) + 1);
-  SPM_CHECK_HIP(hipMemcpyAsync(roff.data(), view.roff, sizeof(int32_t) * (V + 1), hipMemcpyDeviceToHost, g->stream));
-  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   const spm_schedule& s = g->sched;
-  std::vector<int32_t> order(V);
-  int32_t cnt[4] = {0, 0, 0, 0};
-  auto bin_of = [&](int32_t v) {
-    const int32_t d = roff[v + 1] - roff[v];
-    return d <= s.pr_t1 ? 0 : d <= s.pr_t2 ? 1 : d <= s.pr_t3 ? 2 : 3;
-  };
-  for (int32_t v = 0; v < V; ++v) ++cnt[bin_of(v)];
-  int32_t start[4] = {0, cnt[0], cnt[0] + cnt[1], cnt[0] + cnt[1] + cnt[2]};
-  for (int32_t v = 0; v < V; ++v) order[start[bin_of(v)]++] = v;
-  std::vector<int4> rows(V);
-  for (int32_t i = 0; i < V; ++i) {
-    const int32_t v = order[i];
-    rows[i] = make_int4(v, roff[v], roff[v + 1], 0);
+  // Bin vertices by in-degree on the device (once per handle and schedule).
+  {
+    int32_t* cnt = nullptr;
+    const size_t nc = pr_bin_scratch_ints(V);
+    SPM_TRY(dmalloc(g, (void**)&cnt, sizeof(int32_t) * nc, "pr bin counts"));
+    const int rc = pr_bin_device(g, view.roff, w->rows, cnt, g->h_pinned, &w->bins);
+    dfree(g, cnt, sizeof(int32_t) * nc);
+    SPM_TRY(rc);
   }
-  SPM_CHECK_HIP(hipMemcpyAsync(w->rows, rows.data(), sizeof(int4) * V, hipMemcpyHostToDevice, g->stream));
-  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
-  PrBins& b = w->bins;
-  for (int i = 0; i < 4; ++i) b.nb[i] = cnt[i];
-  b.group = s.pr_group;
-  b.blk[0] = 0;
-  b.blk[1] = (cnt[0] + kPrBlock - 1) / kPrBlock;
-  b.blk[2] = b.blk[1] + (cnt[1] + kPrBlock / s.pr_group - 1) / (kPrBlock / s.pr_group);
-  b.blk[3] = b.blk[2] + (cnt[2] + kPrBlock / kWave - 1) / (kPrBlock / kWave);
-  b.blk[4] = b.blk[3] + cnt[3];
+  const PrBins& b = w->bins;
   w->S = 1;
   if (multi) return SPM_OK;
   w->partial_n = std::max(b.blk[4], g->num_cus * 8) + 1;
@@ -641,15 +764,16 @@ int pr_release(spm_graph* g) {
   return SPM_OK;
 }
 
-static int pr_enqueue_iters(spm_graph* g, int R) {
-  PrWork* w = g->pr;
+// R iterations of w's sweep (push when `push`, else the binned pull), each
+// followed by the loop test.
+static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
   const dim3 grid(std::max(w->bins.blk[4], 1)), block(kPrBlock);
   const int pgrid = int(std::min<int64_t>((g->V + kPrBlock - 1) / kPrBlock, int64_t(g->num_cus) * 8));
   for (int r = 0; r < R; ++r) {
     const int p = r & 1;
     const float* pin = w->pr[p];
     const float* cin = w->c[p];
-    if (g->sched.pr_mode == 1) {
+    if (push) {
       SPM_TRY(launch(g, "pr_push_scatter", pr_push_scatter, dim3(pgrid), block, 0, g->V, w->view.off, w->view.adj,
                      cin, w->acc, (const PrState*)w->st));
       SPM_TRY(launch(g, "pr_push_apply", pr_push_apply, dim3(pgrid), block, 0, g->V, (const float*)w->degf, w->acc,
@@ -703,7 +827,7 @@ int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out,
     if (w->exec) { (void)hipGraphExecDestroy(w->exec); w->exec = nullptr; }
     if (w->graph) { (void)hipGraphDestroy(w->graph); w->graph = nullptr; }
     SPM_CHECK_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-    int rc = pr_enqueue_iters(g, R);
+    int rc = pr_enqueue_iters(g, w, R, g->sched.pr_mode == 1);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(g->stream, &graph);
     if (rc != SPM_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
@@ -715,7 +839,7 @@ int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out,
   int32_t* h = g->h_pinned;
   for (int64_t rep = 0;; ++rep) {
     if (use_graph) SPM_CHECK_HIP(hipGraphLaunch(w->exec, g->stream));
-    else SPM_TRY(pr_enqueue_iters(g, R));
+    else SPM_TRY(pr_enqueue_iters(g, w, R, g->sched.pr_mode == 1));
     SPM_CHECK_HIP(hipMemcpyAsync(h, &w->st->done, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
     SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
     if (h[0]) break;

@@ -50,9 +50,12 @@ int spm_version(void);
  * of the same arrays (backend_cuda/dsl_cpp_generator.cpp:4440-4500).
  * Arrays are copied to HBM once; rev_* may be NULL (PR and BC then fail with
  * SPM_ENOGRAPH); edgeLen may be NULL (SSSP then needs spm_graph_set_weights
- * or a weight argument). Offsets must be non-decreasing and every neighbour
- * id in [0, num_nodes), else SPM_EINVAL (checked on the host before any
- * device work). */
+ * or a weight argument). Offsets must be non-decreasing, every row ascending
+ * (as graph.hpp's parseGraph leaves them, :722-734, :810-832) and every
+ * neighbour id in [0, num_nodes), else SPM_EINVAL (checked on the host before
+ * any device work); a reverse CSR that is not the transpose of the forward
+ * one is refused the same way (an edge-multiset hash of both, compared on the
+ * device after the upload). */
 int spm_graph_create(int32_t num_nodes, int64_t num_edges,
                      const int32_t* indexofNodes, const int32_t* edgeList,
                      const int32_t* edgeLen,

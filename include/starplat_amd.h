@@ -82,6 +82,22 @@ int spm_compute_pr(spm_graph* g, float beta, float delta, int32_t maxIter,
 /* Same, result left in device memory d_pageRank (float[V]). */
 int spm_pr_device(spm_graph* g, float beta, float delta, int32_t maxIter,
                   float* d_pageRank, int32_t* iterCount);
+/* Same as spm_pr_device, but allocates nothing: no hipMalloc, hipFree or
+ * allocating library call (graph instantiation included) on any call. No
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid it). The workspace is one file-scope device array of ~40 MiB that
+ * each GPU the library runs on holds until the process ends, so graphs may
+ * have at most 2^20 vertices (else SPM_EINVAL); the handle must have its
+ * reverse CSR and no pending edge updates (SPM_EINVAL: fold them with
+ * spm_graph_compact first). Runs the pull sweep on the original ids with the
+ * handle's lane-mapping thresholds, in the same summation order as
+ * spm_pr_device on graphs it does not relabel (below 2^20 vertices with the
+ * default schedule: bit-identical ranks and iteration count); push and
+ * source segmentation are not used. Every call on one GPU shares that
+ * array: calls that overlap on one GPU (from any handles or threads) race. */
+int spm_pr_device_noalloc(spm_graph* g, float beta, float delta, int32_t maxIter,
+                          float* d_pageRank, int32_t* iterCount);
 /* Many PageRank jobs in one launch sequence: g is the disjoint union of the
  * jobs' graphs, job k owning vertices [bases[k], bases[k+1]) (bases[0] = 0,
  * bases[n] = V, no empty job, no edge between jobs: checked, SPM_EINVAL).

@@ -80,6 +80,7 @@ DEVICE_API = [
     ("spm_graph_device_bytes", i64, [vp]),
     ("spm_compute_pr", C.c_int, [vp, f32, f32, i32, P_f32, P_i32]),
     ("spm_pr_device", C.c_int, [vp, f32, f32, i32, vp, P_i32]),
+    ("spm_pr_device_noalloc", C.c_int, [vp, f32, f32, i32, vp, P_i32]),
     ("spm_pr_multi_device", C.c_int, [vp, P_i32, i32, f32, f32, i32, vp, P_i32]),
     ("spm_compute_sssp", C.c_int, [vp, P_i32, P_i32, i32]),
     ("spm_sssp_device", C.c_int, [vp, i32, vp, P_i32]),

@@ -1283,6 +1283,8 @@ int dyn_settle(spm_graph* g) {
   return dyn_fold(g);
 }
 
+bool dyn_pending(const spm_graph* g) { return dyn_dirty(g); }
+
 void dyn_weights_changed(spm_graph* g) {
   if (g->dyn) g->dyn->rw_ok = false;
 }

@@ -112,6 +112,8 @@ int dyn_release(spm_graph* g);
 // Folds pending edge updates into a clean sorted CSR; every static algorithm
 // calls it first (a no-op when there are none).
 int dyn_settle(spm_graph* g);
+// True while edge updates are waiting to be folded (dyn_settle has work).
+bool dyn_pending(const spm_graph* g);
 // The handle's weights changed: weights derived from them are stale.
 void dyn_weights_changed(spm_graph* g);
 int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32_t* rounds_out);

@@ -26,6 +26,7 @@
 // sequential), which keeps ranks within a few ulps of the restatement.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "spm_internal.hpp"
@@ -994,6 +995,110 @@ int pr_multi_run(spm_graph* g, const int32_t* bases, int32_t n, float beta, floa
   return SPM_OK;
 }
 
+// ---- no-allocation entry point (spm_pr_device_noalloc) ------------------------
+// Callers that cannot allocate device memory (code under graph capture, tight
+// serving loops) run PageRank out of one file-scope device array instead of
+// the handle's workspace. It holds the ranks and contributions (two buffers
+// each), the out-degrees, the binned rows, the per-workgroup diffs, the bin
+// counts and the loop state for graphs of up to kNoAllocMaxV vertices: about
+// 40 MiB, which every GPU the library runs on holds until the process ends.
+constexpr int32_t kNoAllocMaxV = 1 << 20;
+constexpr size_t kNaAlign = 256;
+constexpr size_t na_round(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr size_t kNaFloats = na_round(sizeof(float) * size_t(kNoAllocMaxV));
+constexpr size_t kNaRows = na_round(sizeof(int4) * size_t(kNoAllocMaxV));
+// partial diffs: one per workgroup of a sweep (at most one per vertex plus the
+// three partial bins) or per push workgroup, plus one
+constexpr size_t kNaPartial = na_round(sizeof(float) * (size_t(kNoAllocMaxV) + 8192));
+constexpr size_t kNaCnt = na_round(sizeof(int32_t) * (size_t(4) * ((size_t(kNoAllocMaxV) + kBinChunk - 1) / kBinChunk) + 4));
+constexpr size_t kNaState = na_round(sizeof(PrState));
+constexpr size_t kNoAllocScratchBytes = 5 * kNaFloats + kNaRows + kNaPartial + kNaCnt + kNaState;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// The device address of `scratch` on each GPU (its own copy per device). The
+// name `scratch` in host code is only a host-side stand-in, so kernels get
+// these addresses, looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static int noalloc_scratch(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return set_error(SPM_EINVAL, "spm_pr_device_noalloc: device id out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    SPM_CHECK_HIP(hipGetDevice(&cur));
+    if (cur != dev) SPM_CHECK_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return set_error(SPM_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return SPM_OK;
+}
+
+int pr_run_noalloc(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out, int32_t* iter_out) {
+  if (!g->roff) return set_error(SPM_ENOGRAPH, "PageRank needs the reverse CSR (rev_indexofNodes, srcList)");
+  if (dyn_pending(g))
+    return set_error(SPM_EINVAL, "spm_pr_device_noalloc: pending edge updates (fold them with spm_graph_compact first)");
+  if (g->V > kNoAllocMaxV) return set_error(SPM_EINVAL, "spm_pr_device_noalloc: at most 2^20 vertices");
+  unsigned char* base = nullptr;
+  SPM_TRY(noalloc_scratch(g->device, &base));
+  const int32_t V = g->V;
+  PrWork w;
+  unsigned char* p = base;
+  for (int i = 0; i < 2; ++i) {
+    w.pr[i] = reinterpret_cast<float*>(p), p += kNaFloats;
+    w.c[i] = reinterpret_cast<float*>(p), p += kNaFloats;
+  }
+  w.degf = reinterpret_cast<float*>(p), p += kNaFloats;
+  w.rows = reinterpret_cast<int4*>(p), p += kNaRows;
+  w.partial = reinterpret_cast<float*>(p), p += kNaPartial;
+  int32_t* cnt = reinterpret_cast<int32_t*>(p);
+  p += kNaCnt;
+  w.st = reinterpret_cast<PrState*>(p);
+  w.acc = nullptr;  // pull sweeps only (push and source segments need per-handle memory)
+  w.S = 1;
+  w.view.off = g->off;
+  w.view.adj = g->adj;
+  w.view.w = g->w;
+  w.view.roff = g->roff;
+  w.view.radj = g->radj;
+  w.view.perm = nullptr;
+  w.view.h_perm = nullptr;
+  SPM_TRY(pr_bin_device(g, g->roff, w.rows, cnt, g->h_pinned, &w.bins));
+  w.partial_n = std::max(w.bins.blk[4], g->num_cus * 8) + 1;
+  if (size_t(w.partial_n) * sizeof(float) > kNaPartial) return set_error(SPM_EINVAL, "spm_pr_device_noalloc: too many workgroups");
+  const float num_nodes = float(V);
+  const float init = 1 / num_nodes;
+  const float base_term = (1 - delta) / num_nodes;
+  const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
+  SPM_TRY(launch(g, "pr_init", pr_init, dim3(igrid), dim3(256), 0, V, w.view.off, w.degf, w.pr[0], w.c[0],
+                 (float*)nullptr, w.st, init, int(maxIter), beta, delta, base_term));
+  // Batches of R iterations (even, so each starts on buffer 0), launched
+  // directly -- graph instantiation would allocate -- with the done flag read
+  // back after each batch; iterations after convergence return at once.
+  const int R = std::max(2, std::min(128, (int(std::max(maxIter, 1)) + 1) & ~1));
+  int32_t* h = g->h_pinned;
+  for (int64_t rep = 0;; ++rep) {
+    SPM_TRY(pr_enqueue_iters(g, &w, R, false));
+    SPM_CHECK_HIP(hipMemcpyAsync(h, &w.st->done, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+    if (h[0]) break;
+    if (rep > int64_t(maxIter) / R + 2) return set_error(SPM_EHIP, "pr: loop did not terminate");
+  }
+  const int32_t iters = h[1];
+  SPM_CHECK_HIP(hipMemcpyAsync(d_out, w.pr[iters & 1], sizeof(float) * size_t(V), hipMemcpyDeviceToDevice, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  if (iter_out) *iter_out = iters;
+  SPM_TRY(prof_flush(g));
+  return SPM_OK;
+}
+
 }  // namespace spm
 
 using namespace spm;
@@ -1010,6 +1115,12 @@ int spm_pr_multi_device(spm_graph* g, const int32_t* bases, int32_t n, float bet
                         float* d_pageRank, int32_t* iterCounts) {
   if (!g) return set_error(SPM_EINVAL, "spm_pr_multi_device: NULL handle");
   return pr_multi_run(g, bases, n, beta, delta, maxIter, d_pageRank, iterCounts);
+}
+
+int spm_pr_device_noalloc(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_pageRank,
+                          int32_t* iterCount) {
+  if (!g || !d_pageRank) return set_error(SPM_EINVAL, "spm_pr_device_noalloc: NULL argument");
+  return pr_run_noalloc(g, beta, delta, maxIter, d_pageRank, iterCount);
 }
 
 int spm_compute_pr(spm_graph* g, float beta, float delta, int32_t maxIter, float* pageRank,

@@ -88,6 +88,7 @@ class Prefetch:
 
         self.q = queue.Queue(maxsize=depth)
         self.stop = False
+        self.waited = 0.0  # seconds the consumer spent waiting for a job
         n = threads or max(1, host_threads() - 2)
 
         def work():
@@ -108,7 +109,9 @@ class Prefetch:
 
     def __iter__(self):
         while True:
+            t0 = time.perf_counter()
             item = self.q.get()
+            self.waited += time.perf_counter() - t0
             if item is None:
                 return
             if isinstance(item, BaseException):
@@ -125,27 +128,56 @@ class Prefetch:
         self.t.join()
 
 
-def autotune_cutoff(job, reps=2, cutoff=8.0, outputs=None):
+def autotune_cutoff(job, reps=2, cutoff=8.0, outputs=None, stats=None, probe_iters=0):
     """Time every class: its first run (which also builds the class's plan:
     bins, captured graphs) and then the best of `reps` steady runs. A class
     whose first run is already `cutoff` times slower than the best so far
     keeps that first time and is not repeated -- it cannot be the fastest.
-    With `outputs` (a dict), outputs[k] = (result, iterations) of class k."""
+    With `probe_iters` > 0, a PageRank class after the first is probed with a
+    run of that many iterations (plan build included); if the probe's time
+    per iteration times class 0's iteration count already exceeds `cutoff`
+    times the best, that estimate is kept and the full run skipped (PageRank's
+    cost grows with its iterations, and every class runs the same iterations
+    up to round-off).
+    With `outputs` (a dict), outputs[k] = (result, iterations) of class k.
+    With `stats` (a dict), wall seconds add up under "upload" (the handle's
+    creation), "first_runs", "steady_runs" and "probes"."""
     from . import hip
-    from .schedule import apply_class
+    from .schedule import JOB_PR, apply_class
     out = np.zeros(N_CLASSES)
     best = float("inf")
+    st = stats if stats is not None else {}
+    t0 = time.perf_counter()
+    job.g.device()
+    hip.synchronize()
+    st["upload"] = st.get("upload", 0.0) + time.perf_counter() - t0
+    iters0 = None
     for k in range(N_CLASSES):
         apply_class(job.g.device(), k)
         hip.synchronize()
+        if probe_iters > 0 and job.kind == JOB_PR and iters0 and outputs is None:
+            t0 = time.perf_counter()
+            job.run(max_iter=probe_iters)
+            hip.synchronize()
+            probe = time.perf_counter() - t0
+            st["probes"] = st.get("probes", 0.0) + probe
+            est = probe * 1e3 / max(job.iterations, 1) * iters0
+            if est > cutoff * best:
+                out[k] = est
+                continue
         t0 = time.perf_counter()
         job.run()
         hip.synchronize()
         first = (time.perf_counter() - t0) * 1e3
+        st["first_runs"] = st.get("first_runs", 0.0) + first * 1e-3
+        if k == 0:
+            iters0 = job.iterations
         if first > cutoff * best or reps < 1:
             out[k] = first
         else:
+            t1 = time.perf_counter()
             t = min(first, time_job_steady(job, reps))
+            st["steady_runs"] = st.get("steady_runs", 0.0) + time.perf_counter() - t1
             out[k] = t
             best = min(best, t)
         if outputs is not None:
@@ -269,7 +301,8 @@ def run(n=2000, held_out=400, seed=5, train_reps=1, eval_reps=2, epochs=400, log
 
     1. Training sweep: jobs 0..n-held_out-1 are built (on a background
        thread), featurised and timed under every schedule class
-       (autotune_cutoff with `train_reps` steady runs per class).
+       (autotune_cutoff with `train_reps` steady runs per class, PageRank
+       classes probed with 4 iterations first).
     2. The selector is trained on the GPU on (features, tie-aware labels).
     3. Held-out jobs: each is built, the learned class is picked from its
        features, every class is timed (its output kept and checked against
@@ -291,17 +324,19 @@ def run(n=2000, held_out=400, seed=5, train_reps=1, eval_reps=2, epochs=400, log
     feats = np.zeros((n, 16), np.float32)
     times = np.zeros((n, N_CLASSES))
     t0 = time.perf_counter()
+    sweep_stats = {}
     pf = Prefetch(seed, range(ntr), **ranges)
     try:
         for c, (i, job, f) in enumerate(pf):
             feats[i] = f
-            times[i] = autotune_cutoff(job, train_reps)
+            times[i] = autotune_cutoff(job, train_reps, stats=sweep_stats, probe_iters=4)
             job.release()
             if log and (c % 200 == 0 or c == ntr - 1):
                 log(f"trained-set sweep {c + 1}/{ntr}: {job.name} best class {int(np.argmin(times[i]))} "
                     f"({times[i].min():.2f} ms; fixed {times[i][FIXED_CLASS]:.2f} ms)")
     finally:
         pf.close()
+    sweep_stats["waiting_for_graphs"] = pf.waited
     t_sweep = time.perf_counter() - t0
     sel = Selector()
     t1 = time.perf_counter()
@@ -374,9 +409,12 @@ def run(n=2000, held_out=400, seed=5, train_reps=1, eval_reps=2, epochs=400, log
         "held_out_class_histogram": np.bincount(picks["learned"], minlength=N_CLASSES).tolist(),
         "best_class_histogram": np.bincount(picks["best_autotuned"], minlength=N_CLASSES).tolist(),
         "seconds": {"training_sweep": round(t_sweep, 1), "train_on_gpu": round(t_train, 2),
-                    "held_out_sweep_and_fresh_pass": round(t_eval, 1)},
+                    "held_out_sweep_and_fresh_pass": round(t_eval, 1),
+                    "training_sweep_parts": {k: round(v, 1) for k, v in sweep_stats.items()}},
         "timing": (f"training jobs: every class timed as its first run (plan build) and best of {train_reps} "
-                   f"steady runs; held-out jobs: every class timed likewise with best of {eval_reps}, then the "
+                   "steady runs, a PageRank class after class 0 probed with 4 iterations first and, when that "
+                   "extrapolates past 8x the best class so far, kept at the estimate; held-out jobs: every class "
+                   f"run in full and timed likewise with best of {eval_reps}, then the "
                    f"policies' distinct picks re-timed in a fresh pass (one untimed run, best of {eval_reps}), "
                    "class order rotated per job; makespan = sum of the fresh-pass times"),
     }

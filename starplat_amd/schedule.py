@@ -77,14 +77,15 @@ class Job:
         self._out = None
         self.iterations = None  # PageRank iterations / SSSP rounds of the last run
 
-    def run(self):
+    def run(self, max_iter=100):
+        """One run; `max_iter` caps PageRank's iterations (a short probe)."""
         dev = self.g.device()
         if self._out is None:
             self._out = hip.DeviceArray(self.g.V, np.int32 if self.kind == JOB_SSSP else np.float32)
         if self.kind == JOB_SSSP:
             self.iterations = dev.sssp_device(self.src, self._out)
         else:
-            self.iterations = dev.pr_device(1e-10, 0.85, 100, self._out)
+            self.iterations = dev.pr_device(1e-10, 0.85, max_iter, self._out)
 
     def result(self):
         return self._out.to_host()

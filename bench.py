@@ -146,6 +146,32 @@ class Stash:
         return meta, arrays
 
 
+def warm_library():
+    """Runs every static program once on a tiny graph, so the one-time load
+    of the library's code objects (on a process's first launch from each
+    source file) is not charged to the first call on the benchmark graphs:
+    first_run_ms is the first call on a new graph handle."""
+    g = graph.rmat(10, 8, seed=9, weights=(1, 100))
+    u = graph.rmat(10, 8, seed=9, undirected=True)
+    dev, dev_u = g.device(), u.device()
+    a = hip.DeviceArray(g.V, np.int32)
+    b = hip.DeviceArray(g.V, np.float32)
+    c = hip.DeviceArray(u.V, np.float32)
+    dev.sssp_device(0, a)
+    dev.set_schedule(relabel=1)
+    dev.sssp_device(0, a)
+    dev.pr_device(1e-10, 0.85, 10, b)
+    dev.set_schedule(pr_mode=0)
+    dev.pr_device(1e-10, 0.85, 10, b)
+    dev_u.tc()
+    dev_u.bc_device(sources(u.V, 4, seed=1), c)
+    hip.synchronize()
+    for x in (a, b, c):
+        x.free()
+    g.release_device()
+    u.release_device()
+
+
 # --------------------------------------------------------------------------- SSSP
 def bench_sssp(args, stash):
     log("generating RMAT-22 (directed, weights U[1,100], seed 2)")
@@ -704,6 +730,7 @@ def main():
         log(f"calibration copy of {args.copy_calibration_mib} MiB done")
     import tempfile
 
+    warm_library()
     algos = args.algos.split(",")
     per = {}
     head = None

@@ -245,7 +245,10 @@ typedef struct spm_schedule {
   int32_t sssp_rounds;    /* rounds per captured graph (multiple of 6)       */
   int32_t sssp_grid;      /* workgroups per relaxation launch (0 = auto)     */
   int32_t tc_mode;        /* 1 per-vertex LDS hash (default), 0 edge search */
-  int32_t pr_mode;        /* 0 pull over in-rows, 1 push (scatter + apply)   */
+  int32_t pr_mode;        /* 0 pull over in-rows, 1 push (scatter + apply),
+                             2 pull whose sweeps evaluate the loop test of
+                             the sweep before (one launch per iteration;
+                             sweeps of <= 2048 workgroups, else as 0)     */
   int32_t pr_seg_log2;    /* pull: sweep sources in segments of 2^k vertices
                              so gathers stay in L2 (0 = one sweep; 10..30)   */
   int32_t relabel;        /* PR and SSSP on a copy of the CSR with vertices

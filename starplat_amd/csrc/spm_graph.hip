@@ -50,11 +50,9 @@ static hipEvent_t pool_get(spm_graph* g) {
 void prof_begin(spm_graph* g, const char*, hipEvent_t* a, hipEvent_t* b) {
   *a = pool_get(g);
   *b = pool_get(g);
-  (void)hipEventRecord(*a, g->stream);
 }
 
 void prof_end(spm_graph* g, const char* name, hipEvent_t a, hipEvent_t b) {
-  (void)hipEventRecord(b, g->stream);
   g->prof.pending.push_back({name, a, b});
 }
 
@@ -106,7 +104,10 @@ void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
   s->sssp_rounds = 12;
   s->sssp_grid = 0;
   s->tc_mode = 1;
-  s->pr_mode = 0;
+  // Small graphs fold PageRank's loop test into the next sweep: below 2^17
+  // vertices a sweep takes a few microseconds, about what the separate
+  // loop-test launch costs.
+  s->pr_mode = g->V < (1 << 17) ? 2 : 0;
   // One sweep: measured at RMAT-22, 2 MB source segments double the
   // iteration time (each pass repeats the per-vertex load chain).
   s->pr_seg_log2 = 0;
@@ -125,7 +126,10 @@ int validate_schedule(const spm_schedule* s) {
   if (s->sssp_rounds < 6 || s->sssp_rounds > 96 || s->sssp_rounds % 6) return set_error(SPM_EINVAL, "schedule: sssp_rounds must be a multiple of 6 in [6, 96]");
   if (s->sssp_grid < 0 || s->sssp_grid > 65536) return set_error(SPM_EINVAL, "schedule: sssp_grid out of range");
   if (s->tc_mode < 0 || s->tc_mode > 1) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0 (search) or 1 (hash)");
-  if (s->pr_mode < 0 || s->pr_mode > 1) return set_error(SPM_EINVAL, "schedule: pr_mode must be 0 (pull) or 1 (push)");
+  if (s->pr_mode < 0 || s->pr_mode > 2)
+    return set_error(SPM_EINVAL, "schedule: pr_mode must be 0 (pull), 1 (push) or 2 (pull, loop test in the next sweep)");
+  if (s->pr_mode == 2 && s->pr_seg_log2 != 0)
+    return set_error(SPM_EINVAL, "schedule: pr_mode 2 sweeps all sources at once (pr_seg_log2 must be 0)");
   if (s->pr_seg_log2 != 0 && (s->pr_seg_log2 < 10 || s->pr_seg_log2 > 30))
     return set_error(SPM_EINVAL, "schedule: pr_seg_log2 must be 0 or in [10, 30]");
   if (s->relabel < 0 || s->relabel > 1) return set_error(SPM_EINVAL, "schedule: relabel must be 0 or 1");
@@ -176,9 +180,10 @@ __device__ __forceinline__ unsigned long long pair_hash(uint32_t u, uint32_t v) 
 // sums[dir] += hash(u, v) over every edge u -> v of one direction: the
 // forward CSR (rows are sources) or the reverse CSR (rows are targets).
 // Equal sums mean equal edge multisets (up to a 2^-64 collision chance), i.e.
-// the reverse CSR is the forward one's transpose. One wavefront per row
-// (grid-stride), one 64-bit atomic per workgroup. Rows are already checked
-// (offsets and ids), so every access stays in bounds.
+// the reverse CSR is the forward one's transpose. One wavefront per row, the
+// rows visited in the spread_vertex order (RMAT's heavy rows would otherwise
+// pile onto a few wavefronts), one 64-bit atomic per workgroup. Rows are
+// already checked (offsets and ids), so every access stays in bounds.
 constexpr int kCheckWg = 256;
 __global__ __launch_bounds__(kCheckWg) void csr_pair_hash(int32_t V, const int32_t* __restrict__ off,
                                                           const int32_t* __restrict__ adj, int reverse,
@@ -187,11 +192,13 @@ __global__ __launch_bounds__(kCheckWg) void csr_pair_hash(int32_t V, const int32
   const int lane = __lane_id(), wid = threadIdx.x / kWave;
   const int64_t nw = int64_t(gridDim.x) * (kCheckWg / kWave);
   unsigned long long acc = 0;
-  for (int64_t r = int64_t(blockIdx.x) * (kCheckWg / kWave) + wid; r < V; r += nw)
+  for (int64_t x = int64_t(blockIdx.x) * (kCheckWg / kWave) + wid; x < V; x += nw) {
+    const int64_t r = spread_vertex(x, V);
     for (int32_t e = off[r] + lane; e < off[r + 1]; e += kWave) {
-      const uint32_t x = uint32_t(r), y = uint32_t(adj[e]);
-      acc += reverse ? pair_hash(y, x) : pair_hash(x, y);
+      const uint32_t a = uint32_t(r), b = uint32_t(adj[e]);
+      acc += reverse ? pair_hash(b, a) : pair_hash(a, b);
     }
+  }
   acc = wave_sum(acc);
   if (lane == 0) s_part[wid] = acc;
   __syncthreads();

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <map>
@@ -85,17 +86,25 @@ namespace spm {
 int dmalloc(spm_graph* g, void** p, size_t bytes, const char* what);
 void dfree(spm_graph* g, void* p, size_t bytes);
 
-// Profiling hooks around one launch on g->stream.
+// Profiling: events for one launch on g->stream, and their collection.
 void prof_begin(spm_graph* g, const char* name, hipEvent_t* a, hipEvent_t* b);
 void prof_end(spm_graph* g, const char* name, hipEvent_t a, hipEvent_t b);
 int prof_flush(spm_graph* g);
 
+// With profiling on, the launch carries a start and a stop event that the
+// runtime stamps with the dispatch's own begin and end times (the timestamps
+// a kernel trace reads), so a microsecond kernel is timed without event
+// packets of its own around it.
 template <typename Kernel, typename... Args>
 inline int launch(spm_graph* g, const char* name, Kernel k, dim3 grid, dim3 block, size_t shmem,
                   Args... args) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (g->prof.on) prof_begin(g, name, &a, &b);
-  hipLaunchKernelGGL(k, grid, block, shmem, g->stream, args...);
+  if (g->prof.on) {
+    prof_begin(g, name, &a, &b);
+    hipExtLaunchKernelGGL(k, grid, block, uint32_t(shmem), g->stream, a, b, 0, args...);
+  } else {
+    hipLaunchKernelGGL(k, grid, block, shmem, g->stream, args...);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(SPM_EHIP, std::string("launch ") + name + ": " + hipGetErrorString(e));
   if (g->prof.on) prof_end(g, name, a, b);
@@ -128,7 +137,6 @@ struct CsrView {
   const int32_t* roff;
   const int32_t* radj;
   const int32_t* perm;    // device [V]
-  const int32_t* h_perm;  // host [V]
 };
 int csr_view(spm_graph* g, CsrView* v);
 // Drops the relabeled copy (and the PR / SSSP workspaces that refer to it).

@@ -26,7 +26,6 @@ constexpr int kLayWaves = kLayBlock / kWave;
 
 struct Relabel {
   int32_t* perm = nullptr;  // [V] original id -> new id
-  std::vector<int32_t> h_perm;
   int32_t* eidx = nullptr;  // [E] original CSR position of each new position
   int32_t *off = nullptr, *adj = nullptr, *w = nullptr, *roff = nullptr, *radj = nullptr;
   int64_t E = 0;
@@ -273,13 +272,6 @@ static int relabel_build(spm_graph* g) {
   dfree(g, k1, sizeof(unsigned long long) * n);
   dfree(g, v0, sizeof(int32_t) * n);
   dfree(g, v1, sizeof(int32_t) * n);
-  if (rc == SPM_OK) {
-    r->h_perm.resize(size_t(V));
-    hipError_t e = hipMemcpyAsync(r->h_perm.data(), r->perm, sizeof(int32_t) * size_t(V), hipMemcpyDeviceToHost,
-                                  g->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
-    if (e != hipSuccess) rc = set_error(SPM_EHIP, std::string("relabel map copy: ") + hipGetErrorString(e));
-  }
   if (rc != SPM_OK) {
     relabel_release(g);
     return rc;
@@ -302,7 +294,6 @@ int csr_view(spm_graph* g, CsrView* v) {
   v->roff = r ? r->roff : g->roff;
   v->radj = r ? r->radj : g->radj;
   v->perm = r ? r->perm : nullptr;
-  v->h_perm = r ? r->h_perm.data() : nullptr;
   return SPM_OK;
 }
 

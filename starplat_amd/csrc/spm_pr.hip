@@ -102,14 +102,12 @@ __global__ void pr_init(int32_t V, const int32_t* __restrict__ off, float* __res
   }
 }
 
-__device__ __forceinline__ void pr_store(int32_t v, float sum, const PrState& s,
-                                         const float* __restrict__ pr_in, float* __restrict__ pr_out,
-                                         float* __restrict__ c_out, const float* __restrict__ degf,
-                                         float& mydiff) {
+// v's new rank from its in-row sum; old = pr_in[v] and d = degf[v] are
+// loaded by the caller beside the gather rather than after it.
+__device__ __forceinline__ void pr_store(int32_t v, float sum, const PrState& s, float old, float d,
+                                         float* __restrict__ pr_out, float* __restrict__ c_out, float& mydiff) {
   const float nv = __fadd_rn(s.base, __fmul_rn(s.delta, sum));
-  const float old = pr_in[v];
   pr_out[v] = nv;
-  const float d = degf[v];
   c_out[v] = d > 0.f ? nv / d : 0.f;
   mydiff += nv - old >= 0.f ? nv - old : old - nv;
 }
@@ -131,11 +129,13 @@ __device__ __forceinline__ void pr_finish(float mydiff, float* __restrict__ part
 }
 
 // The loop test (diff > beta && iter < maxIter), one workgroup: sums the
-// per-workgroup partials in a fixed order, advances the iteration count and
-// raises the done flag that makes later iterations return at once.
+// per-workgroup partials in a fixed order, advances the iteration count by
+// `step` (1 after each sweep; the batch length where the sweeps test their
+// predecessors themselves) and raises the done flag that makes later
+// iterations return at once.
 constexpr int kCheckBlock = 1024;
 __global__ __launch_bounds__(kCheckBlock) void pr_check(const float* __restrict__ partial, int n,
-                                                        PrState* __restrict__ st) {
+                                                        PrState* __restrict__ st, int step) {
   __shared__ float s_red[kCheckBlock / kWave];
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kCheckBlock) void pr_check(const float* __restrict_
   if (tid == 0) {
     float d = 0.f;
     for (int w = 0; w < kCheckBlock / kWave; ++w) d += s_red[w];
-    const int it = st->iter + 1;
+    const int it = st->iter + step;
     st->iter = it;
     if (!(d > st->beta && it < st->maxIter)) st->done = 1;
   }
@@ -193,9 +193,8 @@ __device__ __forceinline__ void seg_range(int32_t v, const int32_t* __restrict__
   b = sg.seg == sg.S - 1 ? roff[v + 1] : p[sg.seg];
 }
 
-__device__ __forceinline__ void seg_emit(int32_t v, float sum, const PrSeg& sg, const PrState& s,
-                                         const float* __restrict__ pr_in, float* __restrict__ pr_out,
-                                         float* __restrict__ c_out, const float* __restrict__ degf,
+__device__ __forceinline__ void seg_emit(int32_t v, float sum, const PrSeg& sg, const PrState& s, float old,
+                                         float dg, float* __restrict__ pr_out, float* __restrict__ c_out,
                                          float& mydiff) {
   if (sg.S > 1) {
     if (sg.seg < sg.S - 1) {
@@ -204,25 +203,22 @@ __device__ __forceinline__ void seg_emit(int32_t v, float sum, const PrSeg& sg, 
     }
     sum = sg.acc[v] + sum;
   }
-  pr_store(v, sum, s, pr_in, pr_out, c_out, degf, mydiff);
+  pr_store(v, sum, s, old, dg, pr_out, c_out, mydiff);
 }
 
 // A vertex's new rank: the single-job path through seg_emit; batched jobs use
 // the job's base term and leave |new - old| in dv for the per-job test.
 template <bool MULTI>
-__device__ __forceinline__ void pr_emit(int32_t v, float sum, const PrSeg& sg, const PrState& s,
-                                        const float* __restrict__ pr_in, float* __restrict__ pr_out,
-                                        float* __restrict__ c_out, const float* __restrict__ degf,
-                                        float& mydiff, const PrJobs& jb) {
+__device__ __forceinline__ void pr_emit(int32_t v, float sum, const PrSeg& sg, const PrState& s, float old, float dg,
+                                        float* __restrict__ pr_out, float* __restrict__ c_out, float& mydiff,
+                                        const PrJobs& jb) {
   if (!MULTI) {
-    seg_emit(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff);
+    seg_emit(v, sum, sg, s, old, dg, pr_out, c_out, mydiff);
     return;
   }
   const float nv = __fadd_rn(jb.base[jb.job[v]], __fmul_rn(s.delta, sum));
-  const float old = pr_in[v];
   pr_out[v] = nv;
-  const float d = degf[v];
-  c_out[v] = d > 0.f ? nv / d : 0.f;
+  c_out[v] = dg > 0.f ? nv / dg : 0.f;
   jb.dv[v] = nv - old >= 0.f ? nv - old : old - nv;
 }
 
@@ -232,34 +228,109 @@ __device__ __forceinline__ bool pr_live(int32_t v, const PrJobs& jb) {
   return !MULTI || jb.live[jb.job[v]] != 0;
 }
 
-// Sum of c[radj[e]] for e = ea+start, ea+start+stride, ... < eb, four rows
-// elements at a time: four independent index loads, then four independent
+// Sum of c[radj[e]] for e = ea+start, ea+start+stride, ... < eb, eight row
+// elements at a time: eight independent index loads, then eight independent
 // gathers, so a lane keeps several memory requests in flight instead of a
-// chain of dependent ones.
+// chain of dependent ones (a lane with up to 8 terms waits for two round
+// trips in all).
 __device__ __forceinline__ float gather_sum(const int32_t* __restrict__ radj, const float* __restrict__ c,
                                             int32_t ea, int32_t eb, int32_t start, int32_t stride) {
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int32_t e = ea + start;
-  for (; e + 3 * stride < eb; e += 4 * stride) {
-    const int32_t a0 = radj[e], a1 = radj[e + stride], a2 = radj[e + 2 * stride], a3 = radj[e + 3 * stride];
-    s0 += c[a0];
-    s1 += c[a1];
-    s2 += c[a2];
-    s3 += c[a3];
+  for (; e + 7 * stride < eb; e += 8 * stride) {
+    int32_t a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = radj[e + k * stride];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] += c[a[k]];
   }
-  for (; e < eb; e += stride) s0 += c[radj[e]];
-  return (s0 + s1) + (s2 + s3);
+  // The remaining 0..7 terms, loaded together.
+  int32_t a[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) a[k] = e + k * stride < eb ? radj[e + k * stride] : -1;
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+    if (a[k] >= 0) s[k] += c[a[k]];
+  return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
 // rows[i] = {v, first in-edge, end of in-edges, 0} for the i-th vertex in bin
-// order: one 16-byte load replaces the order -> offsets chain.
-__device__ __forceinline__ void row_of(const int4* __restrict__ rows, int64_t i, const int32_t* __restrict__ roff,
-                                       const PrSeg& sg, int32_t& v, int32_t& ea, int32_t& eb) {
-  const int4 r = rows[i];
-  v = r.x;
-  ea = r.y;
-  eb = r.z;
-  if (sg.S > 1) seg_range(v, roff, sg, ea, eb);
+// order: one 16-byte load replaces the order -> offsets chain. pr_slot gives
+// the i that thread threadIdx.x of workgroup b reads (-1: none): bin 0 one
+// lane per vertex, bin 1 G lanes, bin 2 a wavefront, bin 3 the workgroup.
+template <int G>
+__device__ __forceinline__ int64_t pr_slot(int b, const PrBins& bins) {
+  const int tid = threadIdx.x;
+  if (b < bins.blk[1]) {
+    const int64_t i = int64_t(b) * kPrBlock + tid;
+    return i < bins.nb[0] ? i : -1;
+  }
+  if (b < bins.blk[2]) {
+    const int64_t i = int64_t(b - bins.blk[1]) * (kPrBlock / G) + tid / G;
+    return i < bins.nb[1] ? bins.nb[0] + i : -1;
+  }
+  if (b < bins.blk[3]) {
+    const int64_t i = int64_t(b - bins.blk[2]) * (kPrBlock / kWave) + tid / kWave;
+    return i < bins.nb[2] ? int64_t(bins.nb[0]) + bins.nb[1] + i : -1;
+  }
+  return int64_t(bins.nb[0]) + bins.nb[1] + bins.nb[2] + (b - bins.blk[3]);
+}
+
+template <int G>
+__device__ __forceinline__ int4 pr_row(const int4* __restrict__ rows, int b, const PrBins& bins) {
+  const int64_t i = pr_slot<G>(b, bins);
+  return i >= 0 ? rows[i] : make_int4(-1, 0, 0, 0);
+}
+
+// The sweep of one (virtual) workgroup b of the binned launch layout, given
+// the thread's row (pr_row): the rows of bin 0 one lane each, of bin 1 G
+// lanes each, of bin 2 a wavefront each, of bin 3 the whole workgroup. Adds
+// the workgroup's |new - old| to the calling thread's mydiff. Control flow
+// is uniform across the workgroup.
+template <int G, bool MULTI>
+__device__ __forceinline__ void pr_sweep_block(int b, int4 row, const int32_t* __restrict__ roff,
+                                               const int32_t* __restrict__ radj, const float* __restrict__ degf,
+                                               const PrBins& bins, const float* __restrict__ pr_in,
+                                               const float* __restrict__ c_in, float* __restrict__ pr_out,
+                                               float* __restrict__ c_out, const PrState& s, const PrSeg& sg,
+                                               const PrJobs& jb, float* s_red, float& mydiff) {
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  int32_t v = row.x, ea = row.y, eb = row.z;
+  if (v >= 0 && !pr_live<MULTI>(v, jb)) v = -1;
+  if (v >= 0 && sg.S > 1) seg_range(v, roff, sg, ea, eb);
+  // The old rank and the out-degree are in flight during the gather.
+  float old = 0.f, dg = 0.f;
+  if (v >= 0) {
+    old = pr_in[v];
+    dg = degf[v];
+  }
+  if (b < bins.blk[1]) {
+    // bin 0: one lane per vertex
+    if (v >= 0) pr_emit<MULTI>(v, gather_sum(radj, c_in, ea, eb, 0, 1), sg, s, old, dg, pr_out, c_out, mydiff, jb);
+  } else if (b < bins.blk[2]) {
+    // bin 1: G lanes per vertex
+    const int g = tid % G;
+    float sum = v >= 0 ? gather_sum(radj, c_in, ea, eb, g, G) : 0.f;
+    sum = group_sum<G>(sum);
+    if (v >= 0 && g == 0) pr_emit<MULTI>(v, sum, sg, s, old, dg, pr_out, c_out, mydiff, jb);
+  } else if (b < bins.blk[3]) {
+    // bin 2: one wavefront per vertex
+    float sum = v >= 0 ? gather_sum(radj, c_in, ea, eb, lane, kWave) : 0.f;
+    sum = wave_sum(sum);
+    if (v >= 0 && lane == 0) pr_emit<MULTI>(v, sum, sg, s, old, dg, pr_out, c_out, mydiff, jb);
+  } else if (v >= 0) {
+    // bin 3: one workgroup per vertex (v is the same in every lane)
+    float sum = gather_sum(radj, c_in, ea, eb, tid, kPrBlock);
+    sum = wave_sum(sum);
+    if (lane == 0) s_red[wid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+      float t = 0.f;
+      for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
+      pr_emit<MULTI>(v, t, sg, s, old, dg, pr_out, c_out, mydiff, jb);
+    }
+    __syncthreads();
+  }
 }
 
 template <int G, bool MULTI>
@@ -270,63 +341,97 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
     float* __restrict__ c_out, const PrState* __restrict__ st, float* __restrict__ partial, PrSeg sg,
     PrJobs jb) {
   __shared__ float s_red[kPrBlock / kWave];
-  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  // The row, the run's constants and the flag are loaded together; the flag
+  // is tested before anything that depends on the row.
+  const int4 row = pr_row<G>(rows, blockIdx.x, bins);
   const PrState s = *st;
-  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
-  const int b = blockIdx.x;
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   float mydiff = 0.f;
-  int32_t v, ea, eb;
-
-  if (b < bins.blk[1]) {
-    // bin 0: one lane per vertex
-    const int64_t i = int64_t(b) * kPrBlock + tid;
-    if (i < bins.nb[0]) {
-      row_of(rows, i, roff, sg, v, ea, eb);
-      if (pr_live<MULTI>(v, jb))
-        pr_emit<MULTI>(v, gather_sum(radj, c_in, ea, eb, 0, 1), sg, s, pr_in, pr_out, c_out, degf, mydiff, jb);
-    }
-  } else if (b < bins.blk[2]) {
-    // bin 1: G lanes per vertex
-    const int64_t i = int64_t(b - bins.blk[1]) * (kPrBlock / G) + tid / G;
-    const int g = tid % G;
-    float sum = 0.f;
-    v = -1;
-    if (i < bins.nb[1]) {
-      row_of(rows, bins.nb[0] + i, roff, sg, v, ea, eb);
-      if (!pr_live<MULTI>(v, jb)) v = -1;
-      else sum = gather_sum(radj, c_in, ea, eb, g, G);
-    }
-    sum = group_sum<G>(sum);
-    if (v >= 0 && g == 0) pr_emit<MULTI>(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff, jb);
-  } else if (b < bins.blk[3]) {
-    // bin 2: one wavefront per vertex
-    const int64_t i = int64_t(b - bins.blk[2]) * (kPrBlock / kWave) + wid;
-    float sum = 0.f;
-    v = -1;
-    if (i < bins.nb[2]) {
-      row_of(rows, bins.nb[0] + bins.nb[1] + i, roff, sg, v, ea, eb);
-      if (!pr_live<MULTI>(v, jb)) v = -1;
-      else sum = gather_sum(radj, c_in, ea, eb, lane, kWave);
-    }
-    sum = wave_sum(sum);
-    if (v >= 0 && lane == 0) pr_emit<MULTI>(v, sum, sg, s, pr_in, pr_out, c_out, degf, mydiff, jb);
-  } else {
-    // bin 3: one workgroup per vertex
-    row_of(rows, bins.nb[0] + bins.nb[1] + bins.nb[2] + (b - bins.blk[3]), roff, sg, v, ea, eb);
-    if (pr_live<MULTI>(v, jb)) {  // uniform across the workgroup
-      float sum = gather_sum(radj, c_in, ea, eb, tid, kPrBlock);
-      sum = wave_sum(sum);
-      if (lane == 0) s_red[wid] = sum;
-      __syncthreads();
-      if (tid == 0) {
-        float t = 0.f;
-        for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
-        pr_emit<MULTI>(v, t, sg, s, pr_in, pr_out, c_out, degf, mydiff, jb);
-      }
-      __syncthreads();
-    }
-  }
+  pr_sweep_block<G, MULTI>(blockIdx.x, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb, s_red,
+                           mydiff);
   if (!MULTI && sg.seg == sg.S - 1) pr_finish(mydiff, partial, s_red);
+}
+
+// The loop test folded into the next sweep (schedule pr_mode 2, grids of at
+// most kLagMaxGrid workgroups). Sweep r of a captured batch (r >= 1) starts
+// by summing sweep r-1's shares -- every workgroup the same shares in the
+// same order as pr_check, so all of them reach the same verdict -- and
+// evaluates (diff > beta && iter < maxIter) for iteration st->iter + r. On
+// "stop", workgroup 0 records the count and raises the flag, and the sweep
+// returns without writing. The shares alternate between two buffers, so
+// sweep r reads r-1's while writing its own. Sweep 0 of a batch only reads
+// the flag: the pr_check that ends every batch has judged the batch before.
+// One launch per iteration instead of two; the extra read is at most 8 KB
+// per workgroup.
+constexpr int kLagMaxGrid = 2048;
+
+// pr_check's sum of partial[0..n), reproduced by a 256-lane workgroup: lane
+// tid stands for pr_check's threads tid, tid+256, tid+512 and tid+768, whose
+// lanes sit in waves wid, wid+4, wid+8 and wid+12 of the 1024-lane check.
+__device__ __forceinline__ float lag_sum(const float* __restrict__ partial, int n, float* s_red16) {
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  const float4* p4 = reinterpret_cast<const float4*>(partial);
+  const int n4 = n >> 2;
+  float t[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int vt = tid + k * kPrBlock;
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+    int i = vt;
+    for (; i + kCheckBlock < n4; i += 2 * kCheckBlock) {
+      const float4 x = p4[i], y = p4[i + kCheckBlock];
+      a.x += x.x; a.y += x.y; a.z += x.z; a.w += x.w;
+      b.x += y.x; b.y += y.y; b.z += y.z; b.w += y.w;
+    }
+    if (i < n4) {
+      const float4 x = p4[i];
+      a.x += x.x; a.y += x.y; a.z += x.z; a.w += x.w;
+    }
+    t[k] = ((a.x + b.x) + (a.y + b.y)) + ((a.z + b.z) + (a.w + b.w));
+    if (vt < (n & 3)) t[k] += partial[(n4 << 2) + vt];
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float w = wave_sum(t[k]);
+    if (lane == 0) s_red16[wid + 4 * k] = w;
+  }
+  __syncthreads();
+  float d = 0.f;
+  for (int w = 0; w < kCheckBlock / kWave; ++w) d += s_red16[w];
+  return d;
+}
+
+template <int G>
+__global__ __launch_bounds__(kPrBlock) void pr_iter_lag(
+    const int32_t* __restrict__ roff, const int32_t* __restrict__ radj, const int4* __restrict__ rows,
+    const float* __restrict__ degf, PrBins bins, const float* __restrict__ pr_in, const float* __restrict__ c_in,
+    float* __restrict__ pr_out, float* __restrict__ c_out, PrState* __restrict__ st,
+    const float* __restrict__ prev, float* __restrict__ partial, int r) {
+  __shared__ float s_red[kCheckBlock / kWave];
+  const int4 row = pr_row<G>(rows, blockIdx.x, bins);
+  const PrState s = *st;
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  if (r > 0) {
+    // Every workgroup computes the same d and it. Workgroup 0 alone writes;
+    // a workgroup that reads its new iter or flag also stops (a larger
+    // count fails the test too), so the verdict is the same everywhere.
+    const float d = lag_sum(prev, bins.blk[4], s_red);
+    const int it = s.iter + r;
+    if (!(d > s.beta && it < s.maxIter)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->iter = it;
+        __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return;
+    }
+    __syncthreads();  // s_red is reused by the sweep
+  }
+  const PrSeg sg{nullptr, nullptr, 1, 0};
+  const PrJobs jb{};
+  float mydiff = 0.f;
+  pr_sweep_block<G, false>(blockIdx.x, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb, s_red,
+                           mydiff);
+  pr_finish(mydiff, partial, s_red);
 }
 
 // ---- batched jobs -------------------------------------------------------------
@@ -511,7 +616,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_push_apply(int32_t V, const float
   for (int64_t v = int64_t(blockIdx.x) * kPrBlock + threadIdx.x; v < V; v += stride) {
     const float sum = acc[v];
     acc[v] = 0.f;
-    pr_store(int32_t(v), sum, s, pr_in, pr_out, c_out, degf, mydiff);
+    pr_store(int32_t(v), sum, s, pr_in[v], degf[v], pr_out, c_out, mydiff);
   }
   pr_finish(mydiff, partial, s_red);
 }
@@ -684,7 +789,7 @@ static int pr_build(spm_graph* g, PrWork* w, const CsrView& view, bool multi) {
   const PrBins& b = w->bins;
   w->S = 1;
   if (multi) return SPM_OK;
-  w->partial_n = std::max(b.blk[4], g->num_cus * 8) + 1;
+  w->partial_n = std::max({b.blk[4], g->num_cus * 8, 2 * kLagMaxGrid}) + 1;
   SPM_TRY(dmalloc(g, (void**)&w->partial, sizeof(float) * size_t(w->partial_n), "pr partial diffs"));
   // Source segments for cache blocking.
   if (s.pr_seg_log2 > 0) w->S = int((int64_t(V) + (int64_t(1) << s.pr_seg_log2) - 1) >> s.pr_seg_log2);
@@ -765,6 +870,13 @@ int pr_release(spm_graph* g) {
   return SPM_OK;
 }
 
+// Schedule pr_mode 2 takes effect on pull sweeps of at most kLagMaxGrid
+// workgroups (two share buffers fit in the partial array); larger sweeps run
+// as pr_mode 0.
+static bool pr_lagged(const spm_graph* g, const PrWork* w) {
+  return g->sched.pr_mode == 2 && w->S == 1 && w->bins.blk[4] <= kLagMaxGrid && w->partial_n >= 2 * kLagMaxGrid;
+}
+
 // R iterations of w's sweep (push when `push`, else the binned pull), each
 // followed by the loop test.
 static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
@@ -779,7 +891,33 @@ static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
                      cin, w->acc, (const PrState*)w->st));
       SPM_TRY(launch(g, "pr_push_apply", pr_push_apply, dim3(pgrid), block, 0, g->V, (const float*)w->degf, w->acc,
                      pin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st, w->partial));
-      SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial, pgrid, w->st));
+      SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial, pgrid, w->st,
+                     1));
+      continue;
+    }
+    if (pr_lagged(g, w)) {
+      float* mine = w->partial + (r & 1) * kLagMaxGrid;
+      const float* prev = w->partial + (1 - (r & 1)) * kLagMaxGrid;
+      switch (w->bins.group) {
+#define SPM_PR_LAG_LAUNCH(G)                                                                               \
+  case G:                                                                                                  \
+    SPM_TRY(launch(g, "pr_iter", pr_iter_lag<G>, grid, block, 0, w->view.roff, w->view.radj,               \
+                   (const int4*)w->rows, (const float*)w->degf, w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], \
+                   w->st, prev, mine, r));                                                                 \
+    break;
+        SPM_PR_LAG_LAUNCH(2)
+        SPM_PR_LAG_LAUNCH(4)
+        SPM_PR_LAG_LAUNCH(8)
+        SPM_PR_LAG_LAUNCH(16)
+        SPM_PR_LAG_LAUNCH(32)
+#undef SPM_PR_LAG_LAUNCH
+        default: return set_error(SPM_EINVAL, "pr: bad lane group");
+      }
+      // The batch's last sweep is judged here, with the batch's R
+      // iterations counted at once.
+      if (r == R - 1)
+        SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)mine, int(grid.x),
+                       w->st, R));
       continue;
     }
     for (int seg = 0; seg < w->S; ++seg) {
@@ -802,10 +940,12 @@ static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
 #undef SPM_PR_LAUNCH
     }
     SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial,
-                   int(grid.x), w->st));
+                   int(grid.x), w->st, 1));
   }
   return SPM_OK;
 }
+
+static int pr_finish_run(spm_graph* g, PrWork* w, int32_t iters, float* d_out, float* h_out, int32_t* iter_out);
 
 int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out, float* h_out,
            int32_t* iter_out) {
@@ -821,6 +961,7 @@ int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out,
   const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
   SPM_TRY(launch(g, "pr_init", pr_init, dim3(igrid), dim3(256), 0, V, w->view.off, w->degf,
                  w->pr[0], w->c[0], w->acc, w->st, init, int(maxIter), beta, delta, base));
+  int32_t* h = g->h_pinned;
   // Iterations per replay: even (ping-pong parity), up to 128.
   int R = std::max(2, std::min(128, (int(std::max(maxIter, 1)) + 1) & ~1));
   const bool use_graph = !g->prof.on;
@@ -837,7 +978,6 @@ int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out,
     SPM_CHECK_HIP(hipGraphInstantiate(&w->exec, graph, nullptr, nullptr, 0));
     w->R = R;
   }
-  int32_t* h = g->h_pinned;
   for (int64_t rep = 0;; ++rep) {
     if (use_graph) SPM_CHECK_HIP(hipGraphLaunch(w->exec, g->stream));
     else SPM_TRY(pr_enqueue_iters(g, w, R, g->sched.pr_mode == 1));
@@ -846,7 +986,12 @@ int pr_run(spm_graph* g, float beta, float delta, int32_t maxIter, float* d_out,
     if (h[0]) break;
     if (rep > int64_t(maxIter) / R + 2) return set_error(SPM_EHIP, "pr: loop did not terminate");
   }
-  const int32_t iters = h[1];
+  return pr_finish_run(g, w, h[1], d_out, h_out, iter_out);
+}
+
+// The ranks of the run's last iteration to d_out / h_out, in original ids.
+static int pr_finish_run(spm_graph* g, PrWork* w, int32_t iters, float* d_out, float* h_out, int32_t* iter_out) {
+  const int32_t V = g->V;
   const float* res = w->pr[iters & 1];
   if (w->view.perm) {  // ranks in original ids
     SPM_TRY(gather_back(g, w->view, res, d_out ? d_out : w->out));
@@ -915,7 +1060,6 @@ int pr_multi_run(spm_graph* g, const int32_t* bases, int32_t n, float beta, floa
     view.roff = g->roff;
     view.radj = g->radj;
     view.perm = nullptr;
-    view.h_perm = nullptr;
     SPM_TRY(pr_build(g, &m->w, view, true));
     SPM_TRY(dmalloc(g, (void**)&m->job, sizeof(int32_t) * size_t(V), "pr job ids"));
     SPM_TRY(dmalloc(g, (void**)&m->dv, sizeof(float) * size_t(V), "pr vertex diffs"));
@@ -1069,9 +1213,8 @@ int pr_run_noalloc(spm_graph* g, float beta, float delta, int32_t maxIter, float
   w.view.roff = g->roff;
   w.view.radj = g->radj;
   w.view.perm = nullptr;
-  w.view.h_perm = nullptr;
   SPM_TRY(pr_bin_device(g, g->roff, w.rows, cnt, g->h_pinned, &w.bins));
-  w.partial_n = std::max(w.bins.blk[4], g->num_cus * 8) + 1;
+  w.partial_n = std::max({w.bins.blk[4], g->num_cus * 8, 2 * kLagMaxGrid}) + 1;
   if (size_t(w.partial_n) * sizeof(float) > kNaPartial) return set_error(SPM_EINVAL, "spm_pr_device_noalloc: too many workgroups");
   const float num_nodes = float(V);
   const float init = 1 / num_nodes;
@@ -1079,11 +1222,11 @@ int pr_run_noalloc(spm_graph* g, float beta, float delta, int32_t maxIter, float
   const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
   SPM_TRY(launch(g, "pr_init", pr_init, dim3(igrid), dim3(256), 0, V, w.view.off, w.degf, w.pr[0], w.c[0],
                  (float*)nullptr, w.st, init, int(maxIter), beta, delta, base_term));
+  int32_t* h = g->h_pinned;
   // Batches of R iterations (even, so each starts on buffer 0), launched
   // directly -- graph instantiation would allocate -- with the done flag read
   // back after each batch; iterations after convergence return at once.
   const int R = std::max(2, std::min(128, (int(std::max(maxIter, 1)) + 1) & ~1));
-  int32_t* h = g->h_pinned;
   for (int64_t rep = 0;; ++rep) {
     SPM_TRY(pr_enqueue_iters(g, &w, R, false));
     SPM_CHECK_HIP(hipMemcpyAsync(h, &w.st->done, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));

@@ -171,6 +171,7 @@ __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea
 // the big-round log, and run_start takes the first captured round's stamp).
 // srcs == nullptr means the single source `one`.
 __global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restrict__ srcs, int32_t n, int32_t one,
+                                                        const int32_t* __restrict__ perm,
                                                         const int32_t* __restrict__ off, int32_t* __restrict__ dist,
                                                         SsspQueue q0, unsigned long long* __restrict__ ctr, int slot,
                                                         unsigned long long* __restrict__ band_tot,
@@ -187,6 +188,7 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restric
     int32_t s = -1, deg = 0;
     if (j < n) {
       s = srcs ? srcs[j] : one;
+      if (perm) s = perm[s];  // original id -> the relabeled copy's id
       deg = off[s + 1] - off[s];
       dist[s] = 0;
       if (active) active[s] = 1;
@@ -782,12 +784,6 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   SPM_TRY(sssp_alloc(g, view));
   SsspWork* w = g->sssp;
   w->dist_out_ok = false;
-  std::vector<int32_t> mapped;
-  if (view.h_perm) {  // sources in the kernels' ids
-    mapped.resize(size_t(n));
-    for (int32_t i = 0; i < n; ++i) mapped[i] = view.h_perm[srcs[i]];
-    srcs = mapped.data();
-  }
   const int R = g->sched.sssp_rounds;
   const int dense = g->sched.sssp_mode == 1;
   if (w->host_base > 0xF0000000u) {  // stamp wrap-around: start over
@@ -812,8 +808,10 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   // the captured rounds, and queues its frontier in distance bands.
   const bool banded = g->sched.sssp_mode == 0 && w->maxw >= 0;
   const int slot = banded ? 2 : 0;
+  // sources are original ids; the seed kernel maps them through the
+  // relabeling (view.perm) when there is one
   SPM_TRY(launch(g, "sssp_seed", sssp_seed, dim3(1), dim3(kSsspBlock), 0, n > 1 ? (const int32_t*)w->srcbuf : nullptr,
-                 n, srcs[0], w->view.off, w->dist, w->q[slot], w->ctr, slot, banded ? w->band_tot : nullptr,
+                 n, srcs[0], view.perm, w->view.off, w->dist, w->q[slot], w->ctr, slot, banded ? w->band_tot : nullptr,
                  w->run_start, w->host_base + (banded ? 1u : 0u), w->big, act));
   if (banded) {
     const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);

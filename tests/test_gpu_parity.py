@@ -274,6 +274,41 @@ def test_pr_schedules(oracle, t1, t2, t3, grp, seg, mode):
 
 
 # ------------------------------------------------------------------ relabeled layout
+@pytest.mark.parametrize("kind", ["rmat12", "rmat16", "grid", "star"])
+def test_pr_lagged_loop_test_matches_separate_check(oracle, kind):
+    """pr_mode 2 (each sweep evaluates the loop test of the sweep before, the
+    check kernel runs once per captured batch) sums the same per-vertex terms
+    and the same per-workgroup shares in the same order as pr_mode 0, so ranks
+    and iteration counts are identical: fixed counts (beta < 0 runs to
+    maxIter; 1 and 130 cross the batch length of 128), convergence inside the
+    first batch and after it, and the oracle within tolerance."""
+    if kind == "rmat12":
+        g = graph.rmat(12, 16, seed=2)
+    elif kind == "rmat16":
+        g = graph.rmat(16, 16, seed=1)
+    elif kind == "grid":
+        g = graph.grid2d(100, 130)
+    else:  # one hub row long enough for the workgroup-per-vertex bin
+        n = 6000
+        g = graph.from_edges(np.concatenate([np.arange(1, n), np.arange(1, n)]).astype(np.int32),
+                             np.concatenate([np.zeros(n - 1), np.arange(2, n + 1) % n]).astype(np.int32))
+    dev = g.device()
+    cases = [(-1.0, 37), (-1.0, 1), (-1.0, 2), (-1.0, 130), (1e-10, 100), (1e-10, 300), (1e-6, 300)]
+    out = {}
+    for mode in (0, 2):
+        dev.set_schedule(pr_mode=mode)
+        out[mode] = [dev.pr(beta, 0.85, mi) for beta, mi in cases]
+    for (beta, mi), a, b in zip(cases, out[0], out[2]):
+        assert a[1] == b[1], (beta, mi, a[1], b[1])
+        if beta < 0:
+            assert a[1] == mi
+        np.testing.assert_array_equal(a[0], b[0])
+    ref, it_ref = oracle.compute_pr(g, 1e-10, 0.85, 100)
+    pr, it = out[2][4]
+    assert exits_agree(oracle, g, it, it_ref), (it, it_ref)
+    check_pr_values(oracle, g, pr, it, ref, it_ref)
+
+
 def test_relabel_sssp_bit_identical(oracle):
     """The degree-ordered copy changes vertex ids inside the kernels only:
     distances come back bit-identical in original ids, for both frontier

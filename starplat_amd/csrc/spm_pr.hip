@@ -901,7 +901,7 @@ static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
       switch (w->bins.group) {
 #define SPM_PR_LAG_LAUNCH(G)                                                                               \
   case G:                                                                                                  \
-    SPM_TRY(launch(g, "pr_iter", pr_iter_lag<G>, grid, block, 0, w->view.roff, w->view.radj,               \
+    SPM_TRY(launch(g, "pr_iter_lag", pr_iter_lag<G>, grid, block, 0, w->view.roff, w->view.radj,          \
                    (const int4*)w->rows, (const float*)w->degf, w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], \
                    w->st, prev, mine, r));                                                                 \
     break;

@@ -268,6 +268,24 @@ def bench_tc(args, stash):
 
 
 # --------------------------------------------------------------------------- BC
+def bc_gather_profile(g, batch):
+    """The access pattern of one 64-source batch (host BFS levels): how many
+    sources share each per-source row gather of bc_sigma / bc_back and how
+    many 128-byte lines those gathers touch."""
+    out = np.zeros(7)
+    rc = N.host_lib().spmh_bc_gather_profile(g._native(), N.ptr(np.ascontiguousarray(batch), C.c_int32), len(batch),
+                                             N.ptr(out, C.c_double))
+    if rc != 0:
+        return None
+    dag, gathers, l64, l32, listed, rows, levels = out
+    return {"batch_sources": len(batch), "dag_edges": int(dag), "row_gathers": int(gathers),
+            "mean_sources_per_gather": round(dag / gathers, 3),
+            "lines_per_gather_fp64_row": round(l64 / gathers, 3), "lines_per_gather_fp32_row": round(l32 / gathers, 3),
+            "gather_bytes_per_dag_edge_sigma_fp64": round(l64 * 128 / dag, 2),
+            "gather_bytes_per_dag_edge_coef_fp32": round(l32 * 128 / dag, 2),
+            "listings_per_vertex": round(listed / g.V, 3), "in_edges_read_per_batch": int(rows), "levels": int(levels)}
+
+
 def bench_bc(args, stash):
     log("generating Erdos-Renyi 4M / avg degree 8 for BC")
     g = graph_for("bc", args)
@@ -289,7 +307,8 @@ def bench_bc(args, stash):
            "kernel_ms": {k: round(v[0], 3) for k, v in kt.items()},
            "roofline": roofline(model, kms, launches, f"bc-er{args.bc_n}-{len(src)}", ks),
            "note": "model bytes count every source's traversal separately; the batched kernels share adjacency reads "
-                   "across 64 sources, so achieved model GB/s can exceed the HBM peak"}
+                   "across 64 sources, so achieved model GB/s can exceed the HBM peak",
+           "gather_profile": bc_gather_profile(g, src[:min(64, len(src))])}
     # the first full 64-source batch alone, for the CPU child's parity check and timing sample
     k = min(64, len(src))
     dev.bc_device(src[:k], d_out)

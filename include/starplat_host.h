@@ -124,6 +124,18 @@ double spmh_model_tc_bytes(const spmh_graph* g, int64_t* m_und, int64_t* wedge_w
 double spmh_model_bc_bytes(const spmh_graph* g, const int32_t* sources, int32_t n,
                            int64_t* edges_traversed);
 
+/* The access pattern of one 64-source BC batch (sources[0..n), n <= 64),
+ * from the sources' BFS levels: out[0] DAG edges summed over the sources;
+ * out[1] row gathers -- (in-edge (v, w), level d) pairs with a non-empty set
+ * S of sources for which w is at level d and v at d - 1, i.e. the per-source
+ * rows the sigma sweep fetches (the reverse sweep fetches the same set);
+ * out[2] / out[3] 128-byte lines those gathers touch with 8-byte / 4-byte
+ * per-source entries (64-entry rows); out[4] (vertex, level) listings;
+ * out[5] in-edges read by the sigma sweep (each listing reads the vertex's
+ * whole in-row); out[6] levels. Mean sources per gather = out[0] / out[1].
+ * Returns 0, or -1 on bad arguments. */
+int spmh_bc_gather_profile(const spmh_graph* g, const int32_t* sources, int32_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -144,6 +144,7 @@ HOST_API = [
     ("spmh_model_sssp", f64, [P_spmh, P_i32, i32, P_i64, P_i64, P_i64]),
     ("spmh_model_tc_bytes", f64, [P_spmh, P_i64, P_i64]),
     ("spmh_model_bc_bytes", f64, [P_spmh, P_i32, i32, P_i64]),
+    ("spmh_bc_gather_profile", C.c_int, [P_spmh, P_i32, i32, P_f64]),
 ]
 
 _dev = None

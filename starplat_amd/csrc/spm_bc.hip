@@ -25,10 +25,14 @@
 // The reverse pass walks the level lists backwards, one wavefront per vertex:
 // lane s sums the FP32 terms sigma_v * coef_w over out-neighbours w at the
 // next level for s (ascending neighbour order, as the reference), where
-// coef_w = (1 + delta_w) / sigma_w was left by w's own step in place of
-// sigma_w -- one multiply and one 8-byte read per DAG edge instead of a
-// division and two reads. The 64 per-source dependencies are reduced into
-// BC[v] (pairwise over the batch, then once per batch).
+// coef_w = (1 + delta_w) / sigma_w was left by w's own step in an FP32 row
+// [V][64] -- one multiply and one 4-byte read per DAG edge instead of a
+// division and two reads. A gather touches on average ~2.6 of a 512-byte
+// FP64 row's four 128-byte lines for ~6 sources at RMAT/ER sizes; the FP32
+// row halves the lines it can touch (the coefficient's rounding, 2^-24
+// relative, is below the FP32 accumulation's own). The 64 per-source
+// dependencies are reduced into BC[v] (pairwise over the batch, then once
+// per batch).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -50,6 +54,7 @@ constexpr int kScanStep = 4;
 
 struct BcWork {
   double* sigma = nullptr;          // [V][64]
+  float* coef = nullptr;            // [V][64] (1 + delta) / sigma of the reverse pass
   unsigned long long* visited = nullptr;  // [V]
   unsigned long long* masks = nullptr;    // [mask_levels][V]
   int32_t* lists = nullptr;         // level lists, concatenated
@@ -262,7 +267,8 @@ __global__ __launch_bounds__(kBcBlock) void bc_sigma(
 __global__ __launch_bounds__(kBcBlock) void bc_back(
     const int32_t* __restrict__ list, unsigned int n, const int32_t* __restrict__ off,
     const int32_t* __restrict__ adj, const unsigned long long* __restrict__ md,
-    const unsigned long long* __restrict__ md1, double* __restrict__ sigma, float* __restrict__ BC) {
+    const unsigned long long* __restrict__ md1, const double* __restrict__ sigma, float* __restrict__ coef,
+    float* __restrict__ BC) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kBcWaves;
   for (int64_t i = int64_t(blockIdx.x) * kBcWaves + threadIdx.x / kWave; i < n; i += nw) {
@@ -283,34 +289,33 @@ __global__ __launch_bounds__(kBcBlock) void bc_back(
       unsigned long long act = __ballot(m != 0ull);
       while (act) {
         // Up to kGather active successors per step: loads issued together,
-        // terms added in ascending neighbour order. sigma[w][s] of a successor
-        // already holds coef_w = (1 + delta_w) / sigma_w.
-        double c[kGather];
+        // terms added in ascending neighbour order. coef[w][s] of a successor
+        // holds (1 + delta_w) / sigma_w, written by w's own step.
+        float c[kGather];
         bool on[kGather];
 #pragma unroll
         for (int k = 0; k < kGather; ++k) {
           on[k] = false;
-          c[k] = 0.0;
+          c[k] = 0.f;
           if (act) {
             const int src_lane = __ffsll((long long)act) - 1;
             act &= act - 1;
             const int32_t ww = __shfl(w, src_lane, kWave);
             const unsigned long long mm = __shfl(m, src_lane, kWave);
             on[k] = (mm >> lane) & 1ull;
-            if (on[k]) c[k] = sigma[size_t(ww) * kBatch + lane];
+            if (on[k]) c[k] = coef[size_t(ww) * kBatch + lane];
           }
         }
 #pragma unroll
         for (int k = 0; k < kGather; ++k)
-          if (on[k]) acc += float(sv * c[k]);
+          if (on[k]) acc += float(sv * double(c[k]));
       }
     }
     float part = 0.f;
     if (mine) {
-      // delta_v is final: leave coef_v for the predecessors at level d-1 in
-      // place of sigma_v (only this source's lane; sigma_v for sources where v
-      // sits at another level is untouched).
-      sigma[size_t(v) * kBatch + lane] = (1.0 + double(acc)) / sv;
+      // delta_v is final: leave coef_v for the predecessors at level d-1
+      // (only this source's lane).
+      coef[size_t(v) * kBatch + lane] = float((1.0 + double(acc)) / sv);
       part = acc;
     }
     part = wave_sum(part);
@@ -343,6 +348,7 @@ static int bc_alloc(spm_graph* g) {
   g->bc = w;
   const size_t V = size_t(g->V);
   SPM_TRY(dmalloc(g, (void**)&w->sigma, sizeof(double) * V * kBatch, "bc sigma"));
+  SPM_TRY(dmalloc(g, (void**)&w->coef, sizeof(float) * V * kBatch, "bc coefficients"));
   SPM_TRY(dmalloc(g, (void**)&w->visited, sizeof(unsigned long long) * V, "bc visited"));
   SPM_TRY(dmalloc(g, (void**)&w->cnt, sizeof(unsigned int), "bc counter"));
   SPM_TRY(dmalloc(g, (void**)&w->srcbuf, sizeof(int32_t) * kBatch, "bc sources"));
@@ -386,6 +392,7 @@ int bc_release(spm_graph* g) {
   if (!w) return SPM_OK;
   const size_t V = size_t(g->V);
   dfree(g, w->sigma, sizeof(double) * V * kBatch);
+  dfree(g, w->coef, sizeof(float) * V * kBatch);
   dfree(g, w->visited, sizeof(unsigned long long) * V);
   dfree(g, w->masks, sizeof(unsigned long long) * V * w->mask_levels);
   dfree(g, w->lists, sizeof(int32_t) * w->list_cap);
@@ -458,7 +465,8 @@ static int bc_backward(spm_graph* g, int L, const std::vector<unsigned int>& lst
     SPM_TRY(launch(g, "bc_back", bc_back, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + b), e - b,
                    (const int32_t*)g->off, (const int32_t*)g->adj,
                    (const unsigned long long*)(w->masks + size_t(d) * Vs),
-                   (const unsigned long long*)(w->masks + size_t(d + 1) * Vs), w->sigma, d_BC));
+                   (const unsigned long long*)(w->masks + size_t(d + 1) * Vs), (const double*)w->sigma, w->coef,
+                   d_BC));
   }
   return SPM_OK;
 }

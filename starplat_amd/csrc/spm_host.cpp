@@ -819,4 +819,71 @@ double spmh_model_bc_bytes(const spmh_graph* g, const int32_t* sources, int32_t 
   return total;
 }
 
+int spmh_bc_gather_profile(const spmh_graph* g, const int32_t* sources, int32_t n, double* out) {
+  if (!g || !sources || !out || n < 1 || n > 64) return -1;
+  const int32_t V = g->V;
+  const int nb = n;
+  // lev[v * nb + s]: BFS level of v from source s (-1 unreached).
+  std::vector<int16_t> lev(size_t(V) * nb, int16_t(-1));
+  int32_t depth = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : depth)
+  for (int s = 0; s < nb; ++s) {
+    std::vector<int32_t> q;
+    q.reserve(V);
+    const int32_t src = sources[s];
+    lev[size_t(src) * nb + s] = 0;
+    q.push_back(src);
+    for (size_t h = 0; h < q.size(); ++h) {
+      const int32_t v = q[h];
+      const int16_t lv = lev[size_t(v) * nb + s];
+      depth = std::max(depth, int32_t(lv));
+      for (int32_t e = g->indexofNodes[v]; e < g->indexofNodes[v + 1]; ++e) {
+        const int32_t w = g->edgeList[e];
+        if (lev[size_t(w) * nb + s] < 0) {
+          lev[size_t(w) * nb + s] = int16_t(lv + 1);
+          q.push_back(w);
+        }
+      }
+    }
+  }
+  double dag = 0, gathers = 0, lines64 = 0, lines32 = 0, listed = 0, row_reads = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : dag, gathers, lines64, lines32, listed, row_reads)
+  for (int32_t w = 0; w < V; ++w) {
+    const int16_t* lw = &lev[size_t(w) * nb];
+    // Levels >= 1 at which w is listed (some source reaches w there).
+    int16_t lv[64];
+    int nl = 0;
+    for (int s = 0; s < nb; ++s) {
+      if (lw[s] < 1) continue;
+      bool seen = false;
+      for (int k = 0; k < nl && !seen; ++k) seen = lv[k] == lw[s];
+      if (!seen) lv[nl++] = lw[s];
+    }
+    const int32_t indeg = g->rev_indexofNodes[w + 1] - g->rev_indexofNodes[w];
+    listed += nl;
+    row_reads += double(nl) * indeg;
+    for (int32_t e = g->rev_indexofNodes[w]; e < g->rev_indexofNodes[w + 1]; ++e) {
+      const int16_t* lu = &lev[size_t(g->srcList[e]) * nb];
+      for (int k = 0; k < nl; ++k) {
+        uint64_t m = 0;
+        for (int s = 0; s < nb; ++s)
+          if (lw[s] == lv[k] && lu[s] == lv[k] - 1) m |= 1ull << s;
+        if (!m) continue;
+        gathers += 1;
+        dag += __builtin_popcountll(m);
+        for (int l = 0; l < 4; ++l) lines64 += ((m >> (16 * l)) & 0xffffull) != 0;
+        for (int l = 0; l < 2; ++l) lines32 += ((m >> (32 * l)) & 0xffffffffull) != 0;
+      }
+    }
+  }
+  out[0] = dag;
+  out[1] = gathers;
+  out[2] = lines64;
+  out[3] = lines32;
+  out[4] = listed;
+  out[5] = row_reads;
+  out[6] = double(depth) + 1.0;
+  return 0;
+}
+
 }  // extern "C"

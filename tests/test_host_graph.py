@@ -286,3 +286,48 @@ def test_grid_matches_numpy_restatement():
     d = np.concatenate([v[:, 1:].ravel(), v[1:, :].ravel()])
     g = graph.grid2d(rows, cols, seed=2, weights=(1, 50))
     _check_against(g, _csr_from_samples(s, d, 2, True, (1, 50)))
+
+
+@pytest.mark.parametrize("undirected,nsrc", [(False, 5), (True, 64)])
+def test_bc_gather_profile_matches_brute_force(undirected, nsrc):
+    """spmh_bc_gather_profile against plain loops: BFS levels per source, then
+    for every in-edge (v, w) and every level d at which w is listed, the set
+    of sources with w at d and v at d - 1."""
+    import ctypes as C
+    from collections import deque
+    from starplat_amd import _native as N
+    g = graph.rmat(8, 6, seed=3, undirected=undirected)
+    src = sources(g.V, nsrc, seed=2)
+    off, adj = np.asarray(g.indexofNodes), np.asarray(g.edgeList)
+    roff, radj = np.asarray(g.rev_indexofNodes), np.asarray(g.srcList)
+    lev = np.full((nsrc, g.V), -1)
+    for k, s in enumerate(src):
+        lev[k, s] = 0
+        q = deque([s])
+        while q:
+            v = q.popleft()
+            for w in adj[off[v]:off[v + 1]]:
+                if lev[k, w] < 0:
+                    lev[k, w] = lev[k, v] + 1
+                    q.append(w)
+    dag = gathers = l64 = l32 = listed = rows = 0
+    for w in range(g.V):
+        levels = sorted({int(x) for x in lev[:, w] if x >= 1})
+        listed += len(levels)
+        rows += len(levels) * int(roff[w + 1] - roff[w])
+        for v in radj[roff[w]:roff[w + 1]]:
+            for d in levels:
+                S = [k for k in range(nsrc) if lev[k, w] == d and lev[k, v] == d - 1]
+                if S:
+                    gathers += 1
+                    dag += len(S)
+                    l64 += len({k // 16 for k in S})
+                    l32 += len({k // 32 for k in S})
+    out = np.zeros(7)
+    assert N.host_lib().spmh_bc_gather_profile(g._native(), N.ptr(src, C.c_int32), nsrc, N.ptr(out, C.c_double)) == 0
+    assert list(out[:6]) == [dag, gathers, l64, l32, listed, rows]
+    assert out[6] == lev.max() + 1
+    # DAG edges summed over sources, the per-source count the byte model uses
+    per_source = sum(int(((lev[k, adj] == np.repeat(lev[k], np.diff(off)) + 1) & (np.repeat(lev[k], np.diff(off)) >= 0)).sum())
+                     for k in range(nsrc))
+    assert dag == per_source

@@ -27,7 +27,9 @@
 // next level for s (ascending neighbour order, as the reference), where
 // coef_w = (1 + delta_w) / sigma_w was left by w's own step in an FP32 row
 // [V][64] -- one multiply and one 4-byte read per DAG edge instead of a
-// division and two reads. A gather touches on average ~2.6 of a 512-byte
+// division and two reads (a batch whose path counts reach 2^96, where the
+// reciprocal would leave FP32's normal range, keeps FP64 coefficients in
+// place of sigma instead). A gather touches on average ~2.6 of a 512-byte
 // FP64 row's four 128-byte lines for ~6 sources at RMAT/ER sizes; the FP32
 // row halves the lines it can touch (the coefficient's rounding, 2^-24
 // relative, is below the FP32 accumulation's own). The 64 per-source
@@ -52,13 +54,28 @@ constexpr int kGather = 4;
 // In-edges one lane of the bottom-up scan reads per step.
 constexpr int kScanStep = 4;
 
+// The forward pass runs its levels from device-side state, so the host reads
+// it once per window of levels rather than once per level.
+// The cursor sits on a cache line of its own: the flags are read by every
+// workgroup of every level's launches, the cursor is hit by atomics.
+struct BcCtl {
+  unsigned int cnt;  // list append cursor
+  unsigned int pad_[31];
+  int done;          // no vertex reached at the next level
+  int levels;        // deepest non-empty level, once done
+  int push;          // the current level expands top-down
+  int wide;          // some sigma reached 2^96: the reverse pass keeps FP64 coefficients
+};
+
 struct BcWork {
   double* sigma = nullptr;          // [V][64]
   float* coef = nullptr;            // [V][64] (1 + delta) / sigma of the reverse pass
   unsigned long long* visited = nullptr;  // [V]
   unsigned long long* masks = nullptr;    // [mask_levels][V]
   int32_t* lists = nullptr;         // level lists, concatenated
-  unsigned int* cnt = nullptr;      // [1] list append cursor
+  BcCtl* ctl = nullptr;             // the forward pass's device-side control
+  unsigned int* lstart = nullptr;   // [lstart_cap] level d's list is lstart[d] .. lstart[d+1]
+  size_t lstart_cap = 0;
   int32_t* srcbuf = nullptr;        // [64]
   size_t mask_levels = 0;
   size_t list_cap = 0;
@@ -68,14 +85,48 @@ struct BcWork {
 // a separate launch, so no workgroup can clear a bit another one has set.
 __global__ void bc_init(int32_t V, unsigned long long* __restrict__ visited,
                         unsigned long long* __restrict__ mask0, const int32_t* __restrict__ src, int nb,
-                        int32_t* __restrict__ list0, unsigned int* cnt) {
+                        int32_t* __restrict__ list0, BcCtl* ctl, unsigned int* __restrict__ lstart) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
     visited[v] = 0ull;
     mask0[v] = 0ull;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *cnt = unsigned(nb);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl->cnt = unsigned(nb);
+    ctl->done = 0;
+    ctl->levels = 0;
+    ctl->push = uint64_t(nb) * kPushRatio < uint64_t(V);
+    ctl->wide = 0;
+    lstart[0] = 0;
+    lstart[1] = unsigned(nb);
+  }
   if (blockIdx.x == 0 && int(threadIdx.x) < nb) list0[threadIdx.x] = src[threadIdx.x];
+}
+
+// Flag reads of the control block: written by one thread of an earlier
+// launch (visible after the launch boundary), fixed while a kernel runs.
+__device__ __forceinline__ int ctl_load(const int* p) { return *p; }
+
+// End of level d's expansion (one thread): the list of level d+1 ends at the
+// cursor; an empty one ends the pass, else the size picks how level d+1
+// expands.
+__global__ void bc_level_end(int32_t V, int d, BcCtl* ctl, unsigned int* __restrict__ lstart) {
+  if (ctl->done) return;
+  const unsigned int used = lstart[d + 1], total = ctl->cnt;
+  if (total == used) {
+    ctl->levels = d;
+    ctl->done = 1;
+    return;
+  }
+  lstart[d + 2] = total;
+  ctl->push = uint64_t(total - used) * kPushRatio < uint64_t(V);
+}
+
+// The next level's masks start at zero when the level expands top-down.
+__global__ void bc_clear_if_push(int32_t V, unsigned long long* __restrict__ nxt, const BcCtl* ctl) {
+  if (ctl_load(&ctl->done) || !ctl_load(&ctl->push)) return;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t v = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) nxt[v] = 0ull;
 }
 
 __global__ void bc_seed(const int32_t* __restrict__ src, int nb, unsigned long long* __restrict__ visited,
@@ -93,9 +144,11 @@ __global__ __launch_bounds__(kBcBlock) void bc_scan(
     int32_t V, const int32_t* __restrict__ roff, const int32_t* __restrict__ radj,
     const unsigned long long* __restrict__ cur, unsigned long long* __restrict__ nxt,
     unsigned long long* __restrict__ visited, unsigned long long batchmask,
-    int32_t* __restrict__ list, unsigned int* __restrict__ cnt) {
+    int32_t* __restrict__ list, BcCtl* ctl) {
   __shared__ unsigned int s_cnt[kBcWaves];
   __shared__ unsigned int s_base;
+  if (ctl_load(&ctl->done) || ctl_load(&ctl->push)) return;
+  unsigned int* cnt = &ctl->cnt;
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   const int lane = __lane_id(), wid = threadIdx.x / kWave;
   // Rounded so whole workgroups take part in the aggregated append.
@@ -160,12 +213,16 @@ __global__ __launch_bounds__(kBcBlock) void bc_scan(
 constexpr int kPushTile = 64;
 constexpr int kPushStage = 4096;
 __global__ __launch_bounds__(kBcBlock) void bc_push(
-    const int32_t* __restrict__ front, unsigned int n, const int32_t* __restrict__ off,
+    const unsigned int* __restrict__ lstart, int d, const int32_t* __restrict__ off,
     const int32_t* __restrict__ adj, const unsigned long long* __restrict__ cur,
     unsigned long long* __restrict__ nxt, unsigned long long* __restrict__ visited,
-    int32_t* __restrict__ list, unsigned int* __restrict__ cnt) {
+    int32_t* __restrict__ list, BcCtl* ctl) {
   __shared__ int32_t s_buf[kPushStage];
   __shared__ unsigned int s_n, s_base;
+  if (ctl_load(&ctl->done) || !ctl_load(&ctl->push)) return;
+  unsigned int* cnt = &ctl->cnt;
+  const int32_t* front = list + lstart[d];
+  const unsigned int n = lstart[d + 1] - lstart[d];
   const int lane = __lane_id(), wid = threadIdx.x / kWave;
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
@@ -222,10 +279,14 @@ __global__ __launch_bounds__(kBcBlock) void bc_push(
 }
 
 __global__ __launch_bounds__(kBcBlock) void bc_sigma(
-    const int32_t* __restrict__ list, unsigned int n, const int32_t* __restrict__ roff,
-    const int32_t* __restrict__ radj, const unsigned long long* __restrict__ cur,
+    const int32_t* __restrict__ lists, const unsigned int* __restrict__ lstart, int d, BcCtl* ctl,
+    const int32_t* __restrict__ roff, const int32_t* __restrict__ radj, const unsigned long long* __restrict__ cur,
     const unsigned long long* __restrict__ nxt, double* __restrict__ sigma) {
   const int lane = __lane_id();
+  if (ctl_load(&ctl->done)) return;
+  // level d+1's list
+  const int32_t* list = lists + lstart[d + 1];
+  const unsigned int n = lstart[d + 2] - lstart[d + 1];
   const int64_t nw = int64_t(gridDim.x) * kBcWaves;
   for (int64_t i = int64_t(blockIdx.x) * kBcWaves + threadIdx.x / kWave; i < n; i += nw) {
     const int32_t w = list[i];
@@ -260,15 +321,25 @@ __global__ __launch_bounds__(kBcBlock) void bc_sigma(
         for (int k = 0; k < kGather; ++k) acc += x[k];
       }
     }
-    if ((nm >> lane) & 1ull) sigma[size_t(w) * kBatch + lane] = acc;
+    if ((nm >> lane) & 1ull) {
+      sigma[size_t(w) * kBatch + lane] = acc;
+      // Path counts from 2^96 on would leave (1 + delta) / sigma near FP32's
+      // smallest normal numbers (2^-126): the batch's reverse pass then
+      // keeps FP64 coefficients (a grid's counts are binomials, ~10^59 at
+      // 200 levels).
+      if (acc >= 0x1p96) ctl->wide = 1;
+    }
   }
 }
 
+// CT = float: coefficients in their own FP32 rows (coef); CT = double: in
+// place of sigma_v (coef == sigma), for batches whose path counts are too
+// large for FP32 reciprocals.
+template <typename CT>
 __global__ __launch_bounds__(kBcBlock) void bc_back(
     const int32_t* __restrict__ list, unsigned int n, const int32_t* __restrict__ off,
     const int32_t* __restrict__ adj, const unsigned long long* __restrict__ md,
-    const unsigned long long* __restrict__ md1, const double* __restrict__ sigma, float* __restrict__ coef,
-    float* __restrict__ BC) {
+    const unsigned long long* __restrict__ md1, const double* sigma, CT* coef, float* __restrict__ BC) {
   const int lane = __lane_id();
   const int64_t nw = int64_t(gridDim.x) * kBcWaves;
   for (int64_t i = int64_t(blockIdx.x) * kBcWaves + threadIdx.x / kWave; i < n; i += nw) {
@@ -291,12 +362,12 @@ __global__ __launch_bounds__(kBcBlock) void bc_back(
         // Up to kGather active successors per step: loads issued together,
         // terms added in ascending neighbour order. coef[w][s] of a successor
         // holds (1 + delta_w) / sigma_w, written by w's own step.
-        float c[kGather];
+        CT c[kGather];
         bool on[kGather];
 #pragma unroll
         for (int k = 0; k < kGather; ++k) {
           on[k] = false;
-          c[k] = 0.f;
+          c[k] = CT(0);
           if (act) {
             const int src_lane = __ffsll((long long)act) - 1;
             act &= act - 1;
@@ -315,7 +386,7 @@ __global__ __launch_bounds__(kBcBlock) void bc_back(
     if (mine) {
       // delta_v is final: leave coef_v for the predecessors at level d-1
       // (only this source's lane).
-      coef[size_t(v) * kBatch + lane] = float((1.0 + double(acc)) / sv);
+      coef[size_t(v) * kBatch + lane] = CT((1.0 + double(acc)) / sv);
       part = acc;
     }
     part = wave_sum(part);
@@ -350,16 +421,29 @@ static int bc_alloc(spm_graph* g) {
   SPM_TRY(dmalloc(g, (void**)&w->sigma, sizeof(double) * V * kBatch, "bc sigma"));
   SPM_TRY(dmalloc(g, (void**)&w->coef, sizeof(float) * V * kBatch, "bc coefficients"));
   SPM_TRY(dmalloc(g, (void**)&w->visited, sizeof(unsigned long long) * V, "bc visited"));
-  SPM_TRY(dmalloc(g, (void**)&w->cnt, sizeof(unsigned int), "bc counter"));
+  SPM_TRY(dmalloc(g, (void**)&w->ctl, sizeof(BcCtl), "bc control"));
   SPM_TRY(dmalloc(g, (void**)&w->srcbuf, sizeof(int32_t) * kBatch, "bc sources"));
   return SPM_OK;
 }
 
-// Makes room for `levels` mask levels and `entries` list entries (grow only;
-// contents are kept).
+// Makes room for `levels` mask levels, `entries` list entries and the list
+// bounds of `levels` levels (grow only; contents are kept).
 static int bc_reserve(spm_graph* g, size_t levels, size_t entries) {
   BcWork* w = g->bc;
   const size_t V = size_t(g->V);
+  if (levels + 2 > w->lstart_cap) {
+    const size_t nc = std::max(levels + 2, w->lstart_cap * 2);
+    unsigned int* p = nullptr;
+    SPM_TRY(dmalloc(g, (void**)&p, sizeof(unsigned int) * nc, "bc level bounds"));
+    if (w->lstart) {
+      SPM_CHECK_HIP(hipMemcpyAsync(p, w->lstart, sizeof(unsigned int) * w->lstart_cap, hipMemcpyDeviceToDevice,
+                                   g->stream));
+      SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+      dfree(g, w->lstart, sizeof(unsigned int) * w->lstart_cap);
+    }
+    w->lstart = p;
+    w->lstart_cap = nc;
+  }
   if (levels > w->mask_levels) {
     const size_t nl = std::max(levels, w->mask_levels * 2);
     unsigned long long* p = nullptr;
@@ -396,7 +480,8 @@ int bc_release(spm_graph* g) {
   dfree(g, w->visited, sizeof(unsigned long long) * V);
   dfree(g, w->masks, sizeof(unsigned long long) * V * w->mask_levels);
   dfree(g, w->lists, sizeof(int32_t) * w->list_cap);
-  dfree(g, w->cnt, sizeof(unsigned int));
+  dfree(g, w->ctl, sizeof(BcCtl));
+  dfree(g, w->lstart, sizeof(unsigned int) * w->lstart_cap);
   dfree(g, w->srcbuf, sizeof(int32_t) * kBatch);
   delete w;
   g->bc = nullptr;
@@ -404,9 +489,16 @@ int bc_release(spm_graph* g) {
 }
 
 // Forward pass of one batch; returns the deepest non-empty level and the list
-// start of every level (lstart[d] .. lstart[d+1]).
+// start of every level (lstart[d] .. lstart[d+1]). Levels run in windows of
+// K launches-worth (4, doubling to 32): each level is five launches -- clear
+// (top-down levels), push or scan (the one not chosen returns at once), the
+// level's end (one thread: next list bound, next mode, or done), sigma --
+// reading their list bounds and mode from device memory, and the host reads
+// the control block once per window. Levels after the last one return at
+// once. (One read-back per level costs a round trip per level on
+// high-diameter graphs: ~2,000 levels on a 1024^2 grid.)
 static int bc_forward(spm_graph* g, const int32_t* h_src, int nb, int* levels_out,
-                      std::vector<unsigned int>& lstart) {
+                      std::vector<unsigned int>& lstart, bool* wide = nullptr) {
   BcWork* w = g->bc;
   const int32_t V = g->V;
   const size_t Vs = size_t(V);
@@ -414,59 +506,70 @@ static int bc_forward(spm_graph* g, const int32_t* h_src, int nb, int* levels_ou
   SPM_CHECK_HIP(hipMemcpyAsync(w->srcbuf, h_src, sizeof(int32_t) * nb, hipMemcpyHostToDevice, g->stream));
   const int igrid = int(std::min<int64_t>((V + 255) / 256, 4096));
   SPM_TRY(launch(g, "bc_init", bc_init, dim3(igrid), dim3(256), 0, V, w->visited, w->masks,
-                 (const int32_t*)w->srcbuf, nb, w->lists, w->cnt));
+                 (const int32_t*)w->srcbuf, nb, w->lists, w->ctl, w->lstart));
   SPM_TRY(launch(g, "bc_seed", bc_seed, dim3(1), dim3(kBatch), 0, (const int32_t*)w->srcbuf, nb,
                  w->visited, w->masks, w->sigma));
   const unsigned long long batchmask = nb == 64 ? ~0ull : ((1ull << nb) - 1ull);
-  lstart.assign(1, 0u);
-  lstart.push_back(unsigned(nb));
-  unsigned int* h = reinterpret_cast<unsigned int*>(g->h_pinned);
+  // pinned read-back slots: the cursor, then done, levels, push, wide
+  unsigned int* hc = reinterpret_cast<unsigned int*>(g->h_pinned);
+  int* hf = g->h_pinned + 4;
   const int sgrid = int(std::min<int64_t>((V + kBcBlock - 1) / kBcBlock, int64_t(g->num_cus) * 16));
   const int wgrid = g->num_cus * 8;
-  int d = 0;
-  for (;; ++d) {
-    const unsigned int used = lstart.back();
-    SPM_TRY(bc_reserve(g, size_t(d) + 2, size_t(used) + Vs));
-    unsigned long long* cur = w->masks + size_t(d) * Vs;
-    unsigned long long* nxt = w->masks + size_t(d + 1) * Vs;
-    const unsigned int fb = lstart[lstart.size() - 2], nf = used - fb;  // level d's list
-    if (uint64_t(nf) * kPushRatio < uint64_t(V)) {
-      // Small frontier: push along its out-edges instead of scanning every
-      // vertex's in-edges.
-      SPM_CHECK_HIP(hipMemsetAsync(nxt, 0, sizeof(unsigned long long) * Vs, g->stream));
-      SPM_TRY(launch(g, "bc_push", bc_push, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + fb), nf,
+  unsigned int used = unsigned(nb);  // list entries appended so far (as of the last read)
+  int d = 0, K = 4;
+  for (;;) {
+    // Room for the window: K more levels, each appending at most V entries.
+    SPM_TRY(bc_reserve(g, size_t(d) + size_t(K) + 2, size_t(used) + size_t(K) * Vs + kBatch));
+    for (int k = 0; k < K; ++k, ++d) {
+      unsigned long long* cur = w->masks + size_t(d) * Vs;
+      unsigned long long* nxt = w->masks + size_t(d + 1) * Vs;
+      SPM_TRY(launch(g, "bc_clear", bc_clear_if_push, dim3(igrid), dim3(256), 0, V, nxt, (const BcCtl*)w->ctl));
+      SPM_TRY(launch(g, "bc_push", bc_push, dim3(wgrid), dim3(kBcBlock), 0, (const unsigned int*)w->lstart, d,
                      (const int32_t*)g->off, (const int32_t*)g->adj, (const unsigned long long*)cur, nxt,
-                     w->visited, w->lists, w->cnt));
-    } else {
+                     w->visited, w->lists, w->ctl));
       SPM_TRY(launch(g, "bc_scan", bc_scan, dim3(sgrid), dim3(kBcBlock), 0, V, (const int32_t*)g->roff,
                      (const int32_t*)g->radj, (const unsigned long long*)cur, nxt, w->visited, batchmask,
-                     w->lists, w->cnt));
+                     w->lists, w->ctl));
+      SPM_TRY(launch(g, "bc_level_end", bc_level_end, dim3(1), dim3(1), 0, V, d, w->ctl, w->lstart));
+      SPM_TRY(launch(g, "bc_sigma", bc_sigma, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)w->lists,
+                     (const unsigned int*)w->lstart, d, w->ctl, (const int32_t*)g->roff,
+                     (const int32_t*)g->radj, (const unsigned long long*)cur, (const unsigned long long*)nxt,
+                     w->sigma));
     }
-    SPM_CHECK_HIP(hipMemcpyAsync(h, w->cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipMemcpyAsync(hc, &w->ctl->cnt, sizeof(unsigned int), hipMemcpyDeviceToHost, g->stream));
+    SPM_CHECK_HIP(hipMemcpyAsync(hf, &w->ctl->done, 4 * sizeof(int), hipMemcpyDeviceToHost, g->stream));
     SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
-    const unsigned int total = h[0];
-    if (total == used) break;  // no vertex reached at level d+1
-    lstart.push_back(total);
-    SPM_TRY(launch(g, "bc_sigma", bc_sigma, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + used),
-                   total - used, (const int32_t*)g->roff, (const int32_t*)g->radj,
-                   (const unsigned long long*)cur, (const unsigned long long*)nxt, w->sigma));
+    used = *hc;
+    if (hf[0]) break;
+    K = std::min(2 * K, 32);
   }
-  *levels_out = d;  // levels 0..d are non-empty; masks[d+1] is all zero
+  const int L = hf[1];  // levels 0..L are non-empty; masks[L+1] is all zero
+  if (wide) *wide = hf[3] != 0;
+  lstart.resize(size_t(L) + 2);
+  SPM_CHECK_HIP(hipMemcpyAsync(lstart.data(), w->lstart, sizeof(unsigned int) * (size_t(L) + 2),
+                               hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  *levels_out = L;
   return SPM_OK;
 }
 
-static int bc_backward(spm_graph* g, int L, const std::vector<unsigned int>& lstart, float* d_BC) {
+static int bc_backward(spm_graph* g, int L, const std::vector<unsigned int>& lstart, bool wide, float* d_BC) {
   BcWork* w = g->bc;
   const size_t Vs = size_t(g->V);
   const int wgrid = g->num_cus * 8;
   for (int d = L; d >= 1; --d) {
     const unsigned int b = lstart[d], e = lstart[d + 1];
     if (e == b) continue;
-    SPM_TRY(launch(g, "bc_back", bc_back, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + b), e - b,
-                   (const int32_t*)g->off, (const int32_t*)g->adj,
-                   (const unsigned long long*)(w->masks + size_t(d) * Vs),
-                   (const unsigned long long*)(w->masks + size_t(d + 1) * Vs), (const double*)w->sigma, w->coef,
-                   d_BC));
+    const unsigned long long* md = w->masks + size_t(d) * Vs;
+    const unsigned long long* md1 = w->masks + size_t(d + 1) * Vs;
+    if (wide)
+      SPM_TRY(launch(g, "bc_back", bc_back<double>, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + b),
+                     e - b, (const int32_t*)g->off, (const int32_t*)g->adj, md, md1, (const double*)w->sigma,
+                     w->sigma, d_BC));
+    else
+      SPM_TRY(launch(g, "bc_back", bc_back<float>, dim3(wgrid), dim3(kBcBlock), 0, (const int32_t*)(w->lists + b),
+                     e - b, (const int32_t*)g->off, (const int32_t*)g->adj, md, md1, (const double*)w->sigma,
+                     w->coef, d_BC));
   }
   return SPM_OK;
 }
@@ -493,8 +596,9 @@ int bc_run(spm_graph* g, const int32_t* sources, int32_t nsrc, float* d_BC) {
   for (size_t b = 0; b < s.size(); b += kBatch) {
     const int nb = int(std::min<size_t>(kBatch, s.size() - b));
     int L = 0;
-    SPM_TRY(bc_forward(g, s.data() + b, nb, &L, lstart));
-    SPM_TRY(bc_backward(g, L, lstart, d_BC));
+    bool wide = false;
+    SPM_TRY(bc_forward(g, s.data() + b, nb, &L, lstart, &wide));
+    SPM_TRY(bc_backward(g, L, lstart, wide, d_BC));
   }
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   SPM_TRY(prof_flush(g));

@@ -259,43 +259,6 @@ __global__ __launch_bounds__(kDynBlock) void dyn_emit(int32_t V, const int32_t* 
   }
 }
 
-// adj[i] = low half of keys[i]; w[i] = vals[i]; and the reverse pairs
-// (neighbour << 32 | vertex, weight).
-__global__ void dyn_unpack(int64_t n, const unsigned long long* __restrict__ keys, const int32_t* __restrict__ vals,
-                           int32_t* __restrict__ adj, int32_t* __restrict__ w,
-                           unsigned long long* __restrict__ rkeys, int32_t* __restrict__ rvals) {
-  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const unsigned long long k = keys[i];
-    const uint32_t v = uint32_t(k >> 32), x = uint32_t(k);
-    adj[i] = int32_t(x);
-    if (w) w[i] = vals[i];
-    if (rkeys) {
-      rkeys[i] = (static_cast<unsigned long long>(x) << 32) | v;
-      rvals[i] = vals[i];
-    }
-  }
-}
-
-// Row offsets of sorted keys (row id in the high half): off[x] = first i
-// whose row is >= x.
-__global__ void dyn_segoff(int64_t n, const unsigned long long* __restrict__ keys, int32_t V,
-                           int32_t* __restrict__ off) {
-  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const int64_t r = int64_t(keys[i] >> 32);
-    const int64_t p = i == 0 ? -1 : int64_t(keys[i - 1] >> 32);
-    for (int64_t x = p + 1; x <= r; ++x) off[x] = int32_t(i);
-    if (i == n - 1)
-      for (int64_t x = r + 1; x <= V; ++x) off[x] = int32_t(n);
-  }
-}
-
-__global__ void dyn_fill_i32(int64_t n, int32_t* __restrict__ a, int32_t val) {
-  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) a[i] = val;
-}
-
 // Weight of every reverse-CSR slot, from the CSR: the k-th copy of u in v's
 // in-row gets the weight of the k-th copy of v in u's out-row. One wavefront
 // per vertex u, lanes over its out-edges.
@@ -1157,9 +1120,15 @@ static int dyn_fold(spm_graph* g) {
   SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, w->cnt, noff, V + 1, g->stream));
   SPM_TRY(tmp_reserve(g, need));
   SPM_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(w->tmp, need, w->cnt, noff, V + 1, g->stream));
-  int32_t E2 = 0;
-  SPM_CHECK_HIP(hipMemcpyAsync(&E2, noff + V, sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+  // the largest new row length into cnt[V] (the scan has consumed it)
+  SPM_CHECK_HIP(hipcub::DeviceReduce::Max(nullptr, need, w->cnt, w->cnt + V, V, g->stream));
+  SPM_TRY(tmp_reserve(g, need));
+  SPM_CHECK_HIP(hipcub::DeviceReduce::Max(w->tmp, need, w->cnt, w->cnt + V, V, g->stream));
+  int32_t* h = g->h_pinned;
+  SPM_CHECK_HIP(hipMemcpyAsync(h, noff + V, sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(h + 1, w->cnt + V, sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
+  const int32_t E2 = h[0], maxdeg = h[1];
   const size_t En = size_t(E2);
   // pairs, sorted by (vertex, neighbour); the sort is stable, so copies of
   // an edge keep CSR-before-overflow order
@@ -1175,53 +1144,14 @@ static int dyn_fold(spm_graph* g) {
   if (rc == SPM_OK && g->roff) rc = dmalloc(g, (void**)&nroff, sizeof(int32_t) * (Vs + 1), "rev_indexofNodes");
   if (rc == SPM_OK && g->roff) rc = dmalloc(g, (void**)&nradj, sizeof(int32_t) * En, "srcList");
   if (rc == SPM_OK && g->roff && g->w) rc = dmalloc(g, (void**)&nrw, sizeof(int32_t) * En, "reverse weights");
-  int bits = 1;
-  while ((int64_t(1) << bits) <= int64_t(V)) ++bits;
-  if (rc == SPM_OK && En > 0) {
+  // live entries as (vertex << 32 | neighbour, weight) pairs, then the same
+  // sorted CSR / reverse CSR build the relabeled layout uses (the sort is
+  // stable, so copies of an edge keep CSR-before-overflow order)
+  if (rc == SPM_OK && En > 0)
     rc = launch(g, "dyn_emit", dyn_emit, dim3(grid_for(g, int64_t(V) * kWave)), dim3(kDynBlock), 0, V,
                 (const int32_t*)g->off, (const int32_t*)g->adj, (const int32_t*)g->w, (const uint32_t*)w->dead, out,
                 (const int32_t*)noff, k0, v0);
-    if (rc == SPM_OK) {
-      hipcub::DoubleBuffer<unsigned long long> kb(k0, k1);
-      hipcub::DoubleBuffer<int32_t> vb(v0, v1);
-      size_t sneed = 0;
-      hipError_t e = hipcub::DeviceRadixSort::SortPairs(nullptr, sneed, kb, vb, int(En), 0, 32 + bits, g->stream);
-      if (e == hipSuccess) rc = tmp_reserve(g, sneed);
-      else rc = set_error(SPM_EHIP, std::string("fold sort: ") + hipGetErrorString(e));
-      if (rc == SPM_OK) {
-        e = hipcub::DeviceRadixSort::SortPairs(w->tmp, sneed, kb, vb, int(En), 0, 32 + bits, g->stream);
-        if (e != hipSuccess) rc = set_error(SPM_EHIP, std::string("fold sort: ") + hipGetErrorString(e));
-      }
-      // reverse pairs go to the other buffers
-      unsigned long long* rk = kb.Current() == k0 ? k1 : k0;
-      int32_t* rv = vb.Current() == v0 ? v1 : v0;
-      if (rc == SPM_OK)
-        rc = launch(g, "dyn_unpack", dyn_unpack, dim3(grid_for(g, int64_t(En))), dim3(kDynBlock), 0, int64_t(En),
-                    (const unsigned long long*)kb.Current(), (const int32_t*)vb.Current(), nadj, nw,
-                    g->roff ? rk : (unsigned long long*)nullptr, g->roff ? rv : (int32_t*)nullptr);
-      if (rc == SPM_OK && g->roff) {
-        hipcub::DoubleBuffer<unsigned long long> rkb(rk, kb.Current());
-        hipcub::DoubleBuffer<int32_t> rvb(rv, vb.Current());
-        e = hipcub::DeviceRadixSort::SortPairs(nullptr, sneed, rkb, rvb, int(En), 0, 32 + bits, g->stream);
-        if (e == hipSuccess) rc = tmp_reserve(g, sneed);
-        else rc = set_error(SPM_EHIP, std::string("fold sort: ") + hipGetErrorString(e));
-        if (rc == SPM_OK) {
-          e = hipcub::DeviceRadixSort::SortPairs(w->tmp, sneed, rkb, rvb, int(En), 0, 32 + bits, g->stream);
-          if (e != hipSuccess) rc = set_error(SPM_EHIP, std::string("fold sort: ") + hipGetErrorString(e));
-        }
-        if (rc == SPM_OK)
-          rc = launch(g, "dyn_unpack", dyn_unpack, dim3(grid_for(g, int64_t(En))), dim3(kDynBlock), 0, int64_t(En),
-                      (const unsigned long long*)rkb.Current(), (const int32_t*)rvb.Current(), nradj, nrw,
-                      (unsigned long long*)nullptr, (int32_t*)nullptr);
-        if (rc == SPM_OK)
-          rc = launch(g, "dyn_segoff", dyn_segoff, dim3(grid_for(g, int64_t(En))), dim3(kDynBlock), 0, int64_t(En),
-                      (const unsigned long long*)rkb.Current(), V, nroff);
-      }
-    }
-  } else if (rc == SPM_OK && g->roff) {
-    rc = launch(g, "dyn_fill_i32", dyn_fill_i32, dim3(grid_for(g, int64_t(V) + 1)), dim3(kDynBlock), 0,
-                int64_t(V) + 1, nroff, 0);
-  }
+  if (rc == SPM_OK) rc = csr_from_pairs(g, int64_t(En), k0, k1, v0, v1, noff, nadj, nw, nroff, nradj, nrw);
   if (rc == SPM_OK) {
     hipError_t e = hipStreamSynchronize(g->stream);
     if (e != hipSuccess) rc = set_error(SPM_EHIP, std::string("fold: ") + hipGetErrorString(e));
@@ -1260,6 +1190,8 @@ static int dyn_fold(spm_graph* g) {
   w->rw = nrw;
   w->rw_ok = nrw != nullptr;
   g->E = int64_t(E2);
+  g->max_outdeg = maxdeg;
+  schedule_after_fold(g);
   SPM_TRY(dyn_reset_base(g));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   return SPM_OK;

@@ -169,6 +169,10 @@ static int csr_check(int32_t V, const int32_t* off, const int32_t* adj) {
 
 namespace spm {
 
+void schedule_after_fold(spm_graph* g) {
+  if (g->sched_default) heuristic_schedule(g, &g->sched);
+}
+
 __device__ __forceinline__ unsigned long long pair_hash(uint32_t u, uint32_t v) {
   // splitmix64 finaliser of the packed pair (Steele, Lea, Flood 2014)
   unsigned long long z = (static_cast<unsigned long long>(u) << 32 | v) + 0x9E3779B97F4A7C15ull;
@@ -357,6 +361,7 @@ int spm_set_schedule(spm_graph* g, const spm_schedule* s) {
   if (pr_changed) pr_release(g);
   if (sssp_changed) sssp_release(g);
   g->sched = *s;
+  g->sched_default = false;
   return SPM_OK;
 }
 

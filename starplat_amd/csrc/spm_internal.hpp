@@ -75,7 +75,8 @@ struct spm_graph {
   spm::BcWork* bc = nullptr;
   spm::DynWork* dyn = nullptr;  // edge updates since the CSR was last clean
   spm::Relabel* rl = nullptr;   // degree-ordered copy of the CSR (PR, SSSP)
-  int32_t max_outdeg = 0;       // of the CSR given at creation
+  int32_t max_outdeg = 0;       // of the current clean CSR (creation, then each fold)
+  bool sched_default = true;    // the schedule is still the heuristic one (no spm_set_schedule)
   spm::Profiler prof;
   int32_t* h_pinned = nullptr;  // small pinned scratch for flag readback (16 ints)
 };
@@ -112,6 +113,9 @@ inline int launch(spm_graph* g, const char* name, Kernel k, dim3 grid, dim3 bloc
 }
 
 int pr_release(spm_graph* g);
+// After the CSR changed (a fold): a handle still on the heuristic schedule
+// gets the heuristic of the new CSR (its size and max out-degree).
+void schedule_after_fold(spm_graph* g);
 // Static triangle count (Compute_TC) of the handle's graph, updates folded first.
 int tc_run(spm_graph* g, uint64_t* out);
 int sssp_release(spm_graph* g);

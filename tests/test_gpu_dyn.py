@@ -115,6 +115,30 @@ def test_static_algorithms_after_updates(oracle):
     assert np.max(np.abs(pr - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
+
+def test_fold_refreshes_degree_and_default_schedule(oracle):
+    """The fold rebuilds the CSR with the same sorted-pairs builder as the
+    relabeled layout and refreshes the handle's max out-degree: a handle on
+    the heuristic schedule gets the heuristic of the folded graph (here the
+    relabel decision, V >= 2^20 and max out-degree >= 64x the mean, flips
+    once a hub appears), a handle whose schedule was set keeps it; SSSP on
+    the folded graph matches Dijkstra either way."""
+    g = graph.erdos_renyi(1 << 20, 1 << 21, seed=3, weights=(1, 9))
+    hub_dst = np.arange(1, 401, dtype=np.int32) * 997
+    for explicit in (False, True):
+        dev = g.device()
+        assert dev.schedule().relabel == 0
+        if explicit:
+            dev.set_schedule(relabel=0)
+        dev.add_edges(np.zeros(len(hub_dst), np.int32), hub_dst, np.full(len(hub_dst), 5, np.int32))
+        dev.compact()
+        assert dev.schedule().relabel == (0 if explicit else 1)
+        d = dev.sssp(0)
+        off, adj, w, roff, radj = dev.export()
+        assert off[1] - off[0] == g.indexofNodes[1] - g.indexofNodes[0] + len(hub_dst)
+        np.testing.assert_array_equal(d, oracle.dijkstra(_G(g.V, off, adj, w), w, 0))
+        g.release_device()
+
 @pytest.mark.parametrize("symmetric", [False, True])
 @pytest.mark.parametrize("batch", [1, 64, 5000])
 def test_dyn_sssp_matches_oracle(oracle, symmetric, batch):

@@ -440,12 +440,19 @@ def test_tc_closed_forms_and_directed(oracle):
 
 
 # ------------------------------------------------------------------ BC
-@pytest.mark.parametrize("kind", ["rmat_dir", "rmat_und", "er"])
+@pytest.mark.parametrize("kind", ["rmat_dir", "rmat_und", "er", "grid", "path"])
 def test_bc_paths_bit_exact(oracle, kind):
+    """Levels and path counts of single sources; the grid (up to 178 levels)
+    and the path (2,999) run the forward pass over many windows of levels."""
     if kind == "rmat_dir":
         g = graph.rmat(12, 8, seed=1)
     elif kind == "rmat_und":
         g = graph.rmat(12, 8, seed=2, undirected=True)
+    elif kind == "grid":
+        g = graph.grid2d(80, 100)
+    elif kind == "path":
+        n = 3000
+        g = graph.from_edges(np.arange(n - 1, dtype=np.int32), np.arange(1, n, dtype=np.int32))
     else:
         g = graph.erdos_renyi(10000, 40000, seed=3)
     for s in (0, 1, g.V // 2):
@@ -459,6 +466,18 @@ def test_bc_paths_bit_exact(oracle, kind):
 def test_bc_parity(oracle, nsrc):
     g = graph.erdos_renyi(8000, 32000, seed=4)
     src = sources(g.V, nsrc, seed=4)
+    ref, _, _ = oracle.compute_bc(g, src)
+    bc = np.empty(g.V, np.float32)
+    Compute_BC(g, bc, set(src.tolist()))
+    assert rel_linf(bc, ref) <= BC_TOL, rel_linf(bc, ref)
+
+
+@pytest.mark.parametrize("nsrc", [3, 64, 100])
+def test_bc_parity_high_diameter(oracle, nsrc):
+    """A 90 x 110 grid (up to 198 levels per batch: the level windows grow
+    4, 8, 16, 32 and the levels after the last return at once)."""
+    g = graph.grid2d(90, 110)
+    src = sources(g.V, nsrc, seed=9)
     ref, _, _ = oracle.compute_bc(g, src)
     bc = np.empty(g.V, np.float32)
     Compute_BC(g, bc, set(src.tolist()))

@@ -100,6 +100,31 @@ def roofline(bytes_per_run, kms, launches, workload, kernels):
     return out
 
 
+def copy_bandwidth(mib=1024, reps=10):
+    """A STREAM-style check of the HBM peak the rooflines are priced against:
+    device-to-device copies of `mib` MiB (bytes read + bytes written per copy
+    over the time of `reps` copies, events on the null stream)."""
+    n = mib << 20
+    a = hip.DeviceArray(n // 4, np.int32)
+    b = hip.DeviceArray(n // 4, np.int32)
+    try:
+        hip._chk("hipMemset", hip.lib().hipMemset(a.ptr, 1, n))
+        hip._chk("hipMemcpy", hip.lib().hipMemcpy(b.ptr, a.ptr, n, hip.D2D))
+        hip.synchronize()
+        t0, t1 = hip.Event(), hip.Event()
+        t0.record(None)
+        for _ in range(reps):
+            hip._chk("hipMemcpyAsync", hip.lib().hipMemcpyAsync(b.ptr, a.ptr, n, hip.D2D, None))
+        t1.record(None)
+        t1.synchronize()
+        ms = t0.elapsed_ms(t1)
+    finally:
+        a.free()
+        b.free()
+    return {"bytes_per_copy": 2 * n, "copies": reps, "GBps": round(2 * n * reps / (ms * 1e-3) / 1e9, 1),
+            "frac_of_peak": round(2 * n * reps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3)}
+
+
 def host_cores():
     try:
         import subprocess
@@ -529,7 +554,7 @@ def bench_schedule(args, stash):
 
     res = mixed_batch.run(n=args.sched_graphs, held_out=args.sched_held_out, seed=5, epochs=400, log=log,
                           on_held_out=keep, batched=args.sched_batched, **ranges)
-    res["workload"] = ("mixed batch (BASELINE config 4): RMAT (scale %d-%d, EF 4-32), Erdos-Renyi (2^%d-2^%d vertices, "
+    res["workload"] = ("mixed batch: RMAT (scale %d-%d, EF 4-32), Erdos-Renyi (2^%d-2^%d vertices, "
                        "avg degree 4-16) and 2-D grids (%d^2-%d^2), each with one SSSP (weights U[1,100], src 0) or "
                        "PageRank job (beta 1e-10, delta 0.85, maxIter 100), seed 5; 8 schedule classes timed per job; "
                        "MLP 16-64-64-8 trained on the GPU; makespan = sum of held-out job times"
@@ -763,6 +788,7 @@ def main():
 
     if not args.no_warm_up:
         warm_library()
+    copy_bw = copy_bandwidth() if not args.no_warm_up else None
     algos = args.algos.split(",")
     per = {}
     head = None
@@ -774,7 +800,7 @@ def main():
         if "pr" in algos:
             per["pr"] = bench_pr(args, stash, "pr", "PageRank at RMAT-22")
         if "pr_c0" in algos:
-            per["pr_c0"] = bench_pr(args, stash, "pr_c0", "PageRank config C0")
+            per["pr_c0"] = bench_pr(args, stash, "pr_c0", "PageRank at RMAT-16 (65,536 vertices)")
         if "tc" in algos:
             per["tc"] = bench_tc(args, stash)
         if "bc" in algos:
@@ -816,6 +842,7 @@ def main():
         "roofline": head["roofline"],
         "cpu_baseline": head.get("cpu_baseline"),
         "host": cpu["host"] if cpu else host_cores(),
+        "hbm_copy": copy_bw,
         "per_algo": per,
     }
     if sched is not None:

@@ -48,6 +48,8 @@ struct PrBins {
   int32_t nb[4];   // vertices per bin
   int32_t blk[5];  // first workgroup of each bin; blk[4] = grid size
   int32_t group;   // lanes per vertex in bin 1
+  int32_t zb;      // bin 0 starts with the in-degree-0 vertices: its first zb
+                   // workgroups hold only those (0 for batched jobs)
 };
 
 // Batched jobs (spm_pr_multi_device): vertex v belongs to job job[v], each
@@ -346,6 +348,12 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
   const int4 row = pr_row<G>(rows, blockIdx.x, bins);
   const PrState s = *st;
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  if (!MULTI && int(blockIdx.x) < bins.zb && s.iter >= 2) {
+    // In-degree-0 vertices only: from the third sweep on their rank is base
+    // in both buffers and |new - old| = 0, so the workgroup's share is 0.
+    if (sg.seg == sg.S - 1 && threadIdx.x == 0) partial[blockIdx.x] = 0.f;
+    return;
+  }
   float mydiff = 0.f;
   pr_sweep_block<G, MULTI>(blockIdx.x, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb, s_red,
                            mydiff);
@@ -425,6 +433,10 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter_lag(
       return;
     }
     __syncthreads();  // s_red is reused by the sweep
+  }
+  if (int(blockIdx.x) < bins.zb && s.iter + r >= 2) {  // in-degree-0 vertices only (see pr_iter)
+    if (threadIdx.x == 0) partial[blockIdx.x] = 0.f;
+    return;
   }
   const PrSeg sg{nullptr, nullptr, 1, 0};
   const PrJobs jb{};
@@ -632,40 +644,43 @@ struct PrThresh {
 constexpr int kBinBlock = 256;
 constexpr int kBinChunk = 4 * kBinBlock;  // vertices per binning workgroup
 
+// Placement categories: in-degree 0 first, then the rest of bin 0, bins 1-3;
+// category c > 0 is bin c - 1, and category 0 heads bin 0.
+constexpr int kCat = 5;
 __device__ __forceinline__ int pr_bin_of(int32_t d, const PrThresh& t) {
-  return d <= t.t1 ? 0 : d <= t.t2 ? 1 : d <= t.t3 ? 2 : 3;
+  return d == 0 ? 0 : d <= t.t1 ? 1 : d <= t.t2 ? 2 : d <= t.t3 ? 3 : 4;
 }
 
-// cnt[b * nblk + blk] = vertices of bin b among chunk blk's vertices.
+// cnt[c * nblk + blk] = vertices of category c among chunk blk's vertices.
 __global__ __launch_bounds__(kBinBlock) void pr_bin_count(int32_t V, const int32_t* __restrict__ roff, PrThresh t,
                                                           int32_t nblk, int32_t* __restrict__ cnt) {
-  __shared__ int32_t s_cnt[4];
-  if (threadIdx.x < 4) s_cnt[threadIdx.x] = 0;
+  __shared__ int32_t s_cnt[kCat];
+  if (threadIdx.x < kCat) s_cnt[threadIdx.x] = 0;
   __syncthreads();
-  int32_t mine[4] = {0, 0, 0, 0};
+  int32_t mine[kCat] = {0, 0, 0, 0, 0};
   const int64_t base = int64_t(blockIdx.x) * kBinChunk;
   for (int k = 0; k < kBinChunk / kBinBlock; ++k) {
     const int64_t v = base + k * kBinBlock + threadIdx.x;
     if (v < V) ++mine[pr_bin_of(roff[v + 1] - roff[v], t)];
   }
-  for (int b = 0; b < 4; ++b) {
+  for (int b = 0; b < kCat; ++b) {
     const int32_t w = wave_sum(mine[b]);
     if (__lane_id() == 0 && w) atomicAdd(&s_cnt[b], w);
   }
   __syncthreads();
-  if (threadIdx.x < 4) cnt[int64_t(threadIdx.x) * nblk + blockIdx.x] = s_cnt[threadIdx.x];
+  if (threadIdx.x < kCat) cnt[int64_t(threadIdx.x) * nblk + blockIdx.x] = s_cnt[threadIdx.x];
 }
 
-// One workgroup: cnt (bin-major) becomes the exclusive prefix over all
-// (bin, chunk) pairs, i.e. each chunk's first position within the bin order;
-// nb[b] receives the size of bin b.
+// One workgroup: cnt (category-major) becomes the exclusive prefix over all
+// (category, chunk) pairs, i.e. each chunk's first position within the
+// category order; nb[c] receives the size of category c.
 __global__ __launch_bounds__(1024) void pr_bin_scan(int32_t nblk, int32_t* __restrict__ cnt, int32_t* __restrict__ nb) {
   __shared__ int32_t s_wave[1024 / kWave];
   __shared__ int32_t s_carry;
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   if (tid == 0) s_carry = 0;
   __syncthreads();
-  const int64_t n = int64_t(4) * nblk;
+  const int64_t n = int64_t(kCat) * nblk;
   for (int64_t base = 0; base < n; base += 1024) {
     const int64_t i = base + tid;
     const int32_t x = i < n ? cnt[i] : 0;
@@ -683,23 +698,24 @@ __global__ __launch_bounds__(1024) void pr_bin_scan(int32_t nblk, int32_t* __res
     }
     __syncthreads();
   }
-  // bin sizes: start of the next bin minus start of this one
-  if (tid < 4) {
+  // category sizes: start of the next category minus start of this one
+  if (tid < kCat) {
     const int32_t a = cnt[int64_t(tid) * nblk];
-    const int32_t b = tid < 3 ? cnt[int64_t(tid + 1) * nblk] : s_carry;
+    const int32_t b = tid < kCat - 1 ? cnt[int64_t(tid + 1) * nblk] : s_carry;
     nb[tid] = b - a;
   }
 }
 
 // rows[pos] = {v, first in-edge, end, 0} with pos = the chunk's start in v's
-// bin plus v's rank among the chunk's earlier vertices of the same bin.
+// category plus v's rank among the chunk's earlier vertices of the same
+// category (so bin 0's rows are its in-degree-0 vertices, then the rest).
 __global__ __launch_bounds__(kBinBlock) void pr_bin_place(int32_t V, const int32_t* __restrict__ roff, PrThresh t,
                                                           int32_t nblk, const int32_t* __restrict__ start,
                                                           int4* __restrict__ rows) {
-  __shared__ int32_t s_next[4];
-  __shared__ int32_t s_wave[4][kBinBlock / kWave];
+  __shared__ int32_t s_next[kCat];
+  __shared__ int32_t s_wave[kCat][kBinBlock / kWave];
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
-  if (tid < 4) s_next[tid] = start[int64_t(tid) * nblk + blockIdx.x];
+  if (tid < kCat) s_next[tid] = start[int64_t(tid) * nblk + blockIdx.x];
   __syncthreads();
   const int64_t base = int64_t(blockIdx.x) * kBinChunk;
   const uint64_t below = (uint64_t(1) << lane) - 1;
@@ -713,7 +729,7 @@ __global__ __launch_bounds__(kBinBlock) void pr_bin_place(int32_t V, const int32
       b = pr_bin_of(z - a, t);
     }
     int32_t rank = 0;
-    for (int bb = 0; bb < 4; ++bb) {
+    for (int bb = 0; bb < kCat; ++bb) {
       const uint64_t m = __ballot(b == bb);
       if (b == bb) rank = __popcll(m & below);
       if (lane == 0) s_wave[bb][wid] = __popcll(m);
@@ -725,7 +741,7 @@ __global__ __launch_bounds__(kBinBlock) void pr_bin_place(int32_t V, const int32
       rows[pos] = make_int4(int32_t(v), a, z, 0);
     }
     __syncthreads();
-    if (tid < 4) {
+    if (tid < kCat) {
       int32_t tot = 0;
       for (int w = 0; w < kBinBlock / kWave; ++w) tot += s_wave[tid][w];
       s_next[tid] += tot;
@@ -735,22 +751,24 @@ __global__ __launch_bounds__(kBinBlock) void pr_bin_place(int32_t V, const int32
 }
 
 // Bins the in-rows of `roff` into `rows` (int4[V]) with the schedule's
-// thresholds; cnt is scratch of 4 * ceil(V / kBinChunk) + 4 ints. Reads the
-// four bin sizes back (one synchronisation) and fills the launch layout.
+// thresholds; cnt is scratch of kCat * ceil(V / kBinChunk) + kCat ints. Reads
+// the category sizes back (one synchronisation) and fills the launch layout.
 static int pr_bin_device(spm_graph* g, const int32_t* roff, int4* rows, int32_t* cnt, int32_t* h_nb, PrBins* out) {
   const spm_schedule& s = g->sched;
   const int32_t V = g->V;
   const int32_t nblk = int32_t((int64_t(V) + kBinChunk - 1) / kBinChunk);
   const PrThresh t{s.pr_t1, s.pr_t2, s.pr_t3};
-  int32_t* nb = cnt + int64_t(4) * nblk;
+  int32_t* nb = cnt + int64_t(kCat) * nblk;
   SPM_TRY(launch(g, "pr_bin_count", pr_bin_count, dim3(nblk), dim3(kBinBlock), 0, V, roff, t, nblk, cnt));
   SPM_TRY(launch(g, "pr_bin_scan", pr_bin_scan, dim3(1), dim3(1024), 0, nblk, cnt, nb));
   SPM_TRY(launch(g, "pr_bin_place", pr_bin_place, dim3(nblk), dim3(kBinBlock), 0, V, roff, t, nblk,
                  (const int32_t*)cnt, rows));
-  SPM_CHECK_HIP(hipMemcpyAsync(h_nb, nb, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+  SPM_CHECK_HIP(hipMemcpyAsync(h_nb, nb, kCat * sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   PrBins& b = *out;
-  for (int i = 0; i < 4; ++i) b.nb[i] = h_nb[i];
+  b.nb[0] = h_nb[0] + h_nb[1];
+  for (int i = 1; i < 4; ++i) b.nb[i] = h_nb[i + 1];
+  b.zb = h_nb[0] / kPrBlock;
   b.group = s.pr_group;
   b.blk[0] = 0;
   b.blk[1] = (b.nb[0] + kPrBlock - 1) / kPrBlock;
@@ -760,7 +778,9 @@ static int pr_bin_device(spm_graph* g, const int32_t* roff, int4* rows, int32_t*
   return SPM_OK;
 }
 
-static size_t pr_bin_scratch_ints(int32_t V) { return size_t(4) * size_t((int64_t(V) + kBinChunk - 1) / kBinChunk) + 4; }
+static size_t pr_bin_scratch_ints(int32_t V) {
+  return size_t(kCat) * size_t((int64_t(V) + kBinChunk - 1) / kBinChunk) + kCat;
+}
 
 // Ranks, contributions and in-degree bins over `view`. Batched jobs (multi)
 // use plain pull sweeps: no push accumulators, no source segments.
@@ -786,9 +806,12 @@ static int pr_build(spm_graph* g, PrWork* w, const CsrView& view, bool multi) {
     dfree(g, cnt, sizeof(int32_t) * nc);
     SPM_TRY(rc);
   }
-  const PrBins& b = w->bins;
   w->S = 1;
-  if (multi) return SPM_OK;
+  if (multi) {
+    w->bins.zb = 0;  // batched jobs sweep every vertex every iteration
+    return SPM_OK;
+  }
+  const PrBins& b = w->bins;
   w->partial_n = std::max({b.blk[4], g->num_cus * 8, 2 * kLagMaxGrid}) + 1;
   SPM_TRY(dmalloc(g, (void**)&w->partial, sizeof(float) * size_t(w->partial_n), "pr partial diffs"));
   // Source segments for cache blocking.
@@ -1154,7 +1177,8 @@ constexpr size_t kNaRows = na_round(sizeof(int4) * size_t(kNoAllocMaxV));
 // partial diffs: one per workgroup of a sweep (at most one per vertex plus the
 // three partial bins) or per push workgroup, plus one
 constexpr size_t kNaPartial = na_round(sizeof(float) * (size_t(kNoAllocMaxV) + 8192));
-constexpr size_t kNaCnt = na_round(sizeof(int32_t) * (size_t(4) * ((size_t(kNoAllocMaxV) + kBinChunk - 1) / kBinChunk) + 4));
+constexpr size_t kNaCnt =
+    na_round(sizeof(int32_t) * (size_t(kCat) * ((size_t(kNoAllocMaxV) + kBinChunk - 1) / kBinChunk) + kCat));
 constexpr size_t kNaState = na_round(sizeof(PrState));
 constexpr size_t kNoAllocScratchBytes = 5 * kNaFloats + kNaRows + kNaPartial + kNaCnt + kNaState;
 

@@ -21,6 +21,7 @@ def lib():
         _lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         _lib.hipFree.argtypes = [C.c_void_p]
         _lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _lib.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
         _lib.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         _lib.hipDeviceSynchronize.argtypes = []
         _lib.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]

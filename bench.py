@@ -125,6 +125,18 @@ def copy_bandwidth(mib=1024, reps=10):
             "frac_of_peak": round(2 * n * reps / (ms * 1e-3) / 1e9 / HBM_PEAK_GBPS, 3)}
 
 
+def gather_ceiling(n=1 << 22):
+    """Random 4-byte loads per second the GPU sustains over an array of n
+    floats (the size of RMAT-22's contribution / distance arrays): every
+    workgroup over the whole array, and each XCD over its own eighth."""
+    out = {"array_floats": n}
+    for key, local in (("uniform", 0), ("xcd_local_eighths", 1)):
+        r = C.c_double()
+        N.check("spm_gather_rate", N.device_lib().spm_gather_rate(n, local, C.byref(r)))
+        out[key + "_G_per_s"] = round(r.value / 1e9, 1)
+    return out
+
+
 def host_cores():
     try:
         import subprocess
@@ -258,6 +270,8 @@ def bench_pr(args, stash, leg, label):
            "gteps": g.E * it / (ms * 1e-3) / 1e9,
            "gteps_first_call": g.E * it / (first_ms * 1e-3) / 1e9,
            "pr_mode": int(dev.schedule().pr_mode),
+           # in-edge gathers (E per iteration) per second of sweep-kernel time
+           "gathers_G_per_s": round(g.E * it / (kms * 1e-3) / 1e9, 1) if kms > 0 else 0.0,
            "roofline": roofline(model * it, kms, launches, f"pr-rmat{scale}", [kname])}
     stash.put(leg, {"ranks": d_out.to_host()}, iterations=it)
     d_out.free()
@@ -789,6 +803,7 @@ def main():
     if not args.no_warm_up:
         warm_library()
     copy_bw = copy_bandwidth() if not args.no_warm_up else None
+    gathers = gather_ceiling() if not args.no_warm_up else None
     algos = args.algos.split(",")
     per = {}
     head = None
@@ -843,6 +858,7 @@ def main():
         "cpu_baseline": head.get("cpu_baseline"),
         "host": cpu["host"] if cpu else host_cores(),
         "hbm_copy": copy_bw,
+        "gather_ceiling": gathers,
         "per_algo": per,
     }
     if sched is not None:

@@ -284,13 +284,22 @@ int spm_mlp_train(spm_mlp* m, const float* x, const int32_t* y, int32_t n, int32
                   float lr, uint64_t seed, float* final_loss);
 
 /* ---- profiling -----------------------------------------------------------
- * With profiling on, each kernel launch is bracketed by HIP events on the
- * handle's stream and graphs are not used, so per-kernel device time can be
- * read back. Off by default. */
+ * With profiling on, each kernel launch carries start/stop events that the
+ * dispatch itself stamps (hipExtLaunchKernel) on the handle's stream and
+ * graphs are not used, so per-kernel device time can be read back. Off by
+ * default. */
 int spm_profile_enable(spm_graph* g, int enable);
 int spm_profile_reset(spm_graph* g);
 /* total_ms and launches of the named kernel since the last reset. */
 int spm_profile_query(spm_graph* g, const char* kernel, double* total_ms, int64_t* launches);
+/* No counterpart in the reference: the rate of random 4-byte loads the GPU
+ * sustains (loads per second into *rate), the ceiling of gather-bound
+ * kernels. 2^28 loads at uniformly random indices into an array of n floats
+ * (n >= 8), eight independent loads in flight per lane; with xcd_local the
+ * indices of workgroup b fall in the eighth (b mod 8) of the array, which
+ * keeps each eighth in one XCD's L2 (workgroups are dispatched to the 8 XCDs
+ * round-robin). */
+int spm_gather_rate(int32_t n, int32_t xcd_local, double* rate);
 
 #ifdef __cplusplus
 }

@@ -123,6 +123,7 @@ DEVICE_API = [
     ("spm_profile_enable", C.c_int, [vp, C.c_int]),
     ("spm_profile_reset", C.c_int, [vp]),
     ("spm_profile_query", C.c_int, [vp, C.c_char_p, P_f64, P_i64]),
+    ("spm_gather_rate", C.c_int, [i32, i32, P_f64]),
 ]
 
 HOST_API = [

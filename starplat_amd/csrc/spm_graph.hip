@@ -395,3 +395,68 @@ int spm_profile_query(spm_graph* g, const char* kernel, double* total_ms, int64_
 }
 
 }  // extern "C"
+
+// ---- gather-rate probe --------------------------------------------------------
+namespace spm {
+
+__device__ __forceinline__ uint32_t probe_mix(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+constexpr int kProbeGrid = 8192, kProbeIters = 16;
+
+template <bool LOCAL>
+__global__ __launch_bounds__(256) void gather_probe(const float* __restrict__ a, uint32_t n,
+                                                    float* __restrict__ out) {
+  const uint32_t tid = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t span = LOCAL ? n / 8u : n, lo = LOCAL ? (blockIdx.x % 8u) * span : 0u;
+  float s = 0.f;
+  for (int it = 0; it < kProbeIters; ++it) {
+    uint32_t idx[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) idx[k] = lo + probe_mix(tid * 977u + uint32_t(it * 8 + k) * 0x9E3779B9u) % span;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += a[idx[k]];
+  }
+  out[tid] = s;
+}
+
+}  // namespace spm
+
+extern "C" int spm_gather_rate(int32_t n, int32_t xcd_local, double* rate) {
+  if (n < 8 || !rate) return set_error(SPM_EINVAL, "spm_gather_rate: need n >= 8 and a result pointer");
+  float *a = nullptr, *out = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&a, sizeof(float) * size_t(n));
+  if (e == hipSuccess) e = hipMalloc((void**)&out, sizeof(float) * size_t(kProbeGrid) * 256);
+  if (e == hipSuccess) e = hipMemset(a, 0, sizeof(float) * size_t(n));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  float ms = 0.f;
+  for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {  // the first run warms the caches and the code object
+    e = hipEventRecord(e0, st);
+    if (xcd_local)
+      hipLaunchKernelGGL(gather_probe<true>, dim3(kProbeGrid), dim3(256), 0, st, (const float*)a, uint32_t(n), out);
+    else
+      hipLaunchKernelGGL(gather_probe<false>, dim3(kProbeGrid), dim3(256), 0, st, (const float*)a, uint32_t(n), out);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (out) (void)hipFree(out);
+  if (e != hipSuccess) return set_error(SPM_EHIP, std::string("spm_gather_rate: ") + hipGetErrorString(e));
+  *rate = double(kProbeGrid) * 256.0 * kProbeIters * 8.0 / (double(ms) * 1e-3);
+  return SPM_OK;
+}

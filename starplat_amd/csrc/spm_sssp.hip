@@ -12,10 +12,11 @@
 // splits the frontier's edges into chunks of kChunk edges; the pusher of a
 // vertex records, for every chunk that starts inside its edge range, the
 // queue position that chunk starts at. A workgroup then stages its chunk's
-// vertices (at most kChunk+1) in LDS and every lane locates its edge by a
-// binary search in LDS (lanes take consecutive edges, so loads stay
-// coalesced). Hubs of 10^5 edges are spread over ~100 workgroups, and every
-// relaxation lane does useful work. Frontier distances are read with
+// vertices (at most kChunk+1) in LDS, every entry marks the slot of its
+// first edge and a max-scan over the chunk's slots gives each edge its entry
+// (lanes take consecutive edges, so loads stay coalesced). Hubs of 10^5
+// edges are spread over ~100 workgroups, and every relaxation lane does
+// useful work. Frontier distances are read with
 // device-scope loads (a stale one could lose an improvement); the per-edge
 // test against the neighbour's distance is a cached pre-filter in front of
 // the authoritative atomicMin.
@@ -225,6 +226,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
   __shared__ uint32_t s_off[kChunk + 1];
   __shared__ int32_t s_dv[kChunk + 1];
   __shared__ int32_t s_base[kChunk + 1];
+  __shared__ int16_t s_own[kChunk];  // entry owning each edge slot of the chunk
+  __shared__ int32_t s_wmax[kSsspWaves];
   __shared__ uint32_t s_tot[2 * kSsspWaves];
   __shared__ unsigned long long s_res;
 
@@ -251,11 +254,46 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
     const int32_t p0 = qin.cfirst[chunk];
     const int32_t p1 = chunk + 1 < nchunks ? qin.cfirst[chunk + 1] : int32_t(n) - 1;
     const int cnt = p1 - p0 + 1;
+    const uint32_t x0 = chunk * kChunk;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) s_own[tid * kPer + q] = -1;
+    __syncthreads();
     for (int i = tid; i < cnt; i += kSsspBlock) {
       const int32_t v = qin.vtx[p0 + i];
-      s_off[i] = qin.eoff[p0 + i];
+      const uint32_t o = qin.eoff[p0 + i];
+      s_off[i] = o;
       s_dv[i] = __hip_atomic_load(&dist[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_base[i] = off[v];
+      // Each entry marks the chunk slot of its first edge (the chunk's first
+      // entry may start before it: slot 0; the last may start at the next
+      // chunk: no slot); entries have at least one edge, so marks never
+      // collide.
+      if (o < x0 + kChunk) s_own[o > x0 ? o - x0 : 0] = int16_t(i);
+    }
+    __syncthreads();
+    // Owner of every slot = the last mark at or before it: a max-scan over
+    // the chunk, four consecutive slots per lane (entries are ascending).
+    {
+      int m[kPer];
+      int run = -1;
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        run = max(run, int(s_own[tid * kPer + q]));
+        m[q] = run;
+      }
+      int incl = run;
+#pragma unroll
+      for (int d = 1; d < kWave; d <<= 1) {
+        const int y = __shfl_up(incl, d, kWave);
+        if (__lane_id() >= d) incl = max(incl, y);
+      }
+      if (__lane_id() == kWave - 1) s_wmax[tid / kWave] = incl;
+      int excl = __shfl_up(incl, 1, kWave);
+      if (__lane_id() == 0) excl = -1;
+      __syncthreads();
+      for (int w = 0; w < tid / kWave; ++w) excl = max(excl, s_wmax[w]);
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) s_own[tid * kPer + q] = int16_t(max(excl, m[q]));
     }
     __syncthreads();
 
@@ -266,23 +304,14 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
     uint32_t pc = 0, sd = 0;
     // Lanes take consecutive edges (edge x0 + k*256 + tid in step k), so each
     // load instruction of a wavefront reads one contiguous run of the
-    // frontier's edges. The entry owning edge x is found by binary search in
-    // LDS; entries only move forward with k, so each search starts at the
-    // previous step's entry.
-    const uint32_t x0 = chunk * kChunk;
-    int lo = 0;
+    // frontier's edges. The entry owning edge x is s_own[x - x0].
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const uint32_t x = x0 + k * kSsspBlock + tid;
       pv[k] = -1;
       pd[k] = 0;
       if (x < T) {
-        int hi = min(cnt - 1, lo + kSsspBlock);
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_off[mid] <= x) lo = mid;
-          else hi = mid - 1;
-        }
+        const int lo = s_own[k * kSsspBlock + tid];
         const int32_t e = s_base[lo] + int32_t(x - s_off[lo]);
         int32_t nb, wt;
         if (PACKED) {

@@ -259,9 +259,10 @@ def bench_pr(args, stash, leg, label):
     it = dev.pr_device(1e-10, 0.85, 100, d_out)  # includes the one-time layout (relabeled copy, bins)
     first_ms = (time.perf_counter() - t0) * 1e3
     ms = timed_steps(lambda: dev.pr_device(1e-10, 0.85, 100, d_out), max(1, args.steps // 2), 1)
-    # The sweep kernel: pr_iter, or pr_iter_lag where the schedule folds the
-    # loop test into the sweep (small graphs).
-    kt = kernel_time(dev, lambda: dev.pr_device(1e-10, 0.85, 100, d_out), ["pr_iter", "pr_iter_lag"], reps=1)
+    # The sweep kernel: pr_iter, pr_iter_lag where the schedule folds the
+    # loop test into the sweep (small graphs), or pr_iter_hot where it stages
+    # the lowest ids' contributions in LDS (relabeled graphs).
+    kt = kernel_time(dev, lambda: dev.pr_device(1e-10, 0.85, 100, d_out), ["pr_iter", "pr_iter_lag", "pr_iter_hot"], reps=1)
     kname = max(kt, key=lambda k: kt[k][1])
     kms, launches = kt[kname]
     res = {"workload": f"{label}: pull PageRank, directed RMAT scale {scale} EF16, beta 1e-10, delta 0.85, maxIter 100, seed 1",

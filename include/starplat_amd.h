@@ -248,7 +248,10 @@ typedef struct spm_schedule {
   int32_t pr_mode;        /* 0 pull over in-rows, 1 push (scatter + apply),
                              2 pull whose sweeps evaluate the loop test of
                              the sweep before (one launch per iteration;
-                             sweeps of <= 2048 workgroups, else as 0)     */
+                             sweeps of <= 2048 workgroups, else as 0),
+                             3 pull by persistent 1024-lane workgroups that
+                             read the contributions of the lowest ids (the
+                             highest out-degrees when relabeled) from LDS */
   int32_t pr_seg_log2;    /* pull: sweep sources in segments of 2^k vertices
                              so gathers stay in L2 (0 = one sweep; 10..30)   */
   int32_t relabel;        /* PR and SSSP on a copy of the CSR with vertices

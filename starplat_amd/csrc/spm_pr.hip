@@ -73,6 +73,7 @@ struct PrWork {
   int32_t* sp = nullptr;     // segment split points [V][S-1]
   size_t sp_n = 0;
   int S = 1;                 // source segments per pull sweep
+  bool hot = false;          // layout binned for the staged sweep (pr_iter_hot)
   int4* rows = nullptr;      // {v, in-row start, in-row end, 0} grouped by bin
   float* out = nullptr;      // [V] result in new ids before the gather back (relabeled runs)
   CsrView view{};            // the CSR the kernels run on
@@ -118,15 +119,16 @@ __device__ __forceinline__ void pr_store(int32_t v, float sum, const PrState& s,
 // vertices goes to partial[blockIdx.x] with a plain store. No workgroup
 // touches a shared counter: with ~10^5 workgroups per iteration at RMAT-22,
 // same-address device-scope atomics would serialise the whole iteration.
-__device__ __forceinline__ void pr_finish(float mydiff, float* __restrict__ partial, float* s_red) {
+template <int B = kPrBlock>
+__device__ __forceinline__ void pr_finish(float mydiff, float* __restrict__ partial, float* s_red, int b) {
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   mydiff = wave_sum(mydiff);
   if (lane == 0) s_red[wid] = mydiff;
   __syncthreads();
   if (tid == 0) {
     float t = 0.f;
-    for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
-    partial[blockIdx.x] = t;
+    for (int w = 0; w < B / kWave; ++w) t += s_red[w];
+    partial[b] = t;
   }
 }
 
@@ -235,8 +237,26 @@ __device__ __forceinline__ bool pr_live(int32_t v, const PrJobs& jb) {
 // gathers, so a lane keeps several memory requests in flight instead of a
 // chain of dependent ones (a lane with up to 8 terms waits for two round
 // trips in all).
+//
+// Staged variant (HOT): the contributions of sources [0, H) -- after the
+// degree-ordered relabeling, the sources with the most out-edges -- were
+// copied into LDS by the workgroup, and those terms are read there. The
+// values, and so the sums, are the same as the plain gather's.
+struct PrHot {
+  const float* lds;  // [H] contributions of the sources below H (LDS)
+  int32_t H;
+};
+
+template <bool HOT>
+__device__ __forceinline__ float pr_term(const float* __restrict__ c, const PrHot& hot, int32_t u) {
+  if (HOT && u < hot.H) return hot.lds[u];
+  return c[u];
+}
+
+template <bool HOT = false>
 __device__ __forceinline__ float gather_sum(const int32_t* __restrict__ radj, const float* __restrict__ c,
-                                            int32_t ea, int32_t eb, int32_t start, int32_t stride) {
+                                            int32_t ea, int32_t eb, int32_t start, int32_t stride,
+                                            const PrHot& hot = PrHot{nullptr, 0}) {
   float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
   int32_t e = ea + start;
   for (; e + 7 * stride < eb; e += 8 * stride) {
@@ -244,7 +264,7 @@ __device__ __forceinline__ float gather_sum(const int32_t* __restrict__ radj, co
 #pragma unroll
     for (int k = 0; k < 8; ++k) a[k] = radj[e + k * stride];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) s[k] += c[a[k]];
+    for (int k = 0; k < 8; ++k) s[k] += pr_term<HOT>(c, hot, a[k]);
   }
   // The remaining 0..7 terms, loaded together.
   int32_t a[7];
@@ -252,7 +272,7 @@ __device__ __forceinline__ float gather_sum(const int32_t* __restrict__ radj, co
   for (int k = 0; k < 7; ++k) a[k] = e + k * stride < eb ? radj[e + k * stride] : -1;
 #pragma unroll
   for (int k = 0; k < 7; ++k)
-    if (a[k] >= 0) s[k] += c[a[k]];
+    if (a[k] >= 0) s[k] += pr_term<HOT>(c, hot, a[k]);
   return ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
@@ -260,27 +280,29 @@ __device__ __forceinline__ float gather_sum(const int32_t* __restrict__ radj, co
 // order: one 16-byte load replaces the order -> offsets chain. pr_slot gives
 // the i that thread threadIdx.x of workgroup b reads (-1: none): bin 0 one
 // lane per vertex, bin 1 G lanes, bin 2 a wavefront, bin 3 the workgroup.
-template <int G>
+// B is the workgroup size the layout was binned for (kPrBlock, or kHotBlock
+// for the staged sweep).
+template <int G, int B = kPrBlock>
 __device__ __forceinline__ int64_t pr_slot(int b, const PrBins& bins) {
   const int tid = threadIdx.x;
   if (b < bins.blk[1]) {
-    const int64_t i = int64_t(b) * kPrBlock + tid;
+    const int64_t i = int64_t(b) * B + tid;
     return i < bins.nb[0] ? i : -1;
   }
   if (b < bins.blk[2]) {
-    const int64_t i = int64_t(b - bins.blk[1]) * (kPrBlock / G) + tid / G;
+    const int64_t i = int64_t(b - bins.blk[1]) * (B / G) + tid / G;
     return i < bins.nb[1] ? bins.nb[0] + i : -1;
   }
   if (b < bins.blk[3]) {
-    const int64_t i = int64_t(b - bins.blk[2]) * (kPrBlock / kWave) + tid / kWave;
+    const int64_t i = int64_t(b - bins.blk[2]) * (B / kWave) + tid / kWave;
     return i < bins.nb[2] ? int64_t(bins.nb[0]) + bins.nb[1] + i : -1;
   }
   return int64_t(bins.nb[0]) + bins.nb[1] + bins.nb[2] + (b - bins.blk[3]);
 }
 
-template <int G>
+template <int G, int B = kPrBlock>
 __device__ __forceinline__ int4 pr_row(const int4* __restrict__ rows, int b, const PrBins& bins) {
-  const int64_t i = pr_slot<G>(b, bins);
+  const int64_t i = pr_slot<G, B>(b, bins);
   return i >= 0 ? rows[i] : make_int4(-1, 0, 0, 0);
 }
 
@@ -289,13 +311,14 @@ __device__ __forceinline__ int4 pr_row(const int4* __restrict__ rows, int b, con
 // lanes each, of bin 2 a wavefront each, of bin 3 the whole workgroup. Adds
 // the workgroup's |new - old| to the calling thread's mydiff. Control flow
 // is uniform across the workgroup.
-template <int G, bool MULTI>
+template <int G, bool MULTI, int B = kPrBlock, bool HOT = false>
 __device__ __forceinline__ void pr_sweep_block(int b, int4 row, const int32_t* __restrict__ roff,
                                                const int32_t* __restrict__ radj, const float* __restrict__ degf,
                                                const PrBins& bins, const float* __restrict__ pr_in,
                                                const float* __restrict__ c_in, float* __restrict__ pr_out,
                                                float* __restrict__ c_out, const PrState& s, const PrSeg& sg,
-                                               const PrJobs& jb, float* s_red, float& mydiff) {
+                                               const PrJobs& jb, float* s_red, float& mydiff,
+                                               const PrHot& hot = PrHot{nullptr, 0}) {
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   int32_t v = row.x, ea = row.y, eb = row.z;
   if (v >= 0 && !pr_live<MULTI>(v, jb)) v = -1;
@@ -308,27 +331,28 @@ __device__ __forceinline__ void pr_sweep_block(int b, int4 row, const int32_t* _
   }
   if (b < bins.blk[1]) {
     // bin 0: one lane per vertex
-    if (v >= 0) pr_emit<MULTI>(v, gather_sum(radj, c_in, ea, eb, 0, 1), sg, s, old, dg, pr_out, c_out, mydiff, jb);
+    if (v >= 0)
+      pr_emit<MULTI>(v, gather_sum<HOT>(radj, c_in, ea, eb, 0, 1, hot), sg, s, old, dg, pr_out, c_out, mydiff, jb);
   } else if (b < bins.blk[2]) {
     // bin 1: G lanes per vertex
     const int g = tid % G;
-    float sum = v >= 0 ? gather_sum(radj, c_in, ea, eb, g, G) : 0.f;
+    float sum = v >= 0 ? gather_sum<HOT>(radj, c_in, ea, eb, g, G, hot) : 0.f;
     sum = group_sum<G>(sum);
     if (v >= 0 && g == 0) pr_emit<MULTI>(v, sum, sg, s, old, dg, pr_out, c_out, mydiff, jb);
   } else if (b < bins.blk[3]) {
     // bin 2: one wavefront per vertex
-    float sum = v >= 0 ? gather_sum(radj, c_in, ea, eb, lane, kWave) : 0.f;
+    float sum = v >= 0 ? gather_sum<HOT>(radj, c_in, ea, eb, lane, kWave, hot) : 0.f;
     sum = wave_sum(sum);
     if (v >= 0 && lane == 0) pr_emit<MULTI>(v, sum, sg, s, old, dg, pr_out, c_out, mydiff, jb);
   } else if (v >= 0) {
     // bin 3: one workgroup per vertex (v is the same in every lane)
-    float sum = gather_sum(radj, c_in, ea, eb, tid, kPrBlock);
+    float sum = gather_sum<HOT>(radj, c_in, ea, eb, tid, B, hot);
     sum = wave_sum(sum);
     if (lane == 0) s_red[wid] = sum;
     __syncthreads();
     if (tid == 0) {
       float t = 0.f;
-      for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
+      for (int w = 0; w < B / kWave; ++w) t += s_red[w];
       pr_emit<MULTI>(v, t, sg, s, old, dg, pr_out, c_out, mydiff, jb);
     }
     __syncthreads();
@@ -357,7 +381,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
   float mydiff = 0.f;
   pr_sweep_block<G, MULTI>(blockIdx.x, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb, s_red,
                            mydiff);
-  if (!MULTI && sg.seg == sg.S - 1) pr_finish(mydiff, partial, s_red);
+  if (!MULTI && sg.seg == sg.S - 1) pr_finish(mydiff, partial, s_red, blockIdx.x);
 }
 
 // The loop test folded into the next sweep (schedule pr_mode 2, grids of at
@@ -443,7 +467,56 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter_lag(
   float mydiff = 0.f;
   pr_sweep_block<G, false>(blockIdx.x, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb, s_red,
                            mydiff);
-  pr_finish(mydiff, partial, s_red);
+  pr_finish(mydiff, partial, s_red, blockIdx.x);
+}
+
+// ---- staged sweep (schedule pr_mode 3) ---------------------------------------
+// On the degree-ordered copy of a large skewed graph the sources with the most
+// out-edges have the smallest ids, and a few of them carry much of every
+// sweep's gathers (at RMAT-22 the first 40,832 sources hold 56% of the
+// in-edges). Their contributions are L2 hits that still cost an L1 miss each,
+// and L1 misses are what the gather-bound sweep waits on. The staged sweep runs
+// one persistent workgroup of kHotBlock lanes per CU; each first copies the
+// contributions of sources [0, H) into LDS (159.5 KiB of the CU's 160 KiB) and
+// then reads those terms there, so only the other terms leave the CU. The
+// layout is binned for kHotBlock-lane blocks (a "virtual block" per step); a
+// workgroup walks virtual blocks blockIdx.x, blockIdx.x + gridDim.x, ...
+// without a barrier between them (bin 3, a whole block per vertex, reduces
+// through LDS, so its threshold is set high), and the diff is summed per
+// workgroup at the end. Measured at RMAT-22 (100 iterations): 33.5 ms for the
+// plain sweep, 30.0 ms staging 20,416 sources in two workgroups per CU,
+// 28.4 ms staging 40,832 in one (bin 3 above 4096 in-edges).
+constexpr int kHotBlock = 1024;
+constexpr int kHotPerCu = 1;
+constexpr int kHotMax = 40832;
+
+template <int G>
+__global__ __launch_bounds__(kHotBlock) void pr_iter_hot(
+    const int32_t* __restrict__ roff, const int32_t* __restrict__ radj, const int4* __restrict__ rows,
+    const float* __restrict__ degf, PrBins bins, const float* __restrict__ pr_in, const float* __restrict__ c_in,
+    float* __restrict__ pr_out, float* __restrict__ c_out, const PrState* __restrict__ st,
+    float* __restrict__ partial, int32_t H) {
+  __shared__ float4 s_hot4[kHotMax / 4];
+  __shared__ float s_red[kHotBlock / kWave];
+  const PrState s = *st;
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const int tid = threadIdx.x;
+  // H is a multiple of 4 and at most kHotMax; c_in is 256-byte aligned.
+  const float4* c4 = reinterpret_cast<const float4*>(c_in);
+  for (int i = tid; i < H / 4; i += kHotBlock) s_hot4[i] = c4[i];
+  __syncthreads();
+  const PrHot hot{reinterpret_cast<const float*>(s_hot4), H};
+  const PrSeg sg{nullptr, nullptr, 1, 0};
+  const PrJobs jb{};
+  float mydiff = 0.f;
+  for (int b = blockIdx.x; b < bins.blk[4]; b += gridDim.x) {
+    // In-degree-0 vertices only: from the third sweep on |new - old| = 0.
+    if (b < bins.zb && s.iter >= 2) continue;
+    const int4 row = pr_row<G, kHotBlock>(rows, b, bins);
+    pr_sweep_block<G, false, kHotBlock, true>(b, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb,
+                                              s_red, mydiff, hot);
+  }
+  pr_finish<kHotBlock>(mydiff, partial, s_red, blockIdx.x);
 }
 
 // ---- batched jobs -------------------------------------------------------------
@@ -630,7 +703,7 @@ __global__ __launch_bounds__(kPrBlock) void pr_push_apply(int32_t V, const float
     acc[v] = 0.f;
     pr_store(int32_t(v), sum, s, pr_in[v], degf[v], pr_out, c_out, mydiff);
   }
-  pr_finish(mydiff, partial, s_red);
+  pr_finish(mydiff, partial, s_red, blockIdx.x);
 }
 
 // ---- in-degree bins on the device -------------------------------------------
@@ -752,8 +825,10 @@ __global__ __launch_bounds__(kBinBlock) void pr_bin_place(int32_t V, const int32
 
 // Bins the in-rows of `roff` into `rows` (int4[V]) with the schedule's
 // thresholds; cnt is scratch of kCat * ceil(V / kBinChunk) + kCat ints. Reads
-// the category sizes back (one synchronisation) and fills the launch layout.
-static int pr_bin_device(spm_graph* g, const int32_t* roff, int4* rows, int32_t* cnt, int32_t* h_nb, PrBins* out) {
+// the category sizes back (one synchronisation) and fills the launch layout
+// for workgroups of B lanes.
+static int pr_bin_device(spm_graph* g, const int32_t* roff, int4* rows, int32_t* cnt, int32_t* h_nb, PrBins* out,
+                         int B = kPrBlock) {
   const spm_schedule& s = g->sched;
   const int32_t V = g->V;
   const int32_t nblk = int32_t((int64_t(V) + kBinChunk - 1) / kBinChunk);
@@ -768,12 +843,12 @@ static int pr_bin_device(spm_graph* g, const int32_t* roff, int4* rows, int32_t*
   PrBins& b = *out;
   b.nb[0] = h_nb[0] + h_nb[1];
   for (int i = 1; i < 4; ++i) b.nb[i] = h_nb[i + 1];
-  b.zb = h_nb[0] / kPrBlock;
+  b.zb = h_nb[0] / B;
   b.group = s.pr_group;
   b.blk[0] = 0;
-  b.blk[1] = (b.nb[0] + kPrBlock - 1) / kPrBlock;
-  b.blk[2] = b.blk[1] + (b.nb[1] + kPrBlock / s.pr_group - 1) / (kPrBlock / s.pr_group);
-  b.blk[3] = b.blk[2] + (b.nb[2] + kPrBlock / kWave - 1) / (kPrBlock / kWave);
+  b.blk[1] = (b.nb[0] + B - 1) / B;
+  b.blk[2] = b.blk[1] + (b.nb[1] + B / s.pr_group - 1) / (B / s.pr_group);
+  b.blk[3] = b.blk[2] + (b.nb[2] + B / kWave - 1) / (B / kWave);
   b.blk[4] = b.blk[3] + b.nb[3];
   return SPM_OK;
 }
@@ -802,7 +877,8 @@ static int pr_build(spm_graph* g, PrWork* w, const CsrView& view, bool multi) {
     int32_t* cnt = nullptr;
     const size_t nc = pr_bin_scratch_ints(V);
     SPM_TRY(dmalloc(g, (void**)&cnt, sizeof(int32_t) * nc, "pr bin counts"));
-    const int rc = pr_bin_device(g, view.roff, w->rows, cnt, g->h_pinned, &w->bins);
+    w->hot = !multi && s.pr_mode == 3;
+    const int rc = pr_bin_device(g, view.roff, w->rows, cnt, g->h_pinned, &w->bins, w->hot ? kHotBlock : kPrBlock);
     dfree(g, cnt, sizeof(int32_t) * nc);
     SPM_TRY(rc);
   }
@@ -915,6 +991,28 @@ static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
       SPM_TRY(launch(g, "pr_push_apply", pr_push_apply, dim3(pgrid), block, 0, g->V, (const float*)w->degf, w->acc,
                      pin, w->pr[1 - p], w->c[1 - p], (const PrState*)w->st, w->partial));
       SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial, pgrid, w->st,
+                     1));
+      continue;
+    }
+    if (w->hot) {
+      const int hgrid = std::max(1, std::min(w->bins.blk[4], kHotPerCu * g->num_cus));
+      const int32_t H = std::min<int32_t>(g->V, kHotMax) & ~3;
+      switch (w->bins.group) {
+#define SPM_PR_HOT_LAUNCH(G)                                                                                  \
+  case G:                                                                                                     \
+    SPM_TRY(launch(g, "pr_iter_hot", pr_iter_hot<G>, dim3(hgrid), dim3(kHotBlock), 0, w->view.roff,           \
+                   w->view.radj, (const int4*)w->rows, (const float*)w->degf, w->bins, pin, cin, w->pr[1 - p], \
+                   w->c[1 - p], (const PrState*)w->st, w->partial, H));                                       \
+    break;
+        SPM_PR_HOT_LAUNCH(2)
+        SPM_PR_HOT_LAUNCH(4)
+        SPM_PR_HOT_LAUNCH(8)
+        SPM_PR_HOT_LAUNCH(16)
+        SPM_PR_HOT_LAUNCH(32)
+#undef SPM_PR_HOT_LAUNCH
+        default: return set_error(SPM_EINVAL, "pr: bad lane group");
+      }
+      SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial, hgrid, w->st,
                      1));
       continue;
     }

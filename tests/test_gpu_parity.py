@@ -261,7 +261,8 @@ def test_pr_iteration_count_and_closed_form(oracle):
 
 @pytest.mark.parametrize("t1,t2,t3,grp,seg,mode", [(0, 0, 0, 2, 0, 0), (64, 64, 64, 32, 0, 0), (2, 16, 256, 4, 10, 0),
                                                    (8, 128, 100000, 16, 12, 0), (4, 48, 2048, 8, 11, 0),
-                                                   (4, 48, 2048, 8, 0, 1)])
+                                                   (4, 48, 2048, 8, 0, 1), (0, 0, 0, 2, 0, 3),
+                                                   (64, 64, 64, 32, 0, 3), (4, 128, 1024, 4, 0, 3)])
 def test_pr_schedules(oracle, t1, t2, t3, grp, seg, mode):
     """Lane mapping, source segmentation (8 passes of 1024 sources at seg 10)
     and push vs pull change only the summation order."""
@@ -271,6 +272,41 @@ def test_pr_schedules(oracle, t1, t2, t3, grp, seg, mode):
     dev.set_schedule(pr_t1=t1, pr_t2=t2, pr_t3=t3, pr_group=grp, pr_seg_log2=seg, pr_mode=mode)
     out, it = dev.pr(1e-10, 0.85, 30)
     assert it == 30 and rel_linf(out, ref) <= PR_TOL
+
+
+@pytest.mark.parametrize("kind", ["rmat16", "rmat17", "odd", "star"])
+def test_pr_staged_sweep(oracle, kind):
+    """pr_mode 3 (persistent 1024-lane workgroups reading the contributions of
+    the lowest ids from LDS): more vertices than the 20,416 staged sources
+    (RMAT-16; RMAT-17 on its copy relabeled by out-degree), a
+    vertex count that is not a multiple of four (only the first multiple of
+    four below it is staged) and a hub row summed by a whole workgroup. Ranks
+    within tolerance of the oracle at the restatement's iteration count, and
+    equal to pr_mode 0's up to summation order."""
+    if kind == "rmat16":
+        g = graph.rmat(16, 16, seed=1)
+    elif kind == "rmat17":
+        g = graph.rmat(17, 16, seed=3)
+    elif kind == "odd":
+        g = graph.erdos_renyi(30001, 240000, seed=6, undirected=False)
+    else:
+        n = 25003
+        g = graph.from_edges(np.concatenate([np.arange(1, n), np.arange(1, n)]).astype(np.int32),
+                             np.concatenate([np.zeros(n - 1), np.arange(2, n + 1) % n]).astype(np.int32))
+    dev = g.device()
+    relabel = int(kind == "rmat17")
+    for beta, mi in ((-1.0, 25), (1e-10, 100)):
+        ref, it_ref = oracle.compute_pr(g, beta, 0.85, mi)
+        dev.set_schedule(pr_mode=0, pr_t3=1024, relabel=relabel)
+        base, it0 = dev.pr(beta, 0.85, mi)
+        dev.set_schedule(pr_mode=3, relabel=relabel)
+        out, it = dev.pr(beta, 0.85, mi)
+        if beta < 0:
+            assert it == it0 == it_ref == mi
+        else:
+            assert exits_agree(oracle, g, it, it_ref), (it, it_ref)
+        assert rel_linf(out, base) <= PR_TOL
+        check_pr_values(oracle, g, out, it, ref, it_ref)
 
 
 # ------------------------------------------------------------------ relabeled layout

@@ -241,7 +241,10 @@ typedef struct spm_schedule {
   int32_t pr_group;       /* lanes per vertex in the second bin (2..32)      */
   int32_t sssp_mode;      /* 0 edge-balanced frontier queue with the first
                              frontier in distance bands, 1 dense sweep,
-                             2 queue in vertex order throughout            */
+                             2 queue in vertex order throughout, 3 band-
+                             local rounds in LDS (high-diameter graphs with
+                             local ids; as 1 when a weight is negative or
+                             a row is longer than 64)                      */
   int32_t sssp_rounds;    /* rounds per captured graph (multiple of 6)       */
   int32_t sssp_grid;      /* workgroups per relaxation launch (0 = auto)     */
   int32_t tc_mode;        /* 1 per-vertex LDS hash (default), 0 edge search */
@@ -257,7 +260,9 @@ typedef struct spm_schedule {
   int32_t relabel;        /* PR and SSSP on a copy of the CSR with vertices
                              renumbered by descending out-degree (built once
                              per handle; results in original ids); 0 or 1   */
-  int32_t reserved[5];
+  int32_t sssp_band_rounds; /* sssp_mode 3: rounds a 4096-vertex band runs in
+                             LDS per launch before handing on (1..4096)      */
+  int32_t reserved[4];
 } spm_schedule;
 int spm_get_schedule(spm_graph* g, spm_schedule* s);
 int spm_set_schedule(spm_graph* g, const spm_schedule* s);

@@ -45,7 +45,8 @@ class SpmSchedule(C.Structure):
         ("pr_mode", i32),
         ("pr_seg_log2", i32),
         ("relabel", i32),
-        ("reserved", i32 * 5),
+        ("sssp_band_rounds", i32),
+        ("reserved", i32 * 4),
     ]
 
     def as_dict(self):

@@ -102,6 +102,7 @@ void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
   }
   s->sssp_mode = 0;
   s->sssp_rounds = 12;
+  s->sssp_band_rounds = 16;
   s->sssp_grid = 0;
   s->tc_mode = 1;
   // Small graphs fold PageRank's loop test into the next sweep: below 2^17
@@ -130,9 +131,11 @@ int validate_schedule(const spm_schedule* s) {
   if (s->pr_t1 < 0 || s->pr_t2 < s->pr_t1 || s->pr_t3 < s->pr_t2) return set_error(SPM_EINVAL, "schedule: PR thresholds must be non-decreasing");
   if (s->pr_group != 2 && s->pr_group != 4 && s->pr_group != 8 && s->pr_group != 16 && s->pr_group != 32)
     return set_error(SPM_EINVAL, "schedule: pr_group must be 2, 4, 8, 16 or 32");
-  if (s->sssp_mode < 0 || s->sssp_mode > 2) return set_error(SPM_EINVAL, "schedule: sssp_mode must be 0, 1 or 2");
+  if (s->sssp_mode < 0 || s->sssp_mode > 3) return set_error(SPM_EINVAL, "schedule: sssp_mode must be 0, 1, 2 or 3");
   if (s->sssp_rounds < 6 || s->sssp_rounds > 96 || s->sssp_rounds % 6) return set_error(SPM_EINVAL, "schedule: sssp_rounds must be a multiple of 6 in [6, 96]");
   if (s->sssp_grid < 0 || s->sssp_grid > 65536) return set_error(SPM_EINVAL, "schedule: sssp_grid out of range");
+  if (s->sssp_band_rounds < 1 || s->sssp_band_rounds > 4096)
+    return set_error(SPM_EINVAL, "schedule: sssp_band_rounds must be in [1, 4096]");
   if (s->tc_mode < 0 || s->tc_mode > 1) return set_error(SPM_EINVAL, "schedule: tc_mode must be 0 (search) or 1 (hash)");
   if (s->pr_mode < 0 || s->pr_mode > 3)
     return set_error(SPM_EINVAL,
@@ -366,7 +369,8 @@ int spm_set_schedule(spm_graph* g, const spm_schedule* s) {
                           s->pr_t3 != g->sched.pr_t3 || s->pr_group != g->sched.pr_group ||
                           s->pr_mode != g->sched.pr_mode || s->pr_seg_log2 != g->sched.pr_seg_log2;
   const bool sssp_changed = s->sssp_rounds != g->sched.sssp_rounds || s->sssp_grid != g->sched.sssp_grid ||
-                            s->sssp_mode != g->sched.sssp_mode;
+                            s->sssp_mode != g->sched.sssp_mode ||
+                            s->sssp_band_rounds != g->sched.sssp_band_rounds;
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   if (pr_changed) pr_release(g);
   if (sssp_changed) sssp_release(g);

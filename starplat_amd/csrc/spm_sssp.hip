@@ -92,6 +92,7 @@ struct SsspWork {
   // length fitted to the previous run (a multiple of 3 below R).
   struct Captured {
     int rounds = 0;
+    int mode = -1;              // effective sssp mode the rounds were captured for
     std::vector<uint8_t> comp;  // rounds with compaction kernels (empty: all)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -620,6 +621,142 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_dense(
     atomicOr(ctr_out, 1ull);
 }
 
+// Band-local mode (schedule sssp_mode 3), for graphs of high diameter whose
+// edges join nearby ids (meshes, grids, road networks in a locality-keeping
+// order): there the fixed point takes thousands of rounds of small frontiers,
+// and a round costs a launch and a chain of dependent memory trips, not
+// bandwidth. Workgroup b owns the vertices [b*kBandV, (b+1)*kBandV) and runs
+// up to K rounds (schedule sssp_band_rounds) inside its band in one launch:
+// distances, offsets and the frontier lists live in LDS, rounds are separated
+// by a workgroup barrier, and an in-band improvement queues the vertex for
+// the band's next round. An edge leaving the band lowers the global distance
+// with atomicMin and flags its target for the next launch. At the end the
+// band's distances are written back (atomicMin: another band may have
+// lowered one meanwhile) and vertices still queued are flagged. Launches
+// repeat until one flags nothing. Every improvement is propagated before
+// the fixed point is declared, and any relaxation order reaches the same
+// shortest distances, so the result is bit-identical to the other modes.
+constexpr int kBandBlock = 1024;
+constexpr int kBandV = 4096;
+
+template <bool PACKED>
+__global__ __launch_bounds__(kBandBlock) void sssp_band(
+    int32_t V, const int32_t* __restrict__ off, const int32_t* __restrict__ adj, const int32_t* __restrict__ wgt,
+    const uint32_t* __restrict__ pk, int wb, int32_t* __restrict__ dist, uint8_t* __restrict__ cur,
+    uint8_t* __restrict__ nxt, int K, unsigned long long* __restrict__ ctr_in,
+    unsigned long long* __restrict__ ctr_out, unsigned long long* __restrict__ ctr_clear,
+    unsigned long long* __restrict__ rounds) {
+  __shared__ int32_t s_dist[kBandV];
+  __shared__ int32_t s_off[kBandV + 1];
+  __shared__ uint32_t s_mark[kBandV];    // 1 + the local round whose input list holds the vertex
+  __shared__ int16_t s_list[2][kBandV];  // band-local indices
+  __shared__ int32_t s_cnt[3];
+  __shared__ int32_t s_out;
+
+  const unsigned long long c = *ctr_in;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *ctr_clear = 0ull;
+    if (c) *rounds += 1;
+  }
+  if (c == 0) return;
+  const int tid = threadIdx.x, lane = __lane_id();
+  const int32_t base = int32_t(blockIdx.x) * kBandV;
+  const int32_t nv = min(kBandV, V - base);
+  if (tid < 3) s_cnt[tid] = 0;
+  if (tid == 0) s_out = 0;
+  __syncthreads();
+  // The band's flagged vertices form the first list (their flags cleared:
+  // this buffer is the next launch's output).
+  for (int i = tid; i < nv; i += kBandBlock) {
+    uint32_t m = 0;
+    if (cur[base + i]) {
+      cur[base + i] = 0;
+      m = 1;
+      s_list[0][atomicAdd(&s_cnt[0], 1)] = int16_t(i);
+    }
+    s_mark[i] = m;
+  }
+  __syncthreads();
+  int n = s_cnt[0];
+  if (n == 0) return;  // nothing queued in this band (uniform)
+  for (int i = tid; i < nv; i += kBandBlock) {
+    s_dist[i] = dist[base + i];
+    s_off[i] = off[base + i];
+  }
+  if (tid == 0) s_off[nv] = off[base + nv];
+  __syncthreads();
+
+  int r = 0;  // round r reads s_list[r & 1], counted in s_cnt[r % 3]
+  for (; r < K && n > 0; ++r) {
+    const int16_t* lin = s_list[r & 1];
+    int16_t* lout = s_list[(r + 1) & 1];
+    int32_t* cout = &s_cnt[(r + 1) % 3];
+    if (tid == 0) s_cnt[(r + 2) % 3] = 0;  // the next round's output counter
+    bool out_flag = false;
+    // One lane per queued vertex, its out-edges four at a time with the four
+    // edge loads in flight together (the graphs this mode is for have short
+    // rows, so a round is one global round trip plus LDS work).
+    for (int i = tid; i < n; i += kBandBlock) {
+      const int li = lin[i];
+      const int32_t dv = s_dist[li];
+      const int32_t e1 = s_off[li + 1];
+      for (int32_t e = s_off[li]; e < e1; e += 4) {
+        int32_t nb[4], wt[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          nb[k] = -1;
+          wt[k] = 0;
+          if (e + k < e1) {
+            if (PACKED) {
+              const uint32_t word = pk[e + k];
+              nb[k] = int32_t(word >> wb);
+              wt[k] = int32_t(word & ((1u << wb) - 1u));
+            } else {
+              nb[k] = adj[e + k];
+              wt[k] = wgt[e + k];
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (nb[k] < 0) continue;
+          const int32_t nd = dv + wt[k];
+          const int32_t ln = nb[k] - base;
+          if (ln >= 0 && ln < nv) {
+            // In the band: the LDS minimum decides; a lowered vertex joins
+            // the next round's list once (its mark names that list).
+            if (nd < s_dist[ln] && nd < atomicMin(&s_dist[ln], nd) &&
+                atomicExch(&s_mark[ln], uint32_t(r + 2)) != uint32_t(r + 2))
+              lout[atomicAdd(cout, 1)] = int16_t(ln);
+          } else if (nd < __hip_atomic_load(&dist[nb[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            // Out of the band: cached pre-filter, atomicMin decides, and the
+            // target's band takes it up in the next launch.
+            atomicMin(&dist[nb[k]], nd);
+            nxt[nb[k]] = 1;
+            out_flag = true;
+          }
+        }
+      }
+    }
+    if (__any(out_flag) && lane == 0) s_out = 1;
+    __syncthreads();
+    n = s_cnt[(r + 1) % 3];
+  }
+  // The round bound was reached with vertices queued: the next launch goes on.
+  if (n > 0) {
+    const int16_t* lin = s_list[r & 1];
+    for (int i = tid; i < n; i += kBandBlock) nxt[base + lin[i]] = 1;
+    if (tid == 0) s_out = 1;
+  }
+  for (int i = tid; i < nv; i += kBandBlock) {
+    const int32_t d = s_dist[i];
+    if (d < __hip_atomic_load(&dist[base + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) atomicMin(&dist[base + i], d);
+  }
+  __syncthreads();
+  if (tid == 0 && s_out && __hip_atomic_load(ctr_out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull)
+    atomicOr(ctr_out, 1ull);
+}
+
 // Packs the view's edges as (neighbour << wb | weight) when the weights are
 // non-negative and wb + the bits of a vertex id fit 32 (RMAT-22 with weights
 // up to 100: 22 + 7); otherwise the relaxation reads the two arrays.
@@ -657,6 +794,19 @@ int sssp_weights_changed(spm_graph* g) {
   if (w->wb != old) return sssp_release(g);
   return SPM_OK;
 }
+
+// The band-local mode runs as the dense mode when a weight is negative (its
+// launches are bounded by the rounds Bellman-Ford needs only when every
+// improvement lowers a distance towards a finite minimum) and when a row is
+// longer than kBandMaxDeg (one lane walks a vertex's row, so a hub would
+// hold up its band's every round).
+constexpr int32_t kBandMaxDeg = 64;
+static int sssp_mode_eff(const spm_graph* g) {
+  const int m = g->sched.sssp_mode;
+  return (m == 3 && ((g->sssp && g->sssp->maxw < 0) || g->max_outdeg > kBandMaxDeg)) ? 1 : m;
+}
+
+static int sssp_bands(const spm_graph* g) { return int((int64_t(g->V) + kBandV - 1) / kBandV); }
 
 static int sssp_alloc(spm_graph* g, const CsrView& view) {
   if (g->sssp) return SPM_OK;
@@ -729,7 +879,9 @@ static int sssp_grid(const spm_graph* g) {
 }
 
 // Enqueues R rounds followed by the round-base advance. R is a multiple of 3
-// (queues rotate by 3), and of 6 in dense mode (flags alternate by 2).
+// (queues rotate by 3), and of 6 in the dense and band-local modes (flags
+// alternate by 2); a band-local "round" is one launch of sssp_band (up to
+// kBandRounds rounds inside every band).
 // comp[r] == 0 leaves round r without the compaction kernels: it pushes
 // through the counter whatever its size (correct for any frontier, slower for
 // a big one); comp == nullptr compacts every round.
@@ -738,12 +890,20 @@ static int sssp_enqueue_rounds(spm_graph* g, int R, const uint8_t* comp) {
   const int grid = sssp_grid(g);
   for (int r = 0; r < R; ++r) {
     const int i = r % 3, o = (r + 1) % 3, z = (r + 2) % 3;
-    if (g->sched.sssp_mode == 1) {
+    const int mode = sssp_mode_eff(g);
+    if (mode == 1) {
       uint8_t* cur = w->active + size_t(r & 1) * g->V;
       uint8_t* nxt = w->active + size_t((r + 1) & 1) * g->V;
       SPM_TRY(launch(g, "sssp_dense", sssp_dense, dim3(grid), dim3(kSsspBlock), 0, g->V,
                      w->view.off, w->view.adj, w->view.w, w->dist,
                      cur, nxt, w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3));
+    } else if (mode == 3) {
+      uint8_t* cur = w->active + size_t(r & 1) * g->V;
+      uint8_t* nxt = w->active + size_t((r + 1) & 1) * g->V;
+      SPM_TRY(launch(g, "sssp_band", w->wb ? sssp_band<true> : sssp_band<false>, dim3(sssp_bands(g)),
+                     dim3(kBandBlock), 0, g->V, w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb,
+                     w->dist, cur, nxt, int(g->sched.sssp_band_rounds), w->ctr + i, w->ctr + o, w->ctr + z,
+                     w->ctr + 3));
     } else {
       SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(grid), dim3(kSsspBlock), 0,
                      w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
@@ -769,7 +929,7 @@ static int sssp_enqueue_rounds(spm_graph* g, int R, const uint8_t* comp) {
 // captured on first use.
 static int sssp_capture(spm_graph* g, int k, int R, const std::vector<uint8_t>& comp, hipGraphExec_t* out) {
   SsspWork::Captured& c = g->sssp->cap[k];
-  if (!c.exec || c.rounds != R || c.comp != comp) {
+  if (!c.exec || c.rounds != R || c.comp != comp || c.mode != sssp_mode_eff(g)) {
     if (c.exec) { (void)hipGraphExecDestroy(c.exec); c.exec = nullptr; }
     if (c.graph) { (void)hipGraphDestroy(c.graph); c.graph = nullptr; }
     c.rounds = 0;
@@ -782,6 +942,7 @@ static int sssp_capture(spm_graph* g, int k, int R, const std::vector<uint8_t>& 
     c.graph = graph;
     SPM_CHECK_HIP(hipGraphInstantiate(&c.exec, graph, nullptr, nullptr, 0));
     c.rounds = R;
+    c.mode = sssp_mode_eff(g);
     c.comp = comp;
   }
   *out = c.exec;
@@ -814,7 +975,9 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   SsspWork* w = g->sssp;
   w->dist_out_ok = false;
   const int R = g->sched.sssp_rounds;
-  const int dense = g->sched.sssp_mode == 1;
+  const int mode = sssp_mode_eff(g);
+  const int dense = mode == 1;
+  const bool flags = mode == 1 || mode == 3;  // frontier as per-vertex flags, no queue history
   if (w->host_base > 0xF0000000u) {  // stamp wrap-around: start over
     w->host_base = 1;
     SPM_CHECK_HIP(hipMemsetAsync(w->stamp, 0, sizeof(uint32_t) * g->V, g->stream));
@@ -831,11 +994,11 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
     SPM_CHECK_HIP(hipMemcpyAsync(w->srcbuf, srcs, sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice, g->stream));
   }
   const int igrid = int(std::min<int64_t>((g->V + 255) / 256, 4096));
-  uint8_t* act = dense ? w->active : (uint8_t*)nullptr;
+  uint8_t* act = flags ? w->active : (uint8_t*)nullptr;
   SPM_TRY(launch(g, "sssp_init", sssp_init, dim3(igrid), dim3(256), 0, g->V, w->dist, act));
   // Queue mode with non-negative weights: the seeds' round runs here, outside
   // the captured rounds, and queues its frontier in distance bands.
-  const bool banded = g->sched.sssp_mode == 0 && w->maxw >= 0;
+  const bool banded = mode == 0 && w->maxw >= 0;
   const int slot = banded ? 2 : 0;
   // sources are original ids; the seed kernel maps them through the
   // relabeling (view.perm) when there is one
@@ -871,7 +1034,7 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   // launches cost ~9 us a round).
   int R1 = R;
   std::vector<uint8_t> comp1;
-  if (!dense && w->last_rounds > 0) {
+  if (!flags && w->last_rounds > 0) {
     R1 = std::min(R, 3 * ((w->last_rounds + 3) / 3));
     comp1.resize(size_t(R1));
     for (int r = 0; r < R1; ++r) comp1[r] = r >= 128 || ((w->big_hist[r >> 6] >> (r & 63)) & 1ull);
@@ -903,7 +1066,7 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   if (d_out) SPM_TRY(gather_back(g, w->view, w->dist, d_out));
   SPM_CHECK_HIP(hipStreamSynchronize(g->stream));
   if (rounds_out) *rounds_out = int32_t(h[0]);
-  if (!dense) {
+  if (!flags) {
     w->last_rounds = int(std::max<int64_t>(1, int64_t(h[0]) - (banded ? 1 : 0)));
     w->big_hist[0] = h[1];
     w->big_hist[1] = h[2];

@@ -36,7 +36,8 @@ CLASSES = [
     dict(sssp_mode=1, sssp_rounds=48, sssp_grid=0, pr_mode=1, pr_t1=4, pr_t2=48, pr_t3=2048, pr_group=8),
     dict(sssp_mode=0, sssp_rounds=48, sssp_grid=2048, pr_mode=1, pr_t1=4, pr_t2=48, pr_t3=2048, pr_group=8),
     dict(sssp_mode=0, sssp_rounds=96, sssp_grid=0, pr_mode=0, pr_t1=16, pr_t2=256, pr_t3=8192, pr_group=32),
-    dict(sssp_mode=1, sssp_rounds=6, sssp_grid=0, pr_mode=0, pr_t1=0, pr_t2=0, pr_t3=64, pr_group=2),
+    dict(sssp_mode=3, sssp_rounds=48, sssp_grid=0, sssp_band_rounds=8, pr_mode=2, pr_t1=4, pr_t2=128, pr_t3=256,
+         pr_group=4),
 ]
 FIXED_CLASS = 0
 

@@ -61,7 +61,7 @@ def make(kind, kw, weights=(1, 100)):
 
 
 @pytest.mark.parametrize("kind,kw", SSSP_CASES)
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_sssp_parity(oracle, kind, kw, mode):
     g = make(kind, kw)
     dev = g.device()
@@ -106,7 +106,7 @@ def test_sssp_rounds_fitted_to_previous_run(oracle):
         np.testing.assert_array_equal(dev.sssp(s), ref0 if s == 0 else oracle.compute_sssp(g, g.edgeLen, s))
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 2, 3])
 def test_sssp_negative_weights_and_many_sources(oracle, mode):
     # Negative weights (no negative cycle: every edge goes up in id) leave the
     # edges unpacked and the first frontier unbanded; the fixed point is the
@@ -147,6 +147,29 @@ def test_sssp_negative_weights_and_many_sources(oracle, mode):
     ww = np.concatenate([g.edgeLen, np.zeros(srcs.size, np.int32)])
     gv = graph.from_edges(ss.astype(np.int32), dd.astype(np.int32), ww.astype(np.int32))
     np.testing.assert_array_equal(out.to_host(), oracle.compute_sssp(gv, gv.edgeLen, V)[:V])
+
+
+def test_sssp_band_local_long_paths(oracle):
+    """sssp_mode 3 (each workgroup relaxes its own 4096-vertex band for up to
+    128 rounds per launch): paths far longer than one launch's rounds, so
+    bands hand queued vertices and cross-band improvements to later launches
+    -- a 300 x 300 grid from a corner and from the middle, a 20,000-vertex
+    path (with a shortcut at its end) and a ring walked backwards -- plus a
+    source without out-edges and weights too wide to pack."""
+    n = 20000
+    s = np.concatenate([np.arange(n - 1), [n - 10]]).astype(np.int32)
+    d = np.concatenate([np.arange(1, n), [5]]).astype(np.int32)
+    w = np.concatenate([np.arange(n - 1) % 7 + 1, [1]]).astype(np.int32)
+    ring = np.arange(9000, dtype=np.int32)
+    cases = [(graph.grid2d(300, 300, weights=(1, 100)), (0, 150 * 300 + 150, 300 * 300 - 1)),
+             (graph.from_edges(s, d, w), (0, n // 2, n - 1)),
+             (graph.from_edges(ring, (ring - 1) % 9000, np.full(9000, 3, np.int32)), (0, 4500)),
+             (graph.grid2d(120, 90, weights=(1, 1 << 20)), (0, 5000))]
+    for g, srcs in cases:
+        dev = g.device()
+        dev.set_schedule(sssp_mode=3)
+        for src in srcs:
+            np.testing.assert_array_equal(dev.sssp(src), oracle.compute_sssp(g, g.edgeLen, src))
 
 
 def test_sssp_edge_cases(oracle):

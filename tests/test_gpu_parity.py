@@ -170,6 +170,23 @@ def test_sssp_band_local_long_paths(oracle):
         dev.set_schedule(sssp_mode=3)
         for src in srcs:
             np.testing.assert_array_equal(dev.sssp(src), oracle.compute_sssp(g, g.edgeLen, src))
+    # Several seeds at once, on the grid and on its degree-relabeled copy
+    # (ids no longer local: most edges leave their band), each round cap.
+    from starplat_amd import hip
+    g = graph.grid2d(200, 150, weights=(1, 100))
+    seeds = np.array([0, 7777, 29999], np.int32)
+    V = g.V
+    ss = np.concatenate([np.repeat(np.arange(V, dtype=np.int32), np.diff(g.indexofNodes)), np.full(seeds.size, V, np.int32)])
+    gv = graph.from_edges(ss, np.concatenate([g.edgeList, seeds]).astype(np.int32),
+                          np.concatenate([g.edgeLen, np.zeros(seeds.size, np.int32)]))
+    ref = oracle.compute_sssp(gv, gv.edgeLen, V)[:V]
+    dev = g.device()
+    out = hip.DeviceArray(V, np.int32)
+    for relabel, k in ((0, 1), (0, 8), (0, 4096), (1, 8)):
+        dev.set_schedule(sssp_mode=3, relabel=relabel, sssp_band_rounds=k)
+        dev.sssp_multi_device(seeds, out)
+        np.testing.assert_array_equal(out.to_host(), ref)
+        np.testing.assert_array_equal(dev.sssp(7777), oracle.compute_sssp(g, g.edgeLen, 7777))
 
 
 def test_sssp_edge_cases(oracle):

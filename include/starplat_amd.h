@@ -92,8 +92,9 @@ int spm_pr_device(spm_graph* g, float beta, float delta, int32_t maxIter,
  * reverse CSR and no pending edge updates (SPM_EINVAL: fold them with
  * spm_graph_compact first). Runs the pull sweep on the original ids with the
  * handle's lane-mapping thresholds, in the same summation order as
- * spm_pr_device on graphs it does not relabel (below 2^20 vertices with the
- * default schedule: bit-identical ranks and iteration count); push and
+ * spm_pr_device on graphs it does not relabel (with the default schedule:
+ * below 2^17 vertices, or degrees without a dominant hub; bit-identical
+ * ranks and iteration count); push and
  * source segmentation are not used. Every call on one GPU shares that
  * array: calls that overlap on one GPU (from any handles or threads) race. */
 int spm_pr_device_noalloc(spm_graph* g, float beta, float delta, int32_t maxIter,

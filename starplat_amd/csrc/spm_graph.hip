@@ -112,16 +112,21 @@ void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
   // One sweep: measured at RMAT-22, 2 MB source segments double the
   // iteration time (each pass repeats the per-vertex load chain).
   s->pr_seg_log2 = 0;
-  // Degree-ordered relabeling pays off when PR's contribution / SSSP's
-  // distance array outgrows L2 (4 MB per XCD, ~1M vertices) and the degrees
-  // are skewed enough that a small set of vertices takes most accesses.
+  // Degree-ordered relabeling pays off once the degrees are skewed enough
+  // that a small set of vertices takes most accesses and the graph has from
+  // 2^17 vertices: PageRank, 100 iterations, ms, original -> relabeled ids:
+  // RMAT-17 1.54 -> 1.35, RMAT-18 2.40 -> 1.90, RMAT-19 4.11 -> 3.51,
+  // RMAT-20 8.22 -> 6.62 (edge factor 16); SSSP equal or faster at every
+  // one of these sizes (RMAT-20 0.60 -> 0.57).
   const double avg = g->V > 0 ? double(g->E) / double(g->V) : 0.0;
-  s->relabel = (g->V >= (1 << 20) && double(g->max_outdeg) >= 64.0 * avg) ? 1 : 0;
+  s->relabel = (g->V >= (1 << 17) && double(g->max_outdeg) >= 64.0 * avg) ? 1 : 0;
   // On the relabeled copy the lowest ids are the sources most in-rows
-  // gather from: the staged sweep reads their contributions from LDS, with
-  // rows up to 4096 in-edges on a wavefront (RMAT-22, 100 iterations:
-  // 33.5 -> 28.4 ms; RMAT-21 at edge factor 32: 26.8 -> 21.1 ms).
-  if (s->relabel) {
+  // gather from: from 2^19 vertices the staged sweep reads their
+  // contributions from LDS, with rows up to 4096 in-edges on a wavefront
+  // (100 iterations, ms: RMAT-22 33.5 -> 28.4; RMAT-20 6.62 -> 5.36;
+  // RMAT-19 3.51 -> 3.46, at edge factor 32 5.68 -> 4.25; but RMAT-18
+  // 1.90 -> 2.32, the staging then costing more than it saves).
+  if (s->relabel && g->V >= (1 << 19)) {
     s->pr_mode = 3;
     s->pr_t3 = 4096;
   }

@@ -151,7 +151,8 @@ def test_sssp_negative_weights_and_many_sources(oracle, mode):
 
 def test_sssp_band_local_long_paths(oracle):
     """sssp_mode 3 (each workgroup relaxes its own 4096-vertex band for up to
-    128 rounds per launch): paths far longer than one launch's rounds, so
+    sssp_band_rounds rounds per launch, 8 by default): paths far longer than
+    one launch's rounds, so
     bands hand queued vertices and cross-band improvements to later launches
     -- a 300 x 300 grid from a corner and from the middle, a 20,000-vertex
     path (with a shortcut at its end) and a ring walked backwards -- plus a
@@ -499,6 +500,19 @@ def test_nearly_empty_graph_all_algorithms(oracle):
     bc = dev.bc(np.array([0, 997, 998], np.int32))
     np.testing.assert_allclose(bc, ref_bc, rtol=0, atol=1e-6)
     assert bc[998] == 1.0  # on the path 997 -> 998 -> 999
+    # the band-local SSSP mode and the staged PageRank sweep on the same
+    # ragged input, and on a single vertex
+    dev.set_schedule(sssp_mode=3, pr_mode=3)
+    for s in (0, 997):
+        np.testing.assert_array_equal(dev.sssp(s), oracle.compute_sssp(g, g.edgeLen, s))
+    out, it3 = dev.pr(1e-10, 0.85, 100)
+    assert rel_linf(out, ref) <= PR_TOL
+    one = graph.from_edges(np.array([0], np.int32), np.array([0], np.int32), np.array([5], np.int32))
+    d1 = one.device()
+    d1.set_schedule(sssp_mode=3, pr_mode=3)
+    np.testing.assert_array_equal(d1.sssp(0), [0])
+    out1, _ = d1.pr(1e-10, 0.85, 100)
+    assert rel_linf(out1, oracle.compute_pr(one)[0]) <= PR_TOL
 
 
 def test_tc_closed_forms_and_directed(oracle):

@@ -434,6 +434,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
                                                         const uint2* __restrict__ tile_sum, SsspQueue qout,
                                                         unsigned long long* __restrict__ ctr_out) {
   __shared__ uint32_t s_tot[2 * kSsspWaves];
+  __shared__ int32_t s_vtx[kTileV];
+  __shared__ uint32_t s_eoff[kTileV];
   const uint32_t T = uint32_t(*ctr_in);
   if (T < kSparseEdges) return;
   const uint32_t sv = *round_base + rloc;
@@ -457,16 +459,25 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
   }
   uint32_t ec, ee, tc, te;
   block_scan2(c, e, ec, ee, tc, te, s_tot);
-  uint32_t pos = bc + ec, eo = be + ee;
+  // The tile's entries are staged in LDS in queue order and then written out
+  // as two contiguous runs (a lane's own 16 vertices would make every store
+  // instruction touch 64 scattered lines).
+  uint32_t pos = ec, eo = be + ee;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const uint32_t d = deg[k];
     if (!d) continue;
-    qout.vtx[pos] = int32_t(v0 + k);
-    qout.eoff[pos] = eo;
-    for (uint32_t cc = (eo + kChunk - 1) / kChunk; cc <= (eo + d - 1) / kChunk; ++cc) qout.cfirst[cc] = int32_t(pos);
+    s_vtx[pos] = int32_t(v0 + k);
+    s_eoff[pos] = eo;
+    for (uint32_t cc = (eo + kChunk - 1) / kChunk; cc <= (eo + d - 1) / kChunk; ++cc)
+      qout.cfirst[cc] = int32_t(bc + pos);
     ++pos;
     eo += d;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < tc; i += kSsspBlock) {
+    qout.vtx[bc + i] = s_vtx[i];
+    qout.eoff[bc + i] = s_eoff[i];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
     *ctr_out = (static_cast<unsigned long long>(bc + tc) << 32) | (be + te);

@@ -20,6 +20,11 @@
 // workgroup updates a shared counter: same-address device-scope atomics from
 // 10^5 workgroups cost milliseconds per iteration.)
 //
+// On the degree-ordered copy of a large graph a persistent variant of the
+// sweep (pr_iter_hot, schedule pr_mode 3) first copies the contributions of
+// the lowest ids -- the sources of most in-edges -- into each CU's LDS and
+// reads those terms there.
+//
 // Arithmetic follows the emitted code: the per-term division pr[u]/outdeg(u)
 // is the same IEEE single division, and the update is evaluated without FMA
 // contraction. Only the order of the in-row summation differs (tree vs

@@ -39,6 +39,10 @@
 // of them were big: the first replay of a run is fitted to that, without
 // trailing empty rounds and without compaction kernels in small rounds.
 //
+// For graphs of high diameter with local ids (grids, meshes) a band-local
+// mode (sssp_band, schedule sssp_mode 3) lets each workgroup run several
+// rounds inside its own range of vertices out of LDS per launch.
+//
 // Distances are exact shortest paths, so the result is bit-identical to any
 // correct schedule of the reference's fixed point.
 #include <algorithm>

@@ -102,7 +102,7 @@ void heuristic_schedule(const spm_graph* g, spm_schedule* s) {
   }
   s->sssp_mode = 0;
   s->sssp_rounds = 12;
-  s->sssp_band_rounds = 16;
+  s->sssp_band_rounds = 8;
   s->sssp_grid = 0;
   s->tc_mode = 1;
   // Small graphs fold PageRank's loop test into the next sweep: below 2^17

@@ -23,7 +23,8 @@
 // On the degree-ordered copy of a large graph a persistent variant of the
 // sweep (pr_iter_hot, schedule pr_mode 3) first copies the contributions of
 // the lowest ids -- the sources of most in-edges -- into each CU's LDS and
-// reads those terms there.
+// reads those terms there; it also evaluates the loop test of the sweep
+// before it, so a captured batch has one check kernel in all.
 //
 // Arithmetic follows the emitted code: the per-term division pr[u]/outdeg(u)
 // is the same IEEE single division, and the update is evaluated without FMA
@@ -143,10 +144,10 @@ __device__ __forceinline__ void pr_finish(float mydiff, float* __restrict__ part
 // predecessors themselves) and raises the done flag that makes later
 // iterations return at once.
 constexpr int kCheckBlock = 1024;
-__global__ __launch_bounds__(kCheckBlock) void pr_check(const float* __restrict__ partial, int n,
-                                                        PrState* __restrict__ st, int step) {
-  __shared__ float s_red[kCheckBlock / kWave];
-  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+
+// The sum of partial[0..n) in pr_check's fixed order, computed by a
+// kCheckBlock-lane workgroup; every thread returns the same value.
+__device__ __forceinline__ float check_sum(const float* __restrict__ partial, int n, float* s_red) {
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   // 16-byte loads into four independent sums, two loads per step, so the one
   // workgroup keeps several requests in flight (partial is 256-byte aligned).
@@ -168,9 +169,17 @@ __global__ __launch_bounds__(kCheckBlock) void pr_check(const float* __restrict_
   t = wave_sum(t);
   if (lane == 0) s_red[wid] = t;
   __syncthreads();
-  if (tid == 0) {
-    float d = 0.f;
-    for (int w = 0; w < kCheckBlock / kWave; ++w) d += s_red[w];
+  float d = 0.f;
+  for (int w = 0; w < kCheckBlock / kWave; ++w) d += s_red[w];
+  return d;
+}
+
+__global__ __launch_bounds__(kCheckBlock) void pr_check(const float* __restrict__ partial, int n,
+                                                        PrState* __restrict__ st, int step) {
+  __shared__ float s_red[kCheckBlock / kWave];
+  if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const float d = check_sum(partial, n, s_red);
+  if (threadIdx.x == 0) {
     const int it = st->iter + step;
     st->iter = it;
     if (!(d > st->beta && it < st->maxIter)) st->done = 1;
@@ -500,20 +509,39 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter_lag(
 // workgroup at the end. Measured at RMAT-22 (100 iterations): 33.5 ms for the
 // plain sweep, 30.0 ms staging 20,416 sources in two workgroups per CU,
 // 28.4 ms staging 40,832 in one (bin 3 above 4096 in-edges).
+//
+// The loop test is folded in as in pr_iter_lag: sweep r >= 1 of a captured
+// batch first sums sweep r-1's shares (prev[0..nprev), pr_check's order, so
+// every workgroup reaches pr_check's verdict) and stops the run itself; the
+// batch's last sweep is judged by one pr_check.
 constexpr int kHotBlock = 1024;
 constexpr int kHotPerCu = 1;
 constexpr int kHotMax = 40832;
+static_assert(kHotBlock == kCheckBlock, "the staged sweep reproduces pr_check's sum with its own lanes");
 
 template <int G>
 __global__ __launch_bounds__(kHotBlock) void pr_iter_hot(
     const int32_t* __restrict__ roff, const int32_t* __restrict__ radj, const int4* __restrict__ rows,
     const float* __restrict__ degf, PrBins bins, const float* __restrict__ pr_in, const float* __restrict__ c_in,
-    float* __restrict__ pr_out, float* __restrict__ c_out, const PrState* __restrict__ st,
-    float* __restrict__ partial, int32_t H) {
+    float* __restrict__ pr_out, float* __restrict__ c_out, PrState* __restrict__ st,
+    const float* __restrict__ prev, int nprev, float* __restrict__ partial, int32_t H, int r) {
   __shared__ float4 s_hot4[kHotMax / 4];
   __shared__ float s_red[kHotBlock / kWave];
   const PrState s = *st;
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  if (r > 0) {
+    // Same verdict in every workgroup; workgroup 0 alone writes (see pr_iter_lag).
+    const float d = check_sum(prev, nprev, s_red);
+    const int it = s.iter + r;
+    if (!(d > s.beta && it < s.maxIter)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->iter = it;
+        __hip_atomic_store(&st->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      return;
+    }
+    __syncthreads();  // s_red is reused by the sweep
+  }
   const int tid = threadIdx.x;
   // H is a multiple of 4 and at most kHotMax; c_in is 256-byte aligned.
   const float4* c4 = reinterpret_cast<const float4*>(c_in);
@@ -523,9 +551,11 @@ __global__ __launch_bounds__(kHotBlock) void pr_iter_hot(
   const PrSeg sg{nullptr, nullptr, 1, 0};
   const PrJobs jb{};
   float mydiff = 0.f;
+  // This sweep's iteration: st->iter advances once per batch.
+  const bool skip0 = s.iter + r >= 2;
   for (int b = blockIdx.x; b < bins.blk[4]; b += gridDim.x) {
     // In-degree-0 vertices only: from the third sweep on |new - old| = 0.
-    if (b < bins.zb && s.iter >= 2) continue;
+    if (b < bins.zb && skip0) continue;
     const int4 row = pr_row<G, kHotBlock>(rows, b, bins);
     pr_sweep_block<G, false, kHotBlock, true>(b, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb,
                                               s_red, mydiff, hot);
@@ -1009,14 +1039,16 @@ static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
       continue;
     }
     if (w->hot) {
-      const int hgrid = std::max(1, std::min(w->bins.blk[4], kHotPerCu * g->num_cus));
+      const int hgrid = std::max(1, std::min({w->bins.blk[4], kHotPerCu * g->num_cus, kLagMaxGrid}));
       const int32_t H = std::min<int32_t>(g->V, kHotMax) & ~3;
+      float* mine = w->partial + (r & 1) * kLagMaxGrid;
+      const float* prev = w->partial + (1 - (r & 1)) * kLagMaxGrid;
       switch (w->bins.group) {
 #define SPM_PR_HOT_LAUNCH(G)                                                                                  \
   case G:                                                                                                     \
     SPM_TRY(launch(g, "pr_iter_hot", pr_iter_hot<G>, dim3(hgrid), dim3(kHotBlock), 0, w->view.roff,           \
                    w->view.radj, (const int4*)w->rows, (const float*)w->degf, w->bins, pin, cin, w->pr[1 - p], \
-                   w->c[1 - p], (const PrState*)w->st, w->partial, H));                                       \
+                   w->c[1 - p], w->st, prev, hgrid, mine, H, r));                                             \
     break;
         SPM_PR_HOT_LAUNCH(2)
         SPM_PR_HOT_LAUNCH(4)
@@ -1026,8 +1058,8 @@ static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
 #undef SPM_PR_HOT_LAUNCH
         default: return set_error(SPM_EINVAL, "pr: bad lane group");
       }
-      SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)w->partial, hgrid, w->st,
-                     1));
+      if (r == R - 1)
+        SPM_TRY(launch(g, "pr_check", pr_check, dim3(1), dim3(kCheckBlock), 0, (const float*)mine, hgrid, w->st, R));
       continue;
     }
     if (pr_lagged(g, w)) {

@@ -350,6 +350,40 @@ def test_pr_staged_sweep(oracle, kind):
         check_pr_values(oracle, g, out, it, ref, it_ref)
 
 
+@pytest.mark.parametrize("kind", ["rmat16", "grid"])
+def test_pr_staged_sweep_loop_test(oracle, kind):
+    """pr_mode 3 evaluates each sweep's loop test at the start of the next
+    sweep, and the last sweep of a captured batch of up to 128 in one check
+    kernel. Fixed counts run exactly maxIter sweeps (1, 2 and 130 end or cross
+    a batch; from the third sweep on the in-degree-0 rows are skipped) and
+    match the FP64 oracle at that count. A converging run stops at the first
+    sweep whose diff -- recomputed on the host from the ranks of the runs one
+    and two sweeps shorter, which the deterministic sweep reproduces -- is at
+    most beta, and not one sweep earlier."""
+    g = graph.rmat(16, 16, seed=1) if kind == "rmat16" else graph.grid2d(100, 130)
+    dev = g.device()
+    dev.set_schedule(pr_mode=3)
+    for mi in (1, 2, 37, 130):
+        out, it = dev.pr(-1.0, 0.85, mi)
+        assert it == mi
+        fp64, _ = oracle.pr_fp64(g, -1.0, 0.85, mi)
+        assert rel_linf(out, fp64) <= PR_TOL, mi
+    # delta 0.95 converges slowly enough to stop in the second batch
+    for beta, delta in ((1e-3, 0.85), (1e-6, 0.85), (1e-8, 0.85), (1e-6, 0.95)):
+        out, k = dev.pr(beta, delta, 400)
+        assert 3 <= k < 400, (beta, k)
+        again, k2 = dev.pr(beta, delta, 400)
+        assert k2 == k
+        np.testing.assert_array_equal(again, out)
+        r1, _ = dev.pr(beta, delta, k - 1)
+        r2, _ = dev.pr(beta, delta, k - 2)
+        d_k = float(np.abs(out.astype(np.float64) - r1).sum())
+        d_k1 = float(np.abs(r1.astype(np.float64) - r2).sum())
+        print(f"staged loop test {kind} beta {beta} delta {delta}: stop at {k}, diff {d_k:.4e}, "
+              f"one sweep earlier {d_k1:.4e}")
+        assert d_k <= beta * (1 + 1e-3) and d_k1 > beta * (1 - 1e-3), (beta, k, d_k, d_k1)
+
+
 # ------------------------------------------------------------------ relabeled layout
 @pytest.mark.parametrize("kind", ["rmat12", "rmat16", "grid", "star"])
 def test_pr_lagged_loop_test_matches_separate_check(oracle, kind):

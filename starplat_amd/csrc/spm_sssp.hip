@@ -22,9 +22,11 @@
 // the authoritative atomicMin.
 //
 // Large rounds (frontier over kSparseEdges edges) do not touch a shared
-// counter per pushed vertex: relaxation only stamps improved vertices, and
-// two tiled kernels compact the stamps into the next queue in ascending
-// vertex order (each tile sums the counts of the tiles before it).
+// counter per pushed vertex, nor write anything per improved vertex beyond
+// the atomicMin: two tiled kernels queue, in ascending vertex order, every
+// vertex whose distance is below the one it was last queued with (a second
+// distance array, written by the compaction itself; each tile sums the
+// counts of the tiles before it).
 //
 // Three queues rotate (read k, write k+1, clear k+2), so a round is one
 // kernel and R rounds are captured into one hipGraph; the host reads one
@@ -76,7 +78,9 @@ struct SsspQueue {
 struct SsspWork {
   SsspQueue q[3];
   unsigned long long* ctr = nullptr;  // [4] (entries<<32 | edges) x3, rounds
-  uint32_t* stamp = nullptr;          // [V]
+  uint32_t* stamp = nullptr;          // [V] round of the last push (rounds below kSparseEdges)
+  int32_t* dprev = nullptr;           // [V] distance each vertex was last queued with (>= dist)
+  uint16_t* qmask = nullptr;          // [tiles * kSsspBlock] queued vertices of a compaction lane
   uint32_t* round_base = nullptr;     // [1]
   int32_t* dist = nullptr;            // [V] working distances
   uint8_t* active = nullptr;          // [2][V] dense-mode frontier flags
@@ -108,10 +112,12 @@ struct SsspWork {
   uint32_t host_base = 1;
 };
 
-__global__ void sssp_init(int32_t V, int32_t* __restrict__ dist, uint8_t* __restrict__ active) {
+__global__ void sssp_init(int32_t V, int32_t* __restrict__ dist, int32_t* __restrict__ dprev,
+                          uint8_t* __restrict__ active) {
   const int64_t stride = int64_t(gridDim.x) * blockDim.x;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < V; i += stride) {
     dist[i] = INT_MAX;
+    if (dprev) dprev[i] = INT_MAX;
     if (active) {
       active[i] = 0;
       active[V + i] = 0;
@@ -179,7 +185,7 @@ __device__ __forceinline__ void block_scan2(uint32_t a, uint32_t b, uint32_t& ea
 __global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restrict__ srcs, int32_t n, int32_t one,
                                                         const int32_t* __restrict__ perm,
                                                         const int32_t* __restrict__ off, int32_t* __restrict__ dist,
-                                                        SsspQueue q0, unsigned long long* __restrict__ ctr, int slot,
+                                                        int32_t* __restrict__ dprev, SsspQueue q0, unsigned long long* __restrict__ ctr, int slot,
                                                         unsigned long long* __restrict__ band_tot,
                                                         uint32_t* __restrict__ run_start, uint32_t start_val,
                                                         unsigned long long* __restrict__ big,
@@ -197,6 +203,7 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_seed(const int32_t* __restric
       if (perm) s = perm[s];  // original id -> the relabeled copy's id
       deg = off[s + 1] - off[s];
       dist[s] = 0;
+      if (dprev) dprev[s] = 0;
       if (active) active[s] = 1;
     }
     const uint32_t push = deg > 0 ? 1u : 0u;
@@ -223,7 +230,7 @@ template <bool PACKED>
 __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
     const int32_t* __restrict__ off, const int32_t* __restrict__ adj,
     const int32_t* __restrict__ wgt, const uint32_t* __restrict__ pk, int wb,
-    int32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
+    int32_t* __restrict__ dist, uint32_t* __restrict__ stamp, int32_t* __restrict__ dprev,
     SsspQueue qin, SsspQueue qout, unsigned long long* __restrict__ ctr_in,
     unsigned long long* __restrict__ ctr_out, unsigned long long* __restrict__ ctr_clear,
     unsigned long long* __restrict__ rounds, const uint32_t* __restrict__ round_base, uint32_t rloc,
@@ -251,7 +258,7 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
   const uint32_t nchunks = (T + kChunk - 1) / kChunk;
   const int tid = threadIdx.x;
   // Small rounds (below sparse_max edges) push through the block-aggregated
-  // counter; large rounds only mark improved vertices and leave the queue to
+  // counter; large rounds only lower distances and leave the queue to
   // sssp_count/sssp_emit (or sssp_count_b/sssp_emit_b).
   const bool sparse = T < sparse_max;
 
@@ -330,17 +337,22 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
         const int32_t nd = s_dv[lo] + wt;
         // A cached (possibly stale) read is enough to skip hopeless updates:
         // distances only decrease, so a stale value is never below the true
-        // one and atomicMin below stays the authority. Every lane that passes
-        // the test marks the neighbour, without waiting for the atomic's old
-        // value: the value read is at most the distance at the start of the
-        // round, so if this lane's atomicMin does not lower the distance,
-        // another lane lowered it during this round and marks it too. The
-        // marked set is exactly the set of vertices lowered in this round.
+        // one and atomicMin below stays the authority. In a large round that
+        // is all: the compaction finds every vertex lowered in this round
+        // (its distance is then below the one it was last queued with). In a
+        // small round every lane that passes the test pushes the neighbour
+        // unless this round's stamp is already there, without waiting for
+        // the atomic's old value: the value read is at most the distance at
+        // the start of the round, so if this lane's atomicMin does not lower
+        // the distance, another lane lowered it during this round and pushes
+        // it (once, by the stamp).
         if (nd < __hip_atomic_load(&dist[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
           atomicMin(&dist[nb], nd);  // no return value: the wave does not wait
-          if (!sparse) {
-            stamp[nb] = sv;  // idempotent mark, compacted later
-          } else if (atomicExch(&stamp[nb], sv) != sv) {
+          if (sparse && atomicExch(&stamp[nb], sv) != sv) {
+            // nd bounds the distance it is queued with from above; a lower
+            // one from later in this round queues it once more in the next
+            // compacted round at worst.
+            dprev[nb] = nd;
             const int32_t deg = off[nb + 1] - off[nb];
             if (deg > 0) {
               pv[k] = nb;
@@ -381,60 +393,99 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
 }
 
 // Frontier compaction for large rounds, step 1: each tile of kTileV vertices
-// (16 consecutive per lane) counts the vertices marked in this round that have
-// out-edges, and their out-degrees.
+// (16 consecutive per lane) counts the vertices lowered since they were last
+// queued (dist < dprev) that have out-edges, and their out-degrees; dv
+// receives the distances. Every vertex lowered in the round just relaxed is
+// among them; so, at worst, is a vertex pushed by a small round whose
+// distance fell again later in that round (relaxing it once more is harmless).
 __device__ __forceinline__ void tile_flags(int32_t V, const int32_t* __restrict__ off,
-                                           const uint32_t* __restrict__ stamp, uint32_t sv, int64_t v0,
-                                           uint32_t (&deg)[16]) {
+                                           const int32_t* __restrict__ dist, const int32_t* __restrict__ dprev,
+                                           int64_t v0, uint32_t (&deg)[16], int32_t (&dv)[16],
+                                           bool (&lowered)[4]) {
   if (v0 + 16 <= V) {
-    const uint4* p = reinterpret_cast<const uint4*>(stamp + v0);
+    const int4* pd = reinterpret_cast<const int4*>(dist + v0);
+    const int4* pp = reinterpret_cast<const int4*>(dprev + v0);
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const uint4 s = p[q];
-      deg[4 * q + 0] = s.x == sv;
-      deg[4 * q + 1] = s.y == sv;
-      deg[4 * q + 2] = s.z == sv;
-      deg[4 * q + 3] = s.w == sv;
+      const int4 d = pd[q], b = pp[q];
+      dv[4 * q + 0] = d.x;
+      dv[4 * q + 1] = d.y;
+      dv[4 * q + 2] = d.z;
+      dv[4 * q + 3] = d.w;
+      deg[4 * q + 0] = d.x < b.x;
+      deg[4 * q + 1] = d.y < b.y;
+      deg[4 * q + 2] = d.z < b.z;
+      deg[4 * q + 3] = d.w < b.w;
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) deg[k] = (v0 + k < V) && stamp[v0 + k] == sv;
+    for (int k = 0; k < 16; ++k) {
+      dv[k] = v0 + k < V ? dist[v0 + k] : INT_MAX;
+      deg[k] = (v0 + k < V) && dv[k] < dprev[v0 + k];
+    }
   }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) lowered[q] = deg[4 * q] | deg[4 * q + 1] | deg[4 * q + 2] | deg[4 * q + 3];
 #pragma unroll
   for (int k = 0; k < 16; ++k)
     if (deg[k]) deg[k] = uint32_t(off[v0 + k + 1] - off[v0 + k]);
 }
 
+// dprev := dist over the lane's 16 vertices: each is now queued with its
+// present distance or was not lowered (dist == dprev, as dprev >= dist
+// always). A lane stores 16-byte words, and only those holding a lowered
+// vertex (lowered[q]), so a wavefront writes whole runs of lines where the
+// round lowered many distances and little where it lowered few.
+__device__ __forceinline__ void tile_queued(int32_t V, int32_t* __restrict__ dprev, int64_t v0,
+                                            const int32_t (&dv)[16], const bool (&lowered)[4]) {
+  if (v0 + 16 <= V) {
+    int4* p = reinterpret_cast<int4*>(dprev + v0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (lowered[q]) p[q] = make_int4(dv[4 * q], dv[4 * q + 1], dv[4 * q + 2], dv[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      if (v0 + k < V) dprev[v0 + k] = dv[k];
+  }
+}
+
+// The lane's queued vertices also go to qmask (bit k: vertex v0 + k), which
+// is all the emit step reads, and their distances to dprev.
 __global__ __launch_bounds__(kSsspBlock) void sssp_count(int32_t V, const int32_t* __restrict__ off,
-                                                         const uint32_t* __restrict__ stamp,
+                                                         const int32_t* __restrict__ dist,
+                                                         int32_t* __restrict__ dprev,
                                                          const unsigned long long* __restrict__ ctr_in,
-                                                         const uint32_t* __restrict__ round_base, uint32_t rloc,
-                                                         uint2* __restrict__ tile_sum) {
+                                                         uint2* __restrict__ tile_sum, uint16_t* __restrict__ qmask) {
   __shared__ uint32_t s_tot[2 * kSsspWaves];
   const uint32_t T = uint32_t(*ctr_in);
   if (T < kSparseEdges) return;  // this round pushed through the counter
-  const uint32_t sv = *round_base + rloc;
+  const int64_t v0 = int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
   uint32_t deg[16];
-  tile_flags(V, off, stamp, sv, int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16, deg);
-  uint32_t c = 0, e = 0;
+  int32_t dv[16];
+  bool lowered[4];
+  tile_flags(V, off, dist, dprev, v0, deg, dv, lowered);
+  tile_queued(V, dprev, v0, dv, lowered);
+  uint32_t c = 0, e = 0, m = 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     c += deg[k] > 0;
     e += deg[k];
+    m |= uint32_t(deg[k] > 0) << k;
   }
+  qmask[int64_t(blockIdx.x) * kSsspBlock + threadIdx.x] = uint16_t(m);
   uint32_t ec, ee, tc, te;
   block_scan2(c, e, ec, ee, tc, te, s_tot);
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = make_uint2(tc, te);
 }
 
 // Step 2: every tile sums the counts of the tiles before it (no atomics, no
-// waiting on other workgroups), then writes its vertices in ascending order
-// with their degree prefix and the chunk index; the last tile publishes the
-// frontier size.
+// waiting on other workgroups), then writes the vertices of its lane masks in
+// ascending order with their degree prefix and the chunk index; the last tile
+// publishes the frontier size.
 __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ stamp,
+                                                        const uint16_t* __restrict__ qmask,
                                                         const unsigned long long* __restrict__ ctr_in,
-                                                        const uint32_t* __restrict__ round_base, uint32_t rloc,
                                                         const uint2* __restrict__ tile_sum, SsspQueue qout,
                                                         unsigned long long* __restrict__ ctr_out) {
   __shared__ uint32_t s_tot[2 * kSsspWaves];
@@ -442,7 +493,6 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
   __shared__ uint32_t s_eoff[kTileV];
   const uint32_t T = uint32_t(*ctr_in);
   if (T < kSparseEdges) return;
-  const uint32_t sv = *round_base + rloc;
   uint32_t pc = 0, pe = 0;
   for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kSsspBlock) {
     const uint2 x = tile_sum[t];
@@ -453,11 +503,12 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
   block_scan2(pc, pe, d0, d1, bc, be, s_tot);  // bc, be: sums over the earlier tiles
   __syncthreads();
   const int64_t v0 = int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
+  const uint32_t m = qmask[int64_t(blockIdx.x) * kSsspBlock + threadIdx.x];
   uint32_t deg[16];
-  tile_flags(V, off, stamp, sv, v0, deg);
   uint32_t c = 0, e = 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
+    deg[k] = (m >> k) & 1u ? uint32_t(off[v0 + k + 1] - off[v0 + k]) : 0u;
     c += deg[k] > 0;
     e += deg[k];
   }
@@ -487,8 +538,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
     *ctr_out = (static_cast<unsigned long long>(bc + tc) << 32) | (be + te);
 }
 
-// First frontier in distance bands. The seeds' round marks its improved
-// vertices (all of them at distance seed + one edge, so within [0, maxw]);
+// First frontier in distance bands. The seeds' round lowers the distances
+// of its vertices (all of them to seed + one edge, so within [0, maxw]);
 // instead of ascending vertex order they are queued by band
 // b = min(kBands - 1, dist / delta), tile after tile within a band. The
 // next round's chunks are dispatched roughly nearest first, so the cached
@@ -498,19 +549,17 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
 // tile reserving its place within every band it occupies with one atomic on
 // the band's total (tiles of a band may sit in any order), and the emit.
 __device__ __forceinline__ void tile_bands(int32_t V, const int32_t* __restrict__ off,
-                                           const uint32_t* __restrict__ stamp, const int32_t* __restrict__ dist,
-                                           uint32_t sv, int64_t v0, int32_t delta, uint32_t (&deg)[16],
-                                           uint32_t (&band)[16]) {
-  tile_flags(V, off, stamp, sv, v0, deg);
+                                           const int32_t* __restrict__ dist, const int32_t* __restrict__ dprev,
+                                           int64_t v0, int32_t delta, uint32_t (&deg)[16], int32_t (&dv)[16],
+                                           bool (&lowered)[4], uint32_t (&band)[16]) {
+  tile_flags(V, off, dist, dprev, v0, deg, dv, lowered);
 #pragma unroll
-  for (int k = 0; k < 16; ++k)
-    band[k] = deg[k] ? min(uint32_t(kBands - 1), uint32_t(dist[v0 + k] / delta)) : 0u;
+  for (int k = 0; k < 16; ++k) band[k] = deg[k] ? min(uint32_t(kBands - 1), uint32_t(dv[k] / delta)) : 0u;
 }
 
 __global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int32_t* __restrict__ off,
-                                                           const uint32_t* __restrict__ stamp,
                                                            const int32_t* __restrict__ dist,
-                                                           const uint32_t* __restrict__ round_base, int32_t delta,
+                                                           const int32_t* __restrict__ dprev, int32_t delta,
                                                            int32_t tiles, uint2* __restrict__ tile_b,
                                                            unsigned long long* __restrict__ band_tot) {
   __shared__ uint32_t s_c[kBands], s_e[kBands];
@@ -520,7 +569,9 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int3
   }
   __syncthreads();
   uint32_t deg[16], band[16];
-  tile_bands(V, off, stamp, dist, *round_base, int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16, delta, deg,
+  int32_t dv[16];
+  bool lowered[4];
+  tile_bands(V, off, dist, dprev, int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16, delta, deg, dv, lowered,
              band);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
@@ -544,9 +595,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int3
 // prefixes advance together, so the queue stays consistent whatever order
 // the lanes take. The first workgroup publishes the frontier size.
 __global__ __launch_bounds__(kSsspBlock) void sssp_emit_b(int32_t V, const int32_t* __restrict__ off,
-                                                          const uint32_t* __restrict__ stamp,
                                                           const int32_t* __restrict__ dist,
-                                                          const uint32_t* __restrict__ round_base, int32_t delta,
+                                                          int32_t* __restrict__ dprev, int32_t delta,
                                                           int32_t tiles, const uint2* __restrict__ tile_b,
                                                           const unsigned long long* __restrict__ band_tot,
                                                           SsspQueue qout, unsigned long long* __restrict__ ctr_out) {
@@ -560,7 +610,9 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit_b(int32_t V, const int32
   }
   const int64_t v0 = int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
   uint32_t deg[16], band[16];
-  tile_bands(V, off, stamp, dist, *round_base, v0, delta, deg, band);
+  int32_t dv[16];
+  bool lowered[4];
+  tile_bands(V, off, dist, dprev, v0, delta, deg, dv, lowered, band);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
@@ -572,6 +624,7 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit_b(int32_t V, const int32
     qout.eoff[pos] = eo;
     for (uint32_t cc = (eo + kChunk - 1) / kChunk; cc <= (eo + d - 1) / kChunk; ++cc) qout.cfirst[cc] = int32_t(pos);
   }
+  tile_queued(V, dprev, v0, dv, lowered);
 }
 
 // Dense mode: the reference's full-V sweep of the modified flags, one lane per
@@ -841,6 +894,9 @@ static int sssp_alloc(spm_graph* g, const CsrView& view) {
   SPM_TRY(dmalloc(g, (void**)&w->stamp, sizeof(uint32_t) * V, "sssp stamps"));
   SPM_TRY(dmalloc(g, (void**)&w->round_base, sizeof(uint32_t), "sssp round base"));
   SPM_TRY(dmalloc(g, (void**)&w->dist, sizeof(int32_t) * V, "sssp dist"));
+  SPM_TRY(dmalloc(g, (void**)&w->dprev, sizeof(int32_t) * V, "sssp queued distances"));
+  SPM_TRY(dmalloc(g, (void**)&w->qmask, sizeof(uint16_t) * kSsspBlock * ((size_t(V) + kTileV - 1) / kTileV),
+                  "sssp compaction masks"));
   SPM_TRY(dmalloc(g, (void**)&w->active, 2 * size_t(V), "sssp flags"));
   SPM_TRY(dmalloc(g, (void**)&w->tile_sum, sizeof(uint2) * ((size_t(V) + kTileV - 1) / kTileV), "sssp tiles"));
   SPM_TRY(dmalloc(g, (void**)&w->wstat, sizeof(unsigned long long) * 2, "sssp weight range"));
@@ -872,6 +928,8 @@ int sssp_release(spm_graph* g) {
   dfree(g, w->stamp, sizeof(uint32_t) * g->V);
   dfree(g, w->round_base, sizeof(uint32_t));
   dfree(g, w->dist, sizeof(int32_t) * g->V);
+  dfree(g, w->dprev, sizeof(int32_t) * g->V);
+  dfree(g, w->qmask, sizeof(uint16_t) * kSsspBlock * ((size_t(g->V) + kTileV - 1) / kTileV));
   dfree(g, w->active, 2 * size_t(g->V));
   dfree(g, w->tile_sum, sizeof(uint2) * ((size_t(g->V) + kTileV - 1) / kTileV));
   dfree(g, w->srcbuf, sizeof(int32_t) * w->src_cap);
@@ -922,18 +980,17 @@ static int sssp_enqueue_rounds(spm_graph* g, int R, const uint8_t* comp) {
     } else {
       SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(grid), dim3(kSsspBlock), 0,
                      w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
-                     w->stamp, w->q[i], w->q[o], w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3,
+                     w->stamp, w->dprev, w->q[i], w->q[o], w->ctr + i, w->ctr + o, w->ctr + z, w->ctr + 3,
                      (const uint32_t*)w->round_base, uint32_t(r),
                      comp && !comp[r] ? 0xFFFFFFFFu : kSparseEdges, (const uint32_t*)w->run_start, w->big));
       if (comp && !comp[r]) continue;
       const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
       SPM_TRY(launch(g, "sssp_count", sssp_count, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
-                     (const uint32_t*)w->stamp, (const unsigned long long*)(w->ctr + i),
-                     (const uint32_t*)w->round_base, uint32_t(r), w->tile_sum));
+                     (const int32_t*)w->dist, w->dprev, (const unsigned long long*)(w->ctr + i), w->tile_sum,
+                     w->qmask));
       SPM_TRY(launch(g, "sssp_emit", sssp_emit, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
-                     (const uint32_t*)w->stamp, (const unsigned long long*)(w->ctr + i),
-                     (const uint32_t*)w->round_base, uint32_t(r), (const uint2*)w->tile_sum, w->q[o],
-                     w->ctr + o));
+                     (const uint16_t*)w->qmask, (const unsigned long long*)(w->ctr + i), (const uint2*)w->tile_sum,
+                     w->q[o], w->ctr + o));
     }
   }
   SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, uint32_t(R)));
@@ -991,7 +1048,6 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   w->dist_out_ok = false;
   const int R = g->sched.sssp_rounds;
   const int mode = sssp_mode_eff(g);
-  const int dense = mode == 1;
   const bool flags = mode == 1 || mode == 3;  // frontier as per-vertex flags, no queue history
   if (w->host_base > 0xF0000000u) {  // stamp wrap-around: start over
     w->host_base = 1;
@@ -1010,7 +1066,8 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   }
   const int igrid = int(std::min<int64_t>((g->V + 255) / 256, 4096));
   uint8_t* act = flags ? w->active : (uint8_t*)nullptr;
-  SPM_TRY(launch(g, "sssp_init", sssp_init, dim3(igrid), dim3(256), 0, g->V, w->dist, act));
+  SPM_TRY(launch(g, "sssp_init", sssp_init, dim3(igrid), dim3(256), 0, g->V, w->dist,
+                 flags ? (int32_t*)nullptr : w->dprev, act));
   // Queue mode with non-negative weights: the seeds' round runs here, outside
   // the captured rounds, and queues its frontier in distance bands.
   const bool banded = mode == 0 && w->maxw >= 0;
@@ -1018,21 +1075,21 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   // sources are original ids; the seed kernel maps them through the
   // relabeling (view.perm) when there is one
   SPM_TRY(launch(g, "sssp_seed", sssp_seed, dim3(1), dim3(kSsspBlock), 0, n > 1 ? (const int32_t*)w->srcbuf : nullptr,
-                 n, srcs[0], view.perm, w->view.off, w->dist, w->q[slot], w->ctr, slot, banded ? w->band_tot : nullptr,
+                 n, srcs[0], view.perm, w->view.off, w->dist, flags ? (int32_t*)nullptr : w->dprev, w->q[slot],
+                 w->ctr, slot, banded ? w->band_tot : nullptr,
                  w->run_start, w->host_base + (banded ? 1u : 0u), w->big, act));
   if (banded) {
     const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
     const int32_t delta = int32_t(w->maxw / kBands + 1);
     SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(sssp_grid(g)),
                    dim3(kSsspBlock), 0, w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
-                   w->stamp, w->q[2], w->q[0], w->ctr + 2, w->ctr + 0, w->ctr + 1, w->ctr + 3,
+                   w->stamp, w->dprev, w->q[2], w->q[0], w->ctr + 2, w->ctr + 0, w->ctr + 1, w->ctr + 3,
                    (const uint32_t*)w->round_base, 0u, 0u, (const uint32_t*)w->run_start,
                    (unsigned long long*)nullptr));
     SPM_TRY(launch(g, "sssp_count_b", sssp_count_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
-                   (const uint32_t*)w->stamp, (const int32_t*)w->dist, (const uint32_t*)w->round_base, delta, tiles,
-                   w->tile_b, w->band_tot));
+                   (const int32_t*)w->dist, (const int32_t*)w->dprev, delta, tiles, w->tile_b, w->band_tot));
     SPM_TRY(launch(g, "sssp_emit_b", sssp_emit_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
-                   (const uint32_t*)w->stamp, (const int32_t*)w->dist, (const uint32_t*)w->round_base, delta, tiles,
+                   (const int32_t*)w->dist, w->dprev, delta, tiles,
                    (const uint2*)w->tile_b, (const unsigned long long*)w->band_tot, w->q[0], w->ctr + 0));
     SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, 1u));
     w->host_base += 1;

@@ -548,16 +548,88 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
 // scanned over the whole run). Two kernels: per tile and band counts, each
 // tile reserving its place within every band it occupies with one atomic on
 // the band's total (tiles of a band may sit in any order), and the emit.
+//
+// With a single source the lowered vertices are among its out-neighbours, so
+// the tiles then cover the source's row instead of all V vertices (a row is
+// non-decreasing: a slot holding the same vertex as the slot before it is
+// skipped, so each vertex is queued once).
+struct BandTiles {
+  const int32_t* row;  // the single source's out-row (null: tiles of vertex ids)
+  int32_t n;           // its length
+};
+// A row tile is 1024 slots, four per lane: a hub's row of ~10^5 slots then
+// spreads over ~100 workgroups (vertex tiles are kTileV = 4096, 16 per lane).
+constexpr int kRowPer = 4;
+constexpr int kRowTile = kSsspBlock * kRowPer;
+
+__device__ __forceinline__ int64_t band_tile_start(const BandTiles& bt) {
+  return bt.row ? int64_t(blockIdx.x) * kRowTile + int64_t(threadIdx.x) * kRowPer
+                : int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
+}
+
+__device__ __forceinline__ BandTiles band_tiles(const int32_t* __restrict__ adj, const int32_t* __restrict__ off,
+                                                const int32_t* __restrict__ perm, int32_t src) {
+  if (src < 0) return BandTiles{nullptr, 0};
+  const int32_t s = perm ? perm[src] : src;
+  return BandTiles{adj + off[s], off[s + 1] - off[s]};
+}
+
+// The lane's candidates: vertex ids v0 + k (k < 16), or row slots v0 + k
+// (k < kRowPer; the rest stay empty).
 __device__ __forceinline__ void tile_bands(int32_t V, const int32_t* __restrict__ off,
                                            const int32_t* __restrict__ dist, const int32_t* __restrict__ dprev,
-                                           int64_t v0, int32_t delta, uint32_t (&deg)[16], int32_t (&dv)[16],
-                                           bool (&lowered)[4], uint32_t (&band)[16]) {
-  tile_flags(V, off, dist, dprev, v0, deg, dv, lowered);
+                                           const BandTiles& bt, int64_t v0, int32_t delta, uint32_t (&deg)[16],
+                                           int32_t (&vid)[16], int32_t (&dv)[16], bool (&lowered)[4],
+                                           uint32_t (&band)[16]) {
+  if (!bt.row) {
+    tile_flags(V, off, dist, dprev, v0, deg, dv, lowered);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) vid[k] = int32_t(v0 + k);
+  } else {
+    // Three phases of loads, each unconditional (clamped to slot 0 or
+    // vertex 0), so a lane's slots wait for three round trips in all.
+    int32_t prv[kRowPer], dp[kRowPer], o0[kRowPer], o1[kRowPer];
+#pragma unroll
+    for (int k = 0; k < kRowPer; ++k) {
+      const int64_t i = v0 + k;
+      const bool in = i < bt.n;
+      vid[k] = bt.row[in ? i : 0];
+      prv[k] = bt.row[in && i > 0 ? i - 1 : 0];
+      if (!in) vid[k] = -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kRowPer; ++k) {
+      const int32_t v = max(vid[k], 0);
+      dv[k] = dist[v];
+      dp[k] = dprev[v];
+    }
+#pragma unroll
+    for (int k = 0; k < kRowPer; ++k) {
+      const int32_t v = max(vid[k], 0);
+      o0[k] = off[v];
+      o1[k] = off[v + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (k < kRowPer) {
+        const bool first = vid[k] >= 0 && (v0 + k == 0 || prv[k] != vid[k]);
+        deg[k] = first && dv[k] < dp[k] ? uint32_t(o1[k] - o0[k]) : 0u;
+      } else {
+        vid[k] = -1;
+        dv[k] = 0;
+        deg[k] = 0;
+      }
+    }
+  }
 #pragma unroll
   for (int k = 0; k < 16; ++k) band[k] = deg[k] ? min(uint32_t(kBands - 1), uint32_t(dv[k] / delta)) : 0u;
 }
 
+// src >= 0: tiles over the out-row of that single source (original id,
+// mapped through perm); src < 0: tiles over all vertices.
 __global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int32_t* __restrict__ off,
+                                                           const int32_t* __restrict__ adj,
+                                                           const int32_t* __restrict__ perm, int32_t src,
                                                            const int32_t* __restrict__ dist,
                                                            const int32_t* __restrict__ dprev, int32_t delta,
                                                            int32_t tiles, uint2* __restrict__ tile_b,
@@ -569,10 +641,10 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int3
   }
   __syncthreads();
   uint32_t deg[16], band[16];
-  int32_t dv[16];
+  int32_t vid[16], dv[16];
   bool lowered[4];
-  tile_bands(V, off, dist, dprev, int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16, delta, deg, dv, lowered,
-             band);
+  const BandTiles bt = band_tiles(adj, off, perm, src);
+  tile_bands(V, off, dist, dprev, bt, band_tile_start(bt), delta, deg, vid, dv, lowered, band);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     if (!deg[k]) continue;
@@ -595,6 +667,8 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_count_b(int32_t V, const int3
 // prefixes advance together, so the queue stays consistent whatever order
 // the lanes take. The first workgroup publishes the frontier size.
 __global__ __launch_bounds__(kSsspBlock) void sssp_emit_b(int32_t V, const int32_t* __restrict__ off,
+                                                          const int32_t* __restrict__ adj,
+                                                          const int32_t* __restrict__ perm, int32_t src,
                                                           const int32_t* __restrict__ dist,
                                                           int32_t* __restrict__ dprev, int32_t delta,
                                                           int32_t tiles, const uint2* __restrict__ tile_b,
@@ -608,11 +682,12 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit_b(int32_t V, const int32
     s_cur[threadIdx.x] = start + ((static_cast<unsigned long long>(p.x) << 32) | p.y);
     if (blockIdx.x == 0 && threadIdx.x == kBands - 1) *ctr_out = start + band_tot[kBands - 1];
   }
-  const int64_t v0 = int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
   uint32_t deg[16], band[16];
-  int32_t dv[16];
+  int32_t vid[16], dv[16];
   bool lowered[4];
-  tile_bands(V, off, dist, dprev, v0, delta, deg, dv, lowered, band);
+  const BandTiles bt = band_tiles(adj, off, perm, src);
+  const int64_t v0 = band_tile_start(bt);
+  tile_bands(V, off, dist, dprev, bt, v0, delta, deg, vid, dv, lowered, band);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
@@ -620,11 +695,12 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit_b(int32_t V, const int32
     if (!d) continue;
     const unsigned long long at = atomicAdd(&s_cur[band[k]], (1ull << 32) | d);
     const uint32_t pos = uint32_t(at >> 32), eo = uint32_t(at);
-    qout.vtx[pos] = int32_t(v0 + k);
+    qout.vtx[pos] = vid[k];
     qout.eoff[pos] = eo;
     for (uint32_t cc = (eo + kChunk - 1) / kChunk; cc <= (eo + d - 1) / kChunk; ++cc) qout.cfirst[cc] = int32_t(pos);
+    if (bt.row) dprev[vid[k]] = dv[k];
   }
-  tile_queued(V, dprev, v0, dv, lowered);
+  if (!bt.row) tile_queued(V, dprev, v0, dv, lowered);
 }
 
 // Dense mode: the reference's full-V sweep of the modified flags, one lane per
@@ -876,6 +952,12 @@ static int sssp_mode_eff(const spm_graph* g) {
 
 static int sssp_bands(const spm_graph* g) { return int((int64_t(g->V) + kBandV - 1) / kBandV); }
 
+// Tiles of the banded first frontier: vertex tiles, or row tiles of one
+// source's out-row (at most V slots).
+static size_t sssp_band_tiles(int32_t V) {
+  return std::max((size_t(V) + kTileV - 1) / kTileV, (size_t(V) + kRowTile - 1) / kRowTile);
+}
+
 static int sssp_alloc(spm_graph* g, const CsrView& view) {
   if (g->sssp) return SPM_OK;
   SsspWork* w = new SsspWork();
@@ -900,8 +982,7 @@ static int sssp_alloc(spm_graph* g, const CsrView& view) {
   SPM_TRY(dmalloc(g, (void**)&w->active, 2 * size_t(V), "sssp flags"));
   SPM_TRY(dmalloc(g, (void**)&w->tile_sum, sizeof(uint2) * ((size_t(V) + kTileV - 1) / kTileV), "sssp tiles"));
   SPM_TRY(dmalloc(g, (void**)&w->wstat, sizeof(unsigned long long) * 2, "sssp weight range"));
-  SPM_TRY(dmalloc(g, (void**)&w->tile_b, sizeof(uint2) * kBands * ((size_t(V) + kTileV - 1) / kTileV),
-                  "sssp band tiles"));
+  SPM_TRY(dmalloc(g, (void**)&w->tile_b, sizeof(uint2) * kBands * sssp_band_tiles(V), "sssp band tiles"));
   SPM_TRY(dmalloc(g, (void**)&w->band_tot, sizeof(unsigned long long) * kBands, "sssp band totals"));
   SPM_TRY(dmalloc(g, (void**)&w->run_start, sizeof(uint32_t), "sssp run start"));
   SPM_TRY(dmalloc(g, (void**)&w->big, sizeof(unsigned long long) * 2, "sssp big rounds"));
@@ -936,7 +1017,7 @@ int sssp_release(spm_graph* g) {
   dfree(g, w->dist_out, sizeof(int32_t) * g->V);
   dfree(g, w->pk, sizeof(uint32_t) * size_t(g->E));
   dfree(g, w->wstat, sizeof(unsigned long long) * 2);
-  dfree(g, w->tile_b, sizeof(uint2) * kBands * ((size_t(g->V) + kTileV - 1) / kTileV));
+  dfree(g, w->tile_b, sizeof(uint2) * kBands * sssp_band_tiles(g->V));
   dfree(g, w->band_tot, sizeof(unsigned long long) * kBands);
   dfree(g, w->run_start, sizeof(uint32_t));
   dfree(g, w->big, sizeof(unsigned long long) * 2);
@@ -1079,7 +1160,12 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
                  w->ctr, slot, banded ? w->band_tot : nullptr,
                  w->run_start, w->host_base + (banded ? 1u : 0u), w->big, act));
   if (banded) {
-    const int tiles = int((int64_t(g->V) + kTileV - 1) / kTileV);
+    // One source: tiles over its out-row (at most max_outdeg <= V slots, so
+    // at most sssp_band_tiles(V) tiles, which tile_b is sized for).
+    const bool by_row = n == 1 && g->max_outdeg <= int64_t(g->V);
+    const int tiles = by_row ? std::max(1, int((g->max_outdeg + kRowTile - 1) / kRowTile))
+                             : int((int64_t(g->V) + kTileV - 1) / kTileV);
+    const int32_t src1 = by_row ? srcs[0] : -1;
     const int32_t delta = int32_t(w->maxw / kBands + 1);
     SPM_TRY(launch(g, "sssp_relax", w->wb ? sssp_relax<true> : sssp_relax<false>, dim3(sssp_grid(g)),
                    dim3(kSsspBlock), 0, w->view.off, w->view.adj, w->view.w, (const uint32_t*)w->pk, w->wb, w->dist,
@@ -1087,9 +1173,10 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
                    (const uint32_t*)w->round_base, 0u, 0u, (const uint32_t*)w->run_start,
                    (unsigned long long*)nullptr));
     SPM_TRY(launch(g, "sssp_count_b", sssp_count_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
-                   (const int32_t*)w->dist, (const int32_t*)w->dprev, delta, tiles, w->tile_b, w->band_tot));
+                   w->view.adj, view.perm, src1, (const int32_t*)w->dist, (const int32_t*)w->dprev, delta, tiles,
+                   w->tile_b, w->band_tot));
     SPM_TRY(launch(g, "sssp_emit_b", sssp_emit_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
-                   (const int32_t*)w->dist, w->dprev, delta, tiles,
+                   w->view.adj, view.perm, src1, (const int32_t*)w->dist, w->dprev, delta, tiles,
                    (const uint2*)w->tile_b, (const unsigned long long*)w->band_tot, w->q[0], w->ctr + 0));
     SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, 1u));
     w->host_base += 1;

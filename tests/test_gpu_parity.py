@@ -149,6 +149,56 @@ def test_sssp_negative_weights_and_many_sources(oracle, mode):
     np.testing.assert_array_equal(out.to_host(), oracle.compute_sssp(gv, gv.edgeLen, V)[:V])
 
 
+class _RawCsr:
+    """A CSR handed to spm_graph_create as is (rows non-decreasing, repeated
+    neighbours allowed), without the reverse CSR."""
+
+    def __init__(self, off, adj, w):
+        self.indexofNodes = np.ascontiguousarray(off, np.int32)
+        self.edgeList = np.ascontiguousarray(adj, np.int32)
+        self.edgeLen = np.ascontiguousarray(w, np.int32)
+        self.rev_indexofNodes = None
+        self.srcList = None
+        self.V = len(off) - 1
+        self.E = len(adj)
+
+    def num_nodes(self):
+        return self.V
+
+    def num_edges(self):
+        return self.E
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_sssp_source_row_with_repeated_neighbours(oracle, mode):
+    """The banded first frontier of a single source tiles over that source's
+    row, skipping a slot that repeats the neighbour before it: rows with
+    repeated neighbours (different weights), a self-loop, a row longer than
+    V (the vertex tiles then serve instead) and a source without out-edges."""
+    from starplat_amd.device import DeviceGraph
+    rng = np.random.default_rng(21)
+    for V, hub_len in ((3000, 2500), (5000, 9000), (40, 100)):
+        rows = []
+        for v in range(V):
+            n = hub_len if v == 7 else int(rng.integers(0, 12))
+            if v == V - 1:
+                n = 0
+            r = np.sort(rng.integers(0, V, n))
+            if v == 7:
+                r = np.sort(np.concatenate([r, np.repeat(r[:50], 3), [7, 7]]))
+            rows.append(r)
+        off = np.zeros(V + 1, np.int64)
+        off[1:] = np.cumsum([len(r) for r in rows])
+        adj = np.concatenate(rows).astype(np.int32)
+        w = rng.integers(1, 100, adj.size).astype(np.int32)
+        g = _RawCsr(off, adj, w)
+        dev = DeviceGraph(g)
+        dev.set_schedule(sssp_mode=mode)
+        for src in (7, 0, V - 1):
+            np.testing.assert_array_equal(dev.sssp(src), oracle.compute_sssp(g, g.edgeLen, src), err_msg=f"V={V} src={src}")
+        dev.close()
+
+
 def test_sssp_band_local_long_paths(oracle):
     """sssp_mode 3 (each workgroup relaxes its own 4096-vertex band for up to
     sssp_band_rounds rounds per launch, 8 by default): paths far longer than

@@ -284,13 +284,20 @@ __device__ __forceinline__ float gather_sum(const int32_t* __restrict__ radj, co
 #pragma unroll
     for (int k = 0; k < W; ++k) s[k] += pr_term<HOT>(c, hot, a[k]);
   }
-  // The remaining 0..W-1 terms, loaded together.
+  // The remaining 0..W-1 terms, loaded together: every gather is issued
+  // before any is added (a missing term stays +0, which adds exactly), so
+  // no addition puts one gather's wait in front of the next.
   int32_t a[W - 1];
 #pragma unroll
   for (int k = 0; k < W - 1; ++k) a[k] = e + k * stride < eb ? radj[e + k * stride] : -1;
+  float t[W - 1];
+#pragma unroll
+  for (int k = 0; k < W - 1; ++k) t[k] = 0.f;
 #pragma unroll
   for (int k = 0; k < W - 1; ++k)
-    if (a[k] >= 0) s[k] += pr_term<HOT>(c, hot, a[k]);
+    if (a[k] >= 0) t[k] = pr_term<HOT>(c, hot, a[k]);
+#pragma unroll
+  for (int k = 0; k < W - 1; ++k) s[k] += t[k];
   // pairwise: ((s0 + s1) + (s2 + s3)) + ((s4 + s5) + (s6 + s7)) for W = 8
 #pragma unroll
   for (int h = 1; h < W; h <<= 1)

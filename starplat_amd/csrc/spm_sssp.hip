@@ -392,6 +392,32 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_relax(
   }
 }
 
+// Out-degrees of the lane's 16 vertices v0 .. v0+15 (v0 a multiple of 16,
+// so off + v0 is 64-byte aligned): off[v0 .. v0+16] as four 16-byte loads and
+// one more, all issued at once (loading only the degrees of marked vertices,
+// each under its own condition, put every load's wait before the next).
+__device__ __forceinline__ void tile_degrees(int32_t V, const int32_t* __restrict__ off, int64_t v0,
+                                             uint32_t (&od)[16]) {
+  if (v0 + 16 <= V) {
+    const int4* p = reinterpret_cast<const int4*>(off + v0);
+    int32_t o[17];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int4 x = p[q];
+      o[4 * q] = x.x;
+      o[4 * q + 1] = x.y;
+      o[4 * q + 2] = x.z;
+      o[4 * q + 3] = x.w;
+    }
+    o[16] = off[v0 + 16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) od[k] = uint32_t(o[k + 1] - o[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) od[k] = v0 + k < V ? uint32_t(off[v0 + k + 1] - off[v0 + k]) : 0u;
+  }
+}
+
 // Frontier compaction for large rounds, step 1: each tile of kTileV vertices
 // (16 consecutive per lane) counts the vertices lowered since they were last
 // queued (dist < dprev) that have out-edges, and their out-degrees; dv
@@ -426,9 +452,10 @@ __device__ __forceinline__ void tile_flags(int32_t V, const int32_t* __restrict_
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q) lowered[q] = deg[4 * q] | deg[4 * q + 1] | deg[4 * q + 2] | deg[4 * q + 3];
+  uint32_t od[16];
+  tile_degrees(V, off, v0, od);
 #pragma unroll
-  for (int k = 0; k < 16; ++k)
-    if (deg[k]) deg[k] = uint32_t(off[v0 + k + 1] - off[v0 + k]);
+  for (int k = 0; k < 16; ++k) deg[k] = deg[k] ? od[k] : 0u;
 }
 
 // dprev := dist over the lane's 16 vertices: each is now queued with its
@@ -505,10 +532,11 @@ __global__ __launch_bounds__(kSsspBlock) void sssp_emit(int32_t V, const int32_t
   const int64_t v0 = int64_t(blockIdx.x) * kTileV + int64_t(threadIdx.x) * 16;
   const uint32_t m = qmask[int64_t(blockIdx.x) * kSsspBlock + threadIdx.x];
   uint32_t deg[16];
+  tile_degrees(V, off, v0, deg);
   uint32_t c = 0, e = 0;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    deg[k] = (m >> k) & 1u ? uint32_t(off[v0 + k + 1] - off[v0 + k]) : 0u;
+    deg[k] = (m >> k) & 1u ? deg[k] : 0u;
     c += deg[k] > 0;
     e += deg[k];
   }

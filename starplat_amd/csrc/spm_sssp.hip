@@ -873,21 +873,21 @@ __global__ __launch_bounds__(kBandBlock) void sssp_band(
       const int32_t dv = s_dist[li];
       const int32_t e1 = s_off[li + 1];
       for (int32_t e = s_off[li]; e < e1; e += 4) {
+        // Unconditional loads (clamped to the row's last edge), so the four
+        // are issued together rather than each waiting behind its branch.
         int32_t nb[4], wt[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          nb[k] = -1;
-          wt[k] = 0;
-          if (e + k < e1) {
-            if (PACKED) {
-              const uint32_t word = pk[e + k];
-              nb[k] = int32_t(word >> wb);
-              wt[k] = int32_t(word & ((1u << wb) - 1u));
-            } else {
-              nb[k] = adj[e + k];
-              wt[k] = wgt[e + k];
-            }
+          const int32_t ek = min(e + k, e1 - 1);
+          if (PACKED) {
+            const uint32_t word = pk[ek];
+            nb[k] = int32_t(word >> wb);
+            wt[k] = int32_t(word & ((1u << wb) - 1u));
+          } else {
+            nb[k] = adj[ek];
+            wt[k] = wgt[ek];
           }
+          if (e + k >= e1) nb[k] = -1;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {

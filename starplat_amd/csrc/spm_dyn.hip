@@ -494,16 +494,28 @@ __device__ __forceinline__ float dpr_part_sum(int32_t a, int32_t b, int lane, in
                                               const float* __restrict__ c) {
   float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
   int32_t j = a + lane;
+  // Every gather is issued before its term is added (a removed slot still
+  // names a vertex; its term adds +0, exactly), so no branch puts one
+  // gather's wait in front of the next.
   for (; j + 3 * lanes < b; j += 4 * lanes) {
     const int32_t j1 = j + lanes, j2 = j + 2 * lanes, j3 = j + 3 * lanes;
     const int32_t u0 = adj[j], u1 = adj[j1], u2 = adj[j2], u3 = adj[j3];
-    if (!bit_of(dead, j)) s0 += c[u0];
-    if (!bit_of(dead, j1)) s1 += c[u1];
-    if (!bit_of(dead, j2)) s2 += c[u2];
-    if (!bit_of(dead, j3)) s3 += c[u3];
+    const float t0 = c[u0], t1 = c[u1], t2 = c[u2], t3 = c[u3];
+    s0 += bit_of(dead, j) ? 0.f : t0;
+    s1 += bit_of(dead, j1) ? 0.f : t1;
+    s2 += bit_of(dead, j2) ? 0.f : t2;
+    s3 += bit_of(dead, j3) ? 0.f : t3;
   }
-  for (; j < b; j += lanes)
-    if (!bit_of(dead, j)) s0 += c[adj[j]];
+  // The last 0..3 terms of the lane, loaded together, added in order.
+  float t[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int32_t jk = j + k * lanes;
+    t[k] = 0.f;
+    if (jk < b && !bit_of(dead, jk)) t[k] = c[adj[jk]];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) s0 += t[k];
   return (s0 + s1) + (s2 + s3);
 }
 
@@ -553,19 +565,7 @@ __global__ __launch_bounds__(kDynBlock) void dpr_sweep_light(int32_t V, const ui
 // gathers in flight per lane.
 __device__ __forceinline__ float dpr_row_sum4(int32_t a, int32_t b, const int32_t* __restrict__ adj,
                                               const uint32_t* __restrict__ dead, const float* __restrict__ c) {
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-  int32_t j = a + int32_t(threadIdx.x);
-  for (; j + 3 * kDynBlock < b; j += 4 * kDynBlock) {
-    const int32_t j1 = j + kDynBlock, j2 = j + 2 * kDynBlock, j3 = j + 3 * kDynBlock;
-    const int32_t u0 = adj[j], u1 = adj[j1], u2 = adj[j2], u3 = adj[j3];
-    if (!bit_of(dead, j)) s0 += c[u0];
-    if (!bit_of(dead, j1)) s1 += c[u1];
-    if (!bit_of(dead, j2)) s2 += c[u2];
-    if (!bit_of(dead, j3)) s3 += c[u3];
-  }
-  for (; j < b; j += kDynBlock)
-    if (!bit_of(dead, j)) s0 += c[adj[j]];
-  return (s0 + s1) + (s2 + s3);
+  return dpr_part_sum(a, b, int(threadIdx.x), kDynBlock, adj, dead, c);
 }
 
 // One workgroup per listed vertex (the list is ordered by in-row length,

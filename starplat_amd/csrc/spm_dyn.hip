@@ -655,9 +655,12 @@ __global__ __launch_bounds__(kDynBlock) void dsp_parent(int32_t V, int32_t src, 
         const int32_t a = part == 0 ? roff[v] : in.off[v];
         const int32_t b = part == 0 ? roff[v + 1] : in.off[v + 1];
         for (int32_t j = a + lane; j < b; j += kWave) {
-          if (bit_of(part == 0 ? rdead : in.dead, j)) continue;
+          // The slot's removed bit, neighbour and weight load together (a
+          // removed slot still names a vertex); only then is it skipped.
+          const bool gone = bit_of(part == 0 ? rdead : in.dead, j);
           const int32_t u = part == 0 ? radj[j] : in.adj[j];
           const int32_t wt = part == 0 ? rw[j] : in.w[j];
+          if (gone) continue;
           const int32_t du = dist[u];
           if (du < kInf2 && du + wt == dv && static_cast<unsigned long long>(u) < best) best = u;
         }
@@ -743,9 +746,10 @@ __global__ __launch_bounds__(kDynBlock) void dsp_pull(const int32_t* __restrict_
       const int32_t a = part == 0 ? roff[v] : in.off[v];
       const int32_t b = part == 0 ? roff[v + 1] : in.off[v + 1];
       for (int32_t j = a + lane; j < b; j += kWave) {
-        if (bit_of(part == 0 ? rdead : in.dead, j)) continue;
+        const bool gone = bit_of(part == 0 ? rdead : in.dead, j);
         const int32_t u = part == 0 ? radj[j] : in.adj[j];
         const int32_t wt = part == 0 ? rw[j] : in.w[j];
+        if (gone) continue;
         const int32_t du = ddist(__hip_atomic_load(&dp[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         if (du >= kInf2) continue;
         const unsigned long long k = dpack(du + wt, uint32_t(u));

@@ -1186,7 +1186,7 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
   SPM_TRY(launch(g, "sssp_seed", sssp_seed, dim3(1), dim3(kSsspBlock), 0, n > 1 ? (const int32_t*)w->srcbuf : nullptr,
                  n, srcs[0], view.perm, w->view.off, w->dist, flags ? (int32_t*)nullptr : w->dprev, w->q[slot],
                  w->ctr, slot, banded ? w->band_tot : nullptr,
-                 w->run_start, w->host_base + (banded ? 1u : 0u), w->big, act));
+                 w->run_start, w->host_base, w->big, act));
   if (banded) {
     // One source: tiles over its out-row (at most max_outdeg <= V slots, so
     // at most sssp_band_tiles(V) tiles, which tile_b is sized for).
@@ -1206,8 +1206,9 @@ int sssp_run(spm_graph* g, const int32_t* srcs, int32_t n, int32_t* d_out, int32
     SPM_TRY(launch(g, "sssp_emit_b", sssp_emit_b, dim3(tiles), dim3(kSsspBlock), 0, g->V, w->view.off,
                    w->view.adj, view.perm, src1, (const int32_t*)w->dist, w->dprev, delta, tiles,
                    (const uint2*)w->tile_b, (const unsigned long long*)w->band_tot, w->q[0], w->ctr + 0));
-    SPM_TRY(launch(g, "sssp_advance", sssp_advance, dim3(1), dim3(1), 0, w->round_base, 1u));
-    w->host_base += 1;
+    // The seeds' round takes no round stamp (it neither pushes through the
+    // stamps nor enters the big-round log), so the captured rounds start at
+    // the present base.
   }
   // Rounds after the frontier empties still cost their launches (~13 us a
   // round at RMAT-22), so in queue mode the first replay is as long as the

@@ -268,7 +268,7 @@ __device__ __forceinline__ float pr_term(const float* __restrict__ c, const PrHo
 }
 
 // W partial sums (and W loads in flight per lane); 16 on the staged sweep
-// measured slower than 8 (RMAT-22 28.76 against 28.41 ms per run).
+// measured slower than 8 (RMAT-22 27.92 against 27.22 ms per run).
 template <bool HOT = false, int W = 8>
 __device__ __forceinline__ float gather_sum(const int32_t* __restrict__ radj, const float* __restrict__ c,
                                             int32_t ea, int32_t eb, int32_t start, int32_t stride,

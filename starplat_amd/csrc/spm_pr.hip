@@ -427,30 +427,42 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter(
 // per workgroup.
 constexpr int kLagMaxGrid = 2048;
 
-// pr_check's sum of partial[0..n), reproduced by a 256-lane workgroup: lane
-// tid stands for pr_check's threads tid, tid+256, tid+512 and tid+768, whose
-// lanes sit in waves wid, wid+4, wid+8 and wid+12 of the 1024-lane check.
-__device__ __forceinline__ float lag_sum(const float* __restrict__ partial, int n, float* s_red16) {
-  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+// pr_check's sum of partial[0..n), reproduced by a 256-lane workgroup in two
+// steps (load, then reduce), so the loads can be issued before a sweep's
+// gathers and summed after them. Lane tid stands for pr_check's threads tid,
+// tid+256, tid+512 and tid+768, whose lanes sit in waves wid, wid+4, wid+8
+// and wid+12 of the 1024-lane check. With n <= kLagMaxGrid (n4 <= 512) each
+// of those threads reads at most one 16-byte word (pr_check's second
+// accumulator stays +0) and thread tid < (n & 3) one scalar.
+struct LagShares {
+  float4 x[4];
+  float tail;
+};
+
+__device__ __forceinline__ LagShares lag_load(const float* __restrict__ partial, int n) {
+  const int tid = threadIdx.x;
   const float4* p4 = reinterpret_cast<const float4*>(partial);
   const int n4 = n >> 2;
-  float t[4];
+  LagShares l;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int vt = tid + k * kPrBlock;
-    float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
-    int i = vt;
-    for (; i + kCheckBlock < n4; i += 2 * kCheckBlock) {
-      const float4 x = p4[i], y = p4[i + kCheckBlock];
-      a.x += x.x; a.y += x.y; a.z += x.z; a.w += x.w;
-      b.x += y.x; b.y += y.y; b.z += y.z; b.w += y.w;
-    }
-    if (i < n4) {
-      const float4 x = p4[i];
-      a.x += x.x; a.y += x.y; a.z += x.z; a.w += x.w;
-    }
+    l.x[k] = vt < n4 ? p4[vt] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  l.tail = tid < (n & 3) ? partial[(n4 << 2) + tid] : 0.f;
+  return l;
+}
+
+__device__ __forceinline__ float lag_reduce(const LagShares& l, int n, float* s_red16) {
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  float t[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 b = a;
+    a.x += l.x[k].x; a.y += l.x[k].y; a.z += l.x[k].z; a.w += l.x[k].w;
     t[k] = ((a.x + b.x) + (a.y + b.y)) + ((a.z + b.z) + (a.w + b.w));
-    if (vt < (n & 3)) t[k] += partial[(n4 << 2) + vt];
+    if (k == 0 && tid < (n & 3)) t[k] += l.tail;
   }
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -463,7 +475,62 @@ __device__ __forceinline__ float lag_sum(const float* __restrict__ partial, int 
   return d;
 }
 
+// The sums of pr_sweep_block (single job, one pass, 256-lane blocks) without
+// the writes: the lane that would write vertex ev gets its sum, old rank and
+// out-degree (ev < 0 elsewhere).
 template <int G>
+__device__ __forceinline__ void pr_block_sums(int b, int4 row, const int32_t* __restrict__ radj,
+                                              const float* __restrict__ degf, const PrBins& bins,
+                                              const float* __restrict__ pr_in, const float* __restrict__ c_in,
+                                              float* s_red, int32_t& ev, float& esum, float& eold, float& edg) {
+  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
+  const int32_t v = row.x, ea = row.y, eb = row.z;
+  float old = 0.f, dg = 0.f;
+  if (v >= 0) {
+    old = pr_in[v];
+    dg = degf[v];
+  }
+  ev = -1;
+  eold = old;
+  edg = dg;
+  if (b < bins.blk[1]) {
+    if (v >= 0) {
+      esum = gather_sum<false>(radj, c_in, ea, eb, 0, 1);
+      ev = v;
+    }
+  } else if (b < bins.blk[2]) {
+    const int g = tid % G;
+    float sum = v >= 0 ? gather_sum<false>(radj, c_in, ea, eb, g, G) : 0.f;
+    sum = group_sum<G>(sum);
+    esum = sum;
+    if (v >= 0 && g == 0) ev = v;
+  } else if (b < bins.blk[3]) {
+    float sum = v >= 0 ? gather_sum<false>(radj, c_in, ea, eb, lane, kWave) : 0.f;
+    sum = wave_sum(sum);
+    esum = sum;
+    if (v >= 0 && lane == 0) ev = v;
+  } else if (v >= 0) {
+    float sum = gather_sum<false>(radj, c_in, ea, eb, tid, kPrBlock);
+    sum = wave_sum(sum);
+    if (lane == 0) s_red[wid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+      float t = 0.f;
+      for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
+      esum = t;
+      ev = v;
+    }
+    __syncthreads();
+  }
+}
+
+// OVERLAP: the sums are formed before the verdict and written after it (the
+// shares' loads overlap the gathers); otherwise the verdict comes first and
+// the sweep writes as it goes. The first pays on skewed graphs, whose long
+// rows set the sweep's length (RMAT-16 0.917 -> 0.836 ms a run), the second
+// on even ones, where the barrier between the sums and the writes costs
+// more (ER 2^16 0.826 against 0.859 ms).
+template <int G, bool OVERLAP>
 __global__ __launch_bounds__(kPrBlock) void pr_iter_lag(
     const int32_t* __restrict__ roff, const int32_t* __restrict__ radj, const int4* __restrict__ rows,
     const float* __restrict__ degf, PrBins bins, const float* __restrict__ pr_in, const float* __restrict__ c_in,
@@ -473,11 +540,24 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter_lag(
   const int4 row = pr_row<G>(rows, blockIdx.x, bins);
   const PrState s = *st;
   if (__hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  // The previous sweep's shares load first and the sweep's sums are formed
+  // while they are in flight; the verdict then decides whether the sums are
+  // written (a stopping sweep discards them), so the loop test costs no
+  // round trip of its own.
+  LagShares shares;
+  if (r > 0) shares = lag_load(prev, bins.blk[4]);
+  // In-degree-0 vertices only, from the third sweep on (see pr_iter).
+  const bool zero = int(blockIdx.x) < bins.zb && s.iter + r >= 2;
+  int32_t ev = -1;
+  float esum = 0.f, eold = 0.f, edg = 0.f;
+  if (OVERLAP && !zero)
+    pr_block_sums<G>(blockIdx.x, row, radj, degf, bins, pr_in, c_in, s_red, ev, esum, eold, edg);
   if (r > 0) {
     // Every workgroup computes the same d and it. Workgroup 0 alone writes;
     // a workgroup that reads its new iter or flag also stops (a larger
     // count fails the test too), so the verdict is the same everywhere.
-    const float d = lag_sum(prev, bins.blk[4], s_red);
+    if (OVERLAP) __syncthreads();  // s_red: the sums' reduction is done with it
+    const float d = lag_reduce(shares, bins.blk[4], s_red);
     const int it = s.iter + r;
     if (!(d > s.beta && it < s.maxIter)) {
       if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -486,17 +566,21 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter_lag(
       }
       return;
     }
-    __syncthreads();  // s_red is reused by the sweep
+    __syncthreads();  // s_red is reused below
   }
-  if (int(blockIdx.x) < bins.zb && s.iter + r >= 2) {  // in-degree-0 vertices only (see pr_iter)
+  if (zero) {
     if (threadIdx.x == 0) partial[blockIdx.x] = 0.f;
     return;
   }
   const PrSeg sg{nullptr, nullptr, 1, 0};
   const PrJobs jb{};
   float mydiff = 0.f;
-  pr_sweep_block<G, false>(blockIdx.x, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb, s_red,
-                           mydiff);
+  if (OVERLAP) {
+    if (ev >= 0) pr_emit<false>(ev, esum, sg, s, eold, edg, pr_out, c_out, mydiff, jb);
+  } else {
+    pr_sweep_block<G, false>(blockIdx.x, row, roff, radj, degf, bins, pr_in, c_in, pr_out, c_out, s, sg, jb, s_red,
+                             mydiff);
+  }
   pr_finish(mydiff, partial, s_red, blockIdx.x);
 }
 
@@ -1070,12 +1154,17 @@ static int pr_enqueue_iters(spm_graph* g, PrWork* w, int R, bool push) {
       continue;
     }
     if (pr_lagged(g, w)) {
+      // Skewed in-degrees (the largest out-degree 16x the mean or more, as
+      // a proxy): sums before the verdict.
+      const double avg = g->V > 0 ? double(g->E) / double(g->V) : 0.0;
+      const bool overlap = double(g->max_outdeg) >= 16.0 * avg;
       float* mine = w->partial + (r & 1) * kLagMaxGrid;
       const float* prev = w->partial + (1 - (r & 1)) * kLagMaxGrid;
       switch (w->bins.group) {
 #define SPM_PR_LAG_LAUNCH(G)                                                                               \
   case G:                                                                                                  \
-    SPM_TRY(launch(g, "pr_iter_lag", pr_iter_lag<G>, grid, block, 0, w->view.roff, w->view.radj,          \
+    SPM_TRY(launch(g, "pr_iter_lag", overlap ? pr_iter_lag<G, true> : pr_iter_lag<G, false>, grid, block, 0, \
+                   w->view.roff, w->view.radj,                                                             \
                    (const int4*)w->rows, (const float*)w->degf, w->bins, pin, cin, w->pr[1 - p], w->c[1 - p], \
                    w->st, prev, mine, r));                                                                 \
     break;

@@ -341,14 +341,15 @@ __device__ __forceinline__ int4 pr_row(const int4* __restrict__ rows, int b, con
 // lanes each, of bin 2 a wavefront each, of bin 3 the whole workgroup. Adds
 // the workgroup's |new - old| to the calling thread's mydiff. Control flow
 // is uniform across the workgroup.
+// One block's in-row sums, without writing: the lane that writes vertex ev
+// gets its sum, old rank and out-degree (ev < 0 in the other lanes).
 template <int G, bool MULTI, int B = kPrBlock, bool HOT = false>
-__device__ __forceinline__ void pr_sweep_block(int b, int4 row, const int32_t* __restrict__ roff,
-                                               const int32_t* __restrict__ radj, const float* __restrict__ degf,
-                                               const PrBins& bins, const float* __restrict__ pr_in,
-                                               const float* __restrict__ c_in, float* __restrict__ pr_out,
-                                               float* __restrict__ c_out, const PrState& s, const PrSeg& sg,
-                                               const PrJobs& jb, float* s_red, float& mydiff,
-                                               const PrHot& hot = PrHot{nullptr, 0}) {
+__device__ __forceinline__ void pr_block_sums(int b, int4 row, const int32_t* __restrict__ roff,
+                                              const int32_t* __restrict__ radj, const float* __restrict__ degf,
+                                              const PrBins& bins, const float* __restrict__ pr_in,
+                                              const float* __restrict__ c_in, const PrSeg& sg, const PrJobs& jb,
+                                              float* s_red, int32_t& ev, float& esum, float& eold, float& edg,
+                                              const PrHot& hot = PrHot{nullptr, 0}) {
   const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
   int32_t v = row.x, ea = row.y, eb = row.z;
   if (v >= 0 && !pr_live<MULTI>(v, jb)) v = -1;
@@ -359,21 +360,27 @@ __device__ __forceinline__ void pr_sweep_block(int b, int4 row, const int32_t* _
     old = pr_in[v];
     dg = degf[v];
   }
+  ev = -1;
+  esum = 0.f;
+  eold = old;
+  edg = dg;
   if (b < bins.blk[1]) {
     // bin 0: one lane per vertex
-    if (v >= 0)
-      pr_emit<MULTI>(v, gather_sum<HOT>(radj, c_in, ea, eb, 0, 1, hot), sg, s, old, dg, pr_out, c_out, mydiff, jb);
+    if (v >= 0) {
+      esum = gather_sum<HOT>(radj, c_in, ea, eb, 0, 1, hot);
+      ev = v;
+    }
   } else if (b < bins.blk[2]) {
     // bin 1: G lanes per vertex
     const int g = tid % G;
     float sum = v >= 0 ? gather_sum<HOT>(radj, c_in, ea, eb, g, G, hot) : 0.f;
-    sum = group_sum<G>(sum);
-    if (v >= 0 && g == 0) pr_emit<MULTI>(v, sum, sg, s, old, dg, pr_out, c_out, mydiff, jb);
+    esum = group_sum<G>(sum);
+    if (v >= 0 && g == 0) ev = v;
   } else if (b < bins.blk[3]) {
     // bin 2: one wavefront per vertex
     float sum = v >= 0 ? gather_sum<HOT>(radj, c_in, ea, eb, lane, kWave, hot) : 0.f;
-    sum = wave_sum(sum);
-    if (v >= 0 && lane == 0) pr_emit<MULTI>(v, sum, sg, s, old, dg, pr_out, c_out, mydiff, jb);
+    esum = wave_sum(sum);
+    if (v >= 0 && lane == 0) ev = v;
   } else if (v >= 0) {
     // bin 3: one workgroup per vertex (v is the same in every lane)
     float sum = gather_sum<HOT>(radj, c_in, ea, eb, tid, B, hot);
@@ -383,10 +390,27 @@ __device__ __forceinline__ void pr_sweep_block(int b, int4 row, const int32_t* _
     if (tid == 0) {
       float t = 0.f;
       for (int w = 0; w < B / kWave; ++w) t += s_red[w];
-      pr_emit<MULTI>(v, t, sg, s, old, dg, pr_out, c_out, mydiff, jb);
+      esum = t;
+      ev = v;
     }
     __syncthreads();
   }
+}
+
+// One block's sweep: the sums, then the writes.
+template <int G, bool MULTI, int B = kPrBlock, bool HOT = false>
+__device__ __forceinline__ void pr_sweep_block(int b, int4 row, const int32_t* __restrict__ roff,
+                                               const int32_t* __restrict__ radj, const float* __restrict__ degf,
+                                               const PrBins& bins, const float* __restrict__ pr_in,
+                                               const float* __restrict__ c_in, float* __restrict__ pr_out,
+                                               float* __restrict__ c_out, const PrState& s, const PrSeg& sg,
+                                               const PrJobs& jb, float* s_red, float& mydiff,
+                                               const PrHot& hot = PrHot{nullptr, 0}) {
+  int32_t ev;
+  float esum, eold, edg;
+  pr_block_sums<G, MULTI, B, HOT>(b, row, roff, radj, degf, bins, pr_in, c_in, sg, jb, s_red, ev, esum, eold, edg,
+                                  hot);
+  if (ev >= 0) pr_emit<MULTI>(ev, esum, sg, s, eold, edg, pr_out, c_out, mydiff, jb);
 }
 
 template <int G, bool MULTI>
@@ -475,55 +499,6 @@ __device__ __forceinline__ float lag_reduce(const LagShares& l, int n, float* s_
   return d;
 }
 
-// The sums of pr_sweep_block (single job, one pass, 256-lane blocks) without
-// the writes: the lane that would write vertex ev gets its sum, old rank and
-// out-degree (ev < 0 elsewhere).
-template <int G>
-__device__ __forceinline__ void pr_block_sums(int b, int4 row, const int32_t* __restrict__ radj,
-                                              const float* __restrict__ degf, const PrBins& bins,
-                                              const float* __restrict__ pr_in, const float* __restrict__ c_in,
-                                              float* s_red, int32_t& ev, float& esum, float& eold, float& edg) {
-  const int tid = threadIdx.x, lane = __lane_id(), wid = tid / kWave;
-  const int32_t v = row.x, ea = row.y, eb = row.z;
-  float old = 0.f, dg = 0.f;
-  if (v >= 0) {
-    old = pr_in[v];
-    dg = degf[v];
-  }
-  ev = -1;
-  eold = old;
-  edg = dg;
-  if (b < bins.blk[1]) {
-    if (v >= 0) {
-      esum = gather_sum<false>(radj, c_in, ea, eb, 0, 1);
-      ev = v;
-    }
-  } else if (b < bins.blk[2]) {
-    const int g = tid % G;
-    float sum = v >= 0 ? gather_sum<false>(radj, c_in, ea, eb, g, G) : 0.f;
-    sum = group_sum<G>(sum);
-    esum = sum;
-    if (v >= 0 && g == 0) ev = v;
-  } else if (b < bins.blk[3]) {
-    float sum = v >= 0 ? gather_sum<false>(radj, c_in, ea, eb, lane, kWave) : 0.f;
-    sum = wave_sum(sum);
-    esum = sum;
-    if (v >= 0 && lane == 0) ev = v;
-  } else if (v >= 0) {
-    float sum = gather_sum<false>(radj, c_in, ea, eb, tid, kPrBlock);
-    sum = wave_sum(sum);
-    if (lane == 0) s_red[wid] = sum;
-    __syncthreads();
-    if (tid == 0) {
-      float t = 0.f;
-      for (int w = 0; w < kPrBlock / kWave; ++w) t += s_red[w];
-      esum = t;
-      ev = v;
-    }
-    __syncthreads();
-  }
-}
-
 // OVERLAP: the sums are formed before the verdict and written after it (the
 // shares' loads overlap the gathers); otherwise the verdict comes first and
 // the sweep writes as it goes. The first pays on skewed graphs, whose long
@@ -551,7 +526,8 @@ __global__ __launch_bounds__(kPrBlock) void pr_iter_lag(
   int32_t ev = -1;
   float esum = 0.f, eold = 0.f, edg = 0.f;
   if (OVERLAP && !zero)
-    pr_block_sums<G>(blockIdx.x, row, radj, degf, bins, pr_in, c_in, s_red, ev, esum, eold, edg);
+    pr_block_sums<G, false>(blockIdx.x, row, roff, radj, degf, bins, pr_in, c_in, PrSeg{nullptr, nullptr, 1, 0},
+                            PrJobs{}, s_red, ev, esum, eold, edg);
   if (r > 0) {
     // Every workgroup computes the same d and it. Workgroup 0 alone writes;
     // a workgroup that reads its new iter or flag also stops (a larger

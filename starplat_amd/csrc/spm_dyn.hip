@@ -377,26 +377,33 @@ __global__ __launch_bounds__(kDynBlock) void dtc_count(int32_t n, const int32_t*
     constexpr int Q = kDtcChunk / kWave;
     const int32_t b2 = off[v2], e2 = off[v2 + 1];
     int32_t v3[Q], slo[Q], shi[Q];
+    // The candidates load first, all of them, and are tested afterwards
+    // (testing each as it arrives put every load's wait before the next).
+    int32_t xs[Q];
+    bool lv[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const int64_t p = base + q * kWave + lane;
-      v3[q] = -1;
-      slo[q] = shi[q] = b2;
+      xs[q] = -1;
+      lv[q] = false;
       if (p >= len) continue;
-      int32_t x;
-      bool live;
       if (p < la) {
         const int32_t j = a + int32_t(p);
-        live = !bit_of(dead, j);
-        x = adj[j];
+        lv[q] = !bit_of(dead, j);
+        xs[q] = adj[j];
       } else {
         const int32_t j = da + int32_t(p - la);
-        live = !bit_of(d.dead, j);
-        x = d.adj[j];
+        lv[q] = !bit_of(d.dead, j);
+        xs[q] = d.adj[j];
       }
-      if (!live || x == v1 || x == v2) continue;
-      v3[q] = x;
-      shi[q] = e2;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int32_t x = xs[q];
+      const bool take = lv[q] && x >= 0 && x != v1 && x != v2;
+      v3[q] = take ? x : -1;
+      slo[q] = b2;
+      shi[q] = take ? e2 : b2;
     }
     for (bool more = true; more;) {  // lower bound of each v3 in v2's row
       int32_t mid[Q], val[Q];

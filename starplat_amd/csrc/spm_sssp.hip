@@ -1054,8 +1054,9 @@ int sssp_release(spm_graph* g) {
   return SPM_OK;
 }
 
-// Default: 6 workgroups per CU (measured at RMAT-22: 1536 workgroups 1.635 ms,
-// 1024 1.672 ms, 2048 1.658 ms, 768 1.758 ms).
+// Default: 6 workgroups per CU (measured at RMAT-22, ms per run: 1536
+// workgroups 1.155, 1024 1.205, 1280 1.182, 1792 1.475, 2048 1.414, 2560
+// 1.261 -- more resident relaxations only add contention).
 static int sssp_grid(const spm_graph* g) {
   return g->sched.sssp_grid > 0 ? g->sched.sssp_grid : g->num_cus * 6;
 }
